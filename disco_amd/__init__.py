@@ -1,0 +1,301 @@
+"""MI355X (gfx950) implementation of Disco's per-timestep finite-volume update.
+
+Python mirror of the reference's driver-level interface over the C ABI in
+``include/disco_gpu.h`` (``disco_amd/_lib/libdisco_gpu.so``):
+
+* :func:`read_par_file` / :func:`parse_par`  -- ``read_par_file`` (readpar.c:44-210)
+* :func:`setup_run`                          -- setupGrid/setupDomain/setupCells on the host
+* :class:`Domain`                            -- the device-resident ``struct domain``:
+  ``getmindt`` (misc.c:39-84), ``timestep`` (timestep.c:7-54), ``step`` (the
+  main loop of main.c:68-75 without output), ``state``/``upload``.
+
+There is no CPU fallback: if the HIP library is missing, every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libdisco_gpu.so")
+
+HYDRO = {"euler": 0, "mhd": 1, "grmhd": 2}
+NQ_OF = {0: 5, 1: 8, 2: 8}
+
+
+class Params(C.Structure):
+    """Mirror of ``disco_params`` (disco_amd/csrc/disco_params.h)."""
+
+    _fields_ = [
+        ("t_min", C.c_double), ("t_max", C.c_double),
+        ("Num_R", C.c_int), ("Num_Z", C.c_int),
+        ("aspect", C.c_double),
+        ("rmin", C.c_double), ("rmax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+        ("phimax", C.c_double),
+        ("NoBC_Rmin", C.c_int), ("NoBC_Rmax", C.c_int), ("NoBC_Zmin", C.c_int), ("NoBC_Zmax", C.c_int),
+        ("LogZoning", C.c_int), ("R_Periodic", C.c_int), ("Z_Periodic", C.c_int),
+        ("LogRadius", C.c_double),
+        ("Mesh_Motion", C.c_int), ("Riemann_Solver", C.c_int), ("Timestep", C.c_int),
+        ("CFL", C.c_double), ("PLM", C.c_double), ("maxDT", C.c_double),
+        ("Density_Floor", C.c_double), ("Pressure_Floor", C.c_double),
+        ("Exact_Mesh_Omega", C.c_int),
+        ("Exact_Mesh_Omega_Par", C.c_double),
+        ("Energy_Omega", C.c_int),
+        ("Energy_Omega_Par", C.c_double),
+        ("RotFrame", C.c_int),
+        ("RotOmega", C.c_double), ("RotD", C.c_double),
+        ("Adiabatic_Index", C.c_double),
+        ("isothermal_flag", C.c_int),
+        ("Cs2_Profile", C.c_int),
+        ("Cs2_Par", C.c_double),
+        ("Disk_Mach", C.c_double),
+        ("Mass_Ratio", C.c_double),
+        ("grav2D", C.c_int),
+        ("grav_eps", C.c_double),
+        ("visc_flag", C.c_int),
+        ("CT", C.c_int),
+        ("metricPar0", C.c_int),
+        ("metricPar1", C.c_double), ("metricPar2", C.c_double), ("metricPar3", C.c_double),
+        ("metricPar4", C.c_double),
+        ("initPar0", C.c_int),
+        ("initPar1", C.c_double), ("initPar2", C.c_double), ("initPar3", C.c_double),
+        ("initPar4", C.c_double), ("initPar5", C.c_double), ("initPar6", C.c_double),
+        ("initPar7", C.c_double), ("initPar8", C.c_double),
+        ("noiseType", C.c_int),
+        ("noiseAbs", C.c_double), ("noiseRel", C.c_double),
+        ("hydro", C.c_int), ("boundary", C.c_int), ("planets", C.c_int), ("initial", C.c_int),
+        ("num_c", C.c_int), ("num_n", C.c_int),
+    ]
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.pointer(p)[0] = self
+        return p
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m disco_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    vp = C.c_void_p
+    L.disco_parse_par.argtypes = [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int]
+    L.disco_read_par.argtypes = [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int]
+    L.disco_gpu_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(vp)]
+    L.disco_setup_run.argtypes = [C.POINTER(Params), ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.disco_gpu_info.argtypes = [vp, C.c_int, ip]
+    L.disco_gpu_grid.argtypes = [vp, C.c_int, ip, dp, dp]
+    L.disco_gpu_mindt.argtypes = [vp, dp]
+    L.disco_gpu_step.argtypes = [vp, C.c_int]
+    L.disco_gpu_timestep.argtypes = [vp, dp]
+    L.disco_gpu_sync.argtypes = [vp]
+    L.disco_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
+    L.disco_gpu_upload.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
+    L.disco_gpu_time.argtypes = [vp, dp, dp, dp]
+    L.disco_gpu_kernel_times.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, dp, ip]
+    L.disco_gpu_time_steps.argtypes = [vp, C.c_int, dp]
+    L.disco_gpu_destroy.argtypes = [vp]
+    L.disco_gpu_destroy.restype = None
+    L.disco_gpu_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _err(L) -> str:
+    return L.disco_gpu_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {_err(lib())}")
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def parse_par(text: str) -> Params:
+    """``read_par_file`` on in.par text; raises on missing keys (readpar.c:199-206)."""
+    L = lib()
+    p = Params()
+    buf = C.create_string_buffer(2048)
+    rc = L.disco_parse_par(text.encode(), C.byref(p), buf, len(buf))
+    if rc != 0:
+        raise ValueError(buf.value.decode())
+    return p
+
+
+def read_par_file(path: str) -> Params:
+    with open(path) as f:
+        return parse_par(f.read())
+
+
+@dataclass
+class RunInfo:
+    Nr: int
+    Nz: int
+    NgRa: int
+    NgRb: int
+    NgZa: int
+    NgZb: int
+    ncell: int
+    nreal: int
+    nq: int
+    npl: int
+    nruns: int
+    ncell_total: int
+
+    @staticmethod
+    def from_array(a) -> "RunInfo":
+        return RunInfo(*[int(v) for v in a[:12]])
+
+
+def setup_run(p: Params) -> dict:
+    """Host-only set-up (gridsetup.c + domain.c), returned as numpy arrays."""
+    L = lib()
+    info = np.zeros(12, np.int32)
+    _check(L.disco_setup_run(C.byref(p), _iptr(info), None, None, None, None, None, None, None, None, None),
+           "disco_setup_run")
+    I = RunInfo.from_array(info)
+    out = dict(
+        info=I,
+        Np=np.zeros(I.Nr * I.Nz, np.int32),
+        r_jph=np.zeros(I.Nr + 1),
+        z_kph=np.zeros(I.Nz + 1),
+        prim=np.zeros(I.nq * I.ncell),
+        cons=np.zeros(I.nq * I.ncell),
+        piph=np.zeros(I.ncell),
+        dphi=np.zeros(I.ncell),
+        wiph=np.zeros(I.ncell),
+        planet_phi=np.zeros(2),
+    )
+    _check(L.disco_setup_run(C.byref(p), _iptr(info), _iptr(out["Np"]), _dptr(out["r_jph"]),
+                             _dptr(out["z_kph"]), _dptr(out["prim"]), _dptr(out["cons"]), _dptr(out["piph"]),
+                             _dptr(out["dphi"]), _dptr(out["wiph"]), _dptr(out["planet_phi"])),
+           "disco_setup_run")
+    out["prim"] = out["prim"].reshape(I.nq, I.ncell)
+    out["cons"] = out["cons"].reshape(I.nq, I.ncell)
+    return out
+
+
+class Domain:
+    """Device-resident batch of runs (the reference's ``struct domain``)."""
+
+    def __init__(self, params):
+        if isinstance(params, Params):
+            params = [params]
+        self.params = [p.copy() for p in params]
+        L = lib()
+        arr = (Params * len(self.params))(*self.params)
+        h = C.c_void_p()
+        _check(L.disco_gpu_create(arr, len(self.params), C.byref(h)), "disco_gpu_create")
+        self._h = h
+        self.nruns = len(self.params)
+        self.info = [self._info(r) for r in range(self.nruns)]
+
+    def _info(self, run: int) -> RunInfo:
+        a = np.zeros(12, np.int32)
+        _check(lib().disco_gpu_info(self._h, run, _iptr(a)), "disco_gpu_info")
+        return RunInfo.from_array(a)
+
+    def grid(self, run: int = 0) -> dict:
+        I = self.info[run]
+        Np = np.zeros(I.Nr * I.Nz, np.int32)
+        r = np.zeros(I.Nr + 1)
+        z = np.zeros(I.Nz + 1)
+        _check(lib().disco_gpu_grid(self._h, run, _iptr(Np), _dptr(r), _dptr(z)), "disco_gpu_grid")
+        return dict(Np=Np, r_jph=r, z_kph=z)
+
+    def getmindt(self) -> np.ndarray:
+        dt = np.zeros(self.nruns)
+        _check(lib().disco_gpu_mindt(self._h, _dptr(dt)), "disco_gpu_mindt")
+        return dt
+
+    def timestep(self, dt) -> None:
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, float), (self.nruns,)))
+        _check(lib().disco_gpu_timestep(self._h, _dptr(dt)), "disco_gpu_timestep")
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().disco_gpu_step(self._h, int(nsteps)), "disco_gpu_step")
+
+    def sync(self) -> None:
+        _check(lib().disco_gpu_sync(self._h), "disco_gpu_sync")
+
+    def time(self) -> dict:
+        t = np.zeros(self.nruns)
+        dt = np.zeros(self.nruns)
+        pp = np.zeros(2 * self.nruns)
+        _check(lib().disco_gpu_time(self._h, _dptr(t), _dptr(dt), _dptr(pp)), "disco_gpu_time")
+        return dict(t=t, dt=dt, planet_phi=pp.reshape(self.nruns, 2))
+
+    def state(self, run: int = 0) -> dict:
+        I = self.info[run]
+        prim = np.zeros(I.nq * I.ncell)
+        cons = np.zeros(I.nq * I.ncell)
+        piph = np.zeros(I.ncell)
+        dphi = np.zeros(I.ncell)
+        wiph = np.zeros(I.ncell)
+        _check(lib().disco_gpu_download(self._h, run, _dptr(prim), _dptr(cons), _dptr(piph), _dptr(dphi),
+                                        _dptr(wiph)), "disco_gpu_download")
+        return dict(prim=prim.reshape(I.nq, I.ncell), cons=cons.reshape(I.nq, I.ncell), piph=piph, dphi=dphi,
+                    wiph=wiph)
+
+    def upload(self, run: int, prim=None, cons=None, piph=None, dphi=None, wiph=None) -> None:
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+        prim, cons, piph, dphi, wiph = map(c, (prim, cons, piph, dphi, wiph))
+        _check(lib().disco_gpu_upload(self._h, run, _dptr(prim), _dptr(cons), _dptr(piph), _dptr(dphi),
+                                      _dptr(wiph)), "disco_gpu_upload")
+
+    def kernel_times(self, nsteps: int) -> dict:
+        maxk = 32
+        names = C.create_string_buffer(32 * maxk)
+        avg = np.zeros(maxk)
+        cnt = np.zeros(maxk, np.int32)
+        k = lib().disco_gpu_kernel_times(self._h, int(nsteps), maxk, names, _dptr(avg), _iptr(cnt))
+        if k < 0:
+            raise RuntimeError(f"disco_gpu_kernel_times failed: {_err(lib())}")
+        out = {}
+        for i in range(k):
+            nm = names.raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (float(avg[i]), int(cnt[i]))
+        return out
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(lib().disco_gpu_time_steps(self._h, int(nsteps), C.byref(ms)), "disco_gpu_time_steps")
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().disco_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def config_path(name: str) -> str:
+    """Path of a config shipped in the repository's ``configs/``."""
+    return os.path.join(os.path.dirname(_HERE), "configs", name)

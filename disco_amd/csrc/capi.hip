@@ -1,0 +1,552 @@
+// C ABI (include/disco_gpu.h): device-resident batch of runs, graph-captured
+// RK2 steps, upload/download and timing.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/disco_gpu.h"
+#include "device_state.h"
+#include "host_setup.h"
+
+namespace disco {
+void launch_dt(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_substep(const DevState& S, double RK, double dtfac, int first, hipStream_t st,
+                    const LaunchHook* hook);
+}  // namespace disco
+
+using namespace disco;
+
+static thread_local std::string g_err;
+
+static int fail(const std::string& m) {
+    g_err = m;
+    return -1;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+struct disco_gpu {
+    std::vector<HostRun> runs;
+    std::vector<RunDev> hruns;
+    std::vector<AnnDev> hann;
+    std::vector<int> run_cell0, run_ann0;
+    int nruns = 0, ncell = 0, nann = 0, nq = 5;
+    AnnDev* ann = nullptr;
+    int* cell_ann = nullptr;
+    RunDev* druns = nullptr;
+    double *prim = nullptr, *cons = nullptr, *rkcons = nullptr, *gradp = nullptr, *gradr = nullptr;
+    double *piph[2] = {nullptr, nullptr}, *dphi[2] = {nullptr, nullptr}, *wiph = nullptr;
+    int *lbo = nullptr, *ubi = nullptr, *rot = nullptr;
+    double *fphi = nullptr, *fout = nullptr, *fin = nullptr;
+    unsigned long long* dtbits = nullptr;
+    double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
+    int parity = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+static DevState make_state(disco_gpu* h, int substep) {
+    DevState S{};
+    S.ann = h->ann;
+    S.cell_ann = h->cell_ann;
+    S.runs = h->druns;
+    S.ncell = h->ncell;
+    S.nruns = h->nruns;
+    S.prim = h->prim;
+    S.cons = h->cons;
+    S.rkcons = h->rkcons;
+    S.gradp = h->gradp;
+    S.gradr = h->gradr;
+    int p = h->parity;
+    if (substep == 0) {
+        S.piph = h->piph[p];
+        S.dphi = h->dphi[p];
+        S.piph_n = h->piph[1 - p];
+        S.dphi_n = h->dphi[1 - p];
+    } else {
+        S.piph = h->piph[1 - p];
+        S.dphi = h->dphi[1 - p];
+        S.piph_n = h->piph[1 - p];
+        S.dphi_n = h->dphi[1 - p];
+    }
+    S.wiph = h->wiph;
+    S.lbo = h->lbo;
+    S.ubi = h->ubi;
+    S.rot = h->rot;
+    S.fphi = h->fphi;
+    S.fout = h->fout;
+    S.fin = h->fin;
+    S.dtbits = h->dtbits;
+    S.dt = h->dt;
+    S.planet_phi = h->planet_phi;
+    S.t = h->t;
+    return S;
+}
+
+// One step for the current parity: [getmindt] + onestep(0, dt, first) +
+// onestep(1/2, dt/2, last) (timestep.c:45-48).
+static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
+    DevState S0 = make_state(h, 0);
+    DevState S1 = make_state(h, 1);
+    if (with_dt) launch_dt(S0, h->stream, hook);
+    launch_substep(S0, 0.0, 1.0, 1, h->stream, hook);
+    launch_substep(S1, 0.5, 0.5, 0, h->stream, hook);
+}
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+static void build_tables(disco_gpu* h) {
+    h->hann.clear();
+    h->hruns.clear();
+    h->run_cell0.clear();
+    h->run_ann0.clear();
+    int cell0 = 0, ann0 = 0;
+    for (int r = 0; r < h->nruns; ++r) {
+        const HostRun& R = h->runs[r];
+        RunDev D{};
+        D.ph = R.ph;
+        D.ic = R.ic;
+        D.bc = R.par.boundary;
+        D.planets_mod = R.par.planets;
+        D.cell0 = cell0;
+        D.ncell = R.ncell;
+        D.ann0 = ann0;
+        D.nann = R.Nr * R.Nz;
+        D.Nr = R.Nr;
+        D.Nz = R.Nz;
+        for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) D.planets[p] = R.planets[p];
+        h->hruns.push_back(D);
+        h->run_cell0.push_back(cell0);
+        h->run_ann0.push_back(ann0);
+        // trans_flux ranges, misc.c:459-471 (no CT)
+        int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
+        int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
+        int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
+        for (int k = 0; k < R.Nz; ++k)
+            for (int j = 0; j < R.Nr; ++j) {
+                int jk = j + R.Nr * k;
+                AnnDev A{};
+                A.run = r;
+                A.j = j;
+                A.k = k;
+                A.np = R.Np[jk];
+                A.off = cell0 + R.off[jk];
+                A.outer = (j + 1 < R.Nr) ? ann0 + jk + 1 : -1;
+                A.inner = (j > 0) ? ann0 + jk - 1 : -1;
+                int f = 0;
+                bool kin = k >= kmin && k < kmax;
+                if (R.real_annulus(j, k)) f |= ANN_REAL | ANN_PHIFLUX;
+                if (kin && j >= jmin && j < jmax) f |= ANN_OUTFLUX;
+                if (kin && j - 1 >= jmin && j - 1 < jmax) f |= ANN_INFLUX;
+                bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic;
+                bool zghost = (k < R.NgZa || k >= R.Nz - R.NgZb) && !R.par.Z_Periodic &&
+                              R.par.boundary == DISCO_BC_FIXED3D;
+                if (!(f & ANN_REAL) && (rghost || zghost)) f |= ANN_GHOST_FILL;
+                A.flags = f;
+                A.rm = R.rj(j - 1);
+                A.rp = R.rj(j);
+                A.zm = R.zk(k - 1);
+                A.zp = R.zk(k);
+                A.rc = get_centroid(A.rp, A.rm, 1);
+                A.zc = get_centroid(A.zp, A.zm, 2);
+                if (j + 1 < R.Nr) {
+                    double rmc = get_centroid(R.rj(j), R.rj(j - 1), 1);
+                    double rpc = get_centroid(R.rj(j + 1), R.rj(j), 1);
+                    A.dxL = R.rj(j) - rmc;
+                    A.dxR = rpc - R.rj(j);
+                }
+                if (R.par.boundary == DISCO_BC_FIXED) {
+                    A.rc_b = .5 * (A.rp + A.rm);
+                    A.zc_b = .5 * (A.zp + A.zm);
+                } else {
+                    A.rc_b = A.rc;
+                    A.zc_b = A.zc;
+                }
+                h->hann.push_back(A);
+            }
+        cell0 += R.ncell;
+        ann0 += R.Nr * R.Nz;
+    }
+    h->ncell = cell0;
+    h->nann = ann0;
+}
+
+static int upload_run_state(disco_gpu* h, int r) {
+    const HostRun& R = h->runs[r];
+    size_t c0 = h->run_cell0[r];
+    size_t N = h->ncell;
+    for (int q = 0; q < h->nq; ++q) {
+        HIPCHK(hipMemcpy(h->prim + q * N + c0, R.prim.data() + (size_t)q * R.ncell, R.ncell * sizeof(double),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->cons + q * N + c0, R.cons.data() + (size_t)q * R.ncell, R.ncell * sizeof(double),
+                         hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(h->piph[h->parity] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->dphi[h->parity] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->wiph + c0, R.wiph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->rot + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int), hipMemcpyHostToDevice));
+    double pp[DISCO_MAX_PLANETS] = {0.0, 0.0};
+    for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) pp[p] = R.planets[p].phi;
+    HIPCHK(hipMemcpy(h->planet_phi + r * DISCO_MAX_PLANETS, pp, sizeof(pp), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->t + r, &R.t, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int capture_graphs(disco_gpu* h) {
+    for (int p = 0; p < 2; ++p) {
+        if (h->graph[p]) {
+            (void)hipGraphExecDestroy(h->graph[p]);
+            h->graph[p] = nullptr;
+        }
+    }
+    int saved = h->parity;
+    for (int p = 0; p < 2; ++p) {
+        h->parity = p;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(h, true, nullptr);
+        HIPCHK(hipStreamEndCapture(h->stream, &g));
+        HIPCHK(hipGraphInstantiate(&h->graph[p], g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    h->parity = saved;
+    return 0;
+}
+
+extern "C" {
+
+const char* disco_gpu_last_error(void) { return g_err.c_str(); }
+
+int disco_parse_par(const char* text, disco_params* out, char* errbuf, int errlen) {
+    std::string err;
+    int e = read_par_text(text ? text : "", out, &err);
+    if (errbuf && errlen > 0) {
+        std::snprintf(errbuf, errlen, "%s", err.c_str());
+    }
+    return e;
+}
+
+int disco_read_par(const char* path, disco_params* out, char* errbuf, int errlen) {
+    std::ifstream f(path);
+    if (!f) {
+        if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "cannot open %s", path);
+        return -1;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return disco_parse_par(ss.str().c_str(), out, errbuf, errlen);
+}
+
+void disco_gpu_destroy(disco_gpu_t* h) {
+    if (!h) return;
+    for (int p = 0; p < 2; ++p)
+        if (h->graph[p]) (void)hipGraphExecDestroy(h->graph[p]);
+    void* ptrs[] = {h->ann,  h->cell_ann, h->druns,   h->prim,   h->cons,   h->rkcons, h->gradp,
+                    h->gradr, h->piph[0], h->piph[1], h->dphi[0], h->dphi[1], h->wiph,  h->lbo,
+                    h->ubi,  h->rot,      h->fphi,    h->fout,   h->fin,    h->dtbits, h->dt,
+                    h->planet_phi, h->t};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
+    if (!out || !pars || nruns <= 0) return fail("disco_gpu_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail("no HIP device available");
+    disco_gpu* h = new disco_gpu();
+    h->nruns = nruns;
+    h->runs.resize(nruns);
+    for (int r = 0; r < nruns; ++r) {
+        std::string err;
+        if (setup_run(pars[r], &h->runs[r], &err)) {
+            delete h;
+            return fail("run " + std::to_string(r) + ": " + err);
+        }
+        if (h->runs[r].nq != 5) {
+            delete h;
+            return fail("only NUM_Q = 5 (euler) is built");
+        }
+    }
+    build_tables(h);
+    size_t N = h->ncell;
+    size_t Q = (size_t)h->nq * N;
+    std::vector<int> cell_ann(N);
+    for (int a = 0; a < h->nann; ++a)
+        for (int i = 0; i < h->hann[a].np; ++i) cell_ann[h->hann[a].off + i] = a;
+    int rc = 0;
+    rc |= dalloc(&h->ann, h->nann);
+    rc |= dalloc(&h->cell_ann, N);
+    rc |= dalloc(&h->druns, nruns);
+    rc |= dalloc(&h->prim, Q);
+    rc |= dalloc(&h->cons, Q);
+    rc |= dalloc(&h->rkcons, Q);
+    rc |= dalloc(&h->gradp, Q);
+    rc |= dalloc(&h->gradr, Q);
+    rc |= dalloc(&h->piph[0], N);
+    rc |= dalloc(&h->piph[1], N);
+    rc |= dalloc(&h->dphi[0], N);
+    rc |= dalloc(&h->dphi[1], N);
+    rc |= dalloc(&h->wiph, N);
+    rc |= dalloc(&h->lbo, N);
+    rc |= dalloc(&h->ubi, N);
+    rc |= dalloc(&h->rot, h->nann);
+    rc |= dalloc(&h->fphi, Q);
+    rc |= dalloc(&h->fout, Q);
+    rc |= dalloc(&h->fin, Q);
+    rc |= dalloc(&h->dtbits, nruns);
+    rc |= dalloc(&h->dt, nruns);
+    rc |= dalloc(&h->planet_phi, (size_t)nruns * DISCO_MAX_PLANETS);
+    rc |= dalloc(&h->t, nruns);
+    if (rc) {
+        std::string m = g_err;
+        disco_gpu_destroy(h);
+        return fail(m);
+    }
+    auto cleanup = [&](const std::string& m) {
+        disco_gpu_destroy(h);
+        return fail(m);
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+    if (hipMemcpy(h->ann, h->hann.data(), h->nann * sizeof(AnnDev), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->cell_ann, cell_ann.data(), N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->druns, h->hruns.data(), nruns * sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup("upload of tables failed");
+    // zero the scratch fields once (eager, outside any graph)
+    double* zq[] = {h->rkcons, h->gradp, h->gradr, h->fphi, h->fout, h->fin};
+    for (double* p : zq)
+        if (hipMemset(p, 0, Q * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
+    if (hipMemset(h->lbo, 0, N * sizeof(int)) != hipSuccess || hipMemset(h->ubi, 0, N * sizeof(int)) != hipSuccess ||
+        hipMemset(h->dt, 0, nruns * sizeof(double)) != hipSuccess)
+        return cleanup("hipMemset");
+    for (int r = 0; r < nruns; ++r) {
+        if (upload_run_state(h, r)) return cleanup(g_err);
+        // the other mesh buffer starts equal (substep 2 reads it)
+        size_t c0 = h->run_cell0[r];
+        const HostRun& R = h->runs[r];
+        if (hipMemcpy(h->piph[1] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->dphi[1] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup("upload");
+    }
+    if (capture_graphs(h)) return cleanup(g_err);
+    if (hipDeviceSynchronize() != hipSuccess) return cleanup("sync after create");
+    *out = h;
+    return 0;
+}
+
+int disco_setup_run(const disco_params* par, int* info, int* Np, double* r_jph, double* z_kph, double* prim,
+                    double* cons, double* piph, double* dphi, double* wiph, double* planet_phi) {
+    if (!par || !info) return fail("disco_setup_run: bad arguments");
+    HostRun R;
+    std::string err;
+    if (setup_run(*par, &R, &err)) return fail(err);
+    int v[12] = {R.Nr, R.Nz, R.NgRa, R.NgRb, R.NgZa, R.NgZb, R.ncell, (int)R.nreal, R.nq, (int)R.planets.size(), 1,
+                 R.ncell};
+    std::memcpy(info, v, sizeof v);
+    if (Np) std::memcpy(Np, R.Np.data(), R.Np.size() * sizeof(int));
+    if (r_jph) std::memcpy(r_jph, R.r_jph.data(), R.r_jph.size() * sizeof(double));
+    if (z_kph) std::memcpy(z_kph, R.z_kph.data(), R.z_kph.size() * sizeof(double));
+    if (prim) std::memcpy(prim, R.prim.data(), R.prim.size() * sizeof(double));
+    if (cons) std::memcpy(cons, R.cons.data(), R.cons.size() * sizeof(double));
+    if (piph) std::memcpy(piph, R.piph.data(), R.piph.size() * sizeof(double));
+    if (dphi) std::memcpy(dphi, R.dphi.data(), R.dphi.size() * sizeof(double));
+    if (wiph) std::memcpy(wiph, R.wiph.data(), R.wiph.size() * sizeof(double));
+    if (planet_phi)
+        for (size_t p = 0; p < R.planets.size(); ++p) planet_phi[p] = R.planets[p].phi;
+    return 0;
+}
+
+int disco_gpu_info(disco_gpu_t* h, int run, int* info) {
+    if (!h || run < 0 || run >= h->nruns || !info) return fail("disco_gpu_info: bad arguments");
+    const HostRun& R = h->runs[run];
+    int v[12] = {R.Nr, R.Nz, R.NgRa, R.NgRb, R.NgZa, R.NgZb, R.ncell, (int)R.nreal, R.nq, (int)R.planets.size(),
+                 h->nruns, h->ncell};
+    std::memcpy(info, v, sizeof v);
+    return 0;
+}
+
+int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kph) {
+    if (!h || run < 0 || run >= h->nruns) return fail("disco_gpu_grid: bad arguments");
+    const HostRun& R = h->runs[run];
+    if (Np) std::memcpy(Np, R.Np.data(), R.Np.size() * sizeof(int));
+    if (r_jph) std::memcpy(r_jph, R.r_jph.data(), R.r_jph.size() * sizeof(double));
+    if (z_kph) std::memcpy(z_kph, R.z_kph.data(), R.z_kph.size() * sizeof(double));
+    return 0;
+}
+
+int disco_gpu_mindt(disco_gpu_t* h, double* dt_out) {
+    if (!h) return fail("disco_gpu_mindt: null handle");
+    DevState S = make_state(h, 0);
+    launch_dt(S, h->stream, nullptr);
+    HIPCHK(hipGetLastError());
+    if (dt_out) {
+        HIPCHK(hipMemcpyAsync(dt_out, h->dt, h->nruns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int disco_gpu_step(disco_gpu_t* h, int nsteps) {
+    if (!h || nsteps < 0) return fail("disco_gpu_step: bad arguments");
+    for (int n = 0; n < nsteps; ++n) {
+        HIPCHK(hipGraphLaunch(h->graph[h->parity], h->stream));
+        h->parity ^= 1;
+    }
+    return 0;
+}
+
+int disco_gpu_timestep(disco_gpu_t* h, const double* dt) {
+    if (!h || !dt) return fail("disco_gpu_timestep: bad arguments");
+    HIPCHK(hipMemcpyAsync(h->dt, dt, h->nruns * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    enqueue_step(h, false, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->parity ^= 1;
+    return 0;
+}
+
+int disco_gpu_sync(disco_gpu_t* h) {
+    if (!h) return fail("disco_gpu_sync: null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int disco_gpu_download(disco_gpu_t* h, int run, double* prim, double* cons, double* piph, double* dphi,
+                       double* wiph) {
+    if (!h || run < 0 || run >= h->nruns) return fail("disco_gpu_download: bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const HostRun& R = h->runs[run];
+    size_t c0 = h->run_cell0[run], N = h->ncell, n = R.ncell;
+    for (int q = 0; q < h->nq; ++q) {
+        if (prim) HIPCHK(hipMemcpy(prim + q * n, h->prim + q * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (cons) HIPCHK(hipMemcpy(cons + q * n, h->cons + q * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (piph) HIPCHK(hipMemcpy(piph, h->piph[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (dphi) HIPCHK(hipMemcpy(dphi, h->dphi[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (wiph) HIPCHK(hipMemcpy(wiph, h->wiph + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* cons, const double* piph,
+                     const double* dphi, const double* wiph) {
+    if (!h || run < 0 || run >= h->nruns) return fail("disco_gpu_upload: bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HostRun& R = h->runs[run];
+    size_t n = R.ncell;
+    if (prim) std::memcpy(R.prim.data(), prim, h->nq * n * sizeof(double));
+    if (cons) std::memcpy(R.cons.data(), cons, h->nq * n * sizeof(double));
+    if (piph) std::memcpy(R.piph.data(), piph, n * sizeof(double));
+    if (dphi) std::memcpy(R.dphi.data(), dphi, n * sizeof(double));
+    if (wiph) std::memcpy(R.wiph.data(), wiph, n * sizeof(double));
+    // recompute the rotation point of every annulus for the new mesh
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
+        int o = R.off[jk], np = R.Np[jk];
+        R.rot[jk] = 0;
+        for (int i = 0; i < np; ++i) {
+            int im = i == 0 ? np - 1 : i - 1;
+            if (R.piph[o + i] < R.piph[o + im]) R.rot[jk] = i;
+        }
+    }
+    return upload_run_state(h, run);
+}
+
+int disco_gpu_time(disco_gpu_t* h, double* t, double* dt, double* planet_phi) {
+    if (!h) return fail("disco_gpu_time: null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (t) HIPCHK(hipMemcpy(t, h->t, h->nruns * sizeof(double), hipMemcpyDeviceToHost));
+    if (dt) HIPCHK(hipMemcpy(dt, h->dt, h->nruns * sizeof(double), hipMemcpyDeviceToHost));
+    if (planet_phi)
+        HIPCHK(hipMemcpy(planet_phi, h->planet_phi, h->nruns * DISCO_MAX_PLANETS * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return 0;
+}
+
+struct TimerCtx {
+    hipStream_t st;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
+};
+
+static void timer_before(void* ctx, const char* name) {
+    TimerCtx* T = (TimerCtx*)ctx;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, T->st);
+    T->ev.push_back({name, {a, b}});
+}
+
+static void timer_after(void* ctx, const char* name) {
+    (void)name;
+    TimerCtx* T = (TimerCtx*)ctx;
+    (void)hipEventRecord(T->ev.back().second.second, T->st);
+}
+
+int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, double* avg_ms, int* count) {
+    if (!h || nsteps <= 0) return fail("disco_gpu_kernel_times: bad arguments");
+    TimerCtx T;
+    T.st = h->stream;
+    LaunchHook hook{timer_before, timer_after, &T};
+    for (int n = 0; n < nsteps; ++n) {
+        enqueue_step(h, true, &hook);
+        h->parity ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    for (auto& e : T.ev) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        if (!acc.count(e.first)) order.push_back(e.first);
+        acc[e.first].first += ms;
+        acc[e.first].second += 1;
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    int k = 0;
+    for (auto& nm : order) {
+        if (k >= maxk) break;
+        if (names) std::snprintf(names + 32 * k, 32, "%s", nm.c_str());
+        if (avg_ms) avg_ms[k] = acc[nm].first / acc[nm].second;
+        if (count) count[k] = acc[nm].second;
+        ++k;
+    }
+    return k;
+}
+
+int disco_gpu_time_steps(disco_gpu_t* h, int nsteps, double* ms) {
+    if (!h || nsteps < 0 || !ms) return fail("disco_gpu_time_steps: bad arguments");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, h->stream));
+    if (disco_gpu_step(h, nsteps)) return -1;
+    HIPCHK(hipEventRecord(b, h->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Device-resident layout of a batch of disk runs.
+//
+// All runs of a batch share one flat SoA: per-cell fields are stored
+// field-major, value (q, c) at [q * ncell + c], and cells are ordered run by
+// run, annulus by annulus (jk = j + Nr*k), then by their array index inside
+// the annulus -- the reference's theCells[jk][i] order (domain.c:34-38)
+// flattened with prefix-sum offsets.  Annuli have a global index over the
+// batch; each annulus record carries its run, its neighbours and its static
+// geometry.  Only piph/dphi (the moving mesh) change shape information during
+// a step; the annulus table never changes because Np is fixed (AMR is off in
+// the reference, onestep.c:179-181).
+#pragma once
+
+#include "disco_initial.h"
+#include "phys_common.h"
+
+namespace disco {
+
+enum AnnFlags {
+    ANN_REAL = 1,        // real annulus: conserved update + cons2prim
+    ANN_PHIFLUX = 2,     // phi faces carry fluxes (misc.c:378-436 range)
+    ANN_OUTFLUX = 4,     // the r-track (this | outer) carries fluxes (misc.c:441-519)
+    ANN_INFLUX = 8,      // the r-track (inner | this) carries fluxes
+    ANN_GHOST_FILL = 16, // ghost annulus re-initialised by the boundary module
+};
+
+struct AnnDev {
+    int run;
+    int j, k;
+    int np;
+    int off;    // global index of the annulus' first cell
+    int outer;  // global annulus index of (j+1, k), -1 if none
+    int inner;  // global annulus index of (j-1, k), -1 if none
+    int flags;
+    double rm, rp, zm, zp;  // r_jph[j-1], r_jph[j], z_kph[k-1], z_kph[k]
+    double rc, zc;          // r and z centroids (geometry_functions.c:5-10)
+    double dxL, dxR;        // centroid distances across the (this | outer) track (faces_alt.c:111-121)
+    double rc_b, zc_b;      // boundary-module evaluation point (fixed.c: means, fixed3d: centroids)
+};
+
+struct RunDev {
+    PhysParams ph;
+    disco_ic ic;
+    int bc;
+    int planets_mod;
+    int cell0, ncell;
+    int ann0, nann;
+    int Nr, Nz;
+    Planet planets[DISCO_MAX_PLANETS];
+};
+
+// Pointers handed to every kernel (by value).
+struct DevState {
+    const AnnDev* ann;
+    const int* cell_ann;
+    const RunDev* runs;
+    int ncell;
+    int nruns;
+    double* prim;
+    double* cons;
+    double* rkcons;
+    double* gradp;
+    double* gradr;
+    const double* piph;  // mesh at the start of the substep
+    const double* dphi;
+    double* piph_n;      // mesh after move_cells (first substep only)
+    double* dphi_n;
+    double* wiph;
+    int* lbo;            // per cell: partner in the outer annulus (array index)
+    int* ubi;            // per cell: partner in the inner annulus (array index)
+    int* rot;            // per annulus: array index of the smallest front
+    double* fphi;        // per cell: increment of its phi face (i | i+1)
+    double* fout;        // per cell: increment of its owned face in the outer track
+    double* fin;         // per cell: increment of its owned face in the inner track
+    unsigned long long* dtbits;  // per run: running min of the CFL step (bit pattern)
+    double* dt;          // per run: current step
+    double* planet_phi;  // per run x planet
+    double* t;           // per run: simulation time
+};
+
+// Optional callbacks around every kernel launch (used for per-kernel timing).
+struct LaunchHook {
+    void (*before)(void* ctx, const char* name);
+    void (*after)(void* ctx, const char* name);
+    void* ctx;
+};
+
+}  // namespace disco

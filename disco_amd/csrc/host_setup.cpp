@@ -1,0 +1,493 @@
+// Host-side set-up of one run.  See host_setup.h.
+#include "host_setup.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include "phys_euler.h"
+
+namespace disco {
+
+// ---------------------------------------------------------------- in.par
+// readpar.c:6-42: the first line whose first token equals the name; the
+// value follows the name after any of "\t :=>_"; ints are read as doubles
+// and truncated.
+int read_var(const std::string& text, const char* name, int kind, void* out) {
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        char nm[512] = "";
+        if (std::sscanf(line.c_str(), "%511s", nm) != 1) continue;
+        if (std::strcmp(nm, name) != 0) continue;
+        const char* s2 = line.c_str() + std::strlen(nm);
+        s2 += std::strspn(s2, "\t :=>_");
+        if (kind == 2) {
+            char sv[256] = "";
+            std::sscanf(s2, "%255s", sv);
+            std::strcpy(static_cast<char*>(out), sv);
+            return 0;
+        }
+        double temp = 0.0;
+        std::sscanf(s2, "%lf", &temp);
+        if (kind == 1)
+            *static_cast<int*>(out) = static_cast<int>(temp);
+        else
+            *static_cast<double*>(out) = temp;
+        return 0;
+    }
+    return 1;
+}
+
+int read_par_text(const std::string& text, disco_params* p, std::string* err) {
+    std::memset(p, 0, sizeof(*p));
+    int e = 0;
+    int tTimes_2pi = 0, pTimes_2pi = 0, zTimes_pi = 0;
+    std::string missing;
+    auto I = [&](const char* n, int* v) {
+        if (read_var(text, n, 1, v)) { ++e; missing += std::string(" ") + n; }
+    };
+    auto D = [&](const char* n, double* v) {
+        if (read_var(text, n, 0, v)) { ++e; missing += std::string(" ") + n; }
+    };
+    // the keys the per-timestep update and its set-up depend on
+    I("Num_R", &p->Num_R);
+    I("Num_Z", &p->Num_Z);
+    D("aspect", &p->aspect);
+    D("T_Start", &p->t_min);
+    D("T_End", &p->t_max);
+    D("R_Min", &p->rmin);
+    D("R_Max", &p->rmax);
+    D("Z_Min", &p->zmin);
+    D("Z_Max", &p->zmax);
+    D("Phi_Max", &p->phimax);
+    I("Log_Zoning", &p->LogZoning);
+    I("R_Periodic", &p->R_Periodic);
+    I("Z_Periodic", &p->Z_Periodic);
+    I("NoBC_Rmin", &p->NoBC_Rmin);
+    I("NoBC_Rmax", &p->NoBC_Rmax);
+    I("NoBC_Zmin", &p->NoBC_Zmin);
+    I("NoBC_Zmax", &p->NoBC_Zmax);
+    D("Log_Radius", &p->LogRadius);
+    I("Timestep", &p->Timestep);
+    D("CFL", &p->CFL);
+    D("Max_DT", &p->maxDT);
+    D("PLM", &p->PLM);
+    D("Adiabatic_Index", &p->Adiabatic_Index);
+    I("Isothermal", &p->isothermal_flag);
+    I("Cs2_Profile", &p->Cs2_Profile);
+    D("Cs2_Par", &p->Cs2_Par);
+    D("Density_Floor", &p->Density_Floor);
+    D("Pressure_Floor", &p->Pressure_Floor);
+    I("Mesh_Motion", &p->Mesh_Motion);
+    I("Exact_Mesh_Omega", &p->Exact_Mesh_Omega);
+    D("Exact_Mesh_Omega_Par", &p->Exact_Mesh_Omega_Par);
+    I("Energy_Omega", &p->Energy_Omega);
+    D("Energy_Omega_Par", &p->Energy_Omega_Par);
+    I("RotFrame", &p->RotFrame);
+    D("RotOmega", &p->RotOmega);
+    D("RotD", &p->RotD);
+    I("Riemann_Solver", &p->Riemann_Solver);
+    I("Use_Viscosity", &p->visc_flag);
+    I("P_Times_2pi", &pTimes_2pi);
+    I("T_Times_2pi", &tTimes_2pi);
+    I("Z_Times_pi", &zTimes_pi);
+    D("Mach_Number", &p->Disk_Mach);
+    D("Mass_Ratio", &p->Mass_Ratio);
+    I("Grav2D", &p->grav2D);
+    I("Constrained_Transport", &p->CT);
+    I("Metric_Par0", &p->metricPar0);
+    D("Metric_Par1", &p->metricPar1);
+    D("Metric_Par2", &p->metricPar2);
+    D("Metric_Par3", &p->metricPar3);
+    D("Metric_Par4", &p->metricPar4);
+    I("Init_Par0", &p->initPar0);
+    D("Init_Par1", &p->initPar1);
+    D("Init_Par2", &p->initPar2);
+    D("Init_Par3", &p->initPar3);
+    D("Init_Par4", &p->initPar4);
+    D("Init_Par5", &p->initPar5);
+    D("Init_Par6", &p->initPar6);
+    D("Init_Par7", &p->initPar7);
+    D("Init_Par8", &p->initPar8);
+    I("Noise_Type", &p->noiseType);
+    D("Noise_Abs", &p->noiseAbs);
+    D("Noise_Rel", &p->noiseRel);
+    D("Softening", &p->grav_eps);
+    // module slots (Makefile_opt.in in the reference); default euler/fixed/sun
+    char s[256];
+    p->hydro = DISCO_HYDRO_EULER;
+    if (!read_var(text, "Hydro", 2, s)) {
+        if (!std::strcmp(s, "mhd")) p->hydro = DISCO_HYDRO_MHD;
+        else if (!std::strcmp(s, "grmhd")) p->hydro = DISCO_HYDRO_GRMHD;
+        else if (std::strcmp(s, "euler")) { ++e; missing += " Hydro=" + std::string(s); }
+    }
+    p->boundary = DISCO_BC_FIXED;
+    if (!read_var(text, "Boundary", 2, s)) {
+        if (!std::strcmp(s, "fixed3d")) p->boundary = DISCO_BC_FIXED3D;
+        else if (std::strcmp(s, "fixed")) { ++e; missing += " Boundary=" + std::string(s); }
+    }
+    p->planets = DISCO_PLANETS_SUN;
+    if (!read_var(text, "Planets", 2, s)) {
+        if (!std::strcmp(s, "none")) p->planets = DISCO_PLANETS_NONE;
+        else if (!std::strcmp(s, "binary")) p->planets = DISCO_PLANETS_BINARY;
+        else if (std::strcmp(s, "sun")) { ++e; missing += " Planets=" + std::string(s); }
+    }
+    p->initial = DISCO_IC_KEPLER_NOISE;
+    if (!read_var(text, "Initial", 2, s)) {
+        if (!std::strcmp(s, "iso_binary_disk")) p->initial = DISCO_IC_ISO_BINARY_DISK;
+        else if (!std::strcmp(s, "mhd_disk")) p->initial = DISCO_IC_MHD_DISK;
+        else if (std::strcmp(s, "kepler_noise")) { ++e; missing += " Initial=" + std::string(s); }
+    }
+    p->num_c = p->hydro == DISCO_HYDRO_EULER ? 5 : 8;
+    read_var(text, "Num_C", 1, &p->num_c);
+    p->num_n = 0;
+    read_var(text, "Num_N", 1, &p->num_n);
+
+    // readpar.c:190-198
+    if (tTimes_2pi) {
+        p->t_min *= 2. * M_PI;
+        p->t_max *= 2. * M_PI;
+    }
+    if (pTimes_2pi) p->phimax *= 2. * M_PI;
+    if (zTimes_pi) {
+        p->zmin *= M_PI;
+        p->zmax *= M_PI;
+    }
+    if (e && err) *err = "Read Failed: missing or invalid keys:" + missing;
+    return e;
+}
+
+void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
+    std::memset(ph, 0, sizeof(*ph));
+    ph->gamma_law = par.Adiabatic_Index;
+    ph->rho_floor = par.Density_Floor;
+    ph->pre_floor = par.Pressure_Floor;
+    ph->isothermal = par.isothermal_flag;
+    ph->cs2_choice = par.Cs2_Profile;
+    ph->mach = par.Disk_Mach;
+    ph->gm = par.metricPar2;
+    ph->eps = par.grav_eps;
+    ph->mesh_om_choice = par.Exact_Mesh_Omega;
+    ph->en_om_choice = par.Energy_Omega;
+    ph->omega0 = par.RotOmega;
+    ph->phi_max = par.phimax;
+    ph->plm = par.PLM;
+    ph->cfl = par.CFL;
+    ph->max_dt = par.maxDT;
+    ph->riemann = par.Riemann_Solver;
+    ph->mesh_motion = par.Mesh_Motion;
+    ph->grav2D = par.grav2D;
+    ph->num_c = par.num_c;
+    ph->num_n = par.num_n;
+    disco_ic_from_params(&par, ic);
+}
+
+// ---------------------------------------------------------------- checks
+static int unsupported(const disco_params& p, std::string* err) {
+    std::string m;
+    if (p.hydro != DISCO_HYDRO_EULER) m += " Hydro (only euler is built)";
+    if (p.num_n != 0) m += " Num_N>0";
+    if (p.num_c != 5) m += " Num_C!=5 for euler";
+    if (p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) m += " NoBC_*";
+    if (p.R_Periodic) m += " R_Periodic";
+    if (p.Z_Periodic) m += " Z_Periodic";
+    if (p.Num_Z != 1) m += " Num_Z>1";
+    if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
+    if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
+    if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC) m += " Riemann_Solver";
+    if (p.visc_flag) m += " Use_Viscosity";
+    if (p.RotFrame) m += " RotFrame";
+    if (p.noiseType) m += " Noise_Type";
+    if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";
+    if (p.Num_R < 1 || p.Num_Z < 1) m += " Num_R/Num_Z";
+    if (!m.empty()) {
+        if (err) *err = "unsupported configuration:" + m;
+        return 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- set-up
+static void setup_grid(HostRun& R) {
+    const disco_params& p = R.par;
+    const int Ng = 2;  // NUM_G, paul.h:22
+    int N0r = 0, N1r = p.Num_R;
+    int NgRa = p.NoBC_Rmin ? 0 : Ng;
+    int NgRb = p.NoBC_Rmax ? 0 : Ng;
+    N0r -= NgRa;
+    N1r += NgRb;
+    int N0z = 0, N1z = p.Num_Z;
+    int NgZa = (p.Num_Z == 1 || p.NoBC_Zmin) ? 0 : Ng;
+    int NgZb = (p.Num_Z == 1 || p.NoBC_Zmax) ? 0 : Ng;
+    N0z -= NgZa;
+    N1z += NgZb;
+    R.Nr = N1r - N0r;
+    R.Nz = N1z - N0z;
+    R.NgRa = NgRa;
+    R.NgRb = NgRb;
+    R.NgZa = NgZa;
+    R.NgZb = NgZb;
+    R.N0r = N0r;
+    R.N0z = N0z;
+    R.Nr_glob = p.Num_R;
+    R.Nz_glob = p.Num_Z;
+    R.N0r_glob = 0;
+    R.N0z_glob = 0;
+    if (!p.NoBC_Rmin) { R.Nr_glob += Ng; R.N0r_glob -= Ng; }
+    if (!p.NoBC_Rmax) R.Nr_glob += Ng;
+    if (!p.NoBC_Zmin) { R.Nz_glob += Ng; R.N0z_glob -= Ng; }
+    if (!p.NoBC_Zmax) R.Nz_glob += Ng;
+
+    R.r_jph.assign(R.Nr + 1, 0.0);
+    R.z_kph.assign(R.Nz + 1, 0.0);
+    for (int j = -1; j < R.Nr; ++j) {
+        double x = (N0r + j + 1) / (double)p.Num_R;
+        double v;
+        if (p.LogZoning == 0)
+            v = p.rmin + x * (p.rmax - p.rmin);
+        else if (p.LogZoning == 1)
+            v = p.rmin * std::pow(p.rmax / p.rmin, x);
+        else
+            v = p.LogRadius * (std::pow(p.rmax / p.LogRadius, x) - 1) + p.rmin + (p.LogRadius - p.rmin) * x;
+        R.r_jph[j + 1] = v;
+    }
+    double dz = (p.zmax - p.zmin) / (double)p.Num_Z;
+    double z0 = p.zmin + (double)N0z * dz;
+    for (int k = -1; k < R.Nz; ++k) R.z_kph[k + 1] = z0 + ((double)k + 1.) * dz;
+
+    // gridsetup.c:119-141: Np from the outer edge radius
+    R.Np.assign(R.Nr * R.Nz, 0);
+    for (int k = 0; k < R.Nz; ++k)
+        for (int j = 0; j < R.Nr; ++j) {
+            double rp = R.rj(j), rm = R.rj(j - 1);
+            double dr = rp - rm;
+            double hp = rp;
+            double dp = p.aspect * dr / hp;
+            int np = (int)(p.phimax / dp);
+            if (np < 4) np = 4;
+            R.Np[j + R.Nr * k] = np;
+        }
+    R.off.assign(R.Nr * R.Nz + 1, 0);
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) R.off[jk + 1] = R.off[jk] + R.Np[jk];
+    R.ncell = R.off[R.Nr * R.Nz];
+}
+
+static void setup_planets(HostRun& R) {
+    const disco_params& p = R.par;
+    R.planets.clear();
+    if (p.planets == DISCO_PLANETS_SUN) {
+        // Planets/sun.c:17-27
+        Planet s{};
+        s.M = 1.0;
+        s.omega = 1.0;
+        s.r = 0.0;
+        s.phi = 0.0;
+        s.eps = p.grav_eps;
+        s.type = 0;
+        R.planets.push_back(s);
+    } else if (p.planets == DISCO_PLANETS_BINARY) {
+        // seeded-phase binary modelled on Planets/cmA.c:20-81
+        double a = 1.0;
+        double q = p.Mass_Ratio;
+        double mu = q / (1. + q);
+        double om = std::pow(a, -1.5);
+        Planet b0{}, b1{};
+        b0.M = 1. - mu;
+        b0.omega = om;
+        b0.r = a * mu;
+        b0.phi = p.initPar6 + M_PI;
+        b0.eps = p.grav_eps;
+        b1.M = mu;
+        b1.omega = om;
+        b1.r = a * (1. - mu);
+        b1.phi = p.initPar6;
+        b1.eps = p.grav_eps;
+        R.planets.push_back(b0);
+        R.planets.push_back(b1);
+    }
+    R.ph.npl = (int)R.planets.size();
+}
+
+// domain.c:26-89: one glibc rand() phase per annulus, layout-independent
+static void setup_phases(HostRun& R) {
+    double Pmax = R.par.phimax;
+    std::srand(314159);
+    std::rand();
+    for (int k = R.N0z_glob; k < R.N0z; ++k)
+        for (int j = 0; j < R.Nr_glob; ++j) std::rand();
+    R.piph.assign(R.ncell, 0.0);
+    R.dphi.assign(R.ncell, 0.0);
+    for (int k = 0; k < R.Nz; ++k) {
+        for (int j = R.N0r_glob; j < R.N0r; ++j) std::rand();
+        for (int j = 0; j < R.Nr; ++j) {
+            int jk = k * R.Nr + j;
+            double p0 = Pmax * (double)std::rand() / (double)RAND_MAX;
+            double dp = Pmax / (double)R.Np[jk];
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                double phi = p0 + dp * (double)i;
+                if (phi > Pmax) phi -= Pmax;
+                R.piph[R.off[jk] + i] = phi;
+                R.dphi[R.off[jk] + i] = dp;
+            }
+        }
+        for (int j = R.N0r + R.Nr; j < R.N0r_glob + R.Nr_glob; ++j) std::rand();
+    }
+}
+
+// misc.c:281-298
+static void calc_dp(HostRun& R) {
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
+        int o = R.off[jk], n = R.Np[jk];
+        for (int i = 0; i < n; ++i) {
+            int im = i == 0 ? n - 1 : i - 1;
+            R.dphi[o + i] = get_dp(R.piph[o + i], R.piph[o + im], R.par.phimax);
+        }
+    }
+}
+
+static void cell_x(const HostRun& R, int j, int k, int c, double* x, double* xp, double* xm) {
+    double phip = R.piph[c];
+    double phim = phip - R.dphi[c];
+    xp[0] = R.rj(j);
+    xp[1] = phip;
+    xp[2] = R.zk(k);
+    xm[0] = R.rj(j - 1);
+    xm[1] = phim;
+    xm[2] = R.zk(k - 1);
+    x[0] = get_centroid(R.rj(j), R.rj(j - 1), 1);
+    x[1] = R.piph[c] - .5 * R.dphi[c];
+    x[2] = get_centroid(R.zk(k), R.zk(k - 1), 2);
+}
+
+void host_boundary(HostRun* Rp) {
+    HostRun& R = *Rp;
+    double prim[8];
+    auto fill = [&](int j, int k) {
+        int jk = j + R.Nr * k;
+        for (int i = 0; i < R.Np[jk]; ++i) {
+            int c = R.off[jk] + i;
+            double x[3];
+            if (R.par.boundary == DISCO_BC_FIXED) {
+                // Boundary/fixed.c:25-58: arithmetic-mean r and z
+                x[0] = .5 * (R.rj(j) + R.rj(j - 1));
+                x[1] = R.piph[c] - .5 * R.dphi[c];
+                x[2] = .5 * (R.zk(k) + R.zk(k - 1));
+            } else {
+                // boundary_functions.c:21-31 set_cell_init: centroids
+                double phim = R.piph[c] - R.dphi[c];
+                x[0] = get_centroid(R.rj(j), R.rj(j - 1), 1);
+                x[1] = get_centroid(R.piph[c], phim, 0);
+                x[2] = get_centroid(R.zk(k), R.zk(k - 1), 2);
+            }
+            disco_initial(&R.ic, prim, x);
+            if (R.par.RotFrame) prim[UPP] -= R.par.RotOmega;
+            for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
+        }
+    };
+    for (int k = 0; k < R.Nz; ++k) {
+        for (int j = 0; j < R.NgRa; ++j) fill(j, k);
+        for (int j = R.Nr - R.NgRb; j < R.Nr; ++j) fill(j, k);
+    }
+    if (R.Nz > 1 && !R.par.Z_Periodic && R.par.boundary == DISCO_BC_FIXED3D) {
+        for (int j = 0; j < R.Nr; ++j) {
+            for (int k = 0; k < R.NgZa; ++k) fill(j, k);
+            for (int k = R.Nz - R.NgZb; k < R.Nz; ++k) fill(j, k);
+        }
+    }
+}
+
+// misc.c:101-168 (modes 0, 1, 4)
+static void set_wcell(HostRun& R) {
+    R.wiph.assign(R.ncell, 0.0);
+    for (int k = 0; k < R.Nz; ++k) {
+        double z = get_centroid(R.zk(k), R.zk(k - 1), 2);
+        for (int j = 0; j < R.Nr; ++j) {
+            int jk = j + R.Nr * k;
+            double r = get_centroid(R.rj(j), R.rj(j - 1), 1);
+            int o = R.off[jk], n = R.Np[jk];
+            for (int i = 0; i < n; ++i) {
+                double w = 0.0;
+                double phip = R.piph[o + i];
+                double phim = phip - R.dphi[o + i];
+                double x[3] = {r, 0.5 * (phim + phip), z};
+                if (R.par.Mesh_Motion == 1) {
+                    int ip = (i + 1) % n;
+                    double wL = R.prim[(size_t)UPP * R.ncell + o + i];
+                    double wR = R.prim[(size_t)UPP * R.ncell + o + ip];
+                    w = .5 * (wL + wR);
+                } else if (R.par.Mesh_Motion == 4) {
+                    w = mesh_om(R.ph, x);
+                }
+                R.wiph[o + i] = w;
+            }
+        }
+    }
+}
+
+static void find_rotation(HostRun& R) {
+    R.rot.assign(R.Nr * R.Nz, 0);
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
+        int o = R.off[jk], n = R.Np[jk];
+        for (int i = 0; i < n; ++i) {
+            int im = i == 0 ? n - 1 : i - 1;
+            if (R.piph[o + i] < R.piph[o + im]) R.rot[jk] = i;
+        }
+    }
+}
+
+int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
+    HostRun& R = *Rp;
+    R = HostRun();
+    R.par = par;
+    if (unsupported(par, err)) return 1;
+    derive_params(par, &R.ph, &R.ic);
+    R.nq = par.num_c + par.num_n;
+    setup_grid(R);
+    setup_planets(R);
+    setup_phases(R);
+    calc_dp(R);
+    R.prim.assign((size_t)R.nq * R.ncell, 0.0);
+    R.cons.assign((size_t)R.nq * R.ncell, 0.0);
+    // domain.c:163-209: initial data and prim2cons on real cells
+    double prim[8], cons[8];
+    for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
+        for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
+            int jk = j + R.Nr * k;
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                int c = R.off[jk] + i;
+                double x[3], xp[3], xm[3];
+                cell_x(R, j, k, c, x, xp, xm);
+                double dV = get_dV(xp, xm, R.par.phimax);
+                disco_initial(&R.ic, prim, x);
+                if (par.RotFrame) prim[UPP] -= par.RotOmega;
+                euler::prim2cons<5>(R.ph, prim, cons, x, dV);
+                for (int q = 0; q < R.nq; ++q) R.cons[(size_t)q * R.ncell + c] = cons[q];
+                R.nreal += 1;
+            }
+        }
+    // domain.c:221-238: cons2prim on real cells
+    for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
+        for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
+            int jk = j + R.Nr * k;
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                int c = R.off[jk] + i;
+                double x[3], xp[3], xm[3];
+                cell_x(R, j, k, c, x, xp, xm);
+                double dV = get_dV(xp, xm, R.par.phimax);
+                for (int q = 0; q < R.nq; ++q) cons[q] = R.cons[(size_t)q * R.ncell + c];
+                euler::cons2prim<5>(R.ph, cons, prim, x, dV);
+                for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
+            }
+        }
+    // main.c:52-61 ghost fill (done before set_wcell so ghost w is defined)
+    host_boundary(&R);
+    set_wcell(R);
+    find_rotation(R);
+    R.t = par.t_min;
+    return 0;
+}
+
+}  // namespace disco

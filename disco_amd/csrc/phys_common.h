@@ -1,0 +1,187 @@
+// Geometry, frame-omega profiles and point-mass gravity shared by every
+// physics module.  Host/device (__host__ __device__) so the host set-up and
+// the HIP kernels evaluate the same expressions.
+//
+// Follows: Geometry/cylindrical.c:10-68 (phi wraps, centroid, dL/dA/dV),
+// omega.c:56-209 (mesh_om, get_om, get_om1, get_cs2), planet.c:37-234
+// (fgrav for PLPOINTMASS, planetaryForce, planet_src).
+#pragma once
+
+#include "disco_hd.h"
+#include "disco_params.h"
+
+namespace disco {
+
+// Per-run constants that the reference keeps in static module globals set by
+// set*Params (domain.c:115-126).
+struct PhysParams {
+    double gamma_law;
+    double rho_floor, pre_floor;
+    int isothermal;
+    int cs2_choice;
+    double mach;
+    double gm;          // Metric_Par2, used by get_cs2 profile 4
+    double eps;         // Softening (omega.c, planets)
+    int mesh_om_choice; // Exact_Mesh_Omega
+    int en_om_choice;   // Energy_Omega
+    double omega0;      // RotOmega
+    double phi_max;
+    double plm;
+    double cfl;
+    double max_dt;
+    int riemann;
+    int mesh_motion;
+    int grav2D;
+    int npl;
+    int num_c, num_n;
+};
+
+struct Planet {
+    double r, phi, M, omega, eps;
+    int type;
+    int pad;
+};
+
+// ---------------------------------------------------------------- geometry
+// cylindrical.c:10-15
+DHD double get_dp(double phip, double phim, double phi_max) {
+    double dp = phip - phim;
+    while (dp < 0.0) dp += phi_max;
+    while (dp > phi_max) dp -= phi_max;
+    return dp;
+}
+// cylindrical.c:17-22
+DHD double get_signed_dp(double phip, double phim, double phi_max) {
+    double dp = phip - phim;
+    while (dp < -.5 * phi_max) dp += phi_max;
+    while (dp > .5 * phi_max) dp -= phi_max;
+    return dp;
+}
+// cylindrical.c:24-30; dim 1 = r (area-weighted centroid), else midpoint
+DHD double get_centroid(double xp, double xm, int dim) {
+    if (dim == 1) return 2.0 * (xp * xp + xp * xm + xm * xm) / (3.0 * (xp + xm));
+    return 0.5 * (xp + xm);
+}
+// cylindrical.c:43-54; dim 0 = phi face, 1 = r face, 2 = z face
+DHD double get_dA(const double* xp, const double* xm, int dim, double phi_max) {
+    double r = .5 * (xp[0] + xm[0]);
+    double dr = xp[0] - xm[0];
+    double dphi = get_dp(xp[1], xm[1], phi_max);
+    double dz = xp[2] - xm[2];
+    if (dim == 0) return dr * dz;
+    if (dim == 1) return r * dphi * dz;
+    return r * dr * dphi;
+}
+// cylindrical.c:56-63
+DHD double get_dV(const double* xp, const double* xm, double phi_max) {
+    double r = .5 * (xp[0] + xm[0]);
+    double dr = xp[0] - xm[0];
+    double dphi = get_dp(xp[1], xm[1], phi_max);
+    double dz = xp[2] - xm[2];
+    return r * dr * dphi * dz;
+}
+// cylindrical.c:32-41
+DHD double get_dL(const double* xp, const double* xm, int dim, double phi_max) {
+    double r = .5 * (xp[0] + xm[0]);
+    double dphi = get_dp(xp[1], xm[1], phi_max);
+    if (dim == 0) return r * dphi;
+    if (dim == 1) return xp[0] - xm[0];
+    return xp[2] - xm[2];
+}
+
+// ---------------------------------------------------------------- omega.c
+// omega.c:56-74
+DHD double mesh_om(const PhysParams& P, const double* x) {
+    double r = x[0];
+    r = sqrt(r * r + P.eps * P.eps);
+    if (P.mesh_om_choice == 1) return 1.0;
+    if (P.mesh_om_choice == 2) return pow(r, -1.5);
+    if (P.mesh_om_choice == 3) {
+        double n = 4.0;
+        return 1. / pow(pow(r, 1.5 * n) + 1., 1. / n);
+    }
+    return 0.0;
+}
+// omega.c:76-101
+DHD double get_om(const PhysParams& P, const double* x) {
+    double r = x[0];
+    r = sqrt(r * r + P.eps * P.eps);
+    if (P.en_om_choice == 1) return 1.0 - P.omega0;
+    if (P.en_om_choice == 2) return 1.0 / pow(r, 1.5) - P.omega0;
+    if (P.en_om_choice == 3) {
+        double n = 8.0;
+        return 1. / pow(pow(r, 1.5 * n) + 1., 1. / n) - P.omega0;
+    }
+    if (P.en_om_choice == 4) return 10. * exp(-.5 * r * r) - P.omega0;
+    return 0.0 - P.omega0;
+}
+// omega.c:103-126
+DHD double get_om1(const PhysParams& P, const double* x) {
+    double r = x[0];
+    r = sqrt(r * r + P.eps * P.eps);
+    if (P.en_om_choice == 1) return 0.0;
+    if (P.en_om_choice == 2) return -1.5 / pow(r, 2.5);
+    if (P.en_om_choice == 3) {
+        double n = 8.0;
+        return -1.5 * pow(r, -1 + 1.5 * n) / pow(pow(r, 1.5 * n) + 1., 1.0 + 1. / n);
+    }
+    if (P.en_om_choice == 4) return 10. * r * exp(-.5 * r * r);
+    return 0.0;
+}
+// omega.c:157-209 (profiles 1-4; profile 5 needs the planet list and is not
+// used by the benchmark disks)
+DHD double get_cs2(const PhysParams& P, const double* x) {
+    double r = x[0];
+    if (P.cs2_choice == 1) return 1. / (P.mach * P.mach);
+    if (P.cs2_choice == 2) return .5 / P.mach / P.mach / pow(r, 2. * .5);
+    if (P.cs2_choice == 4) {
+        r = sqrt(r * r + P.eps * P.eps);
+        double v2 = P.gm / r;
+        return v2 / (P.mach * P.mach);
+    }
+    return 1.0;
+}
+
+// ---------------------------------------------------------------- gravity
+// planet.c:97-102 (PLPOINTMASS, PHI_ORDER = 2)
+DHD double fgrav_pointmass(double M, double r, double eps) {
+    double n = 2.0;
+    return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
+}
+
+// planet.c:160-234: force per unit mass from one planet, returned as
+// (fr, fp, fz) in the orthonormal cylindrical basis at (r, phi, z).
+DHD void planetary_force(const Planet& pl, int grav2D, double r, double phi, double z,
+                         double* fr, double* fp, double* fz) {
+    double rp = pl.r;
+    double pp = pl.phi;
+    if (grav2D == 1) {
+        z = 0.0;
+    } else if (grav2D == 2) {
+        rp = r;
+        pp = phi;
+        z = 1.0;
+    }
+    double cosp = cos(phi);
+    double sinp = sin(phi);
+    double dx = r * cosp - rp * cos(pp);
+    double dy = r * sinp - rp * sin(pp);
+    double script_r = sqrt(dx * dx + dy * dy + z * z);
+    double script_r_perp = sqrt(dx * dx + dy * dy);
+    double f1 = -fgrav_pointmass(pl.M, script_r, pl.eps);
+    double cosa = dx / script_r_perp;
+    double sina = dy / script_r_perp;
+    double cosap = cosa * cosp + sina * sinp;
+    double sinap = sina * cosp - cosa * sinp;
+    if (script_r_perp <= 0.0) {
+        cosap = 0.0;
+        sinap = 0.0;
+    }
+    double sint = script_r_perp / script_r;
+    double cost = z / script_r;
+    *fr = cosap * f1 * sint;
+    *fp = sinap * f1 * sint;
+    *fz = f1 * cost;
+}
+
+}  // namespace disco

@@ -1,0 +1,90 @@
+"""Writes the golden fixtures under tests/golden/ from the reference CPU build.
+
+Runs ``oracle/_ref/disco_ref_<variant>`` (the reference compiled from its own
+sources by ``oracle/Makefile``, see ``oracle/ref/ref_driver.c``) on small
+versions of the benchmark disks and stores full-precision states at fixed
+step counts, plus conserved totals of the full-size 2D disk.  Needs the
+reference sources (this container only); the fixtures are committed.
+
+usage: python tests/golden/make_fixtures.py
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from dumpio import read_dump  # noqa: E402
+
+
+def par_with(base: str, **kv) -> str:
+    text = open(os.path.join(ROOT, "configs", base)).read()
+    for k, v in kv.items():
+        pat = re.compile(rf"^{k}\s.*$", re.M)
+        assert pat.search(text), k
+        text = pat.sub(f"{k} {v}", text)
+    return text
+
+
+def run_ref(variant: str, par_text: str, nsteps: int, dumps=()):
+    exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_{variant}")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "in.par"), "w").write(par_text)
+        out = subprocess.run([exe, str(nsteps), "out", *map(str, dumps)], cwd=d, check=True, timeout=600,
+                             capture_output=True, text=True).stdout
+        states = {}
+        for s in [0, *dumps, nsteps]:
+            states[s] = read_dump(os.path.join(d, f"out_{s}.bin"))
+    m = re.search(r"seconds=([0-9.eE+-]+)", out)
+    return states, float(m.group(1))
+
+
+def save_states(name: str, states: dict) -> None:
+    arrs = {}
+    for s, st in states.items():
+        for k, v in st.items():
+            arrs[f"s{s}_{k}"] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, name), steps=np.array(sorted(states)), **arrs)
+
+
+def totals(st: dict) -> np.ndarray:
+    real = st["real"] == 1
+    return st["cons"][:, real].sum(axis=1)
+
+
+def main() -> None:
+    # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
+    txt = par_with("c1_kepler2d.par", Num_R=16)
+    states, _ = run_ref("euler_sun", txt, 100, dumps=(1, 10))
+    save_states("c1_small.npz", states)
+    # same with HLL and analytic mesh motion (Mesh_Motion 4, Kepler)
+    txt = par_with("c1_kepler2d.par", Num_R=16, Riemann_Solver=0, Mesh_Motion=4)
+    states, _ = run_ref("euler_sun", txt, 10, dumps=(1,))
+    save_states("c1_small_hll_mm4.npz", states)
+    # small isothermal binary disk (seeded phase, softened)
+    txt = par_with("c3_binary_disk.par", Num_R=24)
+    states, _ = run_ref("euler_binary", txt, 50, dumps=(1, 10))
+    save_states("c3_small.npz", states)
+    # full-size 2D disk: conserved totals only
+    txt = par_with("c1_kepler2d.par")
+    states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))
+    np.savez(os.path.join(HERE, "c1_full_totals.npz"),
+             steps=np.array([0, 1, 20]),
+             totals=np.stack([totals(states[s]) for s in (0, 1, 20)]),
+             dt=np.array([states[s]["dt"] for s in (0, 1, 20)]),
+             ncell_real=np.array(int((states[0]["real"] == 1).sum())),
+             ref_seconds_20_steps=np.array(secs))
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()
