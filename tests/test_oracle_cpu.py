@@ -1,0 +1,165 @@
+"""CPU tests: the oracle against the reference's own states, the overlap-face
+merge against brute force, the host set-up against the reference's step-0
+state, the parameter parser, and the C ABI's exported symbols."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import disco_amd as D
+from dumpio import load_fixture
+from helpers import ROOT, cell_radius, params, phi_err, state_errors
+from oracle_py import Oracle, track_faces
+
+TOL = {1: 1e-12, 10: 1e-10, 50: 1e-9, 100: 1e-9}
+CASES = [
+    ("c1_small.npz", "c1_kepler2d.par", dict(Num_R=16)),
+    ("c1_small_hll_mm4.npz", "c1_kepler2d.par", dict(Num_R=16, Riemann_Solver=0, Mesh_Motion=4)),
+    ("c3_small.npz", "c3_binary_disk.par", dict(Num_R=24)),
+]
+
+
+def _grid(ref0):
+    return dict(Nr=ref0["Nr"], Nz=ref0["Nz"], NgRa=ref0["NgRa"], NgRb=ref0["NgRb"], NgZa=ref0["NgZa"],
+                NgZb=ref0["NgZb"], Np=ref0["Np"].astype(np.int32), r_jph=ref0["r_jph"], z_kph=ref0["z_kph"])
+
+
+@pytest.mark.parametrize("fixture,base,over", CASES, ids=[c[0] for c in CASES])
+def test_oracle_matches_reference(fixture, base, over):
+    """The restatement, started from the reference's own step-0 state, tracks
+    the reference to round-off (1e-12 after one step)."""
+    ref = load_fixture(fixture)
+    p = params(base, **over)
+    r0 = ref[0]
+    pl = r0["planets"][:, 1] if r0["planets"].size else None
+    o = Oracle(p, _grid(r0), r0, planet_phi=pl, t=r0["t"])
+    rcell = cell_radius(r0["Nr"], r0["Nz"], r0["Np"], r0["r_jph"])
+    done = 0
+    for s in sorted(ref):
+        if s == 0:
+            continue
+        for _ in range(s - done):
+            dt = o.getmindt()
+            o.timestep(dt)
+        done = s
+        st = o.state()
+        real = ref[s]["real"] == 1
+        assert abs(dt - ref[s]["dt"]) <= 1e-12 * ref[s]["dt"]
+        e = state_errors(st, ref[s], rcell, real)
+        assert e["prim"].max() < TOL[s], (s, e)
+        assert e["cons"].max() < TOL[s], (s, e)
+        assert phi_err(st["piph"], ref[s]["piph"]) < TOL[s]
+        if r0["planets"].size:
+            assert np.allclose(st["planet_phi"][:len(ref[s]["planets"])], ref[s]["planets"][:, 1], rtol=0,
+                               atol=1e-12)
+    o.close()
+
+
+def test_oracle_detects_a_planted_sign_error():
+    """The pin is sensitive: flipping the sign of the pressure in one cell's
+    primitive state moves the oracle's next step far outside tolerance."""
+    ref = load_fixture("c1_small.npz")
+    p = params("c1_kepler2d.par", Num_R=16)
+    r0 = ref[0]
+    bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in r0.items()}
+    c = int(np.nonzero(r0["real"] == 1)[0][100])
+    bad["prim"][1, c] *= 1.001
+    o = Oracle(p, _grid(r0), bad, planet_phi=r0["planets"][:, 1], t=0.0)
+    o.timestep(o.getmindt())
+    rcell = cell_radius(r0["Nr"], r0["Nz"], r0["Np"], r0["r_jph"])
+    e = state_errors(o.state(), ref[1], rcell, ref[1]["real"] == 1)
+    assert e["prim"].max() > 1e-9
+
+
+def _brute_overlaps(piA, dpA, piB, dpB, P=2 * np.pi):
+    """All (a, b) pairs whose phi intervals overlap (positive length), by
+    brute force on the circle."""
+    out = set()
+    for a in range(len(piA)):
+        a0, a1 = piA[a] - dpA[a], piA[a]
+        for b in range(len(piB)):
+            b0, b1 = piB[b] - dpB[b], piB[b]
+            for s in (-P, 0.0, P):
+                lo, hi = max(a0, b0 + s), min(a1, b1 + s)
+                if hi - lo > 1e-12:
+                    out.add((a, b))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_track_faces_equal_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    P = 2 * np.pi
+    NA, NB = rng.integers(4, 40), rng.integers(4, 40)
+
+    def ring(n):
+        d = rng.uniform(0.5, 1.5, n)
+        d *= P / d.sum()
+        p0 = rng.uniform(0, P)
+        pi = (p0 + np.cumsum(d)) % P
+        return pi, d
+
+    piA, dpA = ring(NA)
+    piB, dpB = ring(NB)
+    iL, iR, LR, pm, pp = track_faces(piA, dpA, piB, dpB, P)
+    assert len(iL) == NA + NB
+    faces = {(int(a), int(b)) for a, b, lo, hi in zip(iL, iR, pm, pp) if (hi - lo) % P > 1e-12}
+    assert faces == _brute_overlaps(piA, dpA, piB, dpB, P)
+    # total face extent covers the circle exactly once
+    width = np.array([(hi - lo) % P if abs(hi - lo) > 0 else 0.0 for lo, hi in zip(pm, pp)])
+    assert abs(width.sum() - P) < 1e-9
+
+
+@pytest.mark.parametrize("fixture,base,over", CASES, ids=[c[0] for c in CASES])
+def test_host_setup_is_bit_exact(fixture, base, over):
+    """setupGrid/setupDomain/setupCells + first ghost fill on the host give the
+    reference's step-0 state bit for bit (same seeded phases, same IC)."""
+    r0 = load_fixture(fixture)[0]
+    s = D.setup_run(params(base, **over))
+    real = r0["real"] == 1
+    assert np.array_equal(s["Np"], r0["Np"])
+    assert np.array_equal(s["r_jph"], r0["r_jph"])
+    assert np.array_equal(s["piph"], r0["piph"])
+    assert np.array_equal(s["dphi"], r0["dphi"])
+    assert np.array_equal(s["prim"][:, real], r0["prim"][:, real])
+    assert np.array_equal(s["cons"][:, real], r0["cons"][:, real])
+    assert np.array_equal(s["prim"][:, ~real], r0["prim"][:, ~real])
+    assert np.array_equal(s["wiph"][real], r0["wiph"][real])
+
+
+def test_parser_semantics():
+    text = open(os.path.join(ROOT, "configs", "c1_kepler2d.par")).read()
+    p = D.parse_par(text)
+    assert p.Num_R == 256 and p.Num_Z == 1
+    assert abs(p.phimax - 2 * np.pi) < 1e-15          # P_Times_2pi applied
+    assert p.Riemann_Solver == 1 and p.CFL == 0.5
+    # separators "\t :=>_" between key and value (readpar.c:23)
+    q = D.parse_par(text.replace("Num_R           256", "Num_R = 300"))
+    assert q.Num_R == 300
+    # ints are read as doubles and truncated
+    q = D.parse_par(text.replace("Num_R           256", "Num_R 12.9"))
+    assert q.Num_R == 12
+    # a missing key is an error, like "Read Failed"
+    with pytest.raises(ValueError):
+        D.parse_par(re.sub(r"^CFL .*$", "", text, flags=re.M))
+
+
+def test_unsupported_configuration_is_rejected():
+    p = params("c1_kepler2d.par", Timestep=1)
+    with pytest.raises(RuntimeError, match="Timestep"):
+        D.setup_run(p)
+
+
+def test_library_exports_every_declared_symbol():
+    """The C ABI library loads and exports every function include/*.h declares."""
+    L = C.CDLL(D.LIB_PATH)
+    decl = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            decl |= set(re.findall(r"^\s*(?:int|void|const char\*|double)\s+\**(\w+)\s*\(", src, re.M))
+    assert len(decl) >= 15
+    for name in sorted(decl):
+        assert hasattr(L, name), name
