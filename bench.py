@@ -69,7 +69,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="kepler2d_nr256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+
+        n = args.copy_bytes // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 8 * n, "launches": 5}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: one GPU owns the whole domain "
                                    "(no sharding, no collectives); run with --gpus 1"}))

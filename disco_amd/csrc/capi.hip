@@ -15,8 +15,9 @@
 #include "host_setup.h"
 
 namespace disco {
-void launch_dt(const DevState& S, hipStream_t st, const LaunchHook* hook);
-void launch_substep(const DevState& S, double RK, double dtfac, int first, hipStream_t st,
+void launch_mindt(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_substep(const DevState& S, double RK, double dtfac, int first, int want_dt, hipStream_t st,
                     const LaunchHook* hook);
 }  // namespace disco
 
@@ -93,14 +94,25 @@ static DevState make_state(disco_gpu* h, int substep) {
     return S;
 }
 
-// One step for the current parity: [getmindt] + onestep(0, dt, first) +
-// onestep(1/2, dt/2, last) (timestep.c:45-48).
+// One step for the current parity: [dt from the running CFL minimum] +
+// onestep(0, dt, first) + onestep(1/2, dt/2, last) (timestep.c:45-48).
+// Invariant between steps: dtbits holds the CFL minimum over the current
+// state (the last substep's update computes it; create/upload seed it).
 static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
     DevState S0 = make_state(h, 0);
     DevState S1 = make_state(h, 1);
-    if (with_dt) launch_dt(S0, h->stream, hook);
-    launch_substep(S0, 0.0, 1.0, 1, h->stream, hook);
-    launch_substep(S1, 0.5, 0.5, 0, h->stream, hook);
+    if (with_dt) launch_dt_final(S0, h->stream, hook);
+    launch_substep(S0, 0.0, 1.0, 1, 0, h->stream, hook);
+    launch_substep(S1, 0.5, 0.5, 0, 1, h->stream, hook);
+}
+
+// Recomputes the CFL minimum over the current state (after create/upload).
+static int seed_dt(disco_gpu* h) {
+    DevState S = make_state(h, 0);
+    launch_mindt(S, h->stream, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
 }
 
 template <typename T>
@@ -347,6 +359,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
             return cleanup("upload");
     }
     if (capture_graphs(h)) return cleanup(g_err);
+    if (seed_dt(h)) return cleanup(g_err);
     if (hipDeviceSynchronize() != hipSuccess) return cleanup("sync after create");
     *out = h;
     return 0;
@@ -395,7 +408,8 @@ int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kp
 int disco_gpu_mindt(disco_gpu_t* h, double* dt_out) {
     if (!h) return fail("disco_gpu_mindt: null handle");
     DevState S = make_state(h, 0);
-    launch_dt(S, h->stream, nullptr);
+    launch_mindt(S, h->stream, nullptr);
+    launch_dt_final(S, h->stream, nullptr);
     HIPCHK(hipGetLastError());
     if (dt_out) {
         HIPCHK(hipMemcpyAsync(dt_out, h->dt, h->nruns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -465,7 +479,8 @@ int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* 
             if (R.piph[o + i] < R.piph[o + im]) R.rot[jk] = i;
         }
     }
-    return upload_run_state(h, run);
+    if (upload_run_state(h, run)) return -1;
+    return seed_dt(h);
 }
 
 int disco_gpu_time(disco_gpu_t* h, double* t, double* dt, double* planet_phi) {

@@ -9,7 +9,7 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define DHD __host__ __device__ static inline
+#define DHD __host__ __device__ static inline __attribute__((always_inline))
 #else
 #include <math.h>
 #define DHD static inline

@@ -8,7 +8,8 @@
 //   add_source, move_cells, clean_pi,
 //   calc_dp, calc_prim, boundary_trans       -> k_update
 //   movePlanets + clean_pi (planets)         -> k_planets
-//   getmindt                                 -> k_mindt + k_dt_final
+//   getmindt                                 -> fused into the last substep's
+//                                               k_update (k_mindt at start-up)
 //
 // Overlap faces (faces_alt.c:36-179).  The reference merges the two cyclic,
 // sorted front-edge lists of an annulus pair into exactly Np_a + Np_b faces,
@@ -30,7 +31,19 @@
 
 namespace disco {
 
-#define DEV __device__ static inline
+#define DEV __device__ static inline __attribute__((always_inline))
+
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Remap so that XCD x works on one contiguous band of cells: partner cells in
+// the neighbouring annuli (about Np cells away) are then mostly in the same
+// L2 as the block that needs them.
+DEV int cell_index() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, i = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    const int lb = x * per + (x < rem ? x : rem) + i;
+    return lb * blockDim.x + threadIdx.x;
+}
 
 // plm.c:6-13
 DEV double minmod(double a, double b, double c) {
@@ -97,39 +110,76 @@ __global__ void k_dt_init(DevState S) {
     S.dtbits[r] = (unsigned long long)__double_as_longlong(dt);
 }
 
-// misc.c:39-84: min over real cells of mindt(prim, w, xp, xm)
+// Min-reduction of a per-cell CFL candidate into the run's running minimum
+// (bit pattern of a positive double, so unsigned atomicMin orders it).  All
+// threads of the block must call it.  A wavefront whose lanes belong to one
+// run reduces with cross-lane shuffles; a block whose waves agree reduces
+// through LDS and issues one atomic; mixed blocks fall back to per-wave or
+// per-lane atomics (only at run boundaries of a batch).
+DEV void dt_reduce(const DevState& S, int run, double dtc) {
+    __shared__ double wmin[16];
+    __shared__ int wrun[16];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwave = (blockDim.x + 63) >> 6;
+    int run0 = __shfl(run, 0);
+    bool uniform = __all(run == run0);
+    if (uniform) {
+        for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o));
+        if (lane == 0) {
+            wmin[wave] = dtc;
+            wrun[wave] = run0;
+        }
+    } else {
+        if (run >= 0 && dtc < 1.0e300)
+            atomicMin(&S.dtbits[run], (unsigned long long)__double_as_longlong(dtc));
+        if (lane == 0) wrun[wave] = -2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool same = true;
+        for (int w = 1; w < nwave; ++w) same = same && (wrun[w] == wrun[0]);
+        if (same && wrun[0] >= 0) {
+            double m = wmin[0];
+            for (int w = 1; w < nwave; ++w) m = fmin(m, wmin[w]);
+            if (m < 1.0e300) atomicMin(&S.dtbits[wrun[0]], (unsigned long long)__double_as_longlong(m));
+        } else {
+            for (int w = 0; w < nwave; ++w)
+                if (wrun[w] >= 0 && wmin[w] < 1.0e300)
+                    atomicMin(&S.dtbits[wrun[w]], (unsigned long long)__double_as_longlong(wmin[w]));
+        }
+    }
+}
+
+// misc.c:39-84 for one real cell: mindt(prim, w = mean of the two phi-face
+// mesh speeds, xp, xm)
 template <int NQ>
-__global__ void k_mindt(DevState S) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+DEV double cell_mindt(const DevState& S, const AnnDev& A, const PhysParams& P, int c, const double* prim) {
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+    double phip = S.piph[c];
+    double phim = phip - S.dphi[c];
+    double xp[3] = {A.rp, phip, A.zp};
+    double xm[3] = {A.rm, phim, A.zm};
+    double w = .5 * (S.wiph[cm] + S.wiph[c]);
+    return euler::mindt(P, prim, w, xp, xm);
+}
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_mindt(DevState S) {
+    int c = cell_index();
     int run = -1;
     double dtc = 1.0e300;
     if (c < S.ncell) {
         const AnnDev& A = S.ann[S.cell_ann[c]];
         run = A.run;
         if (A.flags & ANN_REAL) {
-            const PhysParams& P = S.runs[run].ph;
-            int i = c - A.off;
-            int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
-            double phip = S.piph[c];
-            double phim = phip - S.dphi[c];
-            double xp[3] = {A.rp, phip, A.zp};
-            double xm[3] = {A.rm, phim, A.zm};
-            double w = .5 * (S.wiph[cm] + S.wiph[c]);
             double prim[NQ];
-            for (int q = 0; q < NQ; ++q) prim[q] = S.prim[(size_t)q * S.ncell + c];
-            dtc = euler::mindt(P, prim, w, xp, xm);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[(size_t)q * S.ncell + c];
+            dtc = cell_mindt<NQ>(S, A, S.runs[run].ph, c, prim);
         }
     }
-    // wavefront min when the whole wave belongs to one run, else per lane
-    int run0 = __shfl(run, 0);
-    bool uniform = __all(run == run0);
-    if (uniform) {
-        for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o));
-        if ((threadIdx.x & 63) == 0 && run0 >= 0 && dtc < 1.0e300)
-            atomicMin(&S.dtbits[run0], (unsigned long long)__double_as_longlong(dtc));
-    } else if (run >= 0 && dtc < 1.0e300) {
-        atomicMin(&S.dtbits[run], (unsigned long long)__double_as_longlong(dtc));
-    }
+    dt_reduce(S, run, dtc);
 }
 
 __global__ void k_dt_final(DevState S) {
@@ -144,8 +194,8 @@ __global__ void k_dt_final(DevState S) {
 // with timestep.c:18-24's copy folded into the first substep), plm_phi
 // (plm.c:15-51) and the partner search of the overlap faces.
 template <int NQ>
-__global__ void k_stage_a(DevState S, double RK, int first) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int first) {
+    int c = cell_index();
     if (c >= S.ncell) return;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const PhysParams& P = S.runs[A.run].ph;
@@ -169,16 +219,16 @@ __global__ void k_stage_a(DevState S, double RK, int first) {
     }
     if (A.flags & ANN_REAL) {
         if (first) {
-            for (int q = 0; q < NQ; ++q) S.rkcons[q * N + c] = S.cons[q * N + c];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.rkcons[q * N + c] = S.cons[q * N + c];
         } else {
-            for (int q = 0; q < NQ; ++q)
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q)
                 S.cons[q * N + c] = (1. - RK) * S.cons[q * N + c] + RK * S.rkcons[q * N + c];
         }
     }
     double dpL = S.dphi[cm];
     double dpC = S.dphi[c];
     double dpR = S.dphi[cp];
-    for (int q = 0; q < NQ; ++q) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double pL = S.prim[q * N + cm];
         double pC = S.prim[q * N + c];
         double pR = S.prim[q * N + cp];
@@ -209,29 +259,33 @@ template <int NQ>
 struct SlopeAcc {
     double sum[NQ];
     double smin[NQ];
-    unsigned char pos[NQ], neg[NQ], zero[NQ];
-    __device__ void init() {
-        for (int q = 0; q < NQ; ++q) {
-            sum[q] = 0.0;
-            smin[q] = 0.0;
-            pos[q] = neg[q] = zero[q] = 0;
-        }
-    }
-    __device__ void add(const double* S, double dA, bool firstface) {
-        for (int q = 0; q < NQ; ++q) {
-            sum[q] += S[q] * dA;
-            if (S[q] > 0.0) pos[q] = 1;
-            else if (S[q] < 0.0) neg[q] = 1;
-            else zero[q] = 1;
-            if (firstface || fabs(S[q]) < fabs(smin[q])) smin[q] = S[q];
-        }
-    }
+    unsigned pos, neg, zero;  // bit q set when some face slope of field q is >0, <0, ==0
 };
+
+template <int NQ>
+DEV void acc_init(SlopeAcc<NQ>& a) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        a.sum[q] = 0.0;
+        a.smin[q] = 0.0;
+    }
+    a.pos = a.neg = a.zero = 0u;
+}
+
+template <int NQ>
+DEV void acc_add(SlopeAcc<NQ>& a, const double* S, double dA, bool firstface) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        a.sum[q] += S[q] * dA;
+        a.pos |= (S[q] > 0.0 ? 1u : 0u) << q;
+        a.neg |= (S[q] < 0.0 ? 1u : 0u) << q;
+        a.zero |= (S[q] == 0.0 ? 1u : 0u) << q;
+        a.smin[q] = (firstface || fabs(S[q]) < fabs(a.smin[q])) ? S[q] : a.smin[q];
+    }
+}
 
 template <int NQ>
 DEV void face_slope(const DevState& S, int cL, int cR, const RFace& F, double dxL, double dxR, double* Sq) {
     const size_t N = S.ncell;
-    for (int q = 0; q < NQ; ++q) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double WL = S.prim[q * N + cL] + F.dpL * S.gradp[q * N + cL];
         double WR = S.prim[q * N + cR] - F.dpR * S.gradp[q * N + cR];
         Sq[q] = (WR - WL) / (dxR + dxL);
@@ -239,15 +293,15 @@ DEV void face_slope(const DevState& S, int cL, int cR, const RFace& F, double dx
 }
 
 template <int NQ>
-__global__ void k_plm_r(DevState S) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_plm_r(DevState S) {
+    int c = cell_index();
     if (c >= S.ncell) return;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const PhysParams& P = S.runs[A.run].ph;
     const size_t N = S.ncell;
     const double Pmax = P.phi_max;
     if (P.plm == 0.0) {
-        for (int q = 0; q < NQ; ++q) S.gradr[q * N + c] = 0.0;
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.gradr[q * N + c] = 0.0;
         return;
     }
     int n = A.np;
@@ -259,7 +313,7 @@ __global__ void k_plm_r(DevState S) {
     double dz = A.zp - A.zm;
 
     SlopeAcc<NQ> acc;
-    acc.init();
+    acc_init(acc);
     bool firstface = true;
     double Sq[NQ];
     if (A.outer >= 0) {  // this cell is L of the (this | outer) track
@@ -272,7 +326,7 @@ __global__ void k_plm_r(DevState S) {
         int cb = B.off + own;
         RFace F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 0, A.rp, dz, Pmax);
         face_slope<NQ>(S, c, cb, F, A.dxL, A.dxR, Sq);
-        acc.add(Sq, F.dA, firstface);
+        acc_add(acc, Sq, F.dA, firstface);
         firstface = false;
         for (int t = 0; t < cnt; ++t) {
             int b = start + t;
@@ -280,7 +334,7 @@ __global__ void k_plm_r(DevState S) {
             cb = B.off + b;
             F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 1, A.rp, dz, Pmax);
             face_slope<NQ>(S, c, cb, F, A.dxL, A.dxR, Sq);
-            acc.add(Sq, F.dA, false);
+            acc_add(acc, Sq, F.dA, false);
         }
     }
     if (A.inner >= 0) {  // this cell is R of the (inner | this) track
@@ -293,7 +347,7 @@ __global__ void k_plm_r(DevState S) {
         int ca = W.off + own;
         RFace F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 1, W.rp, dz, Pmax);
         face_slope<NQ>(S, ca, c, F, W.dxL, W.dxR, Sq);
-        acc.add(Sq, F.dA, firstface);
+        acc_add(acc, Sq, F.dA, firstface);
         firstface = false;
         for (int t = 0; t < cnt; ++t) {
             int a = start + t;
@@ -301,7 +355,7 @@ __global__ void k_plm_r(DevState S) {
             ca = W.off + a;
             F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 0, W.rp, dz, Pmax);
             face_slope<NQ>(S, ca, c, F, W.dxL, W.dxR, Sq);
-            acc.add(Sq, F.dA, false);
+            acc_add(acc, Sq, F.dA, false);
         }
     }
     // plm.c:112-143: total r-face area of the cell
@@ -311,12 +365,12 @@ __global__ void k_plm_r(DevState S) {
     double dAtot = dAp + dAm;
     if (A.inner < 0) dAtot = dAp;
     if (A.outer < 0) dAtot = dAm;
-    for (int q = 0; q < NQ; ++q) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double g = acc.sum[q] / dAtot;
         if (firstface) {
             g = 0.0;
         } else if (g != 0.0) {
-            bool flip = g > 0.0 ? (acc.neg[q] || acc.zero[q]) : (acc.pos[q] || acc.zero[q]);
+            bool flip = g > 0.0 ? (((acc.neg | acc.zero) >> q) & 1u) : (((acc.pos | acc.zero) >> q) & 1u);
             if (flip) {
                 g = 0.0;
             } else {
@@ -337,7 +391,7 @@ DEV void r_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, con
                      double dxR, double r_face, double zc, double dt, double* dU) {
     const size_t N = S.ncell;
     double primL[NQ], primR[NQ];
-    for (int q = 0; q < NQ; ++q) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         primL[q] = S.prim[q * N + cL] + S.gradr[q * N + cL] * dxL + S.gradp[q * N + cL] * F.dpL;
         primR[q] = S.prim[q * N + cR] - S.gradr[q * N + cR] * dxR - S.gradp[q * N + cR] * F.dpR;
     }
@@ -347,8 +401,8 @@ DEV void r_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, con
 }
 
 template <int NQ>
-__global__ void k_flux(DevState S, double dtfac) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_flux(DevState S, double dtfac) {
+    int c = cell_index();
     if (c >= S.ncell) return;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     if (!(A.flags & (ANN_PHIFLUX | ANN_OUTFLUX | ANN_INFLUX))) return;
@@ -367,7 +421,7 @@ __global__ void k_flux(DevState S, double dtfac) {
         int cp = A.off + (i + 1 == n ? 0 : i + 1);
         double primL[NQ], primR[NQ];
         double dphR = S.dphi[cp];
-        for (int q = 0; q < NQ; ++q) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             primL[q] = S.prim[q * N + c] + .5 * S.gradp[q * N + c] * dph;
             primR[q] = S.prim[q * N + cp] - .5 * S.gradp[q * N + cp] * dphR;
         }
@@ -376,21 +430,21 @@ __global__ void k_flux(DevState S, double dtfac) {
         double dA = (A.rp - A.rm) * dz;  // get_dA(xp, xm, 0)
         double w = A.rc * S.wiph[c];     // get_scale_factor(x, 0) * wiph
         euler::riemann<NQ>(P, primL, primR, x, nv, w, dA * dt, dU);
-        for (int q = 0; q < NQ; ++q) S.fphi[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fphi[q * N + c] = dU[q];
     }
     if (A.flags & ANN_OUTFLUX) {
         const AnnDev& B = S.ann[A.outer];
         int cb = B.off + S.lbo[c];
         RFace F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 0, A.rp, dz, Pmax);
         r_face_flux<NQ>(S, P, c, cb, F, A.dxL, A.dxR, A.rp, A.zc, dt, dU);
-        for (int q = 0; q < NQ; ++q) S.fout[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fout[q * N + c] = dU[q];
     }
     if (A.flags & ANN_INFLUX) {
         const AnnDev& W = S.ann[A.inner];
         int ca = W.off + S.ubi[c];
         RFace F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 1, W.rp, dz, Pmax);
         r_face_flux<NQ>(S, P, ca, c, F, W.dxL, W.dxR, W.rp, W.zc, dt, dU);
-        for (int q = 0; q < NQ; ++q) S.fin[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fin[q * N + c] = dU[q];
     }
 }
 
@@ -400,9 +454,25 @@ __global__ void k_flux(DevState S, double dtfac) {
 // 8-36; first substep only), calc_prim (misc.c:300-335) and the fixed
 // boundary (Boundary/fixed.c:11-60, boundary_functions.c:161-266).
 template <int NQ>
-__global__ void k_update(DevState S, double dtfac, int first) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= S.ncell) return;
+DEV double update_cell(const DevState& S, int c, double dtfac, int first, int want_dt);
+
+// The second substep also produces the next step's CFL candidate from the
+// prims it has just written (misc.c:39-84 runs on exactly this state at the
+// start of the next step, with this step's mesh speeds).
+template <int NQ>
+__global__ __launch_bounds__(256) void k_update(DevState S, double dtfac, int first, int want_dt) {
+    int c = cell_index();
+    int run = -1;
+    double dtc = 1.0e300;
+    if (c < S.ncell) {
+        run = S.ann[S.cell_ann[c]].run;
+        dtc = update_cell<NQ>(S, c, dtfac, first, want_dt);
+    }
+    if (want_dt) dt_reduce(S, run, dtc);
+}
+
+template <int NQ>
+DEV double update_cell(const DevState& S, int c, double dtfac, int first, int want_dt) {
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const RunDev& Rn = S.runs[A.run];
     const PhysParams& P = Rn.ph;
@@ -417,44 +487,44 @@ __global__ void k_update(DevState S, double dtfac, int first) {
 
     double prim[NQ], cons[NQ];
     if (A.flags & ANN_REAL) {
-        for (int q = 0; q < NQ; ++q) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             cons[q] = S.cons[q * N + c];
             prim[q] = S.prim[q * N + c];
         }
         // phi faces i-1 (this cell is R) and i (this cell is L)
         if (A.flags & ANN_PHIFLUX) {
             if (i == 0) {
-                for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
-                for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
             } else {
-                for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
-                for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
             }
         }
         // (inner | this) track: this cell is R, gains the flux
         if (A.inner >= 0 && (A.flags & ANN_INFLUX)) {
             const AnnDev& W = S.ann[A.inner];
-            for (int q = 0; q < NQ; ++q) cons[q] += S.fin[q * N + c];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fin[q * N + c];
             int start = S.ubi[cm];
             int cnt = S.ubi[c] - start;
             if (cnt < 0) cnt += W.np;
             for (int t = 0; t < cnt; ++t) {
                 int a = start + t;
                 if (a >= W.np) a -= W.np;
-                for (int q = 0; q < NQ; ++q) cons[q] += S.fout[q * N + W.off + a];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fout[q * N + W.off + a];
             }
         }
         // (this | outer) track: this cell is L, loses the flux
         if (A.outer >= 0 && (A.flags & ANN_OUTFLUX)) {
             const AnnDev& B = S.ann[A.outer];
-            for (int q = 0; q < NQ; ++q) cons[q] -= S.fout[q * N + c];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fout[q * N + c];
             int start = S.lbo[cm];
             int cnt = S.lbo[c] - start;
             if (cnt < 0) cnt += B.np;
             for (int t = 0; t < cnt; ++t) {
                 int b = start + t;
                 if (b >= B.np) b -= B.np;
-                for (int q = 0; q < NQ; ++q) cons[q] -= S.fin[q * N + B.off + b];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fin[q * N + B.off + b];
             }
         }
         // add_source with the pre-move geometry and the substep's prims
@@ -467,7 +537,7 @@ __global__ void k_update(DevState S, double dtfac, int first) {
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
             euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);
         }
-        for (int q = 0; q < NQ; ++q) S.cons[q * N + c] = cons[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.cons[q * N + c] = cons[q];
     }
 
     // moving mesh (first substep): every annulus, ghosts included
@@ -492,7 +562,8 @@ __global__ void k_update(DevState S, double dtfac, int first) {
         double dV = get_dV(xp, xm, Pmax);
         double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
         euler::cons2prim<NQ>(P, cons, prim, x, dV);
-        for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
+        if (want_dt) return cell_mindt<NQ>(S, A, P, c, prim);
     } else if (A.flags & ANN_GHOST_FILL) {
         double x[3];
         x[0] = A.rc_b;
@@ -501,9 +572,11 @@ __global__ void k_update(DevState S, double dtfac, int first) {
             x[1] = phip - .5 * dph;
         else
             x[1] = 0.5 * (phip + (phip - dph));
-        disco_initial(&Rn.ic, prim, x);
-        for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
+        double p8[8];
+        disco_initial(&Rn.ic, p8, x);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = p8[q];
     }
+    return 1.0e300;
 }
 
 // movePlanets (Planets/sun.c: none; seeded binary: phi += omega dt) then the
@@ -522,6 +595,10 @@ __global__ void k_planets(DevState S) {
         S.planet_phi[r * DISCO_MAX_PLANETS + p] = phi;
     }
     S.t[r] += dt;  // timestep.c:47
+    // open the next step's CFL reduction (filled by the second substep)
+    double d0 = Rn.ph.max_dt / Rn.ph.cfl;
+    if (d0 <= 0.0) d0 = 1.0e100;
+    S.dtbits[r] = (unsigned long long)__double_as_longlong(d0);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -534,21 +611,26 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
         if (hook) hook->after(hook->ctx, name);                                    \
     } while (0)
 
-void launch_dt(const DevState& S, hipStream_t st, const LaunchHook* hook) {
+// Full CFL reduction over the current state (first step, getmindt calls).
+void launch_mindt(const DevState& S, hipStream_t st, const LaunchHook* hook) {
     const int B = 256;
     DISCO_LAUNCH(hook, "k_dt_init", k_dt_init, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
     DISCO_LAUNCH(hook, "k_mindt", k_mindt<5>, dim3(nblk(S.ncell, B)), dim3(B), st, S);
+}
+
+// dt = CFL x running minimum (misc.c:79)
+void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook) {
     DISCO_LAUNCH(hook, "k_dt_final", k_dt_final, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
 }
 
-void launch_substep(const DevState& S, double RK, double dtfac, int first, hipStream_t st,
+void launch_substep(const DevState& S, double RK, double dtfac, int first, int want_dt, hipStream_t st,
                     const LaunchHook* hook) {
     const int B = 256;
     dim3 g(nblk(S.ncell, B));
     DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<5>, g, dim3(B), st, S, RK, first);
     DISCO_LAUNCH(hook, "k_plm_r", k_plm_r<5>, g, dim3(B), st, S);
     DISCO_LAUNCH(hook, "k_flux", k_flux<5>, g, dim3(B), st, S, dtfac);
-    DISCO_LAUNCH(hook, "k_update", k_update<5>, g, dim3(B), st, S, dtfac, first);
+    DISCO_LAUNCH(hook, "k_update", k_update<5>, g, dim3(B), st, S, dtfac, first, want_dt);
     if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
 }
 

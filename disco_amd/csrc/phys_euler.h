@@ -39,7 +39,7 @@ DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const 
     cons[SRR] = rho * vr * dV;
     cons[LLL] = r * rho * vp * dV;
     cons[SZZ] = rho * vz * dV;
-    for (int q = NUM_C; q < NQ; ++q) cons[q] = prim[q] * cons[DDD];
+    _Pragma("unroll") for (int q = NUM_C; q < NQ; ++q) cons[q] = prim[q] * cons[DDD];
 }
 
 // euler.c:65-101
@@ -67,7 +67,7 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
     Ustar[LLL] = r * rhostar * (vp + (Ss - vn) * n[1]);
     Ustar[SZZ] = rhostar * (vz + (Ss - vn) * n[2]);
     Ustar[TAU] = .5 * rhostar * v2 + Us + rhostar * Ss_off * (Ss - vn) + Pstar;
-    for (int q = NUM_C; q < NQ; ++q) Ustar[q] = prim[q] * Ustar[DDD];
+    _Pragma("unroll") for (int q = NUM_C; q < NQ; ++q) Ustar[q] = prim[q] * Ustar[DDD];
 }
 
 // euler.c:103-140
@@ -98,7 +98,7 @@ DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const 
     prim[URR] = vr;
     prim[UPP] = vp / r;
     prim[UZZ] = vz;
-    for (int q = NUM_C; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
+    _Pragma("unroll") for (int q = NUM_C; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
 }
 
 // euler.c:142-170
@@ -121,7 +121,7 @@ DHD void flux(const PhysParams& P, const double* prim, double* F, const double* 
     F[LLL] = r * (rho * vp * vn + Pp * n[1]);
     F[SZZ] = rho * vz * vn + Pp * n[2];
     F[TAU] = (.5 * rho * v2 + rhoe + Pp) * vn - Pp * wn;
-    for (int q = NUM_C; q < NQ; ++q) F[q] = prim[q] * F[DDD];
+    _Pragma("unroll") for (int q = NUM_C; q < NQ; ++q) F[q] = prim[q] * F[DDD];
 }
 
 // euler.c:301-326 (wave speeds; Bpack is unused by hydro)
@@ -151,7 +151,7 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
                  const double* n, double w, double dAdt, double* dU) {
     double Flux[NQ];
     double Ustr[NQ];
-    for (int q = 0; q < NQ; ++q) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         Flux[q] = 0.0;
         Ustr[q] = 0.0;
     }
@@ -171,7 +171,7 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
         prim2cons<NQ>(P, primR, Ur, x, 1.0);
         flux<NQ>(P, primL, Fl, x, n);
         flux<NQ>(P, primR, Fr, x, n);
-        for (int q = 0; q < NQ; ++q) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
             Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
         }
@@ -181,15 +181,15 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
             prim2cons<NQ>(P, primL, Uk, x, 1.0);
             getUstar<NQ>(P, primL, Ustr, x, Sl, Ss, n);
             flux<NQ>(P, primL, Fk, x, n);
-            for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
         } else {
             prim2cons<NQ>(P, primR, Uk, x, 1.0);
             getUstar<NQ>(P, primR, Ustr, x, Sr, Ss, n);
             flux<NQ>(P, primR, Fk, x, n);
-            for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
         }
     }
-    for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
 }
 
 // euler.c:172-213 (polar sources are off: NoBC_Rmin = 0)

@@ -573,7 +573,9 @@ static void o_boundary(oracle_run* o) {
                     x[1] = .5 * (c->piph + (c->piph - c->dphi));
                     x[2] = o_cent(ZK(o, k), ZK(o, k - 1), 2);
                 }
-                disco_initial(&o->ic, c->prim, x);
+                double tmp[8];
+                disco_initial(&o->ic, tmp, x);
+                memcpy(c->prim, tmp, sizeof(double) * NQ);
             }
         }
 }

@@ -29,4 +29,9 @@ void setICparams(struct domain* theDomain) {
     the_ic.vnoise = p->initPar5;
 }
 
-void initial(double* prim, double* x) { disco_initial(&the_ic, prim, x); }
+void initial(double* prim, double* x) {
+    double tmp[8];
+    int q;
+    disco_initial(&the_ic, tmp, x);
+    for (q = 0; q < NUM_Q; ++q) prim[q] = tmp[q];
+}
