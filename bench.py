@@ -62,6 +62,40 @@ def cpu_baseline(workload: str) -> dict | None:
             "sample": f"full {cells}-cell grid, {steps} RK2 steps, {secs:.2f} s (reference C build, gcc -O3, serial)"}
 
 
+def bench_batch(steps: int, warmup: int) -> dict:
+    """48 independent binary disks of uneven size (seeded Nr in [32, 256])
+    advanced together, against the same 48 runs advanced one after another
+    (each its own domain, synchronised before the next starts)."""
+    import disco_amd as D
+    from disco_amd.workloads import binary_batch
+
+    runs = binary_batch(48)
+    dom = D.Domain(runs)
+    cells = sum(i.nreal for i in dom.info)
+    dom.step(warmup)
+    dom.sync()
+    ms = dom.time_steps(steps)
+    batched = cells * steps / (ms * 1e-3)
+    kt = dom.kernel_times(5)
+    dom.close()
+    singles = [D.Domain(p) for p in runs]
+    for d in singles:
+        d.step(warmup)
+        d.sync()
+    t0 = time.perf_counter()
+    for d in singles:
+        d.step(steps)
+        d.sync()
+    seq_s = time.perf_counter() - t0
+    for d in singles:
+        d.close()
+    sequential = cells * steps / seq_s
+    return {"workload": "binary_batch_x48", "runs": 48, "real_cells": cells, "steps": steps,
+            "value": batched, "unit": "zone-updates/s", "ms_per_step": ms / steps,
+            "sequential_value": sequential, "speedup_vs_sequential": batched / sequential,
+            "kernels_ms": {k: v[0] for k, v in kt.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +103,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="kepler2d_nr256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -130,9 +165,11 @@ def main() -> int:
         "kernels_ms": {k: v[0] for k, v in kt.items()},
         "wall_s_timed": wall,
     }
+    dom.close()
+    if not args.no_batch:
+        line["batch"] = bench_batch(args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
-    dom.close()
     print(json.dumps(line))
     return 0
 

@@ -113,3 +113,58 @@ def test_full_size_disk_totals():
         for q in (0, 1, 3):
             assert abs(tot[q] - ref[q]) <= 1e-12 * abs(ref[q]), (s, q, tot[q], ref[q])
     dom.close()
+
+
+def test_batch_equals_single_runs():
+    """A batch of independent disks of uneven size, advanced together (one
+    launch per kernel stage for all runs, each run with its own CFL step,
+    time and planet phases), gives every run bit for bit what it gives when
+    advanced alone."""
+    from disco_amd.workloads import binary_batch
+
+    runs = binary_batch(nruns=5, nr_lo=12, nr_hi=40)
+    batch = D.Domain(runs)
+    batch.step(7)
+    tb = batch.time()
+    for r, p in enumerate(runs):
+        one = D.Domain(p)
+        one.step(7)
+        sa, sb = batch.state(r), one.state(0)
+        for k in ("prim", "cons", "piph", "dphi", "wiph"):
+            assert np.array_equal(sa[k], sb[k]), (r, k)
+        t1 = one.time()
+        assert tb["dt"][r] == t1["dt"][0] and tb["t"][r] == t1["t"][0]
+        assert np.array_equal(tb["planet_phi"][r], t1["planet_phi"][0])
+        one.close()
+    # the runs really are different (uneven sizes, their own dt)
+    assert len({i.ncell for i in batch.info}) > 1
+    assert len(set(tb["dt"])) > 1
+    batch.close()
+
+
+def test_batch_matches_oracle_per_run():
+    """Each run of a small batch against the CPU oracle stepped alone."""
+    from disco_amd.workloads import binary_batch
+    from oracle_py import Oracle
+
+    runs = binary_batch(nruns=3, nr_lo=12, nr_hi=24)
+    batch = D.Domain(runs)
+    init = [batch.state(r) for r in range(3)]
+    pp0 = batch.time()["planet_phi"]
+    batch.step(5)
+    for r, p in enumerate(runs):
+        I = batch.info[r]
+        g = batch.grid(r)
+        grid = dict(Nr=I.Nr, Nz=I.Nz, NgRa=I.NgRa, NgRb=I.NgRb, NgZa=I.NgZa, NgZb=I.NgZb, **g)
+        o = Oracle(p, grid, init[r], planet_phi=pp0[r], t=0.0)
+        o.step(5)
+        so = o.state()
+        real = np.zeros(I.ncell, bool)
+        off = np.concatenate([[0], np.cumsum(g["Np"])])
+        for j in range(I.NgRa, I.Nr - I.NgRb):
+            real[off[j]:off[j + 1]] = True
+        rc = cell_radius(I.Nr, I.Nz, g["Np"], g["r_jph"])
+        e = state_errors(batch.state(r), so, rc, real)
+        assert max(e["prim"].max(), e["cons"].max()) < 1e-10, (r, e)
+        o.close()
+    batch.close()
