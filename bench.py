@@ -96,6 +96,74 @@ def bench_batch(steps: int, warmup: int) -> dict:
             "kernels_ms": {k: v[0] for k, v in kt.items()}}
 
 
+# FP64 operations of the plain cons2prim per state, counted by hand from
+# disco_amd/csrc/phys_grmhd.h (divide and sqrt count as 1): Kerr-Schild metric
+# and frame 110, invariants 72, cold start 5, recovery 58 => 245 per state, and
+# 61 per Newton iteration.
+C2P_FLOP_STATE, C2P_FLOP_ITER = 245, 61
+MLP_FLOP_SAMPLE = 2 * (8 * 64 + 3 * 64 * 64 + 64 * 2)  # 25,856 flops (multiply-add = 2)
+FP64_PEAK_TFLOPS, BF16_PEAK_TFLOPS = 78.6, 2500.0
+
+
+def bench_c2p(nstates: int, train_steps: int) -> dict:
+    """Standalone GRMHD cons2prim on 2^24 seeded Kerr-Schild (a = 0.9) states:
+    plain cold start vs learned start (BF16-MFMA MLP + FP64 polish), both with
+    the reference's Nextra = 10 extra iterations and with Nextra = 0."""
+    import numpy as np
+
+    import disco_amd as D
+    from disco_amd.workloads import c4_params
+
+    P = c4_params()
+    h = D.C2P(nstates, P["gamma"], P["M"], P["a"], rmin=P["rmin"], rmax=P["rmax"], theta_cut=P["theta_cut"])
+    t0 = time.perf_counter()
+    loss = h.train(seed=P["seed"] + 1, ntrain=1 << 20, steps=train_steps, batch=8192, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    h.generate(P["seed"], 0, nstates)
+    out = {"workload": "grmhd_c2p_kerr_a0.9_2^24", "states": nstates, "train_steps": train_steps,
+           "train_seconds": train_s, "train_loss_first_last": [float(loss[0]), float(loss[-5:].mean())]}
+    for ne in (0, 10):
+        h.solve(D.PLAIN, nextra=ne)  # warm
+        tp = min(h.solve(D.PLAIN, nextra=ne) for _ in range(3))
+        a = h.download()
+        ep = h.errors(P["seed"], 0)
+        h.solve(D.LEARNED, nextra=ne)
+        tl = [h.solve(D.LEARNED, nextra=ne, split=True) for _ in range(3)]
+        tl, tg = min(t[0] for t in tl), min(t[1] for t in tl)
+        b = h.download()
+        el = h.errors(P["seed"], 0)
+        q, r = a["prim"], b["prim"]
+        rel = np.abs(r - q) / np.maximum(np.abs(q), 1e-300)
+        rel[2:5] = np.abs(r[2:5] - q[2:5]) / np.maximum(np.abs(q[2:5]).max(0), 1e-300)
+        it_p, it_l = float(a["iters"].mean()), float(b["iters"].mean())
+        flops = nstates * (C2P_FLOP_STATE + C2P_FLOP_ITER * it_p)
+        out[f"nextra{ne}"] = {
+            "plain_states_per_s": nstates / (tp * 1e-3), "learned_states_per_s": nstates / (tl * 1e-3),
+            "speedup": tp / tl, "plain_ms": tp, "learned_ms": tl, "learned_network_ms": tg,
+            "plain_iters_mean": it_p, "learned_iters_mean": it_l,
+            "fallback_rate": float((b["flags"] % 2 == 1).mean()),
+            "unconverged_plain": int((a["flags"] >= 2).sum()), "unconverged_learned": int((b["flags"] >= 2).sum()),
+            "max_rel_diff_learned_vs_plain": float(np.nanmax(rel[[0, 2, 3, 4]])),
+            "max_rel_diff_P_learned_vs_plain": float(np.nanmax(rel[1])),
+            "err_vs_truth_plain_p50_p99_max": [float(np.percentile(ep, 50)), float(np.percentile(ep, 99)),
+                                               float(ep.max())],
+            "err_vs_truth_learned_p50_p99_max": [float(np.percentile(el, 50)), float(np.percentile(el, 99)),
+                                                 float(el.max())],
+            "plain_fp64_tflops": flops / (tp * 1e-3) / 1e12,
+            "network_bf16_tflops": nstates * MLP_FLOP_SAMPLE / (tg * 1e-3) / 1e12,
+        }
+    h.close()
+    r0 = out["nextra0"]
+    out["roofline"] = {"bound": "fp64", "kernel": "k_c2p_plain (Nextra=0)", "achieved": r0["plain_fp64_tflops"],
+                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": r0["plain_fp64_tflops"] / FP64_PEAK_TFLOPS,
+                       "flop_per_state": f"{C2P_FLOP_STATE} + {C2P_FLOP_ITER} x iterations"}
+    out["mfma_roofline"] = {"bound": "mfma", "kernel": "k_mlp_guess", "achieved": r0["network_bf16_tflops"],
+                            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": r0["network_bf16_tflops"] / BF16_PEAK_TFLOPS, "flop_per_sample": MLP_FLOP_SAMPLE}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +172,9 @@ def main() -> int:
     ap.add_argument("--workload", default="kepler2d_nr256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-c2p", action="store_true")
+    ap.add_argument("--c2p-states", type=int, default=1 << 24)
+    ap.add_argument("--c2p-train-steps", type=int, default=2000)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -168,6 +239,8 @@ def main() -> int:
     dom.close()
     if not args.no_batch:
         line["batch"] = bench_batch(args.steps, args.warmup)
+    if not args.no_c2p:
+        line["c2p"] = bench_c2p(args.c2p_states, args.c2p_train_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line))

@@ -299,3 +299,147 @@ class Domain:
 def config_path(name: str) -> str:
     """Path of a config shipped in the repository's ``configs/``."""
     return os.path.join(os.path.dirname(_HERE), "configs", name)
+
+
+# ---------------------------------------------------------------- GRMHD cons2prim
+class C2PParams(C.Structure):
+    """Mirror of ``disco_c2p_params`` (include/disco_gpu.h)."""
+
+    _fields_ = [("gamma_law", C.c_double), ("M", C.c_double), ("a", C.c_double), ("prec", C.c_double),
+                ("max_iter", C.c_int), ("nextra", C.c_int), ("rmin", C.c_double), ("rmax", C.c_double),
+                ("theta_cut", C.c_double)]
+
+
+PLAIN, LEARNED = 0, 1
+MLP_NPARAM, MLP_NNORM = 13186, 20
+
+
+def _c2p_lib():
+    L = lib()
+    if getattr(L, "_c2p_ready", False):
+        return L
+    dp, fp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_void_p
+    L.disco_c2p_create.argtypes = [C.POINTER(C2PParams), C.c_long, C.POINTER(vp)]
+    L.disco_c2p_destroy.argtypes = [vp]
+    L.disco_c2p_destroy.restype = None
+    L.disco_c2p_last_error.restype = C.c_char_p
+    L.disco_c2p_generate.argtypes = [vp, C.c_ulonglong, C.c_long, C.c_long]
+    L.disco_c2p_upload.argtypes = [vp, dp, dp, C.c_long]
+    L.disco_c2p_solve.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.disco_c2p_download.argtypes = [vp, dp, ip, ip, C.c_long]
+    L.disco_c2p_errors.argtypes = [vp, C.c_ulonglong, C.c_long, fp]
+    L.disco_mlp_train.argtypes = [vp, C.c_ulonglong, C.c_long, C.c_int, C.c_int, C.c_float, fp]
+    L.disco_mlp_get.argtypes = [vp, fp, fp]
+    L.disco_mlp_set.argtypes = [vp, fp, fp]
+    L.disco_mlp_grad.argtypes = [vp, fp, fp, C.c_int, fp, fp]
+    L.disco_mlp_infer.argtypes = [vp, fp, C.c_long]
+    L.disco_c2p_features.argtypes = [vp, fp, fp, C.c_long]
+    L._c2p_ready = True
+    return L
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class C2P:
+    """Batched GRMHD cons2prim on the GPU (plain and learned starts)."""
+
+    def __init__(self, capacity: int, gamma: float, M: float = 1.0, a: float = 0.9, prec: float = 1e-12,
+                 max_iter: int = 30, nextra: int = 10, rmin: float | None = None, rmax: float = 50.0,
+                 theta_cut: float = 0.1):
+        if rmin is None:
+            rmin = 1.5 * (M + (M * M - a * a) ** 0.5)
+        self.p = C2PParams(gamma, M, a, prec, max_iter, nextra, rmin, rmax, theta_cut)
+        self.capacity = int(capacity)
+        L = _c2p_lib()
+        h = C.c_void_p()
+        self._chk(L.disco_c2p_create(C.byref(self.p), self.capacity, C.byref(h)), "disco_c2p_create")
+        self._h = h
+        self.n = 0
+
+    @staticmethod
+    def _chk(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {_c2p_lib().disco_c2p_last_error().decode()}")
+
+    def generate(self, seed: int, k0: int, n: int) -> None:
+        self._chk(_c2p_lib().disco_c2p_generate(self._h, seed, k0, n), "disco_c2p_generate")
+        self.n = n
+
+    def upload(self, cons: np.ndarray, x: np.ndarray) -> None:
+        cons = np.ascontiguousarray(cons, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        n = cons.shape[-1]
+        self._chk(_c2p_lib().disco_c2p_upload(self._h, _dptr(cons), _dptr(x), n), "disco_c2p_upload")
+        self.n = n
+
+    def solve(self, mode: int = PLAIN, nextra: int | None = None, split: bool = False):
+        """Solve the resident states; returns ms (or (ms, ms_network) with split)."""
+        ms = np.zeros(2)
+        ne = self.p.nextra if nextra is None else nextra
+        self._chk(_c2p_lib().disco_c2p_solve(self._h, mode, ne, _dptr(ms)), "disco_c2p_solve")
+        return (float(ms[0]), float(ms[1])) if split else float(ms[0])
+
+    def download(self, n: int | None = None) -> dict:
+        n = self.n if n is None else n
+        prim = np.zeros(8 * n)
+        it = np.zeros(n, np.int32)
+        fl = np.zeros(n, np.int32)
+        self._chk(_c2p_lib().disco_c2p_download(self._h, _dptr(prim), _iptr(it), _iptr(fl), n), "download")
+        return dict(prim=prim.reshape(8, n), iters=it, flags=fl)
+
+    def errors(self, seed: int, k0: int) -> np.ndarray:
+        e = np.zeros(self.n, np.float32)
+        self._chk(_c2p_lib().disco_c2p_errors(self._h, seed, k0, _fptr(e)), "disco_c2p_errors")
+        return e
+
+    def train(self, seed: int, ntrain: int, steps: int, batch: int, lr: float) -> np.ndarray:
+        loss = np.zeros(steps, np.float32)
+        self._chk(_c2p_lib().disco_mlp_train(self._h, seed, ntrain, steps, batch, lr, _fptr(loss)), "disco_mlp_train")
+        return loss
+
+    def get_mlp(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        nrm = np.zeros(MLP_NNORM, np.float32)
+        self._chk(_c2p_lib().disco_mlp_get(self._h, _fptr(w), _fptr(nrm)), "disco_mlp_get")
+        return w, nrm
+
+    def set_mlp(self, w, nrm) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        self._chk(_c2p_lib().disco_mlp_set(self._h, _fptr(w), _fptr(nrm)), "disco_mlp_set")
+
+    def grad(self, feat: np.ndarray, tgt: np.ndarray):
+        feat = np.ascontiguousarray(feat, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        n = feat.shape[-1]
+        self._chk(_c2p_lib().disco_mlp_grad(self._h, _fptr(feat), _fptr(tgt), n, _fptr(g), _fptr(loss)),
+                  "disco_mlp_grad")
+        return g, float(loss[0])
+
+    def infer(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else n
+        out = np.zeros(2 * n, np.float32)
+        self._chk(_c2p_lib().disco_mlp_infer(self._h, _fptr(out), n), "disco_mlp_infer")
+        return out.reshape(n, 2)
+
+    def features(self, n: int | None = None):
+        n = self.n if n is None else n
+        f = np.zeros(8 * n, np.float32)
+        t = np.zeros(2 * n, np.float32)
+        self._chk(_c2p_lib().disco_c2p_features(self._h, _fptr(f), _fptr(t), n), "disco_c2p_features")
+        return f.reshape(8, n), t.reshape(2, n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _c2p_lib().disco_c2p_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

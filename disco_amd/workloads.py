@@ -39,3 +39,16 @@ def binary_batch(nruns: int = 48, seed: int = 20241203, nr_lo: int = 32, nr_hi: 
     phase = rng.uniform(0.0, 2.0 * np.pi, size=nruns)
     return [params("c3_binary_disk.par", Num_R=int(nr[s]), Init_Par0=s + 1, Init_Par6=repr(float(phase[s])))
             for s in range(nruns)]
+
+
+# Standalone GRMHD cons2prim benchmark: 2^24 states in Kerr-Schild, a = 0.9.
+C4 = dict(seed=20241203, M=1.0, a=0.9, gamma=4.0 / 3.0, theta_cut=0.1, rmax=50.0)
+
+
+def c4_params(**over) -> dict:
+    p = dict(C4)
+    p.update(over)
+    M, a = p["M"], p["a"]
+    r_h = M + (M * M - a * a) ** 0.5
+    p.setdefault("rmin", 1.5 * r_h)
+    return p

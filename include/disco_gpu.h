@@ -94,6 +94,66 @@ int disco_gpu_time_steps(disco_gpu_t* h, int nsteps, double* ms);
 void disco_gpu_destroy(disco_gpu_t* h);
 const char* disco_gpu_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * GRMHD cons2prim (Hydro/grmhd.c:228-245 -> cons2prim_solve_adiabatic_noble2d,
+ * grmhd.c:813-1029) for batches of states, with an optional learned initial
+ * guess.  The Newton solve, its tolerance (prec), Nextra extra iterations and
+ * max_iter are the reference's; only the starting point differs:
+ *   DISCO_C2P_PLAIN:   a cold start (fluid at rest, eta from the energy
+ *                      residual) -- standalone states carry no previous
+ *                      primitives, which the reference uses as its guess;
+ *   DISCO_C2P_LEARNED: an MLP (8 invariants -> 64x4 SiLU -> ln W, ln eta/e)
+ *                      evaluated with BF16 MFMA (FP32 accumulation), FP64
+ *                      Newton polish, and the plain start as the fallback
+ *                      when the polish does not converge.
+ * State layout (n states): cons[8*n] field-major in the GRMHD module's
+ * conserved slots (DDD, TAU, SRR, LLL, SZZ, BRR, BPP, BZZ, volume factor 1),
+ * x[3*n] = {R, phi, z} field-major; prim likewise. */
+typedef struct disco_c2p disco_c2p_t;
+typedef struct disco_c2p_params {
+    double gamma_law;  /* Adiabatic_Index */
+    double M, a;       /* Kerr-Schild mass and spin (a = 0: schw_ks_rpz.c) */
+    double prec;       /* 1e-12 in the reference */
+    int max_iter;      /* 30 */
+    int nextra;        /* 10 */
+    double rmin, rmax, theta_cut;  /* benchmark state generator ranges */
+} disco_c2p_params;
+enum { DISCO_C2P_PLAIN = 0, DISCO_C2P_LEARNED = 1 };
+
+int disco_c2p_create(const disco_c2p_params* p, long capacity, disco_c2p_t** out);
+void disco_c2p_destroy(disco_c2p_t* h);
+const char* disco_c2p_last_error(void);
+/* benchmark states seed/k0..k0+n-1 generated on the device (stay resident) */
+int disco_c2p_generate(disco_c2p_t* h, unsigned long long seed, long k0, long n);
+/* caller's states (host buffers) */
+int disco_c2p_upload(disco_c2p_t* h, const double* cons, const double* x, long n);
+/* solve the resident states; ms[0] = time of the solve, ms[1] = time of a
+ * separate evaluation of the network alone on the same states (learned mode;
+ * not part of ms[0]), HIP events on the handle's stream */
+int disco_c2p_solve(disco_c2p_t* h, int mode, int nextra, double* ms);
+/* prim[8*n], iters[n] (Newton iterations, fallback included), flags[n]:
+ * 0 converged, 1 learned guess failed and the fallback converged,
+ * 2 plain start did not converge, 3 neither converged */
+int disco_c2p_download(disco_c2p_t* h, double* prim, int* iters, int* flags, long n);
+/* per-state error of the resident solution against the generator's exact
+ * primitives (rho, P relative; velocities on the state's velocity scale) */
+int disco_c2p_errors(disco_c2p_t* h, unsigned long long seed, long k0, float* err);
+/* train the MLP on the GPU: the plain solve produces (features, targets) on
+ * states seed/0..ntrain-1, then `steps` Adam steps on minibatches */
+int disco_mlp_train(disco_c2p_t* h, unsigned long long seed, long ntrain, int steps, int batch, float lr,
+                    float* loss_curve);
+/* FP32 master parameters (13186) and normalisation (8+8 feature mean/std,
+ * 2+2 target mean/std) */
+int disco_mlp_get(disco_c2p_t* h, float* params, float* norm);
+int disco_mlp_set(disco_c2p_t* h, const float* params, const float* norm);
+/* gradient of the training loss on caller data (feat[8*n], tgt[2*n],
+ * field-major), for checks against a CPU restatement */
+int disco_mlp_grad(disco_c2p_t* h, const float* feat, const float* tgt, int n, float* grad, float* loss);
+/* BF16-MFMA network outputs (standardised) for the resident states */
+int disco_mlp_infer(disco_c2p_t* h, float* out, long n);
+/* features and plain-solve targets of the resident states */
+int disco_c2p_features(disco_c2p_t* h, float* feat, float* tgt, long n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,35 @@ def totals(st: dict) -> np.ndarray:
     return st["cons"][:, real].sum(axis=1)
 
 
+def grmhd_leaf_vectors(n: int = 512) -> None:
+    """The reference's own grmhd.c / schw_ks_rpz.c (a = 0) on generated states:
+    metric, prim2cons and cons2prim from a guess 1% off the true state."""
+    import ctypes as C
+
+    from disco_amd.workloads import c4_params
+
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "_build", "liboracle.so"))
+    P = c4_params(a=0.0, seed=777)
+    x = np.zeros((n, 3))
+    prim = np.zeros((n, 8))
+    dp = C.POINTER(C.c_double)
+    lib.oracle_c4_states.argtypes = [C.c_uint64] + [C.c_double] * 6 + [C.c_long, C.c_long, dp, dp]
+    lib.oracle_c4_states(P["seed"], P["M"], P["a"], P["gamma"], P["rmin"], P["rmax"], P["theta_cut"], 0, n,
+                         x.ctypes.data_as(dp), prim.ctypes.data_as(dp))
+    rng = np.random.default_rng(5)
+    guess = prim * (1.0 + 0.01 * rng.uniform(-1, 1, prim.shape))
+    exe = os.path.join(ROOT, "oracle", "_ref", "disco_ref_grmhd_leaf")
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.int32(n).tobytes())
+            f.write(np.concatenate([x, prim, guess], axis=1).astype(np.float64).tobytes())
+        subprocess.run([exe, repr(P["gamma"]), repr(P["M"]), fin, fout], check=True, timeout=60)
+        out = np.fromfile(fout, np.float64).reshape(n, 8 + 8 + 23)
+    np.savez_compressed(os.path.join(HERE, "grmhd_leaf_a0.npz"), gamma=P["gamma"], M=P["M"], x=x, prim=prim,
+                        guess=guess, cons=out[:, :8], prim_out=out[:, 8:16], metric=out[:, 16:])
+
+
 def main() -> None:
     # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
     txt = par_with("c1_kepler2d.par", Num_R=16)
@@ -83,6 +112,7 @@ def main() -> None:
              dt=np.array([states[s]["dt"] for s in (0, 1, 20)]),
              ncell_real=np.array(int((states[0]["real"] == 1).sum())),
              ref_seconds_20_steps=np.array(secs))
+    grmhd_leaf_vectors()
     print("fixtures written")
 
 

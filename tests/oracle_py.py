@@ -96,3 +96,54 @@ def track_faces(piA, dpA, piB, dpB, phimax=2 * np.pi):
                              iR.ctypes.data_as(ip), LR.ctypes.data_as(ip), pm.ctypes.data_as(C.POINTER(C.c_double)),
                              pp.ctypes.data_as(C.POINTER(C.c_double)))
     return iL, iR, LR, pm, pp
+
+
+# ---------------------------------------------------------------- GRMHD
+def _grmhd_lib():
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.oracle_metric.argtypes = [dp, C.c_double, C.c_double, dp]
+    L.oracle_grmhd_prim2cons.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double]
+    L.oracle_grmhd_cons2prim.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.c_int, ip, dp]
+    L.oracle_grmhd_cons2prim.restype = C.c_int
+    L.oracle_grmhd_guess.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+    L.oracle_c4_states.argtypes = [C.c_uint64] + [C.c_double] * 6 + [C.c_long, C.c_long, dp, dp]
+    return L
+
+
+def c4_states(P: dict, k0: int, n: int):
+    x = np.zeros((n, 3))
+    prim = np.zeros((n, 8))
+    _grmhd_lib().oracle_c4_states(P["seed"], P["M"], P["a"], P["gamma"], P["rmin"], P["rmax"], P["theta_cut"],
+                                   k0, n, _d(x), _d(prim))
+    return x, prim
+
+
+def metric(x, M, a):
+    out = np.zeros(23)
+    _grmhd_lib().oracle_metric(_d(np.asarray(x, float)), M, a, _d(out))
+    return out
+
+
+def prim2cons(prim, x, gamma, M, a):
+    cons = np.zeros(8)
+    _grmhd_lib().oracle_grmhd_prim2cons(_d(np.asarray(prim, float)), _d(cons), _d(np.asarray(x, float)), gamma, M, a)
+    return cons
+
+
+def cons2prim(cons, x, gamma, M, a, v20, eta0, prec=1e-12, max_iter=30, nextra=10):
+    prim = np.zeros(8)
+    inv = np.zeros(4)
+    conv = C.c_int(0)
+    it = _grmhd_lib().oracle_grmhd_cons2prim(_d(np.asarray(cons, float)), _d(prim), _d(np.asarray(x, float)), gamma,
+                                             M, a, v20, eta0, prec, max_iter, nextra, C.byref(conv), _d(inv))
+    return prim, it, conv.value, inv
+
+
+def guess_from_prim(prim, x, gamma, M, a):
+    v2, eta = C.c_double(), C.c_double()
+    _grmhd_lib().oracle_grmhd_guess(_d(np.asarray(prim, float)), _d(np.asarray(x, float)), gamma, M, a,
+                                    C.byref(v2), C.byref(eta))
+    return v2.value, eta.value
