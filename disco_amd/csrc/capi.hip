@@ -15,10 +15,10 @@
 #include "host_setup.h"
 
 namespace disco {
-void launch_mindt(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
 void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook);
-void launch_substep(const DevState& S, double RK, double dtfac, int first, int want_dt, hipStream_t st,
-                    const LaunchHook* hook);
+void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
+                    hipStream_t st, const LaunchHook* hook);
 }  // namespace disco
 
 using namespace disco;
@@ -45,10 +45,12 @@ struct disco_gpu {
     AnnDev* ann = nullptr;
     int* cell_ann = nullptr;
     RunDev* druns = nullptr;
-    double *prim = nullptr, *cons = nullptr, *rkcons = nullptr, *gradp = nullptr, *gradr = nullptr;
+    StepShape shape{};
+    double *prim = nullptr, *cons = nullptr, *rkcons = nullptr, *gradp = nullptr, *gradr = nullptr,
+           *gradz = nullptr;
     double *piph[2] = {nullptr, nullptr}, *dphi[2] = {nullptr, nullptr}, *wiph = nullptr;
-    int *lbo = nullptr, *ubi = nullptr, *rot = nullptr;
-    double *fphi = nullptr, *fout = nullptr, *fin = nullptr;
+    int *lbo = nullptr, *ubi = nullptr, *lbu = nullptr, *ubd = nullptr, *rot = nullptr;
+    double *fphi = nullptr, *fout = nullptr, *fin = nullptr, *fup = nullptr, *fdn = nullptr;
     unsigned long long* dtbits = nullptr;
     double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
     int parity = 0;
@@ -68,6 +70,7 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.rkcons = h->rkcons;
     S.gradp = h->gradp;
     S.gradr = h->gradr;
+    S.gradz = h->gradz;
     int p = h->parity;
     if (substep == 0) {
         S.piph = h->piph[p];
@@ -83,10 +86,14 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.wiph = h->wiph;
     S.lbo = h->lbo;
     S.ubi = h->ubi;
+    S.lbu = h->lbu;
+    S.ubd = h->ubd;
     S.rot = h->rot;
     S.fphi = h->fphi;
     S.fout = h->fout;
     S.fin = h->fin;
+    S.fup = h->fup;
+    S.fdn = h->fdn;
     S.dtbits = h->dtbits;
     S.dt = h->dt;
     S.planet_phi = h->planet_phi;
@@ -102,14 +109,14 @@ static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
     DevState S0 = make_state(h, 0);
     DevState S1 = make_state(h, 1);
     if (with_dt) launch_dt_final(S0, h->stream, hook);
-    launch_substep(S0, 0.0, 1.0, 1, 0, h->stream, hook);
-    launch_substep(S1, 0.5, 0.5, 0, 1, h->stream, hook);
+    launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook);
+    launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook);
 }
 
 // Recomputes the CFL minimum over the current state (after create/upload).
 static int seed_dt(disco_gpu* h) {
     DevState S = make_state(h, 0);
-    launch_mindt(S, h->stream, nullptr);
+    launch_mindt(h->shape, S, h->stream, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
@@ -146,10 +153,16 @@ static void build_tables(disco_gpu* h) {
         h->hruns.push_back(D);
         h->run_cell0.push_back(cell0);
         h->run_ann0.push_back(ann0);
-        // trans_flux ranges, misc.c:459-471 (no CT)
+        // trans_flux ranges (misc.c:459-497, no CT): r-tracks of real
+        // z-layers, z-tracks of real r-columns
         int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
         int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
         int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
+        int zkmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
+        int zkmax = R.NgZb == 0 ? R.Nz - 1 : R.Nz - R.NgZb;
+        int zjmin = R.NgRa, zjmax = R.Nr - R.NgRb;
+        bool zper = R.Nz > 1 && R.par.Z_Periodic;
+        int Ng = R.NgZa;
         for (int k = 0; k < R.Nz; ++k)
             for (int j = 0; j < R.Nr; ++j) {
                 int jk = j + R.Nr * k;
@@ -161,15 +174,28 @@ static void build_tables(disco_gpu* h) {
                 A.off = cell0 + R.off[jk];
                 A.outer = (j + 1 < R.Nr) ? ann0 + jk + 1 : -1;
                 A.inner = (j > 0) ? ann0 + jk - 1 : -1;
+                A.up = (k + 1 < R.Nz) ? ann0 + jk + R.Nr : -1;
+                A.down = (k > 0) ? ann0 + jk - R.Nr : -1;
+                A.zsrc = -1;
                 int f = 0;
                 bool kin = k >= kmin && k < kmax;
                 if (R.real_annulus(j, k)) f |= ANN_REAL | ANN_PHIFLUX;
                 if (kin && j >= jmin && j < jmax) f |= ANN_OUTFLUX;
                 if (kin && j - 1 >= jmin && j - 1 < jmax) f |= ANN_INFLUX;
+                bool jin = j >= zjmin && j < zjmax;
+                if (R.Nz > 1 && jin && k >= zkmin && k < zkmax) f |= ANN_UPFLUX;
+                if (R.Nz > 1 && jin && k - 1 >= zkmin && k - 1 < zkmax) f |= ANN_DNFLUX;
                 bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic;
                 bool zghost = (k < R.NgZa || k >= R.Nz - R.NgZb) && !R.par.Z_Periodic &&
                               R.par.boundary == DISCO_BC_FIXED3D;
                 if (!(f & ANN_REAL) && (rghost || zghost)) f |= ANN_GHOST_FILL;
+                // periodic z (exchange.c:80-115): ghost k < Ng copies layer
+                // Nz-2Ng+k, ghost k >= Nz-Ng copies layer k-(Nz-2Ng)
+                if (zper && (k < Ng || k >= R.Nz - Ng)) {
+                    int ks = k < Ng ? k + R.Nz - 2 * Ng : k - (R.Nz - 2 * Ng);
+                    A.zsrc = ann0 + j + R.Nr * ks;
+                    f |= ANN_ZCOPY;
+                }
                 A.flags = f;
                 A.rm = R.rj(j - 1);
                 A.rp = R.rj(j);
@@ -177,11 +203,18 @@ static void build_tables(disco_gpu* h) {
                 A.zp = R.zk(k);
                 A.rc = get_centroid(A.rp, A.rm, 1);
                 A.zc = get_centroid(A.zp, A.zm, 2);
+                // faces_alt.c:111-122: centroid distances across the track
                 if (j + 1 < R.Nr) {
                     double rmc = get_centroid(R.rj(j), R.rj(j - 1), 1);
                     double rpc = get_centroid(R.rj(j + 1), R.rj(j), 1);
                     A.dxL = R.rj(j) - rmc;
                     A.dxR = rpc - R.rj(j);
+                }
+                if (k + 1 < R.Nz) {
+                    double zmc = get_centroid(R.zk(k), R.zk(k - 1), 2);
+                    double zpc = get_centroid(R.zk(k + 1), R.zk(k), 2);
+                    A.dzL = R.zk(k) - zmc;
+                    A.dzR = zpc - R.zk(k);
                 }
                 if (R.par.boundary == DISCO_BC_FIXED) {
                     A.rc_b = .5 * (A.rp + A.rm);
@@ -197,6 +230,13 @@ static void build_tables(disco_gpu* h) {
     }
     h->ncell = cell0;
     h->nann = ann0;
+    h->shape.hydro = h->runs.empty() ? 0 : h->runs[0].par.hydro;
+    h->shape.has_z = 0;
+    h->shape.zcopy = 0;
+    for (const HostRun& R : h->runs) {
+        if (R.Nz > 1) h->shape.has_z = 1;
+        if (R.Nz > 1 && R.par.Z_Periodic) h->shape.zcopy = 1;
+    }
 }
 
 static int upload_run_state(disco_gpu* h, int r) {
@@ -269,10 +309,10 @@ void disco_gpu_destroy(disco_gpu_t* h) {
     if (!h) return;
     for (int p = 0; p < 2; ++p)
         if (h->graph[p]) (void)hipGraphExecDestroy(h->graph[p]);
-    void* ptrs[] = {h->ann,  h->cell_ann, h->druns,   h->prim,   h->cons,   h->rkcons, h->gradp,
-                    h->gradr, h->piph[0], h->piph[1], h->dphi[0], h->dphi[1], h->wiph,  h->lbo,
-                    h->ubi,  h->rot,      h->fphi,    h->fout,   h->fin,    h->dtbits, h->dt,
-                    h->planet_phi, h->t};
+    void* ptrs[] = {h->ann,   h->cell_ann, h->druns, h->prim,   h->cons,    h->rkcons,     h->gradp,
+                    h->gradr, h->gradz,    h->piph[0], h->piph[1], h->dphi[0], h->dphi[1], h->wiph,
+                    h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot,     h->fphi,       h->fout,
+                    h->fin,   h->fup,      h->fdn,   h->dtbits, h->dt,      h->planet_phi, h->t};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -313,6 +353,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->rkcons, Q);
     rc |= dalloc(&h->gradp, Q);
     rc |= dalloc(&h->gradr, Q);
+    rc |= dalloc(&h->gradz, Q);
     rc |= dalloc(&h->piph[0], N);
     rc |= dalloc(&h->piph[1], N);
     rc |= dalloc(&h->dphi[0], N);
@@ -320,10 +361,14 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->wiph, N);
     rc |= dalloc(&h->lbo, N);
     rc |= dalloc(&h->ubi, N);
+    rc |= dalloc(&h->lbu, N);
+    rc |= dalloc(&h->ubd, N);
     rc |= dalloc(&h->rot, h->nann);
     rc |= dalloc(&h->fphi, Q);
     rc |= dalloc(&h->fout, Q);
     rc |= dalloc(&h->fin, Q);
+    rc |= dalloc(&h->fup, Q);
+    rc |= dalloc(&h->fdn, Q);
     rc |= dalloc(&h->dtbits, nruns);
     rc |= dalloc(&h->dt, nruns);
     rc |= dalloc(&h->planet_phi, (size_t)nruns * DISCO_MAX_PLANETS);
@@ -343,12 +388,13 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         hipMemcpy(h->druns, h->hruns.data(), nruns * sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup("upload of tables failed");
     // zero the scratch fields once (eager, outside any graph)
-    double* zq[] = {h->rkcons, h->gradp, h->gradr, h->fphi, h->fout, h->fin};
+    double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz, h->fphi, h->fout, h->fin, h->fup, h->fdn};
     for (double* p : zq)
         if (hipMemset(p, 0, Q * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
-    if (hipMemset(h->lbo, 0, N * sizeof(int)) != hipSuccess || hipMemset(h->ubi, 0, N * sizeof(int)) != hipSuccess ||
-        hipMemset(h->dt, 0, nruns * sizeof(double)) != hipSuccess)
-        return cleanup("hipMemset");
+    int* zi[] = {h->lbo, h->ubi, h->lbu, h->ubd};
+    for (int* p : zi)
+        if (hipMemset(p, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
+    if (hipMemset(h->dt, 0, nruns * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
     for (int r = 0; r < nruns; ++r) {
         if (upload_run_state(h, r)) return cleanup(g_err);
         // the other mesh buffer starts equal (substep 2 reads it)
@@ -408,7 +454,7 @@ int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kp
 int disco_gpu_mindt(disco_gpu_t* h, double* dt_out) {
     if (!h) return fail("disco_gpu_mindt: null handle");
     DevState S = make_state(h, 0);
-    launch_mindt(S, h->stream, nullptr);
+    launch_mindt(h->shape, S, h->stream, nullptr);
     launch_dt_final(S, h->stream, nullptr);
     HIPCHK(hipGetLastError());
     if (dt_out) {

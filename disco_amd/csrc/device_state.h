@@ -5,9 +5,9 @@
 // run, annulus by annulus (jk = j + Nr*k), then by their array index inside
 // the annulus -- the reference's theCells[jk][i] order (domain.c:34-38)
 // flattened with prefix-sum offsets.  Annuli have a global index over the
-// batch; each annulus record carries its run, its neighbours and its static
-// geometry.  Only piph/dphi (the moving mesh) change shape information during
-// a step; the annulus table never changes because Np is fixed (AMR is off in
+// batch; each annulus record carries its run, its four neighbours (r and z)
+// and its static geometry.  Only piph/dphi (the moving mesh) change during a
+// step; the annulus table never changes because Np is fixed (AMR is off in
 // the reference, onestep.c:179-181).
 #pragma once
 
@@ -22,7 +22,13 @@ enum AnnFlags {
     ANN_OUTFLUX = 4,     // the r-track (this | outer) carries fluxes (misc.c:441-519)
     ANN_INFLUX = 8,      // the r-track (inner | this) carries fluxes
     ANN_GHOST_FILL = 16, // ghost annulus re-initialised by the boundary module
+    ANN_UPFLUX = 32,     // the z-track (this | up) carries fluxes
+    ANN_DNFLUX = 64,     // the z-track (down | this) carries fluxes
+    ANN_ZCOPY = 128,     // periodic-z ghost: copy of annulus `zsrc` after each substep
 };
+
+// Transverse directions of the overlap faces (the reference's dim 1 and 2).
+enum { DIR_R = 1, DIR_Z = 2 };
 
 struct AnnDev {
     int run;
@@ -31,10 +37,14 @@ struct AnnDev {
     int off;    // global index of the annulus' first cell
     int outer;  // global annulus index of (j+1, k), -1 if none
     int inner;  // global annulus index of (j-1, k), -1 if none
+    int up;     // global annulus index of (j, k+1), -1 if none
+    int down;   // global annulus index of (j, k-1), -1 if none
+    int zsrc;   // periodic-z ghosts: the real annulus they copy, else -1
     int flags;
     double rm, rp, zm, zp;  // r_jph[j-1], r_jph[j], z_kph[k-1], z_kph[k]
     double rc, zc;          // r and z centroids (geometry_functions.c:5-10)
-    double dxL, dxR;        // centroid distances across the (this | outer) track (faces_alt.c:111-121)
+    double dxL, dxR;        // centroid distances across the (this | outer) track (faces_alt.c:111-116)
+    double dzL, dzR;        // centroid distances across the (this | up) track (faces_alt.c:117-122)
     double rc_b, zc_b;      // boundary-module evaluation point (fixed.c: means, fixed3d: centroids)
 };
 
@@ -61,6 +71,7 @@ struct DevState {
     double* rkcons;
     double* gradp;
     double* gradr;
+    double* gradz;
     const double* piph;  // mesh at the start of the substep
     const double* dphi;
     double* piph_n;      // mesh after move_cells (first substep only)
@@ -68,14 +79,25 @@ struct DevState {
     double* wiph;
     int* lbo;            // per cell: partner in the outer annulus (array index)
     int* ubi;            // per cell: partner in the inner annulus (array index)
+    int* lbu;            // per cell: partner in the up annulus
+    int* ubd;            // per cell: partner in the down annulus
     int* rot;            // per annulus: array index of the smallest front
     double* fphi;        // per cell: increment of its phi face (i | i+1)
     double* fout;        // per cell: increment of its owned face in the outer track
     double* fin;         // per cell: increment of its owned face in the inner track
+    double* fup;         // per cell: increment of its owned face in the up track
+    double* fdn;         // per cell: increment of its owned face in the down track
     unsigned long long* dtbits;  // per run: running min of the CFL step (bit pattern)
     double* dt;          // per run: current step
     double* planet_phi;  // per run x planet
     double* t;           // per run: simulation time
+};
+
+// What a batch needs launched each substep (uniform over its runs).
+struct StepShape {
+    int hydro;  // DISCO_HYDRO_* of every run
+    int has_z;  // some run has Num_Z > 1: z-tracks exist
+    int zcopy;  // some run is periodic in z: ghost z-layers are copies
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

@@ -193,8 +193,10 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.num_c != 5) m += " Num_C!=5 for euler";
     if (p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) m += " NoBC_*";
     if (p.R_Periodic) m += " R_Periodic";
-    if (p.Z_Periodic) m += " Z_Periodic";
-    if (p.Num_Z != 1) m += " Num_Z>1";
+    // a serial run fills ghost z-layers only through periodic copies or the
+    // fixed3d boundary (Boundary/fixed.c leaves them unset)
+    if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary != DISCO_BC_FIXED3D) m += " Num_Z>1 needs Z_Periodic or fixed3d";
+    if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
     if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC) m += " Riemann_Solver";
@@ -399,6 +401,33 @@ void host_boundary(HostRun* Rp) {
     }
 }
 
+// Periodic z at set-up (main.c:57-61 -> exchangeData(1), exchange.c:80-115
+// with a self-wraparound rank): ghost layer k < Ng copies real layer
+// Nz-2Ng+k, ghost layer k >= Nz-Ng copies layer k-(Nz-2Ng), every column j.
+static void host_zcopy(HostRun& R, bool mesh_and_w) {
+    if (R.Nz <= 1 || !R.par.Z_Periodic) return;
+    const int Ng = R.NgZa;
+    for (int k = 0; k < R.Nz; ++k) {
+        if (k >= Ng && k < R.Nz - Ng) continue;
+        int ks = k < Ng ? k + R.Nz - 2 * Ng : k - (R.Nz - 2 * Ng);
+        for (int j = 0; j < R.Nr; ++j) {
+            int jk = j + R.Nr * k, js = j + R.Nr * ks;
+            int o = R.off[jk], os = R.off[js];
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                for (int q = 0; q < R.nq; ++q) {
+                    R.prim[(size_t)q * R.ncell + o + i] = R.prim[(size_t)q * R.ncell + os + i];
+                    R.cons[(size_t)q * R.ncell + o + i] = R.cons[(size_t)q * R.ncell + os + i];
+                }
+                if (mesh_and_w) {
+                    R.piph[o + i] = R.piph[os + i];
+                    R.dphi[o + i] = R.dphi[os + i];
+                    R.wiph[o + i] = R.wiph[os + i];
+                }
+            }
+        }
+    }
+}
+
 // misc.c:101-168 (modes 0, 1, 4)
 static void set_wcell(HostRun& R) {
     R.wiph.assign(R.ncell, 0.0);
@@ -482,9 +511,11 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
             }
         }
-    // main.c:52-61 ghost fill (done before set_wcell so ghost w is defined)
+    // main.c:52-61 ghost fill (done before set_wcell so ghost w is defined);
+    // the periodic copy carries the source layers' mesh and mesh speeds
     host_boundary(&R);
     set_wcell(R);
+    host_zcopy(R, true);
     find_rotation(R);
     R.t = par.t_min;
     return 0;

@@ -1,12 +1,12 @@
-// HIP kernels of one RK substep of the moving-mesh finite-volume update for
-// the Newtonian hydro module (gfx950, FP64).
+// HIP kernels of one RK substep of the moving-mesh finite-volume update
+// (gfx950, FP64), for 2D and 3D cylindrical grids.
 //
 // Reference sequence (onestep.c:86-187) and where each step lives here:
 //   set_wcell, adjust_RK_cons, plm_phi       -> k_stage_a
-//   setup_faces + plm_trans (r)              -> k_stage_a (partner search) + k_plm_r
-//   phi_flux, trans_flux (r)                 -> k_flux
+//   setup_faces + plm_trans (r and z)        -> k_stage_a (partner search) + k_plm_trans
+//   phi_flux, trans_flux (r and z)           -> k_flux
 //   add_source, move_cells, clean_pi,
-//   calc_dp, calc_prim, boundary_trans       -> k_update
+//   calc_dp, calc_prim, boundary_trans       -> k_update (+ k_zcopy: periodic z)
 //   movePlanets + clean_pi (planets)         -> k_planets
 //   getmindt                                 -> fused into the last substep's
 //                                               k_update (k_mindt at start-up)
@@ -16,14 +16,16 @@
 // each face ending at the front edge of one of its two cells (LRtype).  Here
 // every cell owns the one face of each neighbouring track that ends at its
 // own front edge; its partner is the neighbour-annulus cell that is current
-// when the reference's merge consumes this front: the first outer cell whose
-// front is >= mine (ties go to the inner list first) or the first inner cell
+// when the reference's merge consumes this front: the first upper cell whose
+// front is >= mine (ties go to the lower list first) or the first lower cell
 // whose front is > mine.  Both are binary searches over the neighbour's
 // fronts in rotated (ascending) order.  The faces a cell shares but does not
 // own are those owned by the neighbour cells whose fronts fall inside its
 // span: a cyclic range bounded by its own partner and its predecessor's.
 // Each face increment is computed once, by its owner, stored per cell, and
 // gathered by both cells: deterministic, no atomics, no face lists in HBM.
+// One code path serves the r-tracks (j | j+1) and the z-tracks (k | k+1),
+// selected by the template direction D.
 #include <hip/hip_runtime.h>
 
 #include "device_state.h"
@@ -33,10 +35,35 @@ namespace disco {
 
 #define DEV __device__ static inline __attribute__((always_inline))
 
+// ------------------------------------------------------------------ physics
+// The Hydro module's per-face and per-cell leaves (hydro.h:6-35) as a policy.
+struct PhysEuler {
+    static constexpr int NQ = 5;
+    DEV void riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
+                     double w, double dAdt, double* dU) {
+        euler::riemann<NQ>(P, pL, pR, x, n, w, dAdt, dU);
+    }
+    DEV void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                    double dVdt) {
+        euler::source(P, prim, cons, xp, xm, dVdt);
+    }
+    DEV void planet_src(const PhysParams& P, const Planet& pl, const double* prim, double* cons, const double* xp,
+                        const double* xm, double dVdt) {
+        euler::planet_src(P, pl, prim, cons, xp, xm, dVdt);
+    }
+    DEV void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+        euler::cons2prim<NQ>(P, cons, prim, x, dV);
+    }
+    DEV double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+        return euler::mindt(P, prim, w, xp, xm);
+    }
+};
+
+// ------------------------------------------------------------------ helpers
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Remap so that XCD x works on one contiguous band of cells: partner cells in
-// the neighbouring annuli (about Np cells away) are then mostly in the same
-// L2 as the block that needs them.
+// the neighbouring annuli (about Np cells away in r, Nr*Np in z) are then
+// mostly in the same L2 as the block that needs them.
 DEV int cell_index() {
     const int nb = gridDim.x, b = blockIdx.x;
     const int x = b & 7, i = b >> 3;
@@ -76,15 +103,48 @@ DEV int front_search(const double* __restrict__ piph, int off, int n, int rot, d
     return idx;
 }
 
-// Geometry of the r-face between L (inner annulus) and R (outer annulus),
-// faces_alt.c:10-34 and :146-168, plus the phi offsets of plm.c:86-91.
-struct RFace {
-    double phic, dA, dpL, dpR;
+// Per-direction accessors: the neighbouring annuli, partner arrays, gradient,
+// owned-face increments, flags and centroid distances of direction D.
+template <int D>
+struct Dir;
+template <>
+struct Dir<DIR_R> {
+    DEV int hi(const AnnDev& A) { return A.outer; }
+    DEV int lo(const AnnDev& A) { return A.inner; }
+    DEV int* part_hi(const DevState& S) { return S.lbo; }
+    DEV int* part_lo(const DevState& S) { return S.ubi; }
+    DEV double* grad(const DevState& S) { return S.gradr; }
+    DEV double* f_hi(const DevState& S) { return S.fout; }
+    DEV double* f_lo(const DevState& S) { return S.fin; }
+    static constexpr int FLAG_HI = ANN_OUTFLUX, FLAG_LO = ANN_INFLUX;
+    DEV double dxL(const AnnDev& A) { return A.dxL; }
+    DEV double dxR(const AnnDev& A) { return A.dxR; }
+};
+template <>
+struct Dir<DIR_Z> {
+    DEV int hi(const AnnDev& A) { return A.up; }
+    DEV int lo(const AnnDev& A) { return A.down; }
+    DEV int* part_hi(const DevState& S) { return S.lbu; }
+    DEV int* part_lo(const DevState& S) { return S.ubd; }
+    DEV double* grad(const DevState& S) { return S.gradz; }
+    DEV double* f_hi(const DevState& S) { return S.fup; }
+    DEV double* f_lo(const DevState& S) { return S.fdn; }
+    static constexpr int FLAG_HI = ANN_UPFLUX, FLAG_LO = ANN_DNFLUX;
+    DEV double dxL(const AnnDev& A) { return A.dzL; }
+    DEV double dxR(const AnnDev& A) { return A.dzR; }
 };
 
-DEV RFace r_face(double piphL, double dphiL, double piphR, double dphiR, int LR, double r_face,
-                 double dz, double Pmax) {
-    RFace F;
+// Geometry of the face between L (annulus AL) and R (the annulus above AL in
+// direction D): addFace and the merge step of faces_alt.c:10-34, :146-168,
+// plus the phi offsets of plm.c:86-91.  x is the face centroid (face cm).
+struct TFace {
+    double phic, dA, dpL, dpR;
+    double x[3];
+};
+
+template <int D>
+DEV TFace t_face(double piphL, double dphiL, double piphR, double dphiR, int LR, const AnnDev& AL, double Pmax) {
+    TFace F;
     double xp1 = LR == 0 ? piphL : piphR;
     double dd = piphL - dphiL - piphR + dphiR;
     while (dd > .5 * Pmax) dd -= Pmax;
@@ -92,7 +152,16 @@ DEV RFace r_face(double piphL, double dphiL, double piphR, double dphiR, int LR,
     double xm1 = dd < 0. ? piphR - dphiR : piphL - dphiL;
     double dp = get_dp(xp1, xm1, Pmax);
     F.phic = get_dp(xp1, .5 * dp, Pmax);
-    F.dA = r_face * dp * dz;
+    if (D == DIR_R) {
+        F.dA = AL.rp * dp * (AL.zp - AL.zm);  // get_dA(.., 1) = r dphi dz, r = r_jph[j]
+        F.x[0] = AL.rp;
+        F.x[2] = AL.zc;
+    } else {
+        F.dA = .5 * (AL.rp + AL.rm) * (AL.rp - AL.rm) * dp;  // get_dA(.., 2) = r dr dphi
+        F.x[0] = AL.rc;
+        F.x[2] = AL.zp;
+    }
+    F.x[1] = F.phic;
     double phiL = piphL - .5 * dphiL;
     double phiR = piphR - .5 * dphiR;
     F.dpL = get_signed_dp(F.phic, phiL, Pmax);
@@ -153,7 +222,7 @@ DEV void dt_reduce(const DevState& S, int run, double dtc) {
 
 // misc.c:39-84 for one real cell: mindt(prim, w = mean of the two phi-face
 // mesh speeds, xp, xm)
-template <int NQ>
+template <class Ph>
 DEV double cell_mindt(const DevState& S, const AnnDev& A, const PhysParams& P, int c, const double* prim) {
     int i = c - A.off;
     int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
@@ -162,11 +231,12 @@ DEV double cell_mindt(const DevState& S, const AnnDev& A, const PhysParams& P, i
     double xp[3] = {A.rp, phip, A.zp};
     double xm[3] = {A.rm, phim, A.zm};
     double w = .5 * (S.wiph[cm] + S.wiph[c]);
-    return euler::mindt(P, prim, w, xp, xm);
+    return Ph::mindt(P, prim, w, xp, xm);
 }
 
-template <int NQ>
+template <class Ph>
 __global__ __launch_bounds__(256) void k_mindt(DevState S) {
+    constexpr int NQ = Ph::NQ;
     int c = cell_index();
     int run = -1;
     double dtc = 1.0e300;
@@ -176,7 +246,7 @@ __global__ __launch_bounds__(256) void k_mindt(DevState S) {
         if (A.flags & ANN_REAL) {
             double prim[NQ];
             _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[(size_t)q * S.ncell + c];
-            dtc = cell_mindt<NQ>(S, A, S.runs[run].ph, c, prim);
+            dtc = cell_mindt<Ph>(S, A, S.runs[run].ph, c, prim);
         }
     }
     dt_reduce(S, run, dtc);
@@ -193,8 +263,9 @@ __global__ void k_dt_final(DevState S) {
 // set_wcell (misc.c:101-168, first substep), adjust_RK_cons (misc.c:226-244,
 // with timestep.c:18-24's copy folded into the first substep), plm_phi
 // (plm.c:15-51) and the partner search of the overlap faces.
-template <int NQ>
+template <class Ph>
 __global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int first) {
+    constexpr int NQ = Ph::NQ;
     int c = cell_index();
     if (c >= S.ncell) return;
     const AnnDev& A = S.ann[S.cell_ann[c]];
@@ -249,41 +320,42 @@ __global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int firs
         const AnnDev& W = S.ann[A.inner];
         S.ubi[c] = front_search(S.piph, W.off, W.np, S.rot[A.inner], front, false);
     }
+    if (A.up >= 0) {
+        const AnnDev& B = S.ann[A.up];
+        S.lbu[c] = front_search(S.piph, B.off, B.np, S.rot[A.up], front, true);
+    }
+    if (A.down >= 0) {
+        const AnnDev& W = S.ann[A.down];
+        S.ubd[c] = front_search(S.piph, W.off, W.np, S.rot[A.down], front, false);
+    }
 }
 
 // ------------------------------------------------------------------ plm_trans
-// plm.c:56-195 for dim = 1: area-weighted mean of the face slopes, divided by
-// the cell's total r-face area, then the per-face limiter (order
+// plm.c:56-195: area-weighted mean of the face slopes, divided by the cell's
+// total face area in the direction, then the per-face limiter (order
 // independent: zero on any sign change, else the smallest |PLM*S|).
 template <int NQ>
 struct SlopeAcc {
     double sum[NQ];
     double smin[NQ];
     unsigned pos, neg, zero;  // bit q set when some face slope of field q is >0, <0, ==0
+    bool any;
 };
 
 template <int NQ>
-DEV void acc_init(SlopeAcc<NQ>& a) {
-    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-        a.sum[q] = 0.0;
-        a.smin[q] = 0.0;
-    }
-    a.pos = a.neg = a.zero = 0u;
-}
-
-template <int NQ>
-DEV void acc_add(SlopeAcc<NQ>& a, const double* S, double dA, bool firstface) {
+DEV void acc_add(SlopeAcc<NQ>& a, const double* S, double dA) {
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         a.sum[q] += S[q] * dA;
         a.pos |= (S[q] > 0.0 ? 1u : 0u) << q;
         a.neg |= (S[q] < 0.0 ? 1u : 0u) << q;
         a.zero |= (S[q] == 0.0 ? 1u : 0u) << q;
-        a.smin[q] = (firstface || fabs(S[q]) < fabs(a.smin[q])) ? S[q] : a.smin[q];
+        a.smin[q] = (!a.any || fabs(S[q]) < fabs(a.smin[q])) ? S[q] : a.smin[q];
     }
+    a.any = true;
 }
 
 template <int NQ>
-DEV void face_slope(const DevState& S, int cL, int cR, const RFace& F, double dxL, double dxR, double* Sq) {
+DEV void face_slope(const DevState& S, int cL, int cR, const TFace& F, double dxL, double dxR, double* Sq) {
     const size_t N = S.ncell;
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double WL = S.prim[q * N + cL] + F.dpL * S.gradp[q * N + cL];
@@ -292,82 +364,88 @@ DEV void face_slope(const DevState& S, int cL, int cR, const RFace& F, double dx
     }
 }
 
-template <int NQ>
-__global__ __launch_bounds__(256) void k_plm_r(DevState S) {
-    int c = cell_index();
-    if (c >= S.ncell) return;
-    const AnnDev& A = S.ann[S.cell_ann[c]];
-    const PhysParams& P = S.runs[A.run].ph;
+// Gradient of cell c in direction D (plm_trans with dim = D).
+template <int NQ, int D>
+DEV void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P, int c, int cm) {
+    using Dr = Dir<D>;
     const size_t N = S.ncell;
     const double Pmax = P.phi_max;
+    double* grad = Dr::grad(S);
     if (P.plm == 0.0) {
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.gradr[q * N + c] = 0.0;
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) grad[q * N + c] = 0.0;
         return;
     }
-    int n = A.np;
-    int i = c - A.off;
-    int cm = A.off + (i == 0 ? n - 1 : i - 1);
     double phip = S.piph[c];
     double dph = S.dphi[c];
-    double phim = phip - dph;
-    double dz = A.zp - A.zm;
-
     SlopeAcc<NQ> acc;
-    acc_init(acc);
-    bool firstface = true;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        acc.sum[q] = 0.0;
+        acc.smin[q] = 0.0;
+    }
+    acc.pos = acc.neg = acc.zero = 0u;
+    acc.any = false;
     double Sq[NQ];
-    if (A.outer >= 0) {  // this cell is L of the (this | outer) track
-        const AnnDev& B = S.ann[A.outer];
+    const int ahi = Dr::hi(A), alo = Dr::lo(A);
+    if (ahi >= 0) {  // this cell is L of the (this | hi) track
+        const AnnDev& B = S.ann[ahi];
+        const int* part = Dr::part_hi(S);
         int nb = B.np;
-        int own = S.lbo[c];
-        int start = S.lbo[cm];
+        int own = part[c];
+        int start = part[cm];
         int cnt = own - start;
         if (cnt < 0) cnt += nb;
         int cb = B.off + own;
-        RFace F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 0, A.rp, dz, Pmax);
-        face_slope<NQ>(S, c, cb, F, A.dxL, A.dxR, Sq);
-        acc_add(acc, Sq, F.dA, firstface);
-        firstface = false;
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        face_slope<NQ>(S, c, cb, F, Dr::dxL(A), Dr::dxR(A), Sq);
+        acc_add(acc, Sq, F.dA);
         for (int t = 0; t < cnt; ++t) {
             int b = start + t;
             if (b >= nb) b -= nb;
             cb = B.off + b;
-            F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 1, A.rp, dz, Pmax);
-            face_slope<NQ>(S, c, cb, F, A.dxL, A.dxR, Sq);
-            acc_add(acc, Sq, F.dA, false);
+            F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 1, A, Pmax);
+            face_slope<NQ>(S, c, cb, F, Dr::dxL(A), Dr::dxR(A), Sq);
+            acc_add(acc, Sq, F.dA);
         }
     }
-    if (A.inner >= 0) {  // this cell is R of the (inner | this) track
-        const AnnDev& W = S.ann[A.inner];
+    if (alo >= 0) {  // this cell is R of the (lo | this) track
+        const AnnDev& W = S.ann[alo];
+        const int* part = Dr::part_lo(S);
         int nw = W.np;
-        int own = S.ubi[c];
-        int start = S.ubi[cm];
+        int own = part[c];
+        int start = part[cm];
         int cnt = own - start;
         if (cnt < 0) cnt += nw;
         int ca = W.off + own;
-        RFace F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 1, W.rp, dz, Pmax);
-        face_slope<NQ>(S, ca, c, F, W.dxL, W.dxR, Sq);
-        acc_add(acc, Sq, F.dA, firstface);
-        firstface = false;
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        face_slope<NQ>(S, ca, c, F, Dr::dxL(W), Dr::dxR(W), Sq);
+        acc_add(acc, Sq, F.dA);
         for (int t = 0; t < cnt; ++t) {
             int a = start + t;
             if (a >= nw) a -= nw;
             ca = W.off + a;
-            F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 0, W.rp, dz, Pmax);
-            face_slope<NQ>(S, ca, c, F, W.dxL, W.dxR, Sq);
-            acc_add(acc, Sq, F.dA, false);
+            F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 0, W, Pmax);
+            face_slope<NQ>(S, ca, c, F, Dr::dxL(W), Dr::dxR(W), Sq);
+            acc_add(acc, Sq, F.dA);
         }
     }
-    // plm.c:112-143: total r-face area of the cell
+    // plm.c:112-143: the cell's total face area in this direction
+    double phim = phip - dph;
     double dpp = get_dp(phip, phim, Pmax);
-    double dAp = A.rp * dpp * dz;
-    double dAm = A.rm * dpp * dz;
+    double dAp, dAm;
+    if (D == DIR_R) {
+        double dz = A.zp - A.zm;
+        dAp = A.rp * dpp * dz;
+        dAm = A.rm * dpp * dz;
+    } else {
+        dAp = .5 * (A.rp + A.rm) * (A.rp - A.rm) * dpp;
+        dAm = dAp;
+    }
     double dAtot = dAp + dAm;
-    if (A.inner < 0) dAtot = dAp;
-    if (A.outer < 0) dAtot = dAm;
+    if (alo < 0) dAtot = dAp;
+    if (ahi < 0) dAtot = dAm;
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double g = acc.sum[q] / dAtot;
-        if (firstface) {
+        if (!acc.any) {
             g = 0.0;
         } else if (g != 0.0) {
             bool flip = g > 0.0 ? (((acc.neg | acc.zero) >> q) & 1u) : (((acc.pos | acc.zero) >> q) & 1u);
@@ -378,101 +456,171 @@ __global__ __launch_bounds__(256) void k_plm_r(DevState S) {
                 if (fabs(lim) < fabs(g)) g = lim;
             }
         }
-        S.gradr[q * N + c] = g;
+        grad[q * N + c] = g;
     }
+}
+
+template <class Ph>
+__global__ __launch_bounds__(256) void k_plm_trans(DevState S, int has_z) {
+    constexpr int NQ = Ph::NQ;
+    int c = cell_index();
+    if (c >= S.ncell) return;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const PhysParams& P = S.runs[A.run].ph;
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+    trans_gradient<NQ, DIR_R>(S, A, P, c, cm);
+    if (has_z) trans_gradient<NQ, DIR_Z>(S, A, P, c, cm);
 }
 
 // ------------------------------------------------------------------ fluxes
 // riemann_phi (riemann.c:39-112) on the cell's phi face, riemann_trans
-// (riemann.c:114-213) on its owned r-faces; solve_riemann's increment
+// (riemann.c:114-213) on its owned faces; solve_riemann's increment
 // (F - w U*) dA dt is stored per owner cell.
-template <int NQ>
-DEV void r_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, const RFace& F, double dxL,
-                     double dxR, double r_face, double zc, double dt, double* dU) {
+template <class Ph, int D>
+DEV void t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, const TFace& F, const AnnDev& AL,
+                     double dt, double* dU) {
+    constexpr int NQ = Ph::NQ;
+    using Dr = Dir<D>;
     const size_t N = S.ncell;
+    const double* grad = Dr::grad(S);
+    const double dxL = Dr::dxL(AL), dxR = Dr::dxR(AL);
     double primL[NQ], primR[NQ];
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-        primL[q] = S.prim[q * N + cL] + S.gradr[q * N + cL] * dxL + S.gradp[q * N + cL] * F.dpL;
-        primR[q] = S.prim[q * N + cR] - S.gradr[q * N + cR] * dxR - S.gradp[q * N + cR] * F.dpR;
+        primL[q] = S.prim[q * N + cL] + grad[q * N + cL] * dxL + S.gradp[q * N + cL] * F.dpL;
+        primR[q] = S.prim[q * N + cR] - grad[q * N + cR] * dxR - S.gradp[q * N + cR] * F.dpR;
     }
-    double x[3] = {r_face, F.phic, zc};
-    const double n[3] = {1.0, 0.0, 0.0};
-    euler::riemann<NQ>(P, primL, primR, x, n, 0.0, F.dA * dt, dU);
+    const double n[3] = {D == DIR_R ? 1.0 : 0.0, 0.0, D == DIR_Z ? 1.0 : 0.0};
+    Ph::riemann(P, primL, primR, F.x, n, 0.0, F.dA * dt, dU);
 }
 
-template <int NQ>
-__global__ __launch_bounds__(256) void k_flux(DevState S, double dtfac) {
-    int c = cell_index();
-    if (c >= S.ncell) return;
-    const AnnDev& A = S.ann[S.cell_ann[c]];
-    if (!(A.flags & (ANN_PHIFLUX | ANN_OUTFLUX | ANN_INFLUX))) return;
-    const PhysParams& P = S.runs[A.run].ph;
+template <class Ph, int D>
+DEV void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& P, int c, double dt) {
+    constexpr int NQ = Ph::NQ;
+    using Dr = Dir<D>;
     const size_t N = S.ncell;
     const double Pmax = P.phi_max;
-    const double dt = S.dt[A.run] * dtfac;
-    int n = A.np;
-    int i = c - A.off;
     double dU[NQ];
     double phip = S.piph[c];
     double dph = S.dphi[c];
-    double dz = A.zp - A.zm;
+    if (A.flags & Dr::FLAG_HI) {
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        int cb = B.off + Dr::part_hi(S)[c];
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        t_face_flux<Ph, D>(S, P, c, cb, F, A, dt, dU);
+        double* f = Dr::f_hi(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+    }
+    if (A.flags & Dr::FLAG_LO) {
+        const AnnDev& W = S.ann[Dr::lo(A)];
+        int ca = W.off + Dr::part_lo(S)[c];
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        t_face_flux<Ph, D>(S, P, ca, c, F, W, dt, dU);
+        double* f = Dr::f_lo(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+    }
+}
+
+template <class Ph>
+__global__ __launch_bounds__(256) void k_flux(DevState S, double dtfac) {
+    constexpr int NQ = Ph::NQ;
+    int c = cell_index();
+    if (c >= S.ncell) return;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(A.flags & (ANN_PHIFLUX | ANN_OUTFLUX | ANN_INFLUX | ANN_UPFLUX | ANN_DNFLUX))) return;
+    const PhysParams& P = S.runs[A.run].ph;
+    const size_t N = S.ncell;
+    const double dt = S.dt[A.run] * dtfac;
+    int n = A.np;
+    int i = c - A.off;
 
     if (A.flags & ANN_PHIFLUX) {
         int cp = A.off + (i + 1 == n ? 0 : i + 1);
+        double dU[NQ];
         double primL[NQ], primR[NQ];
+        double dph = S.dphi[c];
         double dphR = S.dphi[cp];
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             primL[q] = S.prim[q * N + c] + .5 * S.gradp[q * N + c] * dph;
             primR[q] = S.prim[q * N + cp] - .5 * S.gradp[q * N + cp] * dphR;
         }
-        double x[3] = {A.rc, phip, A.zc};
+        double x[3] = {A.rc, S.piph[c], A.zc};
         const double nv[3] = {0.0, 1.0, 0.0};
-        double dA = (A.rp - A.rm) * dz;  // get_dA(xp, xm, 0)
-        double w = A.rc * S.wiph[c];     // get_scale_factor(x, 0) * wiph
-        euler::riemann<NQ>(P, primL, primR, x, nv, w, dA * dt, dU);
+        double dA = (A.rp - A.rm) * (A.zp - A.zm);  // get_dA(xp, xm, 0)
+        double w = A.rc * S.wiph[c];                // get_scale_factor(x, 0) * wiph
+        Ph::riemann(P, primL, primR, x, nv, w, dA * dt, dU);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fphi[q * N + c] = dU[q];
     }
-    if (A.flags & ANN_OUTFLUX) {
-        const AnnDev& B = S.ann[A.outer];
-        int cb = B.off + S.lbo[c];
-        RFace F = r_face(phip, dph, S.piph[cb], S.dphi[cb], 0, A.rp, dz, Pmax);
-        r_face_flux<NQ>(S, P, c, cb, F, A.dxL, A.dxR, A.rp, A.zc, dt, dU);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fout[q * N + c] = dU[q];
-    }
-    if (A.flags & ANN_INFLUX) {
-        const AnnDev& W = S.ann[A.inner];
-        int ca = W.off + S.ubi[c];
-        RFace F = r_face(S.piph[ca], S.dphi[ca], phip, dph, 1, W.rp, dz, Pmax);
-        r_face_flux<NQ>(S, P, ca, c, F, W.dxL, W.dxR, W.rp, W.zc, dt, dU);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fin[q * N + c] = dU[q];
-    }
+    owned_faces_flux<Ph, DIR_R>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_Z>(S, A, P, c, dt);
 }
 
 // ------------------------------------------------------------------ update
+// Gather of the increments of every face of cell c in direction D: the
+// reference subtracts a face's increment from its L cell and adds it to its
+// R cell (riemann.c:314-317).
+template <int NQ, int D>
+DEV void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double* cons) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const int alo = Dr::lo(A), ahi = Dr::hi(A);
+    if (alo >= 0 && (A.flags & Dr::FLAG_LO)) {  // this cell is R: gains
+        const AnnDev& W = S.ann[alo];
+        const double* fo = Dr::f_lo(S);
+        const double* fw = Dr::f_hi(S);
+        const int* part = Dr::part_lo(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fo[q * N + c];
+        int start = part[cm];
+        int cnt = part[c] - start;
+        if (cnt < 0) cnt += W.np;
+        for (int t = 0; t < cnt; ++t) {
+            int a = start + t;
+            if (a >= W.np) a -= W.np;
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fw[q * N + W.off + a];
+        }
+    }
+    if (ahi >= 0 && (A.flags & Dr::FLAG_HI)) {  // this cell is L: loses
+        const AnnDev& B = S.ann[ahi];
+        const double* fo = Dr::f_hi(S);
+        const double* fb = Dr::f_lo(S);
+        const int* part = Dr::part_hi(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fo[q * N + c];
+        int start = part[cm];
+        int cnt = part[c] - start;
+        if (cnt < 0) cnt += B.np;
+        for (int t = 0; t < cnt; ++t) {
+            int b = start + t;
+            if (b >= B.np) b -= B.np;
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fb[q * N + B.off + b];
+        }
+    }
+}
+
+template <class Ph>
+DEV double update_cell(const DevState& S, int c, double dtfac, int first, int want_dt);
+
 // Conservative update from the gathered face increments (riemann.c:314-317),
 // add_source (misc.c:550-593), move_cells/clean_pi/calc_dp (misc.c:266-298,
 // 8-36; first substep only), calc_prim (misc.c:300-335) and the fixed
-// boundary (Boundary/fixed.c:11-60, boundary_functions.c:161-266).
-template <int NQ>
-DEV double update_cell(const DevState& S, int c, double dtfac, int first, int want_dt);
-
-// The second substep also produces the next step's CFL candidate from the
-// prims it has just written (misc.c:39-84 runs on exactly this state at the
-// start of the next step, with this step's mesh speeds).
-template <int NQ>
+// boundaries (Boundary/fixed.c:11-60, boundary_functions.c:161-266).  The
+// second substep also produces the next step's CFL candidate from the prims
+// it has just written (misc.c:39-84 runs on exactly this state at the start
+// of the next step, with this step's mesh speeds).
+template <class Ph>
 __global__ __launch_bounds__(256) void k_update(DevState S, double dtfac, int first, int want_dt) {
     int c = cell_index();
     int run = -1;
     double dtc = 1.0e300;
     if (c < S.ncell) {
         run = S.ann[S.cell_ann[c]].run;
-        dtc = update_cell<NQ>(S, c, dtfac, first, want_dt);
+        dtc = update_cell<Ph>(S, c, dtfac, first, want_dt);
     }
     if (want_dt) dt_reduce(S, run, dtc);
 }
 
-template <int NQ>
+template <class Ph>
 DEV double update_cell(const DevState& S, int c, double dtfac, int first, int want_dt) {
+    constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const RunDev& Rn = S.runs[A.run];
     const PhysParams& P = Rn.ph;
@@ -501,41 +649,17 @@ DEV double update_cell(const DevState& S, int c, double dtfac, int first, int wa
                 _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
             }
         }
-        // (inner | this) track: this cell is R, gains the flux
-        if (A.inner >= 0 && (A.flags & ANN_INFLUX)) {
-            const AnnDev& W = S.ann[A.inner];
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fin[q * N + c];
-            int start = S.ubi[cm];
-            int cnt = S.ubi[c] - start;
-            if (cnt < 0) cnt += W.np;
-            for (int t = 0; t < cnt; ++t) {
-                int a = start + t;
-                if (a >= W.np) a -= W.np;
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fout[q * N + W.off + a];
-            }
-        }
-        // (this | outer) track: this cell is L, loses the flux
-        if (A.outer >= 0 && (A.flags & ANN_OUTFLUX)) {
-            const AnnDev& B = S.ann[A.outer];
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fout[q * N + c];
-            int start = S.lbo[cm];
-            int cnt = S.lbo[c] - start;
-            if (cnt < 0) cnt += B.np;
-            for (int t = 0; t < cnt; ++t) {
-                int b = start + t;
-                if (b >= B.np) b -= B.np;
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fin[q * N + B.off + b];
-            }
-        }
+        gather_trans<NQ, DIR_R>(S, A, c, cm, cons);
+        gather_trans<NQ, DIR_Z>(S, A, c, cm, cons);
         // add_source with the pre-move geometry and the substep's prims
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
         double dV = get_dV(xp, xm, Pmax);
-        euler::source(P, prim, cons, xp, xm, dV * dt);
+        Ph::source(P, prim, cons, xp, xm, dV * dt);
         for (int p = 0; p < P.npl; ++p) {
             Planet pl = Rn.planets[p];
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
-            euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);
+            Ph::planet_src(P, pl, prim, cons, xp, xm, dV * dt);
         }
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.cons[q * N + c] = cons[q];
     }
@@ -552,7 +676,7 @@ DEV double update_cell(const DevState& S, int c, double dtfac, int first, int wa
         dph = get_dp(pn, pm, Pmax);
         S.piph_n[c] = pn;
         S.dphi_n[c] = dph;
-        if (pn < pm) S.rot[S.cell_ann[c]] = i;
+        if (pn < pm && !(A.flags & ANN_ZCOPY)) S.rot[S.cell_ann[c]] = i;
     }
 
     if (A.flags & ANN_REAL) {
@@ -561,9 +685,9 @@ DEV double update_cell(const DevState& S, int c, double dtfac, int first, int wa
         double xm[3] = {A.rm, phim, A.zm};
         double dV = get_dV(xp, xm, Pmax);
         double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
-        euler::cons2prim<NQ>(P, cons, prim, x, dV);
+        Ph::cons2prim(P, cons, prim, x, dV);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
-        if (want_dt) return cell_mindt<NQ>(S, A, P, c, prim);
+        if (want_dt) return cell_mindt<Ph>(S, A, P, c, prim);
     } else if (A.flags & ANN_GHOST_FILL) {
         double x[3];
         x[0] = A.rc_b;
@@ -577,6 +701,35 @@ DEV double update_cell(const DevState& S, int c, double dtfac, int first, int wa
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = p8[q];
     }
     return 1.0e300;
+}
+
+// Periodic z: what a one-rank MPI run's self-wraparound exchange does
+// (exchange.c:173-279 with mpisetup.c:17,38; copy_lite_to_cell
+// exchange.c:58-70).  Every ghost z-annulus becomes a copy of the real
+// annulus one period away -- primitives, mesh speeds and, after the mesh
+// moved, the mesh -- and takes over that annulus' rotation point.  Runs after
+// k_update, so r-ghost columns copy the freshly re-initialised source ghosts
+// exactly as boundary_trans(1) before exchangeData(1) does (onestep.c:166-173).
+template <class Ph>
+__global__ __launch_bounds__(256) void k_zcopy(DevState S, int first) {
+    constexpr int NQ = Ph::NQ;
+    int c = cell_index();
+    if (c >= S.ncell) return;
+    const int a = S.cell_ann[c];
+    const AnnDev& A = S.ann[a];
+    if (!(A.flags & ANN_ZCOPY)) return;
+    const AnnDev& Sr = S.ann[A.zsrc];
+    const size_t N = S.ncell;
+    int i = c - A.off;
+    int s = Sr.off + i;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = S.prim[q * N + s];
+    S.wiph[c] = S.wiph[s];
+    if (first) {
+        S.piph_n[c] = S.piph_n[s];
+        S.dphi_n[c] = S.dphi_n[s];
+        int sm = Sr.off + (i == 0 ? Sr.np - 1 : i - 1);
+        if (S.piph_n[s] < S.piph_n[sm]) S.rot[a] = i;
+    }
 }
 
 // movePlanets (Planets/sun.c: none; seeded binary: phi += omega dt) then the
@@ -604,34 +757,47 @@ __global__ void k_planets(DevState S) {
 // ------------------------------------------------------------------ launchers
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-#define DISCO_LAUNCH(hook, name, kern, grid, block, st, ...)                       \
-    do {                                                                           \
-        if (hook) hook->before(hook->ctx, name);                                   \
-        hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__);                 \
-        if (hook) hook->after(hook->ctx, name);                                    \
+#define DISCO_LAUNCH(hook, name, kern, grid, block, st, ...)       \
+    do {                                                           \
+        if (hook) hook->before(hook->ctx, name);                   \
+        hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__); \
+        if (hook) hook->after(hook->ctx, name);                    \
     } while (0)
 
-// Full CFL reduction over the current state (first step, getmindt calls).
-void launch_mindt(const DevState& S, hipStream_t st, const LaunchHook* hook) {
-    const int B = 256;
-    DISCO_LAUNCH(hook, "k_dt_init", k_dt_init, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
-    DISCO_LAUNCH(hook, "k_mindt", k_mindt<5>, dim3(nblk(S.ncell, B)), dim3(B), st, S);
-}
+template <class Ph>
+struct Launch {
+    // Full CFL reduction over the current state (first step, getmindt calls).
+    static void mindt(const DevState& S, hipStream_t st, const LaunchHook* hook) {
+        const int B = 256;
+        DISCO_LAUNCH(hook, "k_dt_init", k_dt_init, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+        DISCO_LAUNCH(hook, "k_mindt", k_mindt<Ph>, dim3(nblk(S.ncell, B)), dim3(B), st, S);
+    }
+    static void substep(const DevState& S, double RK, double dtfac, int first, int want_dt, const StepShape& sh,
+                        hipStream_t st, const LaunchHook* hook) {
+        const int B = 256;
+        dim3 g(nblk(S.ncell, B));
+        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, g, dim3(B), st, S, RK, first);
+        DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, g, dim3(B), st, S, sh.has_z);
+        DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, g, dim3(B), st, S, dtfac);
+        DISCO_LAUNCH(hook, "k_update", k_update<Ph>, g, dim3(B), st, S, dtfac, first, want_dt);
+        if (sh.zcopy) DISCO_LAUNCH(hook, "k_zcopy", k_zcopy<Ph>, g, dim3(B), st, S, first);
+        if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+    }
+};
 
 // dt = CFL x running minimum (misc.c:79)
 void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook) {
     DISCO_LAUNCH(hook, "k_dt_final", k_dt_final, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
 }
 
-void launch_substep(const DevState& S, double RK, double dtfac, int first, int want_dt, hipStream_t st,
-                    const LaunchHook* hook) {
-    const int B = 256;
-    dim3 g(nblk(S.ncell, B));
-    DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<5>, g, dim3(B), st, S, RK, first);
-    DISCO_LAUNCH(hook, "k_plm_r", k_plm_r<5>, g, dim3(B), st, S);
-    DISCO_LAUNCH(hook, "k_flux", k_flux<5>, g, dim3(B), st, S, dtfac);
-    DISCO_LAUNCH(hook, "k_update", k_update<5>, g, dim3(B), st, S, dtfac, first, want_dt);
-    if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook) {
+    (void)sh;
+    Launch<PhysEuler>::mindt(S, st, hook);
+}
+
+void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
+                    hipStream_t st, const LaunchHook* hook) {
+    Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
 }
 
 }  // namespace disco

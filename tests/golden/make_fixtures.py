@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py
+usage: python tests/golden/make_fixtures.py [--3d]   (--3d: only the 3D sets)
 """
 from __future__ import annotations
 
@@ -90,7 +90,18 @@ def grmhd_leaf_vectors(n: int = 512) -> None:
                         guess=guess, cons=out[:, :8], prim_out=out[:, 8:16], metric=out[:, 16:])
 
 
+def fixtures_3d() -> None:
+    # small 3D Keplerian disk, periodic in z (euler, HLLC, sun)
+    txt = par_with("c1_kepler2d.par", Num_R=12, Num_Z=6, Z_Periodic=1)
+    states, _ = run_ref("euler_sun", txt, 50, dumps=(1, 10))
+    save_states("c2e_small.npz", states)
+
+
 def main() -> None:
+    if "--3d" in sys.argv:
+        fixtures_3d()
+        print("fixtures written")
+        return
     # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
     txt = par_with("c1_kepler2d.par", Num_R=16)
     states, _ = run_ref("euler_sun", txt, 100, dumps=(1, 10))
@@ -113,6 +124,7 @@ def main() -> None:
              ncell_real=np.array(int((states[0]["real"] == 1).sum())),
              ref_seconds_20_steps=np.array(secs))
     grmhd_leaf_vectors()
+    fixtures_3d()
     print("fixtures written")
 
 

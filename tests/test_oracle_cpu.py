@@ -112,7 +112,10 @@ def test_track_faces_equal_brute_force(seed):
     assert abs(width.sum() - P) < 1e-9
 
 
-@pytest.mark.parametrize("fixture,base,over", CASES, ids=[c[0] for c in CASES])
+SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1))]
+
+
+@pytest.mark.parametrize("fixture,base,over", SETUP_CASES, ids=[c[0] for c in SETUP_CASES])
 def test_host_setup_is_bit_exact(fixture, base, over):
     """setupGrid/setupDomain/setupCells + first ghost fill on the host give the
     reference's step-0 state bit for bit (same seeded phases, same IC)."""
@@ -121,6 +124,7 @@ def test_host_setup_is_bit_exact(fixture, base, over):
     real = r0["real"] == 1
     assert np.array_equal(s["Np"], r0["Np"])
     assert np.array_equal(s["r_jph"], r0["r_jph"])
+    assert np.array_equal(s["z_kph"], r0["z_kph"])
     assert np.array_equal(s["piph"], r0["piph"])
     assert np.array_equal(s["dphi"], r0["dphi"])
     assert np.array_equal(s["prim"][:, real], r0["prim"][:, real])
