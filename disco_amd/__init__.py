@@ -105,6 +105,8 @@ def lib() -> C.CDLL:
     L.disco_gpu_sync.argtypes = [vp]
     L.disco_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
     L.disco_gpu_upload.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
+    L.disco_gpu_download_faces.argtypes = [vp, C.c_int, dp]
+    L.disco_gpu_upload_faces.argtypes = [vp, C.c_int, dp]
     L.disco_gpu_time.argtypes = [vp, dp, dp, dp]
     L.disco_gpu_kernel_times.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, dp, ip]
     L.disco_gpu_time_steps.argtypes = [vp, C.c_int, dp]
@@ -255,8 +257,22 @@ class Domain:
         wiph = np.zeros(I.ncell)
         _check(lib().disco_gpu_download(self._h, run, _dptr(prim), _dptr(cons), _dptr(piph), _dptr(dphi),
                                         _dptr(wiph)), "disco_gpu_download")
-        return dict(prim=prim.reshape(I.nq, I.ncell), cons=cons.reshape(I.nq, I.ncell), piph=piph, dphi=dphi,
-                    wiph=wiph)
+        out = dict(prim=prim.reshape(I.nq, I.ncell), cons=cons.reshape(I.nq, I.ncell), piph=piph, dphi=dphi,
+                   wiph=wiph)
+        if self.params[run].hydro == HYDRO["mhd"]:
+            out["Phi"] = self.faces(run)
+        return out
+
+    def faces(self, run: int = 0) -> np.ndarray:
+        """Magnetic face fluxes Phi[5, ncell] (MHD with constrained transport)."""
+        I = self.info[run]
+        Phi = np.zeros(5 * I.ncell)
+        _check(lib().disco_gpu_download_faces(self._h, run, _dptr(Phi)), "disco_gpu_download_faces")
+        return Phi.reshape(5, I.ncell)
+
+    def upload_faces(self, run: int, Phi) -> None:
+        Phi = np.ascontiguousarray(Phi, dtype=np.float64).ravel()
+        _check(lib().disco_gpu_upload_faces(self._h, run, _dptr(Phi)), "disco_gpu_upload_faces")
 
     def upload(self, run: int, prim=None, cons=None, piph=None, dphi=None, wiph=None) -> None:
         def c(a):

@@ -49,8 +49,14 @@ struct disco_gpu {
     double *prim = nullptr, *cons = nullptr, *rkcons = nullptr, *gradp = nullptr, *gradr = nullptr,
            *gradz = nullptr;
     double *piph[2] = {nullptr, nullptr}, *dphi[2] = {nullptr, nullptr}, *wiph = nullptr;
-    int *lbo = nullptr, *ubi = nullptr, *lbu = nullptr, *ubd = nullptr, *rot = nullptr;
+    int *lbo = nullptr, *ubi = nullptr, *lbu = nullptr, *ubd = nullptr;
+    int* rot[2] = {nullptr, nullptr};
     double *fphi = nullptr, *fout = nullptr, *fin = nullptr, *fup = nullptr, *fdn = nullptr;
+    // constrained transport (MHD)
+    double *Phi = nullptr, *rkPhi = nullptr, *emf_phi = nullptr, *emf_t = nullptr, *esum = nullptr,
+           *es1 = nullptr, *efin = nullptr;
+    int* fflags = nullptr;
+    int* err = nullptr;
     unsigned long long* dtbits = nullptr;
     double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
     int parity = 0;
@@ -77,18 +83,21 @@ static DevState make_state(disco_gpu* h, int substep) {
         S.dphi = h->dphi[p];
         S.piph_n = h->piph[1 - p];
         S.dphi_n = h->dphi[1 - p];
+        S.rot = h->rot[p];
+        S.rot_n = h->rot[1 - p];
     } else {
         S.piph = h->piph[1 - p];
         S.dphi = h->dphi[1 - p];
         S.piph_n = h->piph[1 - p];
         S.dphi_n = h->dphi[1 - p];
+        S.rot = h->rot[1 - p];
+        S.rot_n = h->rot[1 - p];
     }
     S.wiph = h->wiph;
     S.lbo = h->lbo;
     S.ubi = h->ubi;
     S.lbu = h->lbu;
     S.ubd = h->ubd;
-    S.rot = h->rot;
     S.fphi = h->fphi;
     S.fout = h->fout;
     S.fin = h->fin;
@@ -98,6 +107,15 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.dt = h->dt;
     S.planet_phi = h->planet_phi;
     S.t = h->t;
+    S.Phi = h->Phi;
+    S.rkPhi = h->rkPhi;
+    S.emf_phi = h->emf_phi;
+    S.emf_t = h->emf_t;
+    S.esum = h->esum;
+    S.es1 = h->es1;
+    S.efin = h->efin;
+    S.fflags = h->fflags;
+    S.err = h->err;
     return S;
 }
 
@@ -131,112 +149,10 @@ static int dalloc(T** p, size_t n) {
 }
 
 static void build_tables(disco_gpu* h) {
-    h->hann.clear();
-    h->hruns.clear();
-    h->run_cell0.clear();
-    h->run_ann0.clear();
-    int cell0 = 0, ann0 = 0;
-    for (int r = 0; r < h->nruns; ++r) {
-        const HostRun& R = h->runs[r];
-        RunDev D{};
-        D.ph = R.ph;
-        D.ic = R.ic;
-        D.bc = R.par.boundary;
-        D.planets_mod = R.par.planets;
-        D.cell0 = cell0;
-        D.ncell = R.ncell;
-        D.ann0 = ann0;
-        D.nann = R.Nr * R.Nz;
-        D.Nr = R.Nr;
-        D.Nz = R.Nz;
-        for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) D.planets[p] = R.planets[p];
-        h->hruns.push_back(D);
-        h->run_cell0.push_back(cell0);
-        h->run_ann0.push_back(ann0);
-        // trans_flux ranges (misc.c:459-497, no CT): r-tracks of real
-        // z-layers, z-tracks of real r-columns
-        int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
-        int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
-        int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
-        int zkmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
-        int zkmax = R.NgZb == 0 ? R.Nz - 1 : R.Nz - R.NgZb;
-        int zjmin = R.NgRa, zjmax = R.Nr - R.NgRb;
-        bool zper = R.Nz > 1 && R.par.Z_Periodic;
-        int Ng = R.NgZa;
-        for (int k = 0; k < R.Nz; ++k)
-            for (int j = 0; j < R.Nr; ++j) {
-                int jk = j + R.Nr * k;
-                AnnDev A{};
-                A.run = r;
-                A.j = j;
-                A.k = k;
-                A.np = R.Np[jk];
-                A.off = cell0 + R.off[jk];
-                A.outer = (j + 1 < R.Nr) ? ann0 + jk + 1 : -1;
-                A.inner = (j > 0) ? ann0 + jk - 1 : -1;
-                A.up = (k + 1 < R.Nz) ? ann0 + jk + R.Nr : -1;
-                A.down = (k > 0) ? ann0 + jk - R.Nr : -1;
-                A.zsrc = -1;
-                int f = 0;
-                bool kin = k >= kmin && k < kmax;
-                if (R.real_annulus(j, k)) f |= ANN_REAL | ANN_PHIFLUX;
-                if (kin && j >= jmin && j < jmax) f |= ANN_OUTFLUX;
-                if (kin && j - 1 >= jmin && j - 1 < jmax) f |= ANN_INFLUX;
-                bool jin = j >= zjmin && j < zjmax;
-                if (R.Nz > 1 && jin && k >= zkmin && k < zkmax) f |= ANN_UPFLUX;
-                if (R.Nz > 1 && jin && k - 1 >= zkmin && k - 1 < zkmax) f |= ANN_DNFLUX;
-                bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic;
-                bool zghost = (k < R.NgZa || k >= R.Nz - R.NgZb) && !R.par.Z_Periodic &&
-                              R.par.boundary == DISCO_BC_FIXED3D;
-                if (!(f & ANN_REAL) && (rghost || zghost)) f |= ANN_GHOST_FILL;
-                // periodic z (exchange.c:80-115): ghost k < Ng copies layer
-                // Nz-2Ng+k, ghost k >= Nz-Ng copies layer k-(Nz-2Ng)
-                if (zper && (k < Ng || k >= R.Nz - Ng)) {
-                    int ks = k < Ng ? k + R.Nz - 2 * Ng : k - (R.Nz - 2 * Ng);
-                    A.zsrc = ann0 + j + R.Nr * ks;
-                    f |= ANN_ZCOPY;
-                }
-                A.flags = f;
-                A.rm = R.rj(j - 1);
-                A.rp = R.rj(j);
-                A.zm = R.zk(k - 1);
-                A.zp = R.zk(k);
-                A.rc = get_centroid(A.rp, A.rm, 1);
-                A.zc = get_centroid(A.zp, A.zm, 2);
-                // faces_alt.c:111-122: centroid distances across the track
-                if (j + 1 < R.Nr) {
-                    double rmc = get_centroid(R.rj(j), R.rj(j - 1), 1);
-                    double rpc = get_centroid(R.rj(j + 1), R.rj(j), 1);
-                    A.dxL = R.rj(j) - rmc;
-                    A.dxR = rpc - R.rj(j);
-                }
-                if (k + 1 < R.Nz) {
-                    double zmc = get_centroid(R.zk(k), R.zk(k - 1), 2);
-                    double zpc = get_centroid(R.zk(k + 1), R.zk(k), 2);
-                    A.dzL = R.zk(k) - zmc;
-                    A.dzR = zpc - R.zk(k);
-                }
-                if (R.par.boundary == DISCO_BC_FIXED) {
-                    A.rc_b = .5 * (A.rp + A.rm);
-                    A.zc_b = .5 * (A.zp + A.zm);
-                } else {
-                    A.rc_b = A.rc;
-                    A.zc_b = A.zc;
-                }
-                h->hann.push_back(A);
-            }
-        cell0 += R.ncell;
-        ann0 += R.Nr * R.Nz;
-    }
-    h->ncell = cell0;
-    h->nann = ann0;
-    h->shape.hydro = h->runs.empty() ? 0 : h->runs[0].par.hydro;
-    h->shape.has_z = 0;
-    h->shape.zcopy = 0;
-    for (const HostRun& R : h->runs) {
-        if (R.Nz > 1) h->shape.has_z = 1;
-        if (R.Nz > 1 && R.par.Z_Periodic) h->shape.zcopy = 1;
-    }
+    build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape);
+    h->ncell = 0;
+    for (const HostRun& R : h->runs) h->ncell += R.ncell;
+    h->nann = (int)h->hann.size();
 }
 
 static int upload_run_state(disco_gpu* h, int r) {
@@ -252,7 +168,13 @@ static int upload_run_state(disco_gpu* h, int r) {
     HIPCHK(hipMemcpy(h->piph[h->parity] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->dphi[h->parity] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->wiph + c0, R.wiph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->rot + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->rot[h->parity] + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    if (h->Phi && !R.Phi.empty()) {
+        for (int f = 0; f < 5; ++f)
+            HIPCHK(hipMemcpy(h->Phi + f * N + c0, R.Phi.data() + (size_t)f * R.ncell, R.ncell * sizeof(double),
+                             hipMemcpyHostToDevice));
+    }
     double pp[DISCO_MAX_PLANETS] = {0.0, 0.0};
     for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) pp[p] = R.planets[p].phi;
     HIPCHK(hipMemcpy(h->planet_phi + r * DISCO_MAX_PLANETS, pp, sizeof(pp), hipMemcpyHostToDevice));
@@ -311,8 +233,10 @@ void disco_gpu_destroy(disco_gpu_t* h) {
         if (h->graph[p]) (void)hipGraphExecDestroy(h->graph[p]);
     void* ptrs[] = {h->ann,   h->cell_ann, h->druns, h->prim,   h->cons,    h->rkcons,     h->gradp,
                     h->gradr, h->gradz,    h->piph[0], h->piph[1], h->dphi[0], h->dphi[1], h->wiph,
-                    h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot,     h->fphi,       h->fout,
-                    h->fin,   h->fup,      h->fdn,   h->dtbits, h->dt,      h->planet_phi, h->t};
+                    h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot[0],  h->rot[1],     h->fphi,
+                    h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
+                    h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
+                    h->efin,  h->fflags,   h->err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -333,11 +257,13 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
             delete h;
             return fail("run " + std::to_string(r) + ": " + err);
         }
-        if (h->runs[r].nq != 5) {
+        if (h->runs[r].par.hydro != h->runs[0].par.hydro) {
             delete h;
-            return fail("only NUM_Q = 5 (euler) is built");
+            return fail("all runs of a batch must use the same Hydro module");
         }
     }
+    h->nq = h->runs[0].nq;
+    const bool mhd = h->runs[0].par.hydro == DISCO_HYDRO_MHD;
     build_tables(h);
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;
@@ -363,7 +289,19 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->ubi, N);
     rc |= dalloc(&h->lbu, N);
     rc |= dalloc(&h->ubd, N);
-    rc |= dalloc(&h->rot, h->nann);
+    rc |= dalloc(&h->rot[0], h->nann);
+    rc |= dalloc(&h->rot[1], h->nann);
+    rc |= dalloc(&h->err, 1);
+    if (mhd) {
+        rc |= dalloc(&h->Phi, 5 * N);
+        rc |= dalloc(&h->rkPhi, 5 * N);
+        rc |= dalloc(&h->emf_phi, 4 * N);
+        rc |= dalloc(&h->emf_t, 12 * N);
+        rc |= dalloc(&h->esum, 8 * N);
+        rc |= dalloc(&h->es1, 8 * N);
+        rc |= dalloc(&h->efin, 4 * N);
+        rc |= dalloc(&h->fflags, N);
+    }
     rc |= dalloc(&h->fphi, Q);
     rc |= dalloc(&h->fout, Q);
     rc |= dalloc(&h->fin, Q);
@@ -395,13 +333,26 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     for (int* p : zi)
         if (hipMemset(p, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
     if (hipMemset(h->dt, 0, nruns * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
+    if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) return cleanup("hipMemset");
+    if (mhd) {
+        struct {
+            double* p;
+            size_t n;
+        } zc[] = {{h->Phi, 5 * N},  {h->rkPhi, 5 * N}, {h->emf_phi, 4 * N}, {h->emf_t, 12 * N},
+                  {h->esum, 8 * N}, {h->es1, 8 * N},   {h->efin, 4 * N}};
+        for (auto& z : zc)
+            if (hipMemset(z.p, 0, z.n * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
+        if (hipMemset(h->fflags, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
+    }
     for (int r = 0; r < nruns; ++r) {
         if (upload_run_state(h, r)) return cleanup(g_err);
         // the other mesh buffer starts equal (substep 2 reads it)
         size_t c0 = h->run_cell0[r];
         const HostRun& R = h->runs[r];
         if (hipMemcpy(h->piph[1] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->dphi[1] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(h->dphi[1] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->rot[1] + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int), hipMemcpyHostToDevice) !=
+                hipSuccess)
             return cleanup("upload");
     }
     if (capture_graphs(h)) return cleanup(g_err);
@@ -483,10 +434,22 @@ int disco_gpu_timestep(disco_gpu_t* h, const double* dt) {
     return 0;
 }
 
+// Device error word: set by kernels that meet a mesh configuration they do
+// not replay (constrained transport: adjacent flux flips in one track).
+static int check_err(disco_gpu* h) {
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, h->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e & 1)
+        return fail("constrained transport: adjacent flux flips in one track (a front overtook two others "
+                    "in one step); this pattern is not replayed");
+    if (e) return fail("device error word " + std::to_string(e));
+    return 0;
+}
+
 int disco_gpu_sync(disco_gpu_t* h) {
     if (!h) return fail("disco_gpu_sync: null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
-    return 0;
+    return check_err(h);
 }
 
 int disco_gpu_download(disco_gpu_t* h, int run, double* prim, double* cons, double* piph, double* dphi,
@@ -502,6 +465,30 @@ int disco_gpu_download(disco_gpu_t* h, int run, double* prim, double* cons, doub
     if (piph) HIPCHK(hipMemcpy(piph, h->piph[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
     if (dphi) HIPCHK(hipMemcpy(dphi, h->dphi[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
     if (wiph) HIPCHK(hipMemcpy(wiph, h->wiph + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    return check_err(h);
+}
+
+int disco_gpu_download_faces(disco_gpu_t* h, int run, double* Phi) {
+    if (!h || run < 0 || run >= h->nruns || !Phi) return fail("disco_gpu_download_faces: bad arguments");
+    if (!h->Phi) return fail("disco_gpu_download_faces: the run has no face fluxes (not MHD)");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const HostRun& R = h->runs[run];
+    size_t c0 = h->run_cell0[run], N = h->ncell, n = R.ncell;
+    for (int f = 0; f < 5; ++f)
+        HIPCHK(hipMemcpy(Phi + f * n, h->Phi + f * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    return check_err(h);
+}
+
+int disco_gpu_upload_faces(disco_gpu_t* h, int run, const double* Phi) {
+    if (!h || run < 0 || run >= h->nruns || !Phi) return fail("disco_gpu_upload_faces: bad arguments");
+    if (!h->Phi) return fail("disco_gpu_upload_faces: the run has no face fluxes (not MHD)");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HostRun& R = h->runs[run];
+    size_t n = R.ncell;
+    std::memcpy(R.Phi.data(), Phi, 5 * n * sizeof(double));
+    size_t c0 = h->run_cell0[run], N = h->ncell;
+    for (int f = 0; f < 5; ++f)
+        HIPCHK(hipMemcpy(h->Phi + f * N + c0, Phi + f * n, n * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 

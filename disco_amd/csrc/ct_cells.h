@@ -1,0 +1,704 @@
+// Constrained transport on the sheared annulus mesh (bfields.c), per cell.
+//
+// The reference keeps one magnetic flux per face (Phi[5] per cell: its phi
+// face and the r/z faces its front edge ends, paul.h:196-216) and updates it
+// by Stokes' theorem from edge EMFs.  It walks explicit face lists in merge
+// order; here every stage is a gather owned by one cell:
+//
+//   emf_sum_cell    riemann_trans's EMF accumulation (riemann.c:172-212):
+//                   sums of .5*E*frac over the faces of each track a cell touches
+//   emf_avg1_cell   avg_Efields, phi-neighbour averages (bfields.c:400-444)
+//   emf_face_cell   avg_Efields, per-face average across the sheared
+//                   neighbour (bfields.c:446-538) and
+//                   subtract_advective_B_fluxes (bfields.c:646-690)
+//   ct_update_cell  update_B_fluxes (bfields.c:313-382) and make_edge_adjust
+//                   (bfields.c:888-1127): each owned face sums the EMFs of its
+//                   edges; the azimuthal edges come from a 4-way merge of the
+//                   fronts of the four annuli around the edge, restricted to
+//                   the face's own span
+//   flip_flag_cell  check_flipped (bfields.c:693-736)
+//   flip_chain_cell flip_fluxes (bfields.c:738-814): order dependent, so each
+//                   chain of interacting flips is replayed by one thread in the
+//                   reference's loop order
+//   b_faces_to_cell B_faces_to_cells (bfields.c:88-309)
+//
+// Faces are named by (owner cell, side): side 0 = the owner is L of the track
+// (this | outer or up) and its front ends the face (LRtype 0); side 1 = the
+// owner is R of the track (inner or down | this) (LRtype 1).
+#pragma once
+
+namespace disco {
+
+struct FaceId {
+    int cell;
+    int side;  // 0: owner is L (hi track of the owner); 1: owner is R (lo track)
+};
+
+DHD bool same_face(FaceId a, FaceId b) { return a.cell == b.cell && a.side == b.side; }
+
+// Sets bits of the device error word (atomic on the device).
+DHD void err_or(int* p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(p, v);
+#else
+    *p |= v;
+#endif
+}
+
+DHD int cell_pred(const AnnDev& A, int c) {
+    int i = c - A.off;
+    return A.off + (i == 0 ? A.np - 1 : i - 1);
+}
+DHD int cell_succ(const AnnDev& A, int c) {
+    int i = c - A.off;
+    return A.off + (i + 1 == A.np ? 0 : i + 1);
+}
+
+// Neighbouring faces in the track's merge order (faces_alt.c:135-173, ties:
+// the lower annulus' front first), from the partner indices of stage A.
+template <int D>
+DHD FaceId face_prev(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    const AnnDev& A = S.ann[S.cell_ann[F.cell]];
+    int Fm = cell_pred(A, F.cell);
+    if (F.side == 0) {
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        const int* part = Dr::part_hi(S);
+        int cnt = wrap_idx(part[F.cell] - part[Fm], B.np);
+        if (cnt == 0) return FaceId{Fm, 0};
+        return FaceId{B.off + wrap_idx(part[F.cell] - 1, B.np), 1};
+    }
+    const AnnDev& W = S.ann[Dr::lo(A)];
+    const int* part = Dr::part_lo(S);
+    int cnt = wrap_idx(part[F.cell] - part[Fm], W.np);
+    if (cnt == 0) return FaceId{Fm, 1};
+    return FaceId{W.off + wrap_idx(part[F.cell] - 1, W.np), 0};
+}
+
+template <int D>
+DHD FaceId face_next(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    const AnnDev& A = S.ann[S.cell_ann[F.cell]];
+    int Fp = cell_succ(A, F.cell);
+    if (F.side == 0) {
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        const int* part = Dr::part_hi(S);
+        int cnt = wrap_idx(part[Fp] - part[F.cell], B.np);
+        if (cnt == 0) return FaceId{Fp, 0};
+        return FaceId{B.off + part[F.cell], 1};
+    }
+    const AnnDev& W = S.ann[Dr::lo(A)];
+    const int* part = Dr::part_lo(S);
+    int cnt = wrap_idx(part[Fp] - part[F.cell], W.np);
+    if (cnt == 0) return FaceId{Fp, 1};
+    return FaceId{W.off + part[F.cell], 0};
+}
+
+// The face's other cell, as the owner of its own face in the same track
+// (what get_phi_pointer returns after the LRtype toggle, bfields.c:738-758).
+template <int D>
+DHD FaceId face_other(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    const AnnDev& A = S.ann[S.cell_ann[F.cell]];
+    if (F.side == 0) {
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        return FaceId{B.off + Dr::part_hi(S)[F.cell], 1};
+    }
+    const AnnDev& W = S.ann[Dr::lo(A)];
+    return FaceId{W.off + Dr::part_lo(S)[F.cell], 0};
+}
+
+// First face of the track in the reference's face list: the merge starts at
+// the first cell in quadrant I of each annulus (faces_alt.c:57-79), which is
+// the annulus' smallest front (its rotation point).
+template <int D>
+DHD FaceId face_zero(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    const AnnDev& A = S.ann[S.cell_ann[F.cell]];
+    int la, ua;
+    if (F.side == 0) {
+        la = S.cell_ann[F.cell];
+        ua = Dr::hi(A);
+    } else {
+        la = Dr::lo(A);
+        ua = S.cell_ann[F.cell];
+    }
+    const AnnDev& L = S.ann[la];
+    const AnnDev& U = S.ann[ua];
+    int cl = L.off + S.rot[la];
+    int cu = U.off + S.rot[ua];
+    if (S.piph[cl] <= S.piph[cu]) return FaceId{cl, 0};
+    return FaceId{cu, 1};
+}
+
+template <int D>
+DHD size_t face_slot(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    return (size_t)(F.side == 0 ? Dr::PHI_HI : Dr::PHI_LO) * S.ncell + F.cell;
+}
+
+template <int D>
+DHD bool face_flag(const DevState& S, FaceId F) {
+    using Dr = Dir<D>;
+    return (S.fflags[F.cell] & (F.side == 0 ? Dr::FL_HI : Dr::FL_LO)) != 0;
+}
+
+// ------------------------------------------------------------------ EMF sums
+// riemann_trans (riemann.c:172-212): every face of a flux-carrying track adds
+// .5*E*frac (frac = face dphi / cell dphi) to the edge slots of its two cells.
+template <int D>
+DHD void emf_sum_dir(const DevState& S, const AnnDev& A, const PhysParams& P, int c, int cm) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double Pmax = P.phi_max;
+    const double phip = S.piph[c], dph = S.dphi[c];
+    double sE = 0.0, sB = 0.0;
+    if (Dr::lo(A) >= 0 && (A.flags & Dr::FLAG_LO)) {  // this cell is R
+        const AnnDev& W = S.ann[Dr::lo(A)];
+        const int* part = Dr::part_lo(S);
+        int ca = W.off + part[c];
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        double fr = F.dp / dph;
+        sE += .5 * S.emf_t[(Dr::EMF_LO + 0) * N + c] * fr;
+        sB += .5 * S.emf_t[(Dr::EMF_LO + 1) * N + c] * fr;
+        int start = part[cm];
+        int cnt = wrap_idx(part[c] - start, W.np);
+        for (int t = 0; t < cnt; ++t) {
+            ca = W.off + wrap_idx(start + t, W.np);
+            F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 0, W, Pmax);
+            fr = F.dp / dph;
+            sE += .5 * S.emf_t[(Dr::EMF_HI + 0) * N + ca] * fr;
+            sB += .5 * S.emf_t[(Dr::EMF_HI + 1) * N + ca] * fr;
+        }
+    }
+    S.esum[Dr::SUM_LO * N + c] = sE;
+    S.esum[(Dr::SUM_LO + 4) * N + c] = sB;
+    sE = 0.0;
+    sB = 0.0;
+    if (Dr::hi(A) >= 0 && (A.flags & Dr::FLAG_HI)) {  // this cell is L
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        const int* part = Dr::part_hi(S);
+        int cb = B.off + part[c];
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        double fr = F.dp / dph;
+        sE += .5 * S.emf_t[(Dr::EMF_HI + 0) * N + c] * fr;
+        sB += .5 * S.emf_t[(Dr::EMF_HI + 1) * N + c] * fr;
+        int start = part[cm];
+        int cnt = wrap_idx(part[c] - start, B.np);
+        for (int t = 0; t < cnt; ++t) {
+            cb = B.off + wrap_idx(start + t, B.np);
+            F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 1, A, Pmax);
+            fr = F.dp / dph;
+            sE += .5 * S.emf_t[(Dr::EMF_LO + 0) * N + cb] * fr;
+            sB += .5 * S.emf_t[(Dr::EMF_LO + 1) * N + cb] * fr;
+        }
+    }
+    S.esum[Dr::SUM_HI * N + c] = sE;
+    S.esum[(Dr::SUM_HI + 4) * N + c] = sB;
+}
+
+DHD void emf_sum_cell(const DevState& S, int c, int has_z) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const PhysParams& P = S.runs[A.run].ph;
+    int cm = cell_pred(A, c);
+    emf_sum_dir<DIR_R>(S, A, P, c, cm);
+    if (has_z) emf_sum_dir<DIR_Z>(S, A, P, c, cm);
+}
+
+// avg_Efields, first loop (bfields.c:400-444): the edge values at the cell's
+// front are the mean of the cell's and its phi-neighbour's edge slots; each
+// slot holds .5 x the phi face's EMF plus the track sums.
+DHD void emf_avg1_cell(const DevState& S, int c) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const size_t N = S.ncell;
+    int cp = cell_succ(A, c);
+    double Ez = S.emf_phi[0 * N + c], Br = S.emf_phi[1 * N + c];
+    double Er = S.emf_phi[2 * N + c], Bz = S.emf_phi[3 * N + c];
+    const double* s = S.esum;
+    S.es1[0 * N + c] = .5 * ((.5 * Ez + s[0 * N + c]) + (.5 * Ez + s[0 * N + cp]));
+    S.es1[1 * N + c] = .5 * ((.5 * Ez + s[1 * N + c]) + (.5 * Ez + s[1 * N + cp]));
+    S.es1[2 * N + c] = .5 * ((.5 * Er + s[2 * N + c]) + (.5 * Er + s[2 * N + cp]));
+    S.es1[3 * N + c] = .5 * ((.5 * Er + s[3 * N + c]) + (.5 * Er + s[3 * N + cp]));
+    S.es1[4 * N + c] = .5 * ((.5 * Br + s[4 * N + c]) + (.5 * Br + s[4 * N + cp]));
+    S.es1[5 * N + c] = .5 * ((.5 * Br + s[5 * N + c]) + (.5 * Br + s[5 * N + cp]));
+    S.es1[6 * N + c] = .5 * ((.5 * Bz + s[6 * N + c]) + (.5 * Bz + s[6 * N + cp]));
+    S.es1[7 * N + c] = .5 * ((.5 * Bz + s[7 * N + c]) + (.5 * Bz + s[7 * N + cp]));
+}
+
+// avg_Efields, face loops (bfields.c:446-538): the edge at the owner's front
+// averages the owner's slot with the other cell's two edge values
+// interpolated to that front; then the mesh-advection part of the EMF is
+// removed (bfields.c:646-690).
+template <int D>
+DHD void emf_face_dir(const DevState& S, const AnnDev& A, int c, double* Ehi, double* Bhi, double* Elo,
+                      double* Blo) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double Pmax = S.runs[A.run].ph.phi_max;
+    const double* e = S.es1;
+    *Ehi = e[Dr::E_HI * N + c];
+    *Bhi = e[(Dr::E_HI + 4) * N + c];
+    *Elo = e[Dr::E_LO * N + c];
+    *Blo = e[(Dr::E_LO + 4) * N + c];
+    if (Dr::hi(A) >= 0) {  // owned face, LRtype 0: other cell c2 = R
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        int c2 = B.off + Dr::part_hi(S)[c];
+        int c2m = cell_pred(B, c2);
+        double dp1 = get_dp(S.piph[c2], S.piph[c], Pmax);
+        double dp2 = S.dphi[c2] - dp1;
+        double Eavg = (dp2 * e[Dr::E_LO * N + c2] + dp1 * e[Dr::E_LO * N + c2m]) / (dp1 + dp2);
+        double Bavg = (dp2 * e[(Dr::E_LO + 4) * N + c2] + dp1 * e[(Dr::E_LO + 4) * N + c2m]) / (dp1 + dp2);
+        *Ehi = .5 * (e[Dr::E_HI * N + c] + Eavg);
+        *Bhi = .5 * (e[(Dr::E_HI + 4) * N + c] + Bavg);
+    }
+    if (Dr::lo(A) >= 0) {  // owned face, LRtype 1: other cell c2 = L
+        const AnnDev& W = S.ann[Dr::lo(A)];
+        int c2 = W.off + Dr::part_lo(S)[c];
+        int c2m = cell_pred(W, c2);
+        double dp1 = get_dp(S.piph[c2], S.piph[c], Pmax);
+        double dp2 = S.dphi[c2] - dp1;
+        double Eavg = (dp2 * e[Dr::E_HI * N + c2] + dp1 * e[Dr::E_HI * N + c2m]) / (dp1 + dp2);
+        double Bavg = (dp2 * e[(Dr::E_HI + 4) * N + c2] + dp1 * e[(Dr::E_HI + 4) * N + c2m]) / (dp1 + dp2);
+        *Elo = .5 * (e[Dr::E_LO * N + c] + Eavg);
+        *Blo = .5 * (e[(Dr::E_LO + 4) * N + c] + Bavg);
+    }
+}
+
+DHD void emf_face_cell(const DevState& S, int c, int has_z) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const size_t N = S.ncell;
+    const double w = S.wiph[c];
+    double E1, B1, E0, B0;
+    emf_face_dir<DIR_R>(S, A, c, &E1, &B1, &E0, &B0);
+    E0 -= A.rm * w * B0;  // hL = r_jph[j-1]
+    E1 -= A.rp * w * B1;  // hR = r_jph[j]
+    S.efin[0 * N + c] = E0;
+    S.efin[1 * N + c] = E1;
+    if (has_z) {
+        double E5, B5, E4, B4;
+        emf_face_dir<DIR_Z>(S, A, c, &E5, &B5, &E4, &B4);
+        E4 += A.rc * w * B4;  // hD = hU = r centroid
+        E5 += A.rc * w * B5;
+        S.efin[2 * N + c] = E4;
+        S.efin[3 * N + c] = E5;
+    }
+}
+
+// ------------------------------------------------------------------ edges
+// One azimuthal edge (r_jph[j], z_kph[k]) and its four annuli by rank:
+// 0 = (j,k), 1 = (j+1,k), 2 = (j,k+1), 3 = (j+1,k+1) (bfields.c:922-935).
+struct Corner {
+    int ann[4];
+    double r;  // r_jph[j]
+};
+
+// EMF along the part of the edge that ends at the front of the cell of rank
+// `w4` (make_edge_adjust's segment, bfields.c:946-1108), given the four
+// current cells; returns E * dl * dt.
+DHD double edge_segment(const DevState& S, const int* cur, int w4, double r, double dt, double Pmax) {
+    const size_t N = S.ncell;
+    double ph[4], bk[4];
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {
+        ph[q] = S.piph[cur[q]];
+        bk[q] = ph[q] - S.dphi[cur[q]];
+    }
+    // get_which4 mode 0 toggles (bfields.c:834-885)
+    int LR_D = get_signed_dp(ph[0], ph[1], Pmax) > 0.0;
+    int LR_U = get_signed_dp(ph[2], ph[3], Pmax) > 0.0;
+    int LR_alt = (w4 == 0 || w4 == 1) ? LR_U : LR_D;
+    int UD_alt = (w4 == 0 || w4 == 2) ? (get_signed_dp(ph[1], ph[3], Pmax) > 0.0)
+                                      : (get_signed_dp(ph[0], ph[2], Pmax) > 0.0);
+    // owned-face E_phi: r-out [1], r-in [0], z-up [3], z-down [2]
+    const double* Ep = S.emf_t;
+#define EO(k) Ep[2 * N + cur[k]]
+#define EI(k) Ep[5 * N + cur[k]]
+#define EU(k) Ep[8 * N + cur[k]]
+#define ED(k) Ep[11 * N + cur[k]]
+    double E;
+    if (w4 == 0) {
+        E = .25 * (EU(0) + EO(0));
+        E += .25 * (LR_alt == 0 ? EO(2) : EI(3));
+        E += .25 * (UD_alt == 0 ? EU(1) : ED(3));
+    } else if (w4 == 1) {
+        E = .25 * (EU(1) + EI(1));
+        E += .25 * (LR_alt == 0 ? EO(2) : EI(3));
+        E += .25 * (UD_alt == 0 ? EU(0) : ED(2));
+    } else if (w4 == 2) {
+        E = .25 * (ED(2) + EO(2));
+        E += .25 * (LR_alt == 0 ? EO(0) : EI(1));
+        E += .25 * (UD_alt == 0 ? EU(1) : ED(3));
+    } else {
+        E = .25 * (ED(3) + EI(3));
+        E += .25 * (LR_alt == 0 ? EO(0) : EI(1));
+        E += .25 * (UD_alt == 0 ? EU(0) : ED(2));
+    }
+#undef EO
+#undef EI
+#undef EU
+#undef ED
+    // get_which4 mode 1 on the back edges: the latest back is the segment start
+    double b1 = get_signed_dp(bk[0], bk[1], Pmax) < 0.0 ? bk[1] : bk[0];
+    double b2 = get_signed_dp(bk[2], bk[3], Pmax) < 0.0 ? bk[3] : bk[2];
+    double xm1 = get_signed_dp(b1, b2, Pmax) < 0.0 ? b2 : b1;
+    double dl = r * get_dp(ph[w4], xm1, Pmax);
+    return E * dl * dt;
+}
+
+// Number of fronts of annulus `a` (rank qr in the corner) that precede the
+// front value v of rank vr in the 4-way order (ties: lower rank first).
+DHD int corner_count(const DevState& S, int a, int qr, double v, int vr) {
+    const AnnDev& Q = S.ann[a];
+    return front_count(S.piph, Q.off, Q.np, S.rot[a], v, qr > vr);
+}
+
+DHD int corner_cell(const DevState& S, int a, int pos) {
+    const AnnDev& Q = S.ann[a];
+    return Q.off + (S.rot[a] + pos) % Q.np;
+}
+
+// Sum of E dl dt over the segments of corner K inside face F, whose two
+// cells are pc0 (rank p0) and pc1 (rank p1), which ends at the front of its
+// owner (rank orank) and starts at the front fprev of rank prank.  The other
+// two annuli (ranks q0 < q1) contribute the fronts that split the face.
+DHD double edge_face_sum(const DevState& S, const Corner& K, int p0, int p1, int pc0, int pc1, int orank,
+                         double fown, int prank, double fprev, double dt, double Pmax) {
+    int q0, q1;
+    if (p0 == 0 && p1 == 1) { q0 = 2; q1 = 3; }
+    else if (p0 == 2 && p1 == 3) { q0 = 0; q1 = 1; }
+    else if (p0 == 0 && p1 == 2) { q0 = 1; q1 = 3; }
+    else { q0 = 0; q1 = 2; }
+    const int aq0 = K.ann[q0], aq1 = K.ann[q1];
+    const int n0 = S.ann[aq0].np, n1 = S.ann[aq1].np;
+    int a0 = corner_count(S, aq0, q0, fprev, prank);
+    int b0 = corner_count(S, aq0, q0, fown, orank);
+    int a1 = corner_count(S, aq1, q1, fprev, prank);
+    int b1 = corner_count(S, aq1, q1, fown, orank);
+    int m0 = wrap_idx(b0 - a0, n0);
+    int m1 = wrap_idx(b1 - a1, n1);
+    int cur[4];
+    cur[p0] = pc0;
+    cur[p1] = pc1;
+    double sum = 0.0;
+    int i0 = 0, i1 = 0;
+    while (i0 < m0 || i1 < m1) {
+        int c0 = corner_cell(S, aq0, a0 + i0);
+        int c1 = corner_cell(S, aq1, a1 + i1);
+        bool take0;
+        if (i0 >= m0)
+            take0 = false;
+        else if (i1 >= m1)
+            take0 = true;
+        else
+            take0 = get_signed_dp(S.piph[c0], S.piph[c1], Pmax) <= 0.0;  // tie: lower rank q0 first
+        cur[q0] = c0;
+        cur[q1] = c1;
+        sum += edge_segment(S, cur, take0 ? q0 : q1, K.r, dt, Pmax);
+        if (take0)
+            ++i0;
+        else
+            ++i1;
+    }
+    cur[q0] = corner_cell(S, aq0, a0 + m0);
+    cur[q1] = corner_cell(S, aq1, a1 + m1);
+    sum += edge_segment(S, cur, orank, K.r, dt, Pmax);
+    return sum;
+}
+
+// ------------------------------------------------------------------ Stokes
+// update_B_fluxes (bfields.c:313-382): the edge at each face's end changes
+// the owned face, the owner's phi face and the next face of the track; as a
+// gather, an owned face gains its predecessor's end EMF and loses its own.
+// make_edge_adjust (3D) adds the azimuthal edges on the face's two other
+// sides: D +, U -, L -, R + (add_E_phi, bfields.c:384-389).
+template <int D>
+DHD double ct_face_update(const DevState& S, const AnnDev& A, int c, int side, double dt, int has_z,
+                          double* dPhi0) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double Pmax = S.runs[A.run].ph.phi_max;
+    const double dl = Dr::dl(A);
+    FaceId F{c, side};
+    FaceId Pv = face_prev<D>(S, F);
+    double Eown = S.efin[(side == 0 ? Dr::E_HI : Dr::E_LO) * N + c];
+    double Eprev = S.efin[(Pv.side == 0 ? Dr::E_HI : Dr::E_LO) * N + Pv.cell];
+    double dOwn = Eown * dl * dt;
+    double dPrev = Eprev * dl * dt;
+    // r tracks: own -= E, next += E, phi face (LR0 +, LR1 -); z tracks opposite
+    double delta = Dr::SGN * (dPrev - dOwn);
+    *dPhi0 += (side == 0 ? Dr::SGN : -Dr::SGN) * dOwn;
+    if (!has_z) return delta;
+
+    // azimuthal edges of this face (make_edge_adjust)
+    int other = side == 0 ? S.ann[Dr::hi(A)].off + Dr::part_hi(S)[c] : S.ann[Dr::lo(A)].off + Dr::part_lo(S)[c];
+    int la = side == 0 ? S.cell_ann[c] : Dr::lo(A);    // lower annulus of the track
+    int ua = side == 0 ? Dr::hi(A) : S.cell_ann[c];    // upper annulus
+    int lcell = side == 0 ? c : other;
+    int ucell = side == 0 ? other : c;
+    const AnnDev& L = S.ann[la];
+    const AnnDev& U = S.ann[ua];
+    double fown = S.piph[c];
+    double fprev = S.piph[Pv.cell];
+    // ranks of the owner / predecessor within the track pair (0 = lower annulus)
+    int o_up = side;                     // owner in the upper annulus of the track?
+    int p_up = Pv.side;
+    double sum = 0.0;
+    if (D == DIR_R) {
+        // r face between L=(j,k) and U=(j+1,k): top edge corner (j,k) [D],
+        // bottom edge corner (j,k-1) [U]
+        if (L.up >= 0 && U.up >= 0) {
+            Corner K{{la, ua, L.up, U.up}, L.rp};
+            sum += edge_face_sum(S, K, 0, 1, lcell, ucell, o_up, fown, p_up, fprev, dt, Pmax);
+        }
+        if (L.down >= 0 && U.down >= 0) {
+            Corner K{{L.down, U.down, la, ua}, L.rp};
+            sum -= edge_face_sum(S, K, 2, 3, lcell, ucell, 2 + o_up, fown, 2 + p_up, fprev, dt, Pmax);
+        }
+    } else {
+        // z face between L=(j,k) and U=(j,k+1): edge at r_jph[j] is corner
+        // (j,k) [L], edge at r_jph[j-1] is corner (j-1,k) [R]
+        if (L.outer >= 0 && U.outer >= 0) {
+            Corner K{{la, L.outer, ua, U.outer}, L.rp};
+            sum -= edge_face_sum(S, K, 0, 2, lcell, ucell, 2 * o_up, fown, 2 * p_up, fprev, dt, Pmax);
+        }
+        if (L.inner >= 0 && U.inner >= 0) {
+            Corner K{{L.inner, la, U.inner, ua}, L.rm};
+            sum += edge_face_sum(S, K, 1, 3, lcell, ucell, 1 + 2 * o_up, fown, 1 + 2 * p_up, fprev, dt, Pmax);
+        }
+    }
+    return delta + sum;
+}
+
+DHD void ct_update_cell(const DevState& S, int c, double dtfac, int has_z) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const size_t N = S.ncell;
+    const double dt = S.dt[A.run] * dtfac;
+    double dPhi0 = 0.0;
+    if (A.outer >= 0) S.Phi[2 * N + c] += ct_face_update<DIR_R>(S, A, c, 0, dt, has_z, &dPhi0);
+    if (A.inner >= 0) S.Phi[1 * N + c] += ct_face_update<DIR_R>(S, A, c, 1, dt, has_z, &dPhi0);
+    if (has_z) {
+        if (A.up >= 0) S.Phi[4 * N + c] += ct_face_update<DIR_Z>(S, A, c, 0, dt, has_z, &dPhi0);
+        if (A.down >= 0) S.Phi[3 * N + c] += ct_face_update<DIR_Z>(S, A, c, 1, dt, has_z, &dPhi0);
+    }
+    S.Phi[0 * N + c] += dPhi0;
+}
+
+// ------------------------------------------------------------------ flips
+// check_flipped (bfields.c:693-736): after the mesh moved, a face whose
+// owner's front now lies behind the previous face's owner's front is flagged.
+// Reads the partners of the mesh before the move and both meshes.
+template <int D>
+DHD int flip_flag_dir(const DevState& S, const AnnDev& A, int c, const PhysParams& P) {
+    using Dr = Dir<D>;
+    int f = 0;
+    for (int side = 0; side < 2; ++side) {
+        int nb = side == 0 ? Dr::hi(A) : Dr::lo(A);
+        if (nb < 0) continue;
+        FaceId Pv = face_prev<D>(S, FaceId{c, side});
+        double dp = get_signed_dp(S.piph_n[c], S.piph_n[Pv.cell], P.phi_max);
+        if (dp < 0.) f |= side == 0 ? Dr::FL_HI : Dr::FL_LO;
+    }
+    return f;
+}
+
+DHD void flip_flag_cell(const DevState& S, int c, int has_z) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const PhysParams& P = S.runs[A.run].ph;
+    int f = flip_flag_dir<DIR_R>(S, A, c, P);
+    if (has_z) f |= flip_flag_dir<DIR_Z>(S, A, c, P);
+    S.fflags[c] = f;
+}
+
+// One flip at face F (bfields.c:781-808), with the next face's slot taken
+// through its toggled pointer when a later face of the chain was already
+// flipped (the chain crossed the start of the track's face list).
+template <int D>
+DHD void flip_one(const DevState& S, FaceId F, bool next_toggled) {
+    FaceId Fm = face_prev<D>(S, F);
+    FaceId Fp = face_next<D>(S, F);
+    if (next_toggled) Fp = face_other<D>(S, Fp);
+    size_t sP = face_slot<D>(S, F), sPm = face_slot<D>(S, Fm), sPp = face_slot<D>(S, Fp);
+    double* Ph = S.Phi;
+    double* Rk = S.rkPhi;
+    double Phi = Ph[sP];
+    double RK_Phi = Rk[sP];
+    Ph[sP] = Ph[sPm] + Phi;
+    Rk[sP] = Rk[sPm] + RK_Phi;
+    Ph[sPm] = -Phi;
+    Rk[sPm] = -RK_Phi;
+    Ph[sPp] = Ph[sPp] + Phi;
+    Rk[sPp] = Rk[sPp] + RK_Phi;
+}
+
+// flip_fluxes (bfields.c:760-814).  A flip reads and writes the fluxes of the
+// faces before and after it, so flips two faces apart interact and must run
+// in the reference's loop order (face list from its first face); flips at
+// least three faces apart are independent.  The thread of the first flip of
+// each chain replays the chain; a chain that runs across the start of the
+// face list is replayed by the thread of its first face after the start, its
+// part before the end of the list last.  Adjacent flipped faces cannot occur
+// when every annulus rotates rigidly and no front overtakes two others in a
+// step; they are reported in S.err instead of being replayed.
+template <int D>
+DHD void flip_chain_face(const DevState& S, FaceId F) {
+    const int MAXCHAIN = 64;
+    FaceId z0 = face_zero<D>(S, F);
+    FaceId n1 = face_next<D>(S, F);
+    if (face_flag<D>(S, n1)) {
+        err_or(S.err, 1);
+        return;
+    }
+    FaceId p1 = face_prev<D>(S, F);
+    FaceId p2 = face_prev<D>(S, p1);
+    bool cross_in = same_face(F, z0) || same_face(p1, z0);
+    bool pred = face_flag<D>(S, p2);
+    if (pred && !cross_in) return;  // not the first flip of its chain
+    bool wrap_head = pred && cross_in;
+    // forward part of the chain
+    FaceId M = F;
+    int len = 1;
+    for (;;) {
+        FaceId m1 = face_next<D>(S, M);
+        FaceId m2 = face_next<D>(S, m1);
+        if (!face_flag<D>(S, m2)) break;
+        bool cross = same_face(m1, z0) || same_face(m2, z0);
+        if (cross) {
+            if (!wrap_head) return;  // continues into the list start: replayed by that chain's head
+            err_or(S.err, 1);      // a chain all around the track
+            return;
+        }
+        M = m2;
+        if (++len > MAXCHAIN) {
+            err_or(S.err, 1);
+            return;
+        }
+    }
+    // replay: forward part in order
+    M = F;
+    for (int k = 0; k < len; ++k) {
+        flip_one<D>(S, M, false);
+        M = face_next<D>(S, face_next<D>(S, M));
+    }
+    if (!wrap_head) return;
+    // the part before the end of the list: from its first flip up to p2
+    FaceId T = p2;
+    int tl = 1;
+    for (;;) {
+        FaceId q1 = face_prev<D>(S, T);
+        FaceId q2 = face_prev<D>(S, q1);
+        if (!face_flag<D>(S, q2) || same_face(T, z0) || same_face(q1, z0)) break;
+        T = q2;
+        if (++tl > MAXCHAIN) {
+            err_or(S.err, 1);
+            return;
+        }
+    }
+    for (int k = 0; k < tl; ++k) {
+        // the last one (p2) reads the face after it through the pointer that
+        // the head's flip toggled
+        flip_one<D>(S, T, k == tl - 1);
+        T = face_next<D>(S, face_next<D>(S, T));
+    }
+}
+
+template <int D>
+DHD void flip_chain_dir(const DevState& S, const AnnDev& A, int c) {
+    using Dr = Dir<D>;
+    int f = S.fflags[c];
+    if (Dr::hi(A) >= 0 && (f & Dr::FL_HI)) flip_chain_face<D>(S, FaceId{c, 0});
+    if (Dr::lo(A) >= 0 && (f & Dr::FL_LO)) flip_chain_face<D>(S, FaceId{c, 1});
+}
+
+DHD void flip_chain_cell(const DevState& S, int c, int has_z) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(S.fflags[c])) return;
+    flip_chain_dir<DIR_R>(S, A, c);
+    if (has_z) flip_chain_dir<DIR_Z>(S, A, c);
+}
+
+// ------------------------------------------------------------------ B_faces_to_cells
+// bfields.c:88-309 (type 1): cell field from the face fluxes, using the face
+// list of the mesh before the move (with the flips' LRtype toggles) and the
+// cell volume after it.
+template <int D>
+DHD bool face_toggled(const DevState& S, FaceId G, int first) {
+    if (!first) return false;
+    return face_flag<D>(S, G) != face_flag<D>(S, face_next<D>(S, G));
+}
+
+template <int D>
+DHD void b_sum_dir(const DevState& S, const AnnDev& A, int c, int cm, int first, double Pmax, double* sPhi,
+                   double* sdA) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double phip = S.piph[c], dph = S.dphi[c];
+    double P = 0.0, dA = 0.0;
+    if (Dr::lo(A) >= 0) {  // this cell is R of the (lo | this) track
+        const AnnDev& W = S.ann[Dr::lo(A)];
+        const int* part = Dr::part_lo(S);
+        int ca = W.off + part[c];
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        dA += F.dA;
+        FaceId G{c, 1};
+        P += face_toggled<D>(S, G, first) ? S.Phi[Dr::PHI_HI * N + ca] : S.Phi[Dr::PHI_LO * N + c];
+        int start = part[cm];
+        int cnt = wrap_idx(part[c] - start, W.np);
+        for (int t = 0; t < cnt; ++t) {
+            ca = W.off + wrap_idx(start + t, W.np);
+            F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 0, W, Pmax);
+            dA += F.dA;
+            FaceId H{ca, 0};
+            P += face_toggled<D>(S, H, first) ? S.Phi[Dr::PHI_LO * N + c] : S.Phi[Dr::PHI_HI * N + ca];
+        }
+    }
+    if (Dr::hi(A) >= 0) {  // this cell is L of the (this | hi) track
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        const int* part = Dr::part_hi(S);
+        int cb = B.off + part[c];
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        dA += F.dA;
+        FaceId G{c, 0};
+        P += face_toggled<D>(S, G, first) ? S.Phi[Dr::PHI_LO * N + cb] : S.Phi[Dr::PHI_HI * N + c];
+        int start = part[cm];
+        int cnt = wrap_idx(part[c] - start, B.np);
+        for (int t = 0; t < cnt; ++t) {
+            cb = B.off + wrap_idx(start + t, B.np);
+            F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 1, A, Pmax);
+            dA += F.dA;
+            FaceId H{cb, 1};
+            P += face_toggled<D>(S, H, first) ? S.Phi[Dr::PHI_HI * N + c] : S.Phi[Dr::PHI_LO * N + cb];
+        }
+    }
+    *sPhi = P;
+    *sdA = dA;
+}
+
+template <class Ph>
+DHD void b_faces_to_cell(const DevState& S, const AnnDev& A, int c, int first, double* cons) {
+    const size_t N = S.ncell;
+    const double Pmax = S.runs[A.run].ph.phi_max;
+    int cm = cell_pred(A, c);
+    double sPr, sAr;
+    b_sum_dir<DIR_R>(S, A, c, cm, first, Pmax, &sPr, &sAr);
+    double xp[3] = {A.rp, S.piph_n[c], A.zp};
+    double xm[3] = {A.rm, S.piph_n[cm], A.zm};
+    double dV = get_dV(xp, xm, Pmax);
+    double rc = A.rc;
+    double rfac = 1.0 / rc;  // bfield_scale_factor(r, 0)
+    double hr = 1.0, hp = rc, hz = 1.0;
+    double dA0 = (A.rp - A.rm) * (A.zp - A.zm);
+    cons[BRR] = sPr * (dV * rfac / (sAr * hr));
+    cons[BPP] = .5 * (S.Phi[0 * N + c] + S.Phi[0 * N + cm]) * dV / (dA0 * hp);
+    if (A.up >= 0 || A.down >= 0) {
+        double sPz, sAz;
+        b_sum_dir<DIR_Z>(S, A, c, cm, first, Pmax, &sPz, &sAz);
+        double zfac = 1.0;
+        cons[BZZ] = sPz * (dV * zfac / (sAz * hz));
+    } else {
+        cons[BZZ] = 0.0;
+    }
+    S.cons[BRR * N + c] = cons[BRR];
+    S.cons[BPP * N + c] = cons[BPP];
+    S.cons[BZZ * N + c] = cons[BZZ];
+}
+
+}  // namespace disco

@@ -81,7 +81,8 @@ struct DevState {
     int* ubi;            // per cell: partner in the inner annulus (array index)
     int* lbu;            // per cell: partner in the up annulus
     int* ubd;            // per cell: partner in the down annulus
-    int* rot;            // per annulus: array index of the smallest front
+    const int* rot;      // per annulus: array index of the smallest front (mesh at the start of the substep)
+    int* rot_n;          // per annulus: the same after move_cells (written by the first substep)
     double* fphi;        // per cell: increment of its phi face (i | i+1)
     double* fout;        // per cell: increment of its owned face in the outer track
     double* fin;         // per cell: increment of its owned face in the inner track
@@ -91,6 +92,20 @@ struct DevState {
     double* dt;          // per run: current step
     double* planet_phi;  // per run x planet
     double* t;           // per run: simulation time
+    // ---- constrained transport (MHD with CT; null otherwise) ----
+    // Face fluxes (bfields.c, paul.h:196-216 Phi[NUM_FACES]): slot 0 = the
+    // cell's phi face (at piph), 1 / 2 = its owned r face in the inner / outer
+    // track, 3 / 4 = its owned z face in the down / up track; a face is owned
+    // by the cell whose front edge ends it (LRtype, faces_alt.c:144-163).
+    double* Phi;         // [5][ncell]
+    double* rkPhi;       // [5][ncell] (RK_Phi)
+    double* emf_phi;     // [4][ncell] the cell's phi face: E_z, B_r, E_r, B_z
+    double* emf_t;       // [12][ncell] owned faces: (E1, B1, E_phi) for r-out, r-in, z-up, z-down
+    double* esum;        // [8][ncell] sums of .5*E1*frac over the faces of a track: E r-in, r-out, z-down, z-up; B likewise
+    double* es1;         // [8][ncell] edge values after the phi average: E0, E1, E4, E5, B0, B1, B4, B5
+    double* efin;        // [4][ncell] final edge EMFs E0, E1, E4, E5 (face average, advective part removed)
+    int* fflags;         // per cell: flip flag bits of its owned faces (1 r-out, 2 r-in, 4 z-up, 8 z-down)
+    int* err;            // device error word (bit 1: unsupported flip pattern)
 };
 
 // What a batch needs launched each substep (uniform over its runs).
@@ -98,6 +113,7 @@ struct StepShape {
     int hydro;  // DISCO_HYDRO_* of every run
     int has_z;  // some run has Num_Z > 1: z-tracks exist
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
+    int ct;     // MHD with constrained transport
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

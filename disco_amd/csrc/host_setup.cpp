@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "phys_euler.h"
+#include "phys_mhd.h"
 
 namespace disco {
 
@@ -188,9 +189,19 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
-    if (p.hydro != DISCO_HYDRO_EULER) m += " Hydro (only euler is built)";
+    if (p.hydro == DISCO_HYDRO_GRMHD) m += " Hydro=grmhd (the FV step is built for euler and mhd)";
     if (p.num_n != 0) m += " Num_N>0";
-    if (p.num_c != 5) m += " Num_C!=5 for euler";
+    if (p.hydro == DISCO_HYDRO_EULER && p.num_c != 5) m += " Num_C!=5 for euler";
+    if (p.hydro == DISCO_HYDRO_MHD) {
+        if (p.num_c != 8) m += " Num_C!=8 for mhd";
+        // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
+        if (!p.CT) m += " mhd needs Constrained_Transport 1";
+        if (p.Num_Z < 2) m += " mhd with CT_MODE 2 needs Num_Z > 1";
+        if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
+            m += " mhd needs rigidly rotating annuli (Mesh_Motion 0 or 4) for its flux flips";
+    } else if (p.Riemann_Solver == DISCO_HLLD) {
+        m += " Riemann_Solver=HLLD needs mhd (riemann.c:19-22)";
+    }
     if (p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) m += " NoBC_*";
     if (p.R_Periodic) m += " R_Periodic";
     // a serial run fills ghost z-layers only through periodic copies or the
@@ -199,7 +210,8 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
-    if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC) m += " Riemann_Solver";
+    if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
+        m += " Riemann_Solver";
     if (p.visc_flag) m += " Use_Viscosity";
     if (p.RotFrame) m += " RotFrame";
     if (p.noiseType) m += " Noise_Type";
@@ -418,6 +430,8 @@ static void host_zcopy(HostRun& R, bool mesh_and_w) {
                     R.prim[(size_t)q * R.ncell + o + i] = R.prim[(size_t)q * R.ncell + os + i];
                     R.cons[(size_t)q * R.ncell + o + i] = R.cons[(size_t)q * R.ncell + os + i];
                 }
+                if (!R.Phi.empty())
+                    for (int f = 0; f < 5; ++f) R.Phi[(size_t)f * R.ncell + o + i] = R.Phi[(size_t)f * R.ncell + os + i];
                 if (mesh_and_w) {
                     R.piph[o + i] = R.piph[os + i];
                     R.dphi[o + i] = R.dphi[os + i];
@@ -467,6 +481,199 @@ static void find_rotation(HostRun& R) {
     }
 }
 
+// ---------------------------------------------------------------- MHD set-up
+// Face list of every track in the reference's order (faces_alt.c:36-179):
+// tracks k-major then j; inside a track the cyclic merge of the two annuli's
+// front edges from their first cell in quadrant I.
+struct HostFace {
+    int L, R;  // cell indices
+    int LR;    // 0: the face ends at L's front, 1: at R's front
+    double dA;
+    double cm[3];
+};
+
+static std::vector<HostFace> build_faces(const HostRun& R, int dim) {
+    const double Pmax = R.par.phimax;
+    std::vector<int> I0(R.Nr * R.Nz, 0);
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
+        int found = 0, quad_prev = 0;
+        for (int i = 0; i < R.Np[jk] && !found; ++i) {
+            double convert = 2. * M_PI / Pmax;
+            double p = R.piph[R.off[jk] + i];
+            double sn = std::sin(p * convert);
+            double cs = std::cos(p * convert);
+            if (sn > 0. && cs > 0. && quad_prev) {
+                found = 1;
+                I0[jk] = i;
+            } else if (sn < 0. && cs > 0.) {
+                quad_prev = 1;
+            }
+        }
+        if (!found) I0[jk] = 0;
+    }
+    std::vector<HostFace> faces;
+    int Nrmax = dim == 1 ? R.Nr - 1 : R.Nr;
+    int Nzmax = dim == 1 ? R.Nz : R.Nz - 1;
+    for (int k = 0; k < Nzmax; ++k)
+        for (int j = 0; j < Nrmax; ++j) {
+            int jk = j + R.Nr * k;
+            int jkp = dim == 1 ? j + 1 + R.Nr * k : j + R.Nr * (k + 1);
+            double xp[3] = {R.rj(j), 0.0, R.zk(k)};
+            double xm[3] = {R.rj(j), 0.0, R.zk(k - 1)};
+            if (dim == 2) {
+                xm[0] = R.rj(j - 1);
+                xm[2] = R.zk(k);
+            }
+            int i = I0[jk], ip = I0[jkp];
+            for (int f = 0; f < R.Np[jk] + R.Np[jkp]; ++f) {
+                int cL = R.off[jk] + i, cR = R.off[jkp] + ip;
+                double dphi = R.piph[cL] - R.piph[cR];
+                while (dphi > .5 * Pmax) dphi -= Pmax;
+                while (dphi < -.5 * Pmax) dphi += Pmax;
+                int LR = dphi > 0. ? 1 : 0;
+                dphi = R.piph[cL] - R.dphi[cL] - R.piph[cR] + R.dphi[cR];
+                while (dphi > .5 * Pmax) dphi -= Pmax;
+                while (dphi < -.5 * Pmax) dphi += Pmax;
+                int LR_back = dphi < 0. ? 1 : 0;
+                xp[1] = LR == 0 ? R.piph[cL] : R.piph[cR];
+                xm[1] = LR_back == 0 ? R.piph[cL] - R.dphi[cL] : R.piph[cR] - R.dphi[cR];
+                // addFace (faces_alt.c:10-34)
+                HostFace F;
+                double dp = get_dp(xp[1], xm[1], Pmax);
+                F.cm[0] = dim == 1 ? 0.5 * (xp[0] + xm[0]) : get_centroid(xp[0], xm[0], 1);
+                F.cm[1] = get_dp(xp[1], .5 * dp, Pmax);
+                F.cm[2] = dim == 2 ? 0.5 * (xp[2] + xm[2]) : get_centroid(xp[2], xm[2], 2);
+                F.L = cL;
+                F.R = cR;
+                F.LR = LR;
+                F.dA = get_dA(xp, xm, dim, Pmax);
+                faces.push_back(F);
+                if (LR == 0) {
+                    if (++i == R.Np[jk]) i = 0;
+                } else {
+                    if (++ip == R.Np[jkp]) ip = 0;
+                }
+            }
+        }
+    return faces;
+}
+
+// B_faces_to_cells (bfields.c:88-309) on the host, type 0 (prim) or 1 (cons),
+// for every cell.
+static void host_b_faces_to_cells(HostRun& R, const std::vector<HostFace>& f1, const std::vector<HostFace>& f2,
+                                  int type) {
+    const size_t N = R.ncell;
+    std::vector<double>& X = type == 0 ? R.prim : R.cons;
+    std::vector<double> tmp(N, 0.0);
+    for (size_t c = 0; c < N; ++c) {
+        X[BRR * N + c] = 0.0;
+        X[BPP * N + c] = 0.0;
+        X[BZZ * N + c] = 0.0;
+    }
+    for (const HostFace& F : f1) {
+        double Phi = F.LR == 0 ? R.Phi[2 * N + F.L] : R.Phi[1 * N + F.R];
+        tmp[F.L] += F.dA;
+        tmp[F.R] += F.dA;
+        X[BRR * N + F.L] += Phi;
+        X[BRR * N + F.R] += Phi;
+    }
+    for (int k = 0; k < R.Nz; ++k)
+        for (int j = 0; j < R.Nr; ++j) {
+            int jk = j + R.Nr * k;
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                int c = R.off[jk] + i;
+                int cm = R.off[jk] + (i == 0 ? R.Np[jk] - 1 : i - 1);
+                double xp[3] = {R.rj(j), R.piph[c], R.zk(k)};
+                double xm[3] = {R.rj(j - 1), R.piph[cm], R.zk(k - 1)};
+                double x0 = get_centroid(xp[0], xm[0], 1);
+                double dA = get_dA(xp, xm, 0, R.par.phimax);
+                double dV = get_dV(xp, xm, R.par.phimax);
+                if (type == 0) {
+                    X[BRR * N + c] /= tmp[c];
+                    X[BPP * N + c] = .5 * (R.Phi[c] + R.Phi[cm]) / dA;
+                } else {
+                    double rfac = 1.0 / x0, hr = 1.0, hp = x0;
+                    X[BRR * N + c] *= dV * rfac / (tmp[c] * hr);
+                    X[BPP * N + c] = .5 * (R.Phi[c] + R.Phi[cm]) * dV / (dA * hp);
+                }
+            }
+        }
+    if (R.Nz > 1) {
+        std::fill(tmp.begin(), tmp.end(), 0.0);
+        for (const HostFace& F : f2) {
+            double Phi = F.LR == 0 ? R.Phi[4 * N + F.L] : R.Phi[3 * N + F.R];
+            tmp[F.L] += F.dA;
+            tmp[F.R] += F.dA;
+            X[BZZ * N + F.L] += Phi;
+            X[BZZ * N + F.R] += Phi;
+        }
+        for (int k = 0; k < R.Nz; ++k)
+            for (int j = 0; j < R.Nr; ++j) {
+                int jk = j + R.Nr * k;
+                for (int i = 0; i < R.Np[jk]; ++i) {
+                    int c = R.off[jk] + i;
+                    int cm = R.off[jk] + (i == 0 ? R.Np[jk] - 1 : i - 1);
+                    double xp[3] = {R.rj(j), R.piph[c], R.zk(k)};
+                    double xm[3] = {R.rj(j - 1), R.piph[cm], R.zk(k - 1)};
+                    double dV = get_dV(xp, xm, R.par.phimax);
+                    if (type == 0)
+                        X[BZZ * N + c] /= tmp[c];
+                    else
+                        X[BZZ * N + c] *= dV * 1.0 / (tmp[c] * 1.0);
+                }
+            }
+    }
+}
+
+// set_B_fields (bfields.c:13-86): face fluxes from the initial field sampled
+// at the phi-face position and at the r/z face centroids, then the cell
+// fields from the faces.
+static void host_set_B_fields(HostRun& R) {
+    const size_t N = R.ncell;
+    R.Phi.assign(5 * N, 0.0);
+    double prim[8];
+    for (int k = 0; k < R.Nz; ++k) {
+        double z = get_centroid(R.zk(k), R.zk(k - 1), 2);
+        for (int j = 0; j < R.Nr; ++j) {
+            double r = get_centroid(R.rj(j), R.rj(j - 1), 1);
+            int jk = j + R.Nr * k;
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                int c = R.off[jk] + i;
+                double phip = R.piph[c];
+                double phim = phip - R.dphi[c];
+                double xp[3] = {R.rj(j), phip, R.zk(k)};
+                double xm[3] = {R.rj(j - 1), phim, R.zk(k - 1)};
+                double x[3] = {r, phip, z};
+                disco_initial(&R.ic, prim, x);
+                R.Phi[c] = prim[BPP] * get_dA(xp, xm, 0, R.par.phimax);
+            }
+        }
+    }
+    std::vector<HostFace> f1 = build_faces(R, 1);
+    for (const HostFace& F : f1) {
+        disco_initial(&R.ic, prim, F.cm);
+        double Phi = prim[BRR] * F.dA;
+        if (F.LR == 0)
+            R.Phi[2 * N + F.L] = Phi;
+        else
+            R.Phi[1 * N + F.R] = Phi;
+    }
+    std::vector<HostFace> f2;
+    if (R.Nz > 1) {
+        f2 = build_faces(R, 2);
+        for (const HostFace& F : f2) {
+            disco_initial(&R.ic, prim, F.cm);
+            double Phi = prim[BZZ] * F.dA;
+            if (F.LR == 0)
+                R.Phi[4 * N + F.L] = Phi;
+            else
+                R.Phi[3 * N + F.R] = Phi;
+        }
+    }
+    host_b_faces_to_cells(R, f1, f2, 0);
+    host_b_faces_to_cells(R, f1, f2, 1);
+}
+
 int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     HostRun& R = *Rp;
     R = HostRun();
@@ -480,6 +687,8 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     calc_dp(R);
     R.prim.assign((size_t)R.nq * R.ncell, 0.0);
     R.cons.assign((size_t)R.nq * R.ncell, 0.0);
+    R.wiph.assign(R.ncell, 0.0);
+    const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD;
     // domain.c:163-209: initial data and prim2cons on real cells
     double prim[8], cons[8];
     for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
@@ -492,11 +701,20 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 double dV = get_dV(xp, xm, R.par.phimax);
                 disco_initial(&R.ic, prim, x);
                 if (par.RotFrame) prim[UPP] -= par.RotOmega;
-                euler::prim2cons<5>(R.ph, prim, cons, x, dV);
+                for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
+                if (mhd_ct)
+                    mhd::prim2cons(R.ph, prim, cons, x, dV);
+                else
+                    euler::prim2cons<5>(R.ph, prim, cons, x, dV);
                 for (int q = 0; q < R.nq; ++q) R.cons[(size_t)q * R.ncell + c] = cons[q];
                 R.nreal += 1;
             }
         }
+    if (mhd_ct) {
+        // domain.c:211-219: ghost meshes from the exchange, then set_B_fields
+        host_zcopy(R, true);
+        host_set_B_fields(R);
+    }
     // domain.c:221-238: cons2prim on real cells
     for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
         for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
@@ -507,7 +725,10 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 cell_x(R, j, k, c, x, xp, xm);
                 double dV = get_dV(xp, xm, R.par.phimax);
                 for (int q = 0; q < R.nq; ++q) cons[q] = R.cons[(size_t)q * R.ncell + c];
-                euler::cons2prim<5>(R.ph, cons, prim, x, dV);
+                if (mhd_ct)
+                    mhd::cons2prim(R.ph, cons, prim, x, dV);
+                else
+                    euler::cons2prim<5>(R.ph, cons, prim, x, dV);
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
             }
         }
@@ -519,6 +740,136 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     find_rotation(R);
     R.t = par.t_min;
     return 0;
+}
+
+// ---------------------------------------------------------------- tables
+// Annulus and run tables of a batch (device_state.h): neighbours, static
+// geometry, and the flags that mark which sweeps touch each annulus.
+void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
+                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape) {
+    hann->clear();
+    hruns->clear();
+    run_cell0->clear();
+    run_ann0->clear();
+    int cell0 = 0, ann0 = 0;
+    for (int r = 0; r < (int)runs.size(); ++r) {
+        const HostRun& R = runs[r];
+        RunDev D{};
+        D.ph = R.ph;
+        D.ic = R.ic;
+        D.bc = R.par.boundary;
+        D.planets_mod = R.par.planets;
+        D.cell0 = cell0;
+        D.ncell = R.ncell;
+        D.ann0 = ann0;
+        D.nann = R.Nr * R.Nz;
+        D.Nr = R.Nr;
+        D.Nz = R.Nz;
+        for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) D.planets[p] = R.planets[p];
+        hruns->push_back(D);
+        run_cell0->push_back(cell0);
+        run_ann0->push_back(ann0);
+        // phi_flux and trans_flux ranges (misc.c:393-407, 459-500): r-tracks
+        // of real z-layers, z-tracks of real r-columns; with constrained
+        // transport every range grows by one ghost layer across the track
+        bool ct = R.par.hydro == DISCO_HYDRO_MHD && R.par.CT;
+        int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
+        int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
+        int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
+        if (ct) {
+            kmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
+            kmax = R.NgZb == 0 ? R.Nz : R.Nz - R.NgZb + 1;
+        }
+        int zkmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
+        int zkmax = R.NgZb == 0 ? R.Nz - 1 : R.Nz - R.NgZb;
+        int zjmin = R.NgRa, zjmax = R.Nr - R.NgRb;
+        if (ct) {
+            zjmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
+            zjmax = R.NgRb == 0 ? R.Nr : R.Nr - R.NgRb + 1;
+        }
+        int pjmin = R.NgRa, pjmax = R.Nr - R.NgRb, pkmin = R.NgZa, pkmax = R.Nz - R.NgZb;
+        if (ct) {
+            pjmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
+            pjmax = R.NgRb == 0 ? R.Nr : R.Nr - R.NgRb + 1;
+            pkmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
+            pkmax = R.NgZb == 0 ? R.Nz : R.Nz - R.NgZb + 1;
+        }
+        bool zper = R.Nz > 1 && R.par.Z_Periodic;
+        int Ng = R.NgZa;
+        for (int k = 0; k < R.Nz; ++k)
+            for (int j = 0; j < R.Nr; ++j) {
+                int jk = j + R.Nr * k;
+                AnnDev A{};
+                A.run = r;
+                A.j = j;
+                A.k = k;
+                A.np = R.Np[jk];
+                A.off = cell0 + R.off[jk];
+                A.outer = (j + 1 < R.Nr) ? ann0 + jk + 1 : -1;
+                A.inner = (j > 0) ? ann0 + jk - 1 : -1;
+                A.up = (k + 1 < R.Nz) ? ann0 + jk + R.Nr : -1;
+                A.down = (k > 0) ? ann0 + jk - R.Nr : -1;
+                A.zsrc = -1;
+                int f = 0;
+                bool kin = k >= kmin && k < kmax;
+                if (R.real_annulus(j, k)) f |= ANN_REAL;
+                if (j >= pjmin && j < pjmax && k >= pkmin && k < pkmax) f |= ANN_PHIFLUX;
+                if (kin && j >= jmin && j < jmax) f |= ANN_OUTFLUX;
+                if (kin && j - 1 >= jmin && j - 1 < jmax) f |= ANN_INFLUX;
+                bool jin = j >= zjmin && j < zjmax;
+                if (R.Nz > 1 && jin && k >= zkmin && k < zkmax) f |= ANN_UPFLUX;
+                if (R.Nz > 1 && jin && k - 1 >= zkmin && k - 1 < zkmax) f |= ANN_DNFLUX;
+                bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic;
+                bool zghost = (k < R.NgZa || k >= R.Nz - R.NgZb) && !R.par.Z_Periodic &&
+                              R.par.boundary == DISCO_BC_FIXED3D;
+                if (!(f & ANN_REAL) && (rghost || zghost)) f |= ANN_GHOST_FILL;
+                // periodic z (exchange.c:80-115): ghost k < Ng copies layer
+                // Nz-2Ng+k, ghost k >= Nz-Ng copies layer k-(Nz-2Ng)
+                if (zper && (k < Ng || k >= R.Nz - Ng)) {
+                    int ks = k < Ng ? k + R.Nz - 2 * Ng : k - (R.Nz - 2 * Ng);
+                    A.zsrc = ann0 + j + R.Nr * ks;
+                    f |= ANN_ZCOPY;
+                }
+                A.flags = f;
+                A.rm = R.rj(j - 1);
+                A.rp = R.rj(j);
+                A.zm = R.zk(k - 1);
+                A.zp = R.zk(k);
+                A.rc = get_centroid(A.rp, A.rm, 1);
+                A.zc = get_centroid(A.zp, A.zm, 2);
+                // faces_alt.c:111-122: centroid distances across the track
+                if (j + 1 < R.Nr) {
+                    double rmc = get_centroid(R.rj(j), R.rj(j - 1), 1);
+                    double rpc = get_centroid(R.rj(j + 1), R.rj(j), 1);
+                    A.dxL = R.rj(j) - rmc;
+                    A.dxR = rpc - R.rj(j);
+                }
+                if (k + 1 < R.Nz) {
+                    double zmc = get_centroid(R.zk(k), R.zk(k - 1), 2);
+                    double zpc = get_centroid(R.zk(k + 1), R.zk(k), 2);
+                    A.dzL = R.zk(k) - zmc;
+                    A.dzR = zpc - R.zk(k);
+                }
+                if (R.par.boundary == DISCO_BC_FIXED) {
+                    A.rc_b = .5 * (A.rp + A.rm);
+                    A.zc_b = .5 * (A.zp + A.zm);
+                } else {
+                    A.rc_b = A.rc;
+                    A.zc_b = A.zc;
+                }
+                hann->push_back(A);
+            }
+        cell0 += R.ncell;
+        ann0 += R.Nr * R.Nz;
+    }
+    shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
+    shape->has_z = 0;
+    shape->zcopy = 0;
+    shape->ct = (!runs.empty() && runs[0].par.hydro == DISCO_HYDRO_MHD) ? 1 : 0;
+    for (const HostRun& R : runs) {
+        if (R.Nz > 1) shape->has_z = 1;
+        if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;
+    }
 }
 
 }  // namespace disco

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "device_state.h"
 #include "disco_initial.h"
 #include "disco_params.h"
 #include "phys_common.h"
@@ -29,6 +30,7 @@ struct HostRun {
     // cell state, annulus-major, SoA per field
     std::vector<double> piph, dphi, wiph;
     std::vector<double> prim, cons;  // [q*ncell + c]
+    std::vector<double> Phi;         // MHD with CT: face fluxes [5*ncell] (bfields.c slots)
     std::vector<int> rot;            // array index of the smallest front per annulus
     std::vector<Planet> planets;
     double t = 0.0;
@@ -54,5 +56,9 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic);
 
 // boundary_trans for the fixed / fixed3d modules on the host (ghost prims).
 void host_boundary(HostRun* run);
+
+// Annulus/run tables of a batch and the launch shape of its substeps.
+void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
+                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape);
 
 }  // namespace disco

@@ -36,6 +36,12 @@ struct PhysParams {
     int num_c, num_n;
 };
 
+// EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face
+// {E_z, B_r, E_r, B_z}, r face {E_z, B_r, E_phi, -}, z face {E_r, B_z, E_phi, -}.
+struct FaceEMF {
+    double E1, B1, E2, B2;
+};
+
 struct Planet {
     double r, phi, M, omega, eps;
     int type;

@@ -1,0 +1,660 @@
+// Ideal MHD module: the leaf functions of the reference's Hydro slot for mhd
+// (hydro.h:6-35), host/device, plus the HLLD solver of hlld.c.
+//
+// Follows Hydro/mhd.c:30-483 (get_omega, prim2cons, getUstar, cons2prim,
+// flux, source, flux_to_E, vel, mindt, bfield_scale_factor), hlld.c:20-401
+// (get_velocities, prim_to_cons, get_flux, get_single_star, get_double_star,
+// get_Ustar_HLLD) and the three solver branches of solve_riemann
+// (riemann.c:216-320).  Conserved variables are volume-integrated, prim[UPP]
+// is the angular velocity, and the field is stored as cons[BRR] = B_r dV / r,
+// cons[BPP] = B_phi dV / r, cons[BZZ] = B_z dV (mhd.c:61-63,518-528).
+// NQ = 8 (NUM_C = 8, no passive scalars).
+#pragma once
+
+#include "phys_common.h"
+
+namespace disco {
+namespace mhd {
+
+constexpr int NQ = 8;
+
+// mhd.c:518-528 for dim 0 (r); 1 for phi and z
+DHD double bscale_r(double r) { return 1.0 / r; }
+
+// mhd.c:35-69
+DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const double* x, double dV) {
+    double r = x[0];
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double vr = prim[URR];
+    double vp = prim[UPP] * r;
+    double vz = prim[UZZ];
+    double om = get_om(P, x);
+    double vp_off = vp - om * r;
+    double Br = prim[BRR];
+    double Bp = prim[BPP];
+    double Bz = prim[BZZ];
+    double v2 = vr * vr + vp_off * vp_off + vz * vz;
+    double B2 = Br * Br + Bp * Bp + Bz * Bz;
+    double rhoe = Pp / (P.gamma_law - 1.);
+    cons[DDD] = rho * dV;
+    cons[TAU] = (.5 * rho * v2 + rhoe + .5 * B2) * dV;
+    cons[SRR] = rho * vr * dV;
+    cons[LLL] = r * rho * vp * dV;
+    cons[SZZ] = rho * vz * dV;
+    cons[BRR] = Br * dV * bscale_r(r);
+    cons[BPP] = Bp * dV / r;
+    cons[BZZ] = Bz * dV;
+}
+
+// mhd.c:71-136; Bpack = {Bn, Br, Bp, Bz, v.B} of the HLL state (vel)
+DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const double* x, double Sk, double Ss,
+                  const double* n, const double* Bpack) {
+    double r = x[0];
+    double Bsn = Bpack[0];
+    double Bsr = Bpack[1];
+    double Bsp = Bpack[2];
+    double Bsz = Bpack[3];
+    double vBs = Bpack[4];
+    double rho = prim[RHO];
+    double vr = prim[URR];
+    double vp = prim[UPP] * r;
+    double vz = prim[UZZ];
+    double Pp = prim[PPP];
+    double Br = prim[BRR];
+    double Bp = prim[BPP];
+    double Bz = prim[BZZ];
+    double v2 = vr * vr + vp * vp + vz * vz;
+    double B2 = Br * Br + Bp * Bp + Bz * Bz;
+    double vn = vr * n[0] + vp * n[1] + vz * n[2];
+    double Bn = Br * n[0] + Bp * n[1] + Bz * n[2];
+    double vB = vr * Br + vp * Bp + vz * Bz;
+    double rhoe = Pp / (P.gamma_law - 1.);
+    double D = rho;
+    double mr = rho * vr;
+    double mp = rho * vp;
+    double mz = rho * vz;
+    double E = .5 * rho * v2 + rhoe + .5 * B2;
+    double Bs2 = Bsr * Bsr + Bsp * Bsp + Bsz * Bsz;
+    double Ps = rho * (Sk - vn) * (Ss - vn) + (Pp + .5 * B2 - Bn * Bn) - .5 * Bs2 + Bsn * Bsn;
+    double Dstar = (Sk - vn) * D / (Sk - Ss);
+    double Msr = ((Sk - vn) * mr + (Br * Bn - Bsr * Bsn)) / (Sk - Ss);
+    double Msp = ((Sk - vn) * mp + (Bp * Bn - Bsp * Bsn)) / (Sk - Ss);
+    double Msz = ((Sk - vn) * mz + (Bz * Bn - Bsz * Bsn)) / (Sk - Ss);
+    double Estar = ((Sk - vn) * E + (Ps + .5 * Bs2) * Ss - (Pp + .5 * B2) * vn - vBs * Bsn + vB * Bn) / (Sk - Ss);
+    double Msn = Dstar * Ss;
+    double mn = Msr * n[0] + Msp * n[1] + Msz * n[2];
+    Msr += n[0] * (Msn - mn);
+    Msp += n[1] * (Msn - mn);
+    Msz += n[2] * (Msn - mn);
+    Ustar[DDD] = Dstar;
+    Ustar[SRR] = Msr;
+    Ustar[LLL] = r * Msp;
+    Ustar[SZZ] = Msz;
+    Ustar[TAU] = Estar;
+    Ustar[BRR] = Bsr * bscale_r(r);
+    Ustar[BPP] = Bsp / r;
+    Ustar[BZZ] = Bsz;
+}
+
+// mhd.c:138-185
+DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+    double r = x[0];
+    double rho = cons[DDD] / dV;
+    if (rho < P.rho_floor) rho = P.rho_floor;
+    double Sr = cons[SRR] / dV;
+    double Sp = cons[LLL] / dV / r;
+    double Sz = cons[SZZ] / dV;
+    double E = cons[TAU] / dV;
+    double om = get_om(P, x);
+    double vr = Sr / rho;
+    double vp = Sp / rho;
+    double vp_off = vp - om * r;
+    double vz = Sz / rho;
+    double Br = cons[BRR] / (dV * bscale_r(r));
+    double Bp = cons[BPP] / dV * r;
+    double Bz = cons[BZZ] / dV;
+    double B2 = Br * Br + Bp * Bp + Bz * Bz;
+    double KE = .5 * (Sr * vr + rho * vp_off * vp_off + Sz * vz);
+    double rhoe = E - KE - .5 * B2;
+    double Pp = (P.gamma_law - 1.) * rhoe;
+    if (Pp < P.pre_floor * rho) Pp = P.pre_floor * rho;
+    if (P.isothermal) {
+        double cs2 = get_cs2(P, x);
+        Pp = cs2 * rho / P.gamma_law;
+    }
+    prim[RHO] = rho;
+    prim[PPP] = Pp;
+    prim[URR] = vr;
+    prim[UPP] = vp / r;
+    prim[UZZ] = vz;
+    prim[BRR] = Br;
+    prim[BPP] = Bp;
+    prim[BZZ] = Bz;
+}
+
+// mhd.c:187-223
+DHD void flux(const PhysParams& P, const double* prim, double* F, const double* x, const double* n) {
+    double r = x[0];
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double vr = prim[URR];
+    double vp = prim[UPP] * r;
+    double vz = prim[UZZ];
+    double Br = prim[BRR];
+    double Bp = prim[BPP];
+    double Bz = prim[BZZ];
+    double vn = vr * n[0] + vp * n[1] + vz * n[2];
+    double Bn = Br * n[0] + Bp * n[1] + Bz * n[2];
+    double vB = vr * Br + vp * Bp + vz * Bz;
+    double rhoe = Pp / (P.gamma_law - 1.);
+    double v2 = vr * vr + vp * vp + vz * vz;
+    double B2 = Br * Br + Bp * Bp + Bz * Bz;
+    F[DDD] = rho * vn;
+    F[SRR] = rho * vr * vn + (Pp + .5 * B2) * n[0] - Br * Bn;
+    F[LLL] = r * (rho * vp * vn + (Pp + .5 * B2) * n[1] - Bp * Bn);
+    F[SZZ] = rho * vz * vn + (Pp + .5 * B2) * n[2] - Bz * Bn;
+    F[TAU] = (.5 * rho * v2 + rhoe + Pp + B2) * vn - vB * Bn;
+    F[BRR] = (Br * vn - vr * Bn) * bscale_r(r);
+    F[BPP] = (Bp * vn - vp * Bn) / r;
+    F[BZZ] = Bz * vn - vz * Bn;
+}
+
+// mhd.c:318-449: fast-magnetosonic wave-speed bounds, contact speed, and the
+// HLL-averaged field and v.B used by getUstar
+DHD void vel(const PhysParams& P, const double* prim1, const double* prim2, double* Sl, double* Sr, double* Ss,
+             const double* n, const double* x, double* Bpack) {
+    double r = x[0];
+    const double g = P.gamma_law;
+    double P1 = prim1[PPP];
+    double rho1 = prim1[RHO];
+    double vr1 = prim1[URR];
+    double vp1 = r * prim1[UPP];
+    double vz1 = prim1[UZZ];
+    double vn1 = vr1 * n[0] + vp1 * n[1] + vz1 * n[2];
+    double cs1 = sqrt(g * (P1 / rho1));
+    double Br1 = prim1[BRR];
+    double Bp1 = prim1[BPP];
+    double Bz1 = prim1[BZZ];
+    double Bn1 = Br1 * n[0] + Bp1 * n[1] + Bz1 * n[2];
+    double B21 = (Br1 * Br1 + Bp1 * Bp1 + Bz1 * Bz1);
+    double b21 = B21 / rho1;
+    double FrL = vn1 * Br1 - Bn1 * vr1;
+    double FpL = vn1 * Bp1 - Bn1 * vp1;
+    double FzL = vn1 * Bz1 - Bn1 * vz1;
+    double mrL = rho1 * vr1;
+    double mpL = rho1 * vp1;
+    double mzL = rho1 * vz1;
+    double FmrL = rho1 * vr1 * vn1 + (P1 + .5 * B21) * n[0] - Br1 * Bn1;
+    double FmpL = rho1 * vp1 * vn1 + (P1 + .5 * B21) * n[1] - Bp1 * Bn1;
+    double FmzL = rho1 * vz1 * vn1 + (P1 + .5 * B21) * n[2] - Bz1 * Bn1;
+    double a1 = cs1 * cs1 + b21;
+    double cf21 = .5 * (a1 + sqrt(fabs(a1 * a1 - 4.0 * cs1 * cs1 * Bn1 * Bn1 / rho1)));
+    double L_Mins = vn1 - sqrt(cf21);
+    double L_Plus = vn1 + sqrt(cf21);
+
+    double P2 = prim2[PPP];
+    double rho2 = prim2[RHO];
+    double vr2 = prim2[URR];
+    double vp2 = r * prim2[UPP];
+    double vz2 = prim2[UZZ];
+    double vn2 = vr2 * n[0] + vp2 * n[1] + vz2 * n[2];
+    double cs2 = sqrt(g * (P2 / rho2));
+    double Br2 = prim2[BRR];
+    double Bp2 = prim2[BPP];
+    double Bz2 = prim2[BZZ];
+    double Bn2 = Br2 * n[0] + Bp2 * n[1] + Bz2 * n[2];
+    double B22 = (Br2 * Br2 + Bp2 * Bp2 + Bz2 * Bz2);
+    double b22 = B22 / rho2;
+    double FrR = vn2 * Br2 - Bn2 * vr2;
+    double FpR = vn2 * Bp2 - Bn2 * vp2;
+    double FzR = vn2 * Bz2 - Bn2 * vz2;
+    double mrR = rho2 * vr2;
+    double mpR = rho2 * vp2;
+    double mzR = rho2 * vz2;
+    double FmrR = rho2 * vr2 * vn2 + (P2 + .5 * B22) * n[0] - Br2 * Bn2;
+    double FmpR = rho2 * vp2 * vn2 + (P2 + .5 * B22) * n[1] - Bp2 * Bn2;
+    double FmzR = rho2 * vz2 * vn2 + (P2 + .5 * B22) * n[2] - Bz2 * Bn2;
+    double a2 = cs2 * cs2 + b22;
+    double cf22 = .5 * (a2 + sqrt(fabs(a2 * a2 - 4.0 * cs2 * cs2 * Bn2 * Bn2 / rho2)));
+    if (L_Mins > vn2 - sqrt(cf22)) L_Mins = vn2 - sqrt(cf22);
+    if (L_Plus < vn2 + sqrt(cf22)) L_Plus = vn2 + sqrt(cf22);
+
+    double aL = L_Plus;
+    double aR = -L_Mins;
+    double Br = (aR * Br1 + aL * Br2 + FrL - FrR) / (aL + aR);
+    double Bp = (aR * Bp1 + aL * Bp2 + FpL - FpR) / (aL + aR);
+    double Bz = (aR * Bz1 + aL * Bz2 + FzL - FzR) / (aL + aR);
+    double Bn = Br * n[0] + Bp * n[1] + Bz * n[2];
+    double mr = (aR * mrL + aL * mrR + FmrL - FmrR) / (aL + aR);
+    double mp = (aR * mpL + aL * mpR + FmpL - FmpR) / (aL + aR);
+    double mz = (aR * mzL + aL * mzR + FmzL - FmzR) / (aL + aR);
+    double mnL = mrL * n[0] + mpL * n[1] + mzL * n[2];
+    double mnR = mrR * n[0] + mpR * n[1] + mzR * n[2];
+    double rho = (aR * rho1 + aL * rho2 + mnL - mnR) / (aL + aR);
+    double L_Star = (rho2 * vn2 * (L_Plus - vn2) - rho1 * vn1 * (L_Mins - vn1) + (P1 + .5 * B21 - Bn1 * Bn1) -
+                     (P2 + .5 * B22 - Bn2 * Bn2)) /
+                    (rho2 * (L_Plus - vn2) - rho1 * (L_Mins - vn1));
+    double vr = mr / rho;
+    double vp = mp / rho;
+    double vz = mz / rho;
+    double vdotB = vr * Br + vp * Bp + vz * Bz;
+    Bpack[0] = Bn;
+    Bpack[1] = Br;
+    Bpack[2] = Bp;
+    Bpack[3] = Bz;
+    Bpack[4] = vdotB;
+    *Sl = L_Mins;
+    *Sr = L_Plus;
+    *Ss = L_Star;
+}
+
+// ------------------------------------------------------------------ HLLD
+// hlld.c works on Cartesian-normal states: prim slots as above with the
+// velocities multiplied by the scale factors (1, r, 1), hlld.c:316-321.
+namespace hlld {
+
+// hlld.c:20-70
+DHD void velocities(double g, const double* pL, const double* pR, const double* n, double* v, double* Pt_star) {
+    double rho_L = pL[RHO];
+    double P_L = pL[PPP];
+    double u_L = pL[URR] * n[0] + pL[UPP] * n[1] + pL[UZZ] * n[2];
+    double Bn_L = pL[BRR] * n[0] + pL[BPP] * n[1] + pL[BZZ] * n[2];
+    double B2_L = pL[BRR] * pL[BRR] + pL[BPP] * pL[BPP] + pL[BZZ] * pL[BZZ];
+    double Pt_L = P_L + .5 * B2_L;
+    double rho_R = pR[RHO];
+    double P_R = pR[PPP];
+    double u_R = pR[URR] * n[0] + pR[UPP] * n[1] + pR[UZZ] * n[2];
+    double Bn_R = pR[BRR] * n[0] + pR[BPP] * n[1] + pR[BZZ] * n[2];
+    double B2_R = pR[BRR] * pR[BRR] + pR[BPP] * pR[BPP] + pR[BZZ] * pR[BZZ];
+    double Pt_R = P_R + .5 * B2_R;
+    double Bn = .5 * (Bn_L + Bn_R);
+    double aL = g * P_L + B2_L;
+    double aR = g * P_R + B2_R;
+    double cf2_L = .5 * (aL + sqrt(fabs(aL * aL - 4. * g * P_L * Bn_L * Bn_L))) / rho_L;
+    double cf2_R = .5 * (aR + sqrt(fabs(aR * aR - 4. * g * P_R * Bn_R * Bn_R))) / rho_R;
+    double cf2 = cf2_L > cf2_R ? cf2_L : cf2_R;
+    double umin, umax;
+    if (u_L < u_R) {
+        umin = u_L;
+        umax = u_R;
+    } else {
+        umin = u_R;
+        umax = u_L;
+    }
+    double S_R = umax + sqrt(fabs(cf2));
+    double S_L = umin - sqrt(fabs(cf2));
+    double S_M = ((S_R - u_R) * rho_R * u_R - (S_L - u_L) * rho_L * u_L - Pt_R + Pt_L) /
+                 ((S_R - u_R) * rho_R - (S_L - u_L) * rho_L);
+    double rhostar_L = rho_L * (S_L - u_L) / (S_L - S_M);
+    double rhostar_R = rho_R * (S_R - u_R) / (S_R - S_M);
+    double Ss_L = S_M - fabs(Bn) / sqrt(rhostar_L);
+    double Ss_R = S_M + fabs(Bn) / sqrt(rhostar_R);
+    v[0] = S_L;
+    v[1] = Ss_L;
+    v[2] = S_M;
+    v[3] = Ss_R;
+    v[4] = S_R;
+    double num = (S_R - u_R) * rho_R * Pt_L - (S_L - u_L) * rho_L * Pt_R +
+                 rho_L * rho_R * (S_R - u_R) * (S_L - u_L) * (u_R - u_L);
+    double denom = (S_R - u_R) * rho_R - (S_L - u_L) * rho_L;
+    *Pt_star = num / denom;
+}
+
+// hlld.c:72-100
+DHD void prim_to_cons(double g, const double* prim, double* cons) {
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double vx = prim[URR];
+    double vy = prim[UPP];
+    double vz = prim[UZZ];
+    double Bx = prim[BRR];
+    double By = prim[BPP];
+    double Bz = prim[BZZ];
+    double v2 = vx * vx + vy * vy + vz * vz;
+    double rhoe = Pp / (g - 1.);
+    double B2 = Bx * Bx + By * By + Bz * Bz;
+    cons[DDD] = rho;
+    cons[SRR] = rho * vx;
+    cons[LLL] = rho * vy;
+    cons[SZZ] = rho * vz;
+    cons[TAU] = (.5 * rho * v2 + rhoe + .5 * B2);
+    cons[BRR] = Bx;
+    cons[BPP] = By;
+    cons[BZZ] = Bz;
+}
+
+// hlld.c:102-133
+DHD void get_flux(double g, const double* prim, double* flux, const double* n) {
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double vx = prim[URR];
+    double vy = prim[UPP];
+    double vz = prim[UZZ];
+    double Bx = prim[BRR];
+    double By = prim[BPP];
+    double Bz = prim[BZZ];
+    double v2 = vx * vx + vy * vy + vz * vz;
+    double vB = vx * Bx + vy * By + vz * Bz;
+    double B2 = Bx * Bx + By * By + Bz * Bz;
+    double vn = vx * n[0] + vy * n[1] + vz * n[2];
+    double Bn = Bx * n[0] + By * n[1] + Bz * n[2];
+    double rhoe = Pp / (g - 1.);
+    flux[DDD] = rho * vn;
+    flux[SRR] = rho * vx * vn + (Pp + .5 * B2) * n[0] - Bx * Bn;
+    flux[LLL] = rho * vy * vn + (Pp + .5 * B2) * n[1] - By * Bn;
+    flux[SZZ] = rho * vz * vn + (Pp + .5 * B2) * n[2] - Bz * Bn;
+    flux[TAU] = (.5 * rho * v2 + rhoe + Pp + B2) * vn - vB * Bn;
+    flux[BRR] = Bx * vn - vx * Bn;
+    flux[BPP] = By * vn - vy * Bn;
+    flux[BZZ] = Bz * vn - vz * Bn;
+}
+
+// hlld.c:135-193 (TOL = 1e-10)
+DHD void single_star(double g, const double* prim, double* Ustar, double S_K, double S_M, double Pt_star,
+                     const double* n) {
+    double rho = prim[RHO];
+    double vx = prim[URR];
+    double vy = prim[UPP];
+    double vz = prim[UZZ];
+    double Pp = prim[PPP];
+    double Bx = prim[BRR];
+    double By = prim[BPP];
+    double Bz = prim[BZZ];
+    double u_K = vx * n[0] + vy * n[1] + vz * n[2];
+    double Bn = Bx * n[0] + By * n[1] + Bz * n[2];
+    double vdotB = vx * Bx + vy * By + vz * Bz;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    double B2 = Bx * Bx + By * By + Bz * Bz;
+    double Pt = Pp + .5 * B2;
+    double e = .5 * rho * v2 + Pp / (g - 1.) + .5 * B2;
+    double rhostar = rho * (S_K - u_K) / (S_K - S_M);
+    double denom = rho * (S_K - u_K) * (S_K - S_M) - Bn * Bn;
+    if (fabs(denom) < 1e-10) denom = 1e-10;
+    double vx_star = vx - Bn * Bx * (S_M - u_K) / denom;
+    double vy_star = vy - Bn * By * (S_M - u_K) / denom;
+    double vz_star = vz - Bn * Bz * (S_M - u_K) / denom;
+    double sk2 = (S_K - u_K) * (S_K - u_K);
+    double Bx_star = Bx * (rho * sk2 - Bn * Bn) / denom;
+    double By_star = By * (rho * sk2 - Bn * Bn) / denom;
+    double Bz_star = Bz * (rho * sk2 - Bn * Bn) / denom;
+    double vn_star = vx_star * n[0] + vy_star * n[1] + vz_star * n[2];
+    double Bn_star = Bx_star * n[0] + By_star * n[1] + Bz_star * n[2];
+    vx_star += (S_M - vn_star) * n[0];
+    vy_star += (S_M - vn_star) * n[1];
+    vz_star += (S_M - vn_star) * n[2];
+    Bx_star += (Bn - Bn_star) * n[0];
+    By_star += (Bn - Bn_star) * n[1];
+    Bz_star += (Bn - Bn_star) * n[2];
+    double vBstar = vx_star * Bx_star + vy_star * By_star + vz_star * Bz_star;
+    double e_star = ((S_K - u_K) * e - Pt * u_K + Pt_star * S_M + Bn * (vdotB - vBstar)) / (S_K - S_M);
+    Ustar[DDD] = rhostar;
+    Ustar[SRR] = rhostar * vx_star;
+    Ustar[LLL] = rhostar * vy_star;
+    Ustar[SZZ] = rhostar * vz_star;
+    Ustar[TAU] = e_star;
+    Ustar[BRR] = Bx_star;
+    Ustar[BPP] = By_star;
+    Ustar[BZZ] = Bz_star;
+}
+
+// hlld.c:195-279
+DHD void double_star(const double* UsL, const double* UsR, double* Uss, const double* v, const double* n,
+                     int LR) {
+    double rho_L = UsL[DDD];
+    double rho_R = UsR[DDD];
+    double rhostar_K = LR == 0 ? rho_L : rho_R;
+    double rrh_L = sqrt(rho_L);
+    double rrh_R = sqrt(rho_R);
+    double rrh_K = sqrt(rhostar_K);
+    double vx_L = UsL[SRR] / rho_L;
+    double vy_L = UsL[LLL] / rho_L;
+    double vz_L = UsL[SZZ] / rho_L;
+    double Bx_L = UsL[BRR];
+    double By_L = UsL[BPP];
+    double Bz_L = UsL[BZZ];
+    double vx_R = UsR[SRR] / rho_R;
+    double vy_R = UsR[LLL] / rho_R;
+    double vz_R = UsR[SZZ] / rho_R;
+    double Bx_R = UsR[BRR];
+    double By_R = UsR[BPP];
+    double Bz_R = UsR[BZZ];
+    double Bn_L = Bx_L * n[0] + By_L * n[1] + Bz_L * n[2];
+    double Bn_R = Bx_R * n[0] + By_R * n[1] + Bz_R * n[2];
+    double Bn = .5 * (Bn_L + Bn_R);
+    double signBn = Bn > 0.0 ? 1.0 : -1.0;
+    double denom = rrh_L + rrh_R;
+    double vx_ss = (rrh_L * vx_L + rrh_R * vx_R + (Bx_R - Bx_L) * signBn) / denom;
+    double vy_ss = (rrh_L * vy_L + rrh_R * vy_R + (By_R - By_L) * signBn) / denom;
+    double vz_ss = (rrh_L * vz_L + rrh_R * vz_R + (Bz_R - Bz_L) * signBn) / denom;
+    double vn_ss = vx_ss * n[0] + vy_ss * n[1] + vz_ss * n[2];
+    vx_ss += (v[2] - vn_ss) * n[0];
+    vy_ss += (v[2] - vn_ss) * n[1];
+    vz_ss += (v[2] - vn_ss) * n[2];
+    double Bx_ss = (rrh_L * Bx_R + rrh_R * Bx_L + rrh_L * rrh_R * (vx_R - vx_L) * signBn) / denom;
+    double By_ss = (rrh_L * By_R + rrh_R * By_L + rrh_L * rrh_R * (vy_R - vy_L) * signBn) / denom;
+    double Bz_ss = (rrh_L * Bz_R + rrh_R * Bz_L + rrh_L * rrh_R * (vz_R - vz_L) * signBn) / denom;
+    double Bn_ss = Bx_ss * n[0] + By_ss * n[1] + Bz_ss * n[2];
+    Bx_ss += (Bn - Bn_ss) * n[0];
+    By_ss += (Bn - Bn_ss) * n[1];
+    Bz_ss += (Bn - Bn_ss) * n[2];
+    double vB_ss = vx_ss * Bx_ss + vy_ss * By_ss + vz_ss * Bz_ss;
+    double vBstar, e_star;
+    if (LR == 0) {
+        vBstar = vx_L * Bx_L + vy_L * By_L + vz_L * Bz_L;
+        e_star = UsL[TAU];
+    } else {
+        vBstar = vx_R * Bx_R + vy_R * By_R + vz_R * Bz_R;
+        e_star = UsR[TAU];
+    }
+    double plusminus = LR == 1 ? 1.0 : -1.0;
+    double e_ss = e_star + plusminus * rrh_K * (vBstar - vB_ss) * signBn;
+    Uss[DDD] = rhostar_K;
+    Uss[SRR] = rhostar_K * vx_ss;
+    Uss[LLL] = rhostar_K * vy_ss;
+    Uss[SZZ] = rhostar_K * vz_ss;
+    Uss[TAU] = e_ss;
+    Uss[BRR] = Bx_ss;
+    Uss[BPP] = By_ss;
+    Uss[BZZ] = Bz_ss;
+}
+
+// hlld.c:282-401 (cylindrical scale factors h = (1, r, 1))
+DHD void solve(const PhysParams& P, double w, const double* primL, const double* primR, double* F, double* U,
+               const double* x, const double* n) {
+    const double g = P.gamma_law;
+    const double hr = 1.0, hp = x[0], hz = 1.0;
+    double pL[NQ], pR[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        pL[q] = primL[q];
+        pR[q] = primR[q];
+    }
+    pL[URR] *= hr;
+    pR[URR] *= hr;
+    pL[UPP] *= hp;
+    pR[UPP] *= hp;
+    pL[UZZ] *= hz;
+    pR[UZZ] *= hz;
+    double v[5], Pt_star;
+    velocities(g, pL, pR, n, v, &Pt_star);
+    if (w < v[2]) {
+        double UL[NQ];
+        get_flux(g, pL, F, n);
+        prim_to_cons(g, pL, UL);
+        if (w < v[0]) {
+            prim_to_cons(g, pL, U);
+        } else if (w < v[1]) {
+            single_star(g, pL, U, v[0], v[2], Pt_star, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[0] * (U[q] - UL[q]);
+        } else {
+            double UsL[NQ], UsR[NQ];
+            single_star(g, pL, UsL, v[0], v[2], Pt_star, n);
+            single_star(g, pR, UsR, v[4], v[2], Pt_star, n);
+            double_star(UsL, UsR, U, v, n, 0);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[1] * (U[q] - UsL[q]) + v[0] * (UsL[q] - UL[q]);
+        }
+    } else {
+        double UR[NQ];
+        get_flux(g, pR, F, n);
+        prim_to_cons(g, pR, UR);
+        if (w > v[4]) {
+            prim_to_cons(g, pR, U);
+        } else if (w > v[3]) {
+            single_star(g, pR, U, v[4], v[2], Pt_star, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[4] * (U[q] - UR[q]);
+        } else {
+            double UsL[NQ], UsR[NQ];
+            single_star(g, pL, UsL, v[0], v[2], Pt_star, n);
+            single_star(g, pR, UsR, v[4], v[2], Pt_star, n);
+            double_star(UsL, UsR, U, v, n, 1);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[3] * (U[q] - UsR[q]) + v[4] * (UsR[q] - UR[q]);
+        }
+    }
+    const double brfac = bscale_r(x[0]), bpfac = 1.0, bzfac = 1.0;
+    F[SRR] *= hr;
+    F[LLL] *= hp;
+    F[SZZ] *= hz;
+    F[BRR] *= brfac / hr;
+    F[BPP] *= bpfac / hp;
+    F[BZZ] *= bzfac / hz;
+    U[SRR] *= hr;
+    U[LLL] *= hp;
+    U[SZZ] *= hz;
+    U[BRR] *= brfac / hr;
+    U[BPP] *= bpfac / hp;
+    U[BZZ] *= bzfac / hz;
+}
+
+}  // namespace hlld
+
+// Electromotive forces of one face from its flux and star state
+// (flux_to_E, mhd.c:293-316); dim 0 = phi face, 1 = r face, 2 = z face.
+DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, int dim) {
+    double r = x[0];
+    double irfac = 1.0 / bscale_r(x[0]);
+    FaceEMF e;
+    if (dim == 0) {
+        e.E1 = Flux[BRR] * irfac;  // E_z
+        e.B1 = Ustr[BRR] * irfac;  // B_r
+        e.E2 = -Flux[BZZ];         // E_r
+        e.B2 = Ustr[BZZ];          // B_z
+    } else if (dim == 1) {
+        e.E1 = -Flux[BPP] * r;     // E_z
+        e.B1 = Ustr[BRR] * irfac;  // B_r
+        e.E2 = Flux[BZZ];          // E_phi
+        e.B2 = 0.0;
+    } else {
+        e.E1 = Flux[BPP] * r;       // E_r
+        e.B1 = Ustr[BZZ];           // B_z
+        e.E2 = -Flux[BRR] * irfac;  // E_phi
+        e.B2 = 0.0;
+    }
+    return e;
+}
+
+// solve_riemann (riemann.c:216-320) for MHD: HLL / HLLC (mhd.c vel, getUstar)
+// or HLLD (hlld.c).  Returns the per-face increment dU = (F - w U*) dAdt
+// that the reference subtracts from L and adds to R, and the face's EMFs.
+DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
+                    const double* n, double w, double dAdt, int dim, double* dU) {
+    double Flux[NQ], Ustr[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        Flux[q] = 0.0;
+        Ustr[q] = 0.0;
+    }
+    if (P.riemann == DISCO_HLLD) {
+        hlld::solve(P, w, primL, primR, Flux, Ustr, x, n);
+    } else {
+        double Sl, Sr, Ss, Bpack[5];
+        vel(P, primL, primR, &Sl, &Sr, &Ss, n, x, Bpack);
+        if (w < Sl) {
+            flux(P, primL, Flux, x, n);
+            prim2cons(P, primL, Ustr, x, 1.0);
+        } else if (w > Sr) {
+            flux(P, primR, Flux, x, n);
+            prim2cons(P, primR, Ustr, x, 1.0);
+        } else if (P.riemann == DISCO_HLL) {
+            double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
+            double aL = Sr;
+            double aR = -Sl;
+            prim2cons(P, primL, Ul, x, 1.0);
+            prim2cons(P, primR, Ur, x, 1.0);
+            flux(P, primL, Fl, x, n);
+            flux(P, primR, Fr, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+                Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
+                Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+            }
+        } else {
+            double Uk[NQ], Fk[NQ];
+            if (w < Ss) {
+                prim2cons(P, primL, Uk, x, 1.0);
+                getUstar(P, primL, Ustr, x, Sl, Ss, n, Bpack);
+                flux(P, primL, Fk, x, n);
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
+            } else {
+                prim2cons(P, primR, Uk, x, 1.0);
+                getUstar(P, primR, Ustr, x, Sr, Ss, n, Bpack);
+                flux(P, primR, Fk, x, n);
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
+            }
+        }
+    }
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+    return flux_to_E(Flux, Ustr, x, dim);
+}
+
+// mhd.c:225-271 (polar sources off: NgRa > 0; viscosity off)
+DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                double dVdt) {
+    double rp = xp[0];
+    double rm = xm[0];
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double r_1 = .5 * (rp + rm);
+    double r2_3 = (rp * rp + rp * rm + rm * rm) / 3.;
+    double vr = prim[URR];
+    double omega = prim[UPP];
+    double r = get_centroid(rp, rm, 1);
+    double z = get_centroid(xp[2], xm[2], 2);
+    double x[3] = {r, 0.5 * (xm[1] + xp[1]), z};
+    double Br = prim[BRR];
+    double Bp = prim[BPP];
+    double Bz = prim[BZZ];
+    double B2 = Br * Br + Bp * Bp + Bz * Bz;
+    double centrifugal = rho * omega * omega * r - Bp * Bp / r;
+    double press_bal = (Pp + .5 * B2) / r_1;
+    cons[SRR] += dVdt * (centrifugal + press_bal);
+    double om = get_om(P, x);
+    double om1 = get_om1(P, x);
+    cons[TAU] += dVdt * rho * vr * (om * om * r2_3 / r_1 - om1 * (omega - om) * r2_3);
+}
+
+// mhd.c:452-483
+DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+    double r = get_centroid(xp[0], xm[0], 1);
+    double Pp = prim[PPP];
+    double rho = prim[RHO];
+    double vp = (prim[UPP] - w) * r;
+    double vr = prim[URR];
+    double vz = prim[UZZ];
+    double cs = sqrt(P.gamma_law * Pp / rho);
+    double Br = prim[BRR];
+    double Bp = prim[BPP];
+    double Bz = prim[BZZ];
+    double b2 = (Br * Br + Bp * Bp + Bz * Bz) / rho;
+    double cf = sqrt(cs * cs + b2);
+    double maxvr = cf + fabs(vr);
+    double maxvp = cf + fabs(vp);
+    double maxvz = cf + fabs(vz);
+    double dtr = get_dL(xp, xm, 1, P.phi_max) / maxvr;
+    double dtp = get_dL(xp, xm, 0, P.phi_max) / maxvp;
+    double dtz = get_dL(xp, xm, 2, P.phi_max) / maxvz;
+    double dt = dtr;
+    if (dt > dtp) dt = dtp;
+    if (dt > dtz) dt = dtz;
+    return dt;
+}
+
+}  // namespace mhd
+}  // namespace disco

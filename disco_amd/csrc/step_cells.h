@@ -1,0 +1,776 @@
+// Per-cell bodies of one RK substep of the moving-mesh finite-volume update
+// (FP64), for 2D and 3D cylindrical grids.  Every function here works on one
+// cell of the flat SoA state (device_state.h) and is __host__ __device__; the
+// HIP kernels in hydro_step.hip are one-thread-per-cell wrappers around them.
+//
+// Reference sequence (onestep.c:86-187) and where each step lives:
+//   set_wcell, adjust_RK_cons, plm_phi       -> stage_a_cell
+//   setup_faces + plm_trans (r and z)        -> stage_a_cell (partner search) + plm_trans_cell
+//   phi_flux, trans_flux (r and z)           -> flux_cell
+//   CT: avg_Efields, subtract_advective,
+//       update_B_fluxes, make_edge_adjust    -> ct_cells.h
+//   add_source, move_cells, clean_pi,
+//   calc_dp                                  -> update_a_cell
+//   CT: check_flipped, flip_fluxes           -> ct_cells.h
+//   B_faces_to_cells, calc_prim,
+//   boundary_trans                           -> update_b_cell (+ zcopy_cell: periodic z)
+//   getmindt                                 -> cell_mindt (fused into the last substep)
+//
+// Overlap faces (faces_alt.c:36-179).  The reference merges the two cyclic,
+// sorted front-edge lists of an annulus pair into exactly Np_a + Np_b faces,
+// each face ending at the front edge of one of its two cells (LRtype).  Here
+// every cell owns the one face of each neighbouring track that ends at its
+// own front edge; its partner is the neighbour-annulus cell that is current
+// when the reference's merge consumes this front: the first upper cell whose
+// front is >= mine (ties go to the lower list first) or the first lower cell
+// whose front is > mine.  Both are binary searches over the neighbour's
+// fronts in rotated (ascending) order.  The faces a cell shares but does not
+// own are those owned by the neighbour cells whose fronts fall inside its
+// span: a cyclic range bounded by its own partner and its predecessor's.
+// Each face increment is computed once, by its owner, stored per cell, and
+// gathered by both cells: deterministic, no atomics, no face lists in HBM.
+// One code path serves the r-tracks (j | j+1) and the z-tracks (k | k+1),
+// selected by the template direction D.
+#pragma once
+
+#include "device_state.h"
+#include "phys_euler.h"
+#include "phys_mhd.h"
+
+namespace disco {
+
+// ------------------------------------------------------------------ physics
+// The Hydro module's per-face and per-cell leaves (hydro.h:6-35) as policies.
+struct PhysEuler {
+    static constexpr int NQ = 5;
+    static constexpr bool MHD = false;
+    DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
+                        double w, double dAdt, int dim, double* dU) {
+        (void)dim;
+        euler::riemann<NQ>(P, pL, pR, x, n, w, dAdt, dU);
+        return FaceEMF{0.0, 0.0, 0.0, 0.0};
+    }
+    DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                    double dVdt) {
+        euler::source(P, prim, cons, xp, xm, dVdt);
+    }
+    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+        euler::cons2prim<NQ>(P, cons, prim, x, dV);
+    }
+    DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+        return euler::mindt(P, prim, w, xp, xm);
+    }
+};
+
+struct PhysMHD {
+    static constexpr int NQ = 8;
+    static constexpr bool MHD = true;
+    DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
+                        double w, double dAdt, int dim, double* dU) {
+        return mhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
+    }
+    DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                    double dVdt) {
+        mhd::source(P, prim, cons, xp, xm, dVdt);
+    }
+    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+        mhd::cons2prim(P, cons, prim, x, dV);
+    }
+    DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+        return mhd::mindt(P, prim, w, xp, xm);
+    }
+};
+
+// ------------------------------------------------------------------ helpers
+// plm.c:6-13
+DHD double minmod(double a, double b, double c) {
+    double m = a;
+    if (a * b < 0.0) m = 0.0;
+    if (fabs(b) < fabs(m)) m = b;
+    if (b * c < 0.0) m = 0.0;
+    if (fabs(c) < fabs(m)) m = c;
+    return m;
+}
+
+// Number of fronts of annulus B (n cells from off, smallest front at `rot`)
+// strictly below x (strict = true) or at or below x (strict = false).
+DHD int front_count(const double* __restrict__ piph, int off, int n, int rot, double x, bool strict) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int idx = rot + mid;
+        if (idx >= n) idx -= n;
+        double f = piph[off + idx];
+        bool below = strict ? (f < x) : (f <= x);
+        if (below)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// The same count, returned as the array index of the cell at that rotated
+// position (mod Np): the first cell whose front is not counted.
+DHD int front_search(const double* __restrict__ piph, int off, int n, int rot, double x, bool strict) {
+    int idx = rot + front_count(piph, off, n, rot, x, strict);
+    while (idx >= n) idx -= n;
+    return idx;
+}
+
+DHD int wrap_idx(int i, int n) {
+    if (i >= n) i -= n;
+    if (i < 0) i += n;
+    return i;
+}
+
+// Per-direction accessors: the neighbouring annuli, partner arrays, gradient,
+// owned-face increments, flags, centroid distances, and the constrained
+// transport slots of direction D.  "hi" is the track (this | outer or up) in
+// which this cell is L, "lo" the track (inner or down | this) in which it is R.
+template <int D>
+struct Dir;
+template <>
+struct Dir<DIR_R> {
+    DHD int hi(const AnnDev& A) { return A.outer; }
+    DHD int lo(const AnnDev& A) { return A.inner; }
+    DHD int* part_hi(const DevState& S) { return S.lbo; }
+    DHD int* part_lo(const DevState& S) { return S.ubi; }
+    DHD double* grad(const DevState& S) { return S.gradr; }
+    DHD double* f_hi(const DevState& S) { return S.fout; }
+    DHD double* f_lo(const DevState& S) { return S.fin; }
+    static constexpr int FLAG_HI = ANN_OUTFLUX, FLAG_LO = ANN_INFLUX;
+    DHD double dxL(const AnnDev& A) { return A.dxL; }
+    DHD double dxR(const AnnDev& A) { return A.dxR; }
+    // constrained transport
+    static constexpr int PHI_HI = 2, PHI_LO = 1;     // Phi slots of the owned faces
+    static constexpr int EMF_HI = 0, EMF_LO = 3;     // emf_t base slots
+    static constexpr int SUM_HI = 1, SUM_LO = 0;     // esum slots (E; +4 for B)
+    static constexpr int E_HI = 1, E_LO = 0;         // es1 / efin slots (E1, E0)
+    static constexpr int FL_HI = 1, FL_LO = 2;       // fflags bits
+    static constexpr double SGN = 1.0;               // update_B_fluxes orientation (bfields.c:321-346)
+    DHD double dl(const AnnDev& A) { return A.zp - A.zm; }  // face dl = get_dL(.., 2) (faces_alt.c:26-27)
+};
+template <>
+struct Dir<DIR_Z> {
+    DHD int hi(const AnnDev& A) { return A.up; }
+    DHD int lo(const AnnDev& A) { return A.down; }
+    DHD int* part_hi(const DevState& S) { return S.lbu; }
+    DHD int* part_lo(const DevState& S) { return S.ubd; }
+    DHD double* grad(const DevState& S) { return S.gradz; }
+    DHD double* f_hi(const DevState& S) { return S.fup; }
+    DHD double* f_lo(const DevState& S) { return S.fdn; }
+    static constexpr int FLAG_HI = ANN_UPFLUX, FLAG_LO = ANN_DNFLUX;
+    DHD double dxL(const AnnDev& A) { return A.dzL; }
+    DHD double dxR(const AnnDev& A) { return A.dzR; }
+    static constexpr int PHI_HI = 4, PHI_LO = 3;
+    static constexpr int EMF_HI = 6, EMF_LO = 9;
+    static constexpr int SUM_HI = 3, SUM_LO = 2;
+    static constexpr int E_HI = 3, E_LO = 2;
+    static constexpr int FL_HI = 4, FL_LO = 8;
+    static constexpr double SGN = -1.0;                     // bfields.c:352-375
+    DHD double dl(const AnnDev& A) { return A.rp - A.rm; }  // get_dL(.., 1)
+};
+
+// Geometry of the face between L (annulus AL) and R (the annulus above AL in
+// direction D): addFace and the merge step of faces_alt.c:10-34, :146-168,
+// plus the phi offsets of plm.c:86-91.  x is the face centroid (face cm).
+struct TFace {
+    double phic, dA, dpL, dpR, dp;
+    double x[3];
+};
+
+template <int D>
+DHD TFace t_face(double piphL, double dphiL, double piphR, double dphiR, int LR, const AnnDev& AL, double Pmax) {
+    TFace F;
+    double xp1 = LR == 0 ? piphL : piphR;
+    double dd = piphL - dphiL - piphR + dphiR;
+    while (dd > .5 * Pmax) dd -= Pmax;
+    while (dd < -.5 * Pmax) dd += Pmax;
+    double xm1 = dd < 0. ? piphR - dphiR : piphL - dphiL;
+    double dp = get_dp(xp1, xm1, Pmax);
+    F.dp = dp;
+    F.phic = get_dp(xp1, .5 * dp, Pmax);
+    if (D == DIR_R) {
+        F.dA = AL.rp * dp * (AL.zp - AL.zm);  // get_dA(.., 1) = r dphi dz, r = r_jph[j]
+        F.x[0] = AL.rp;
+        F.x[2] = AL.zc;
+    } else {
+        F.dA = .5 * (AL.rp + AL.rm) * (AL.rp - AL.rm) * dp;  // get_dA(.., 2) = r dr dphi
+        F.x[0] = AL.rc;
+        F.x[2] = AL.zp;
+    }
+    F.x[1] = F.phic;
+    double phiL = piphL - .5 * dphiL;
+    double phiR = piphR - .5 * dphiR;
+    F.dpL = get_signed_dp(F.phic, phiL, Pmax);
+    F.dpR = get_signed_dp(phiR, F.phic, Pmax);
+    return F;
+}
+
+// ------------------------------------------------------------------ dt
+// misc.c:39-84 for one real cell: mindt(prim, w = mean of the two phi-face
+// mesh speeds, xp, xm)
+template <class Ph>
+DHD double cell_mindt(const DevState& S, const AnnDev& A, const PhysParams& P, int c, const double* prim) {
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+    double phip = S.piph[c];
+    double phim = phip - S.dphi[c];
+    double xp[3] = {A.rp, phip, A.zp};
+    double xm[3] = {A.rm, phim, A.zm};
+    double w = .5 * (S.wiph[cm] + S.wiph[c]);
+    return Ph::mindt(P, prim, w, xp, xm);
+}
+
+// CFL candidate of cell c over the current state (1e300 for ghosts)
+template <class Ph>
+DHD double mindt_cell(const DevState& S, int c) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(A.flags & ANN_REAL)) return 1.0e300;
+    double prim[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[(size_t)q * S.ncell + c];
+    return cell_mindt<Ph>(S, A, S.runs[A.run].ph, c, prim);
+}
+
+// ------------------------------------------------------------------ stage A
+// set_wcell (misc.c:101-168, first substep), adjust_RK_cons (misc.c:226-244,
+// with timestep.c:18-24's copy folded into the first substep), plm_phi
+// (plm.c:15-51) and the partner search of the overlap faces.
+template <class Ph>
+DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const PhysParams& P = S.runs[A.run].ph;
+    const size_t N = S.ncell;
+    int n = A.np;
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? n - 1 : i - 1);
+    int cp = A.off + (i + 1 == n ? 0 : i + 1);
+
+    if (first) {
+        double w = 0.0;
+        if (P.mesh_motion == 1) {
+            w = .5 * (S.prim[UPP * N + c] + S.prim[UPP * N + cp]);
+        } else if (P.mesh_motion == 4) {
+            double phip = S.piph[c];
+            double phim = phip - S.dphi[c];
+            double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
+            w = mesh_om(P, x);
+        }
+        S.wiph[c] = w;
+    }
+    if (A.flags & ANN_REAL) {
+        if (first) {
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.rkcons[q * N + c] = S.cons[q * N + c];
+        } else {
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q)
+                S.cons[q * N + c] = (1. - RK) * S.cons[q * N + c] + RK * S.rkcons[q * N + c];
+        }
+    }
+    if (Ph::MHD && S.Phi) {
+        // face fluxes of every cell (misc.c:239-241; timestep.c:22 at the first substep)
+        _Pragma("unroll") for (int f = 0; f < 5; ++f) {
+            if (first)
+                S.rkPhi[f * N + c] = S.Phi[f * N + c];
+            else
+                S.Phi[f * N + c] = (1. - RK) * S.Phi[f * N + c] + RK * S.rkPhi[f * N + c];
+        }
+    }
+    double dpL = S.dphi[cm];
+    double dpC = S.dphi[c];
+    double dpR = S.dphi[cp];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        double pL = S.prim[q * N + cm];
+        double pC = S.prim[q * N + c];
+        double pR = S.prim[q * N + cp];
+        double sL = pC - pL;
+        sL /= .5 * (dpC + dpL);
+        double sR = pR - pC;
+        sR /= .5 * (dpR + dpC);
+        double sC = pR - pL;
+        sC /= .5 * (dpL + dpR) + dpC;
+        S.gradp[q * N + c] = minmod(P.plm * sL, sC, P.plm * sR);
+    }
+    double front = S.piph[c];
+    if (A.outer >= 0) {
+        const AnnDev& B = S.ann[A.outer];
+        S.lbo[c] = front_search(S.piph, B.off, B.np, S.rot[A.outer], front, true);
+    }
+    if (A.inner >= 0) {
+        const AnnDev& W = S.ann[A.inner];
+        S.ubi[c] = front_search(S.piph, W.off, W.np, S.rot[A.inner], front, false);
+    }
+    if (A.up >= 0) {
+        const AnnDev& B = S.ann[A.up];
+        S.lbu[c] = front_search(S.piph, B.off, B.np, S.rot[A.up], front, true);
+    }
+    if (A.down >= 0) {
+        const AnnDev& W = S.ann[A.down];
+        S.ubd[c] = front_search(S.piph, W.off, W.np, S.rot[A.down], front, false);
+    }
+}
+
+// ------------------------------------------------------------------ plm_trans
+// plm.c:56-195: area-weighted mean of the face slopes, divided by the cell's
+// total face area in the direction, then the per-face limiter (order
+// independent: zero on any sign change, else the smallest |PLM*S|).
+template <int NQ>
+struct SlopeAcc {
+    double sum[NQ];
+    double smin[NQ];
+    unsigned pos, neg, zero;  // bit q set when some face slope of field q is >0, <0, ==0
+    bool any;
+};
+
+template <int NQ>
+DHD void acc_add(SlopeAcc<NQ>& a, const double* S, double dA) {
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        a.sum[q] += S[q] * dA;
+        a.pos |= (S[q] > 0.0 ? 1u : 0u) << q;
+        a.neg |= (S[q] < 0.0 ? 1u : 0u) << q;
+        a.zero |= (S[q] == 0.0 ? 1u : 0u) << q;
+        a.smin[q] = (!a.any || fabs(S[q]) < fabs(a.smin[q])) ? S[q] : a.smin[q];
+    }
+    a.any = true;
+}
+
+template <int NQ>
+DHD void face_slope(const DevState& S, int cL, int cR, const TFace& F, double dxL, double dxR, double* Sq) {
+    const size_t N = S.ncell;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        double WL = S.prim[q * N + cL] + F.dpL * S.gradp[q * N + cL];
+        double WR = S.prim[q * N + cR] - F.dpR * S.gradp[q * N + cR];
+        Sq[q] = (WR - WL) / (dxR + dxL);
+    }
+}
+
+// Gradient of cell c in direction D (plm_trans with dim = D).
+template <int NQ, int D>
+DHD void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P, int c, int cm) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double Pmax = P.phi_max;
+    double* grad = Dr::grad(S);
+    if (P.plm == 0.0) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) grad[q * N + c] = 0.0;
+        return;
+    }
+    double phip = S.piph[c];
+    double dph = S.dphi[c];
+    SlopeAcc<NQ> acc;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        acc.sum[q] = 0.0;
+        acc.smin[q] = 0.0;
+    }
+    acc.pos = acc.neg = acc.zero = 0u;
+    acc.any = false;
+    double Sq[NQ];
+    const int ahi = Dr::hi(A), alo = Dr::lo(A);
+    if (ahi >= 0) {  // this cell is L of the (this | hi) track
+        const AnnDev& B = S.ann[ahi];
+        const int* part = Dr::part_hi(S);
+        int nb = B.np;
+        int own = part[c];
+        int start = part[cm];
+        int cnt = own - start;
+        if (cnt < 0) cnt += nb;
+        int cb = B.off + own;
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        face_slope<NQ>(S, c, cb, F, Dr::dxL(A), Dr::dxR(A), Sq);
+        acc_add(acc, Sq, F.dA);
+        for (int t = 0; t < cnt; ++t) {
+            int b = start + t;
+            if (b >= nb) b -= nb;
+            cb = B.off + b;
+            F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 1, A, Pmax);
+            face_slope<NQ>(S, c, cb, F, Dr::dxL(A), Dr::dxR(A), Sq);
+            acc_add(acc, Sq, F.dA);
+        }
+    }
+    if (alo >= 0) {  // this cell is R of the (lo | this) track
+        const AnnDev& W = S.ann[alo];
+        const int* part = Dr::part_lo(S);
+        int nw = W.np;
+        int own = part[c];
+        int start = part[cm];
+        int cnt = own - start;
+        if (cnt < 0) cnt += nw;
+        int ca = W.off + own;
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        face_slope<NQ>(S, ca, c, F, Dr::dxL(W), Dr::dxR(W), Sq);
+        acc_add(acc, Sq, F.dA);
+        for (int t = 0; t < cnt; ++t) {
+            int a = start + t;
+            if (a >= nw) a -= nw;
+            ca = W.off + a;
+            F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 0, W, Pmax);
+            face_slope<NQ>(S, ca, c, F, Dr::dxL(W), Dr::dxR(W), Sq);
+            acc_add(acc, Sq, F.dA);
+        }
+    }
+    // plm.c:112-143: the cell's total face area in this direction
+    double phim = phip - dph;
+    double dpp = get_dp(phip, phim, Pmax);
+    double dAp, dAm;
+    if (D == DIR_R) {
+        double dz = A.zp - A.zm;
+        dAp = A.rp * dpp * dz;
+        dAm = A.rm * dpp * dz;
+    } else {
+        dAp = .5 * (A.rp + A.rm) * (A.rp - A.rm) * dpp;
+        dAm = dAp;
+    }
+    double dAtot = dAp + dAm;
+    if (alo < 0) dAtot = dAp;
+    if (ahi < 0) dAtot = dAm;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        double g = acc.sum[q] / dAtot;
+        if (!acc.any) {
+            g = 0.0;
+        } else if (g != 0.0) {
+            bool flip = g > 0.0 ? (((acc.neg | acc.zero) >> q) & 1u) : (((acc.pos | acc.zero) >> q) & 1u);
+            if (flip) {
+                g = 0.0;
+            } else {
+                double lim = P.plm * acc.smin[q];
+                if (fabs(lim) < fabs(g)) g = lim;
+            }
+        }
+        grad[q * N + c] = g;
+    }
+}
+
+template <class Ph>
+DHD void plm_trans_cell(const DevState& S, int c, int has_z) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const PhysParams& P = S.runs[A.run].ph;
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+    trans_gradient<NQ, DIR_R>(S, A, P, c, cm);
+    if (has_z) trans_gradient<NQ, DIR_Z>(S, A, P, c, cm);
+}
+
+// ------------------------------------------------------------------ fluxes
+// riemann_phi (riemann.c:39-112) on the cell's phi face, riemann_trans
+// (riemann.c:114-213) on its owned faces; solve_riemann's increment
+// (F - w U*) dA dt is stored per owner cell, and for MHD the face's EMFs
+// (flux_to_E) beside it.
+template <class Ph, int D>
+DHD FaceEMF t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, const TFace& F, const AnnDev& AL,
+                        double dt, double* dU) {
+    constexpr int NQ = Ph::NQ;
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double* grad = Dr::grad(S);
+    const double dxL = Dr::dxL(AL), dxR = Dr::dxR(AL);
+    double primL[NQ], primR[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        primL[q] = S.prim[q * N + cL] + grad[q * N + cL] * dxL + S.gradp[q * N + cL] * F.dpL;
+        primR[q] = S.prim[q * N + cR] - grad[q * N + cR] * dxR - S.gradp[q * N + cR] * F.dpR;
+    }
+    if (Ph::MHD) {  // riemann.c:146-152: the normal field is single-valued on the face
+        const int BT = D == DIR_R ? BRR : BZZ;
+        double Bavg = .5 * (primL[BT] + primR[BT]);
+        primL[BT] = Bavg;
+        primR[BT] = Bavg;
+    }
+    const double n[3] = {D == DIR_R ? 1.0 : 0.0, 0.0, D == DIR_Z ? 1.0 : 0.0};
+    return Ph::riemann(P, primL, primR, F.x, n, 0.0, F.dA * dt, D, dU);
+}
+
+template <class Ph, int D>
+DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& P, int c, double dt) {
+    constexpr int NQ = Ph::NQ;
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const double Pmax = P.phi_max;
+    double dU[NQ];
+    double phip = S.piph[c];
+    double dph = S.dphi[c];
+    if (A.flags & Dr::FLAG_HI) {
+        const AnnDev& B = S.ann[Dr::hi(A)];
+        int cb = B.off + Dr::part_hi(S)[c];
+        TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
+        FaceEMF e = t_face_flux<Ph, D>(S, P, c, cb, F, A, dt, dU);
+        double* f = Dr::f_hi(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+        if (Ph::MHD && S.emf_t) {
+            S.emf_t[(Dr::EMF_HI + 0) * N + c] = e.E1;
+            S.emf_t[(Dr::EMF_HI + 1) * N + c] = e.B1;
+            S.emf_t[(Dr::EMF_HI + 2) * N + c] = e.E2;
+        }
+    }
+    if (A.flags & Dr::FLAG_LO) {
+        const AnnDev& W = S.ann[Dr::lo(A)];
+        int ca = W.off + Dr::part_lo(S)[c];
+        TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
+        FaceEMF e = t_face_flux<Ph, D>(S, P, ca, c, F, W, dt, dU);
+        double* f = Dr::f_lo(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+        if (Ph::MHD && S.emf_t) {
+            S.emf_t[(Dr::EMF_LO + 0) * N + c] = e.E1;
+            S.emf_t[(Dr::EMF_LO + 1) * N + c] = e.B1;
+            S.emf_t[(Dr::EMF_LO + 2) * N + c] = e.E2;
+        }
+    }
+}
+
+template <class Ph>
+DHD void flux_cell(const DevState& S, int c, double dtfac) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(A.flags & (ANN_PHIFLUX | ANN_OUTFLUX | ANN_INFLUX | ANN_UPFLUX | ANN_DNFLUX))) return;
+    const PhysParams& P = S.runs[A.run].ph;
+    const size_t N = S.ncell;
+    const double dt = S.dt[A.run] * dtfac;
+    int n = A.np;
+    int i = c - A.off;
+
+    if (A.flags & ANN_PHIFLUX) {
+        int cp = A.off + (i + 1 == n ? 0 : i + 1);
+        double dU[NQ];
+        double primL[NQ], primR[NQ];
+        double dph = S.dphi[c];
+        double dphR = S.dphi[cp];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+            primL[q] = S.prim[q * N + c] + .5 * S.gradp[q * N + c] * dph;
+            primR[q] = S.prim[q * N + cp] - .5 * S.gradp[q * N + cp] * dphR;
+        }
+        if (Ph::MHD) {  // riemann.c:55-59
+            double Bp = .5 * (primL[BPP] + primR[BPP]);
+            primL[BPP] = Bp;
+            primR[BPP] = Bp;
+        }
+        double x[3] = {A.rc, S.piph[c], A.zc};
+        const double nv[3] = {0.0, 1.0, 0.0};
+        double dA = (A.rp - A.rm) * (A.zp - A.zm);  // get_dA(xp, xm, 0)
+        double w = A.rc * S.wiph[c];                // get_scale_factor(x, 0) * wiph
+        FaceEMF e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fphi[q * N + c] = dU[q];
+        if (Ph::MHD && S.emf_phi) {
+            S.emf_phi[0 * N + c] = e.E1;  // E_z
+            S.emf_phi[1 * N + c] = e.B1;  // B_r
+            S.emf_phi[2 * N + c] = e.E2;  // E_r
+            S.emf_phi[3 * N + c] = e.B2;  // B_z
+        }
+    }
+    owned_faces_flux<Ph, DIR_R>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_Z>(S, A, P, c, dt);
+}
+
+// ------------------------------------------------------------------ update
+// Gather of the increments of every face of cell c in direction D: the
+// reference subtracts a face's increment from its L cell and adds it to its
+// R cell (riemann.c:314-317).
+template <int NQ, int D>
+DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double* cons) {
+    using Dr = Dir<D>;
+    const size_t N = S.ncell;
+    const int alo = Dr::lo(A), ahi = Dr::hi(A);
+    if (alo >= 0 && (A.flags & Dr::FLAG_LO)) {  // this cell is R: gains
+        const AnnDev& W = S.ann[alo];
+        const double* fo = Dr::f_lo(S);
+        const double* fw = Dr::f_hi(S);
+        const int* part = Dr::part_lo(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fo[q * N + c];
+        int start = part[cm];
+        int cnt = part[c] - start;
+        if (cnt < 0) cnt += W.np;
+        for (int t = 0; t < cnt; ++t) {
+            int a = start + t;
+            if (a >= W.np) a -= W.np;
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fw[q * N + W.off + a];
+        }
+    }
+    if (ahi >= 0 && (A.flags & Dr::FLAG_HI)) {  // this cell is L: loses
+        const AnnDev& B = S.ann[ahi];
+        const double* fo = Dr::f_hi(S);
+        const double* fb = Dr::f_lo(S);
+        const int* part = Dr::part_hi(S);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fo[q * N + c];
+        int start = part[cm];
+        int cnt = part[c] - start;
+        if (cnt < 0) cnt += B.np;
+        for (int t = 0; t < cnt; ++t) {
+            int b = start + t;
+            if (b >= B.np) b -= B.np;
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fb[q * N + B.off + b];
+        }
+    }
+}
+
+// First half of the update: conservative update from the gathered face
+// increments (riemann.c:314-317), add_source (misc.c:550-593) and, in the
+// first substep, move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns
+// the updated conserved state of a real cell in cons[] (and stores it).
+template <class Ph>
+DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double* cons) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const RunDev& Rn = S.runs[A.run];
+    const PhysParams& P = Rn.ph;
+    const size_t N = S.ncell;
+    const double Pmax = P.phi_max;
+    const double dt = S.dt[A.run] * dtfac;
+    int n = A.np;
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? n - 1 : i - 1);
+    double phip = S.piph[c];
+    double dph = S.dphi[c];
+
+    if (A.flags & ANN_REAL) {
+        double prim[NQ];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+            cons[q] = S.cons[q * N + c];
+            prim[q] = S.prim[q * N + c];
+        }
+        // phi faces i-1 (this cell is R) and i (this cell is L)
+        if (A.flags & ANN_PHIFLUX) {
+            if (i == 0) {
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
+            } else {
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
+            }
+        }
+        gather_trans<NQ, DIR_R>(S, A, c, cm, cons);
+        gather_trans<NQ, DIR_Z>(S, A, c, cm, cons);
+        // add_source with the pre-move geometry and the substep's prims
+        double xp[3] = {A.rp, phip, A.zp};
+        double xm[3] = {A.rm, phip - dph, A.zm};
+        double dV = get_dV(xp, xm, Pmax);
+        Ph::source(P, prim, cons, xp, xm, dV * dt);
+        for (int p = 0; p < P.npl; ++p) {
+            Planet pl = Rn.planets[p];
+            pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
+            euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);  // planet.c:138-158 (module independent)
+        }
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.cons[q * N + c] = cons[q];
+    }
+
+    // moving mesh (first substep): every annulus, ghosts included
+    if (first) {
+        double pn = phip + S.wiph[c] * dt;
+        while (pn > Pmax) pn -= Pmax;
+        while (pn < 0.0) pn += Pmax;
+        double pm = S.piph[cm] + S.wiph[cm] * dt;
+        while (pm > Pmax) pm -= Pmax;
+        while (pm < 0.0) pm += Pmax;
+        S.piph_n[c] = pn;
+        S.dphi_n[c] = get_dp(pn, pm, Pmax);
+        if (pn < pm && !(A.flags & ANN_ZCOPY)) S.rot_n[S.cell_ann[c]] = i;
+    }
+}
+
+template <class Ph>
+DHD void b_faces_to_cell(const DevState& S, const AnnDev& A, int c, int first, double* cons);
+
+// Second half of the update: (MHD with CT) B_faces_to_cells (bfields.c:88-309),
+// calc_prim (misc.c:300-335) and the fixed boundaries (Boundary/fixed.c:11-60,
+// boundary_functions.c:161-266) on the mesh after the move.  The second
+// substep also produces the next step's CFL candidate from the prims it has
+// just written (misc.c:39-84 runs on exactly this state at the start of the
+// next step, with this step's mesh speeds).  cons[] holds the real cell's
+// conserved state from update_a_cell.
+template <class Ph>
+DHD double update_b_cell(const DevState& S, int c, int first, int want_dt, double* cons) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const RunDev& Rn = S.runs[A.run];
+    const PhysParams& P = Rn.ph;
+    const size_t N = S.ncell;
+    const double Pmax = P.phi_max;
+    const double phip = S.piph_n[c];
+    const double dph = S.dphi_n[c];
+
+    if (A.flags & ANN_REAL) {
+        if (Ph::MHD && S.Phi) b_faces_to_cell<Ph>(S, A, c, first, cons);
+        double phim = phip - dph;
+        double xp[3] = {A.rp, phip, A.zp};
+        double xm[3] = {A.rm, phim, A.zm};
+        double dV = get_dV(xp, xm, Pmax);
+        double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
+        double prim[NQ];
+        Ph::cons2prim(P, cons, prim, x, dV);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
+        if (want_dt) {
+            // the CFL step of the next step reads this mesh (piph_n is the
+            // current mesh after the first substep)
+            int i = c - A.off;
+            int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+            double w = .5 * (S.wiph[cm] + S.wiph[c]);
+            return Ph::mindt(P, prim, w, xp, xm);
+        }
+    } else if (A.flags & ANN_GHOST_FILL) {
+        double x[3];
+        x[0] = A.rc_b;
+        x[2] = A.zc_b;
+        if (Rn.bc == DISCO_BC_FIXED)
+            x[1] = phip - .5 * dph;
+        else
+            x[1] = 0.5 * (phip + (phip - dph));
+        double p8[8];
+        disco_initial(&Rn.ic, p8, x);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = p8[q];
+    }
+    return 1.0e300;
+}
+
+// Periodic z: what a one-rank MPI run's self-wraparound exchange does
+// (exchange.c:173-279 with mpisetup.c:17,38; copy_lite_to_cell
+// exchange.c:58-70).  Every ghost z-annulus becomes a copy of the real
+// annulus one period away -- primitives, mesh speeds, face fluxes and, after
+// the mesh moved, the mesh -- and takes over that annulus' rotation point.
+// Runs after the update, so r-ghost columns copy the freshly re-initialised
+// source ghosts exactly as boundary_trans(1) before exchangeData(1) does
+// (onestep.c:166-173).
+template <class Ph>
+DHD void zcopy_cell(const DevState& S, int c, int first) {
+    constexpr int NQ = Ph::NQ;
+    const int a = S.cell_ann[c];
+    const AnnDev& A = S.ann[a];
+    if (!(A.flags & ANN_ZCOPY)) return;
+    const AnnDev& Sr = S.ann[A.zsrc];
+    const size_t N = S.ncell;
+    int i = c - A.off;
+    int s = Sr.off + i;
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = S.prim[q * N + s];
+    S.wiph[c] = S.wiph[s];
+    if (Ph::MHD && S.Phi) {
+        _Pragma("unroll") for (int f = 0; f < 5; ++f) {
+            S.Phi[f * N + c] = S.Phi[f * N + s];
+            S.rkPhi[f * N + c] = S.rkPhi[f * N + s];
+        }
+    }
+    if (first) {
+        S.piph_n[c] = S.piph_n[s];
+        S.dphi_n[c] = S.dphi_n[s];
+        int sm = Sr.off + (i == 0 ? Sr.np - 1 : i - 1);
+        if (S.piph_n[s] < S.piph_n[sm]) S.rot_n[a] = i;
+    }
+}
+
+// movePlanets (Planets/sun.c: none; seeded binary: phi += omega dt) then the
+// planet wrap of clean_pi (misc.c:26-33), t += dt (timestep.c:47), and the
+// opening of the next step's CFL reduction.
+DHD void planets_run(const DevState& S, int r) {
+    const RunDev& Rn = S.runs[r];
+    double Pmax = Rn.ph.phi_max;
+    double dt = S.dt[r];
+    for (int p = 0; p < Rn.ph.npl; ++p) {
+        double phi = S.planet_phi[r * DISCO_MAX_PLANETS + p];
+        if (Rn.planets_mod == DISCO_PLANETS_BINARY) phi += Rn.planets[p].omega * dt;
+        while (phi > Pmax) phi -= Pmax;
+        while (phi < 0.0) phi += Pmax;
+        S.planet_phi[r * DISCO_MAX_PLANETS + p] = phi;
+    }
+    S.t[r] += dt;
+}
+
+}  // namespace disco
+
+#include "ct_cells.h"

@@ -67,6 +67,8 @@ int disco_gpu_step(disco_gpu_t* h, int nsteps);
 /* timestep(dom, dt) with caller-given dt per run (no CFL reduction). */
 int disco_gpu_timestep(disco_gpu_t* h, const double* dt);
 
+/* Waits for the handle's stream; fails if a kernel reported a mesh
+ * configuration it does not replay (see disco_gpu_last_error). */
 int disco_gpu_sync(disco_gpu_t* h);
 
 /* Cell state of run `run` (layout above).  Any pointer may be NULL.
@@ -75,6 +77,15 @@ int disco_gpu_download(disco_gpu_t* h, int run, double* prim, double* cons, doub
                        double* wiph);
 int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* cons, const double* piph,
                      const double* dphi, const double* wiph);
+
+/* Magnetic face fluxes of run `run` (MHD with constrained transport; the
+ * reference's Phi[NUM_FACES] per cell, paul.h:196-216 / bfields.c): Phi[5*ncell]
+ * field-major, slot 0 = the cell's phi face, 1 / 2 = its owned r face in the
+ * inner / outer track, 3 / 4 = its owned z face in the down / up track.
+ * Download replaces reading theCells[..].Phi for output (h5out.c:77-85),
+ * upload the restart path (h5in.c:101-368). */
+int disco_gpu_download_faces(disco_gpu_t* h, int run, double* Phi);
+int disco_gpu_upload_faces(disco_gpu_t* h, int run, const double* Phi);
 
 /* Simulation time t and last dt per run (nruns each, either may be NULL);
  * planets: phi per run x 2. */

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--3d]   (--3d: only the 3D sets)
+usage: python tests/golden/make_fixtures.py [--3d | --mhd | --mhd-full]   (only the named set)
 """
 from __future__ import annotations
 
@@ -97,9 +97,38 @@ def fixtures_3d() -> None:
     save_states("c2e_small.npz", states)
 
 
+def fixtures_mhd() -> None:
+    # small 3D MHD disk (mhd, HLLD, CT_MODE 2, periodic z, Keplerian mesh);
+    # CFL 0.3 keeps the shear shift per step below one cell on this coarse grid
+    txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3)
+    states, _ = run_ref("mhd_sun", txt, 20, dumps=(1, 5))
+    save_states("c2_mhd_small.npz", states)
+
+
+def fixtures_mhd_full() -> None:
+    # full-size 3D MHD disk (5.5 M cells): conserved totals and dt after 1 and
+    # 2 steps (about 90 s of reference CPU time and 3.6 GB of temporary dumps)
+    txt = par_with("c2_mhd3d.par")
+    states, secs = run_ref("mhd_sun", txt, 2, dumps=(1,))
+    np.savez(os.path.join(HERE, "c2_full_totals.npz"),
+             steps=np.array([0, 1, 2]),
+             totals=np.stack([totals(states[s]) for s in (0, 1, 2)]),
+             dt=np.array([states[s]["dt"] for s in (0, 1, 2)]),
+             ncell_real=np.array(int((states[0]["real"] == 1).sum())),
+             ref_seconds_2_steps=np.array(secs))
+
+
 def main() -> None:
     if "--3d" in sys.argv:
         fixtures_3d()
+        print("fixtures written")
+        return
+    if "--mhd" in sys.argv:
+        fixtures_mhd()
+        print("fixtures written")
+        return
+    if "--mhd-full" in sys.argv:
+        fixtures_mhd_full()
         print("fixtures written")
         return
     # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
@@ -125,6 +154,8 @@ def main() -> None:
              ref_seconds_20_steps=np.array(secs))
     grmhd_leaf_vectors()
     fixtures_3d()
+    fixtures_mhd()
+    fixtures_mhd_full()
     print("fixtures written")
 
 

@@ -75,3 +75,71 @@ def state_errors(st: dict, ref: dict, rcell: np.ndarray, mask: np.ndarray) -> di
         sp[5:8] = bscale
         sc[5:8] = np.sqrt((rc[5:8] ** 2).sum(axis=0)).max()
     return {"prim": ep / sp, "cons": ec / sc}
+
+
+def track_partners(piph, offL, nL, offU, nU):
+    """Other cell of every face of the track between annulus L (lower) and U.
+
+    A face is owned by the cell whose front edge ends it (faces_alt.c:140-163,
+    ties: the lower annulus' front first).  For an owner in L the other cell is
+    the first U cell whose front is >= the owner's; for an owner in U, the
+    first L cell whose front is > the owner's.  Returns (other of each L cell's
+    face, other of each U cell's face) as global cell indices."""
+    fL = piph[offL:offL + nL]
+    fU = piph[offU:offU + nU]
+    oL = np.argsort(fL, kind="stable")
+    oU = np.argsort(fU, kind="stable")
+    pU = np.searchsorted(fU[oU], fL, side="left") % nU
+    pL = np.searchsorted(fL[oL], fU, side="right") % nL
+    return offU + oU[pU], offL + oL[pL]
+
+
+def div_b(Phi, piph, Np, Nr, Nz):
+    """Net outward magnetic flux of every cell from the face fluxes, the
+    quantity Python/discopy/geom.py:370-520 (calcDivB) integrates: the phi
+    face at the cell's front minus the one at its back, plus the r and z faces
+    of the tracks on both sides, each face taken once from its owner's slot
+    (Phi[1]/Phi[2]: owned r face in the inner/outer track, Phi[3]/Phi[4]:
+    z faces).  Returns (div, scale) with scale = sum of |face flux| per cell."""
+    N = piph.size
+    off = np.concatenate([[0], np.cumsum(Np)])
+    div = np.zeros(N)
+    scale = np.zeros(N)
+    for jk in range(Nr * Nz):
+        c = np.arange(off[jk], off[jk + 1])
+        cm = np.roll(c, 1)
+        div[c] += Phi[0][c] - Phi[0][cm]
+        scale[c] += np.abs(Phi[0][c]) + np.abs(Phi[0][cm])
+    for k in range(Nz):
+        for j in range(Nr):
+            jk = j + Nr * k
+            tracks = []
+            if j + 1 < Nr:
+                tracks.append((jk + 1, 2, 1))
+            if k + 1 < Nz:
+                tracks.append((jk + Nr, 4, 3))
+            for jk2, sL, sU in tracks:
+                oL, oU = track_partners(piph, off[jk], Np[jk], off[jk2], Np[jk2])
+                cL = np.arange(off[jk], off[jk + 1])
+                cU = np.arange(off[jk2], off[jk2 + 1])
+                f = Phi[sL][cL]  # faces owned by lower cells: outward for them, inward for the other
+                np.add.at(div, cL, f)
+                np.add.at(div, oL, -f)
+                np.add.at(scale, cL, np.abs(f))
+                np.add.at(scale, oL, np.abs(f))
+                f = Phi[sU][cU]
+                np.add.at(div, oU, f)
+                np.add.at(div, cU, -f)
+                np.add.at(scale, oU, np.abs(f))
+                np.add.at(scale, cU, np.abs(f))
+    return div, scale
+
+
+def real_mask(Nr, Nz, NgRa, NgRb, NgZa, NgZb, Np):
+    off = np.concatenate([[0], np.cumsum(Np)])
+    m = np.zeros(int(off[-1]), bool)
+    for k in range(NgZa, Nz - NgZb):
+        for j in range(NgRa, Nr - NgRb):
+            jk = j + Nr * k
+            m[off[jk]:off[jk + 1]] = True
+    return m
