@@ -6,6 +6,9 @@ A step is one full RK2 timestep (getmindt + 2 substeps, timestep.c:7-54) of
 every real cell of the workload, with the state resident in HBM; the value is
 zone-updates/s = real cells x K / wall time of the K graph-launched steps
 (HIP events on the library's stream, plus a device sync on both sides).
+The headline workload is the 3D ideal-MHD disk (HLLD, constrained
+transport, Nr=128 x Nz=64, 5,512,000 real cells); the 2D hydro disk, the
+48-disk batch and the standalone GRMHD cons2prim are reported beside it.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -24,14 +27,27 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-# compulsory HBM bytes of one zone-substep of the 2D/3D hydro update (NUM_Q=5,
-# no CT): read prim, cons, RKcons (15 doubles) + piph, dphi (2); write cons,
-# prim (10) + piph, dphi (2) = 29 doubles
-BYTES_PER_ZONE_SUBSTEP_HYDRO = 232
+# compulsory HBM bytes of one zone-substep of a perfectly fused update:
+# hydro (NUM_Q=5, no CT): read prim, cons, RKcons (15 doubles) + piph, dphi
+# (2); write cons, prim (10) + piph, dphi (2) = 29 doubles = 232 B.
+# MHD with 3D CT (NUM_Q=8, NUM_FACES=5): read prim, cons, RKcons (24) + Phi,
+# RK_Phi (10) + piph, dphi (2); write cons, prim (16) + Phi (5) + piph, dphi
+# (2) = 59 doubles = 472 B.
+BYTES_PER_ZONE_SUBSTEP = {"euler": 232, "mhd": 472}
 
 WORKLOADS = {
+    # 3D cylindrical ideal MHD disk, HLLD + constrained transport, Nr=128 in
+    # r in [1, 4], Nz=64 in z in [-0.5, 0.5] periodic (5,512,000 real cells)
+    "mhd3d_nr128_nz64": ("c2_mhd3d.par", {}, "mhd", "mhd_sun"),
     # 2D cylindrical Newtonian hydro Keplerian disk, Nr=256, r in [0.5, 3]
-    "kepler2d_nr256": ("c1_kepler2d.par", {}),
+    "kepler2d_nr256": ("c1_kepler2d.par", {}, "euler", "euler_sun"),
+}
+# bounded CPU sample of each workload for the reference build: the same grid
+# spacing and physics on a slab of the domain, a few steps
+CPU_SAMPLES = {
+    "mhd3d_nr128_nz64": ({"Num_Z": 8, "Z_Min": -0.0625, "Z_Max": 0.0625}, 3,
+                         "8 of the 64 z-layers (same dz, periodic), 689,000 real cells, 3 RK2 steps"),
+    "kepler2d_nr256": ({}, 20, "full 288,917-cell grid, 20 RK2 steps"),
 }
 
 
@@ -44,22 +60,65 @@ def par_text(base: str, **kv) -> str:
 
 def cpu_baseline(workload: str) -> dict | None:
     """The reference CPU build (oracle/_ref, compiled from the reference's own
-    sources) on a bounded sample of the same workload: the full grid for a
-    fixed number of steps, serial (the reference has no threads)."""
-    base, over = WORKLOADS[workload]
-    exe = os.path.join(ROOT, "oracle", "_ref", "disco_ref_euler_sun")
+    sources) on a bounded sample of the same workload, serial (the reference
+    has no threads), timed around its step loop."""
+    base, over, _, variant = WORKLOADS[workload]
+    samp, nsteps, what = CPU_SAMPLES[workload]
+    exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_{variant}")
     if not os.path.exists(exe):
         return None
-    nsteps = 20
     with tempfile.TemporaryDirectory() as d:
-        open(os.path.join(d, "in.par"), "w").write(par_text(base, **over))
+        open(os.path.join(d, "in.par"), "w").write(par_text(base, **{**over, **samp}))
         out = subprocess.run([exe, str(nsteps), "-"], cwd=d, capture_output=True, text=True, timeout=300).stdout
     m = re.search(r"steps=(\d+) cells=(\d+) seconds=([0-9.eE+-]+)", out)
     if not m:
         return None
     steps, cells, secs = int(m.group(1)), int(m.group(2)), float(m.group(3))
     return {"value": cells * steps / secs, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": f"full {cells}-cell grid, {steps} RK2 steps, {secs:.2f} s (reference C build, gcc -O3, serial)"}
+            "sample": f"{what}: {cells} cells x {steps} steps in {secs:.2f} s (reference C build, gcc -O3, serial)"}
+
+
+def bench_fv(workload: str, steps: int, warmup: int) -> dict:
+    """Zone-updates/s of one workload, its per-kernel times and the roofline
+    of one RK substep (all kernels of the substep) against HBM."""
+    import disco_amd as D
+
+    base, over, hydro, _ = WORKLOADS[workload]
+    dom = D.Domain(D.parse_par(par_text(base, **over)))
+    nreal = sum(i.nreal for i in dom.info)
+    dom.step(warmup)
+    dom.sync()
+    t0 = time.perf_counter()
+    ms = dom.time_steps(steps)
+    dom.sync()
+    wall = time.perf_counter() - t0
+    value = nreal * steps / (ms * 1e-3)
+    # per-kernel durations (HIP events around eager launches on the same stream)
+    kt = dom.kernel_times(5)
+    dom.close()
+    per_step = {"k_dt_final", "k_planets"}
+    sub = [k for k in kt if k not in per_step and not k.startswith(("k_flip", "k_mindt", "k_dt_init"))]
+    # average substep: every substep kernel once, the flip kernels every other substep
+    substep_ms = sum(kt[k][0] for k in sub) + 0.5 * sum(kt[k][0] for k in kt if k.startswith("k_flip"))
+    dominant = max(kt, key=lambda k: kt[k][0])
+    bpz = BYTES_PER_ZONE_SUBSTEP[hydro]
+    achieved = bpz * nreal / (substep_ms * 1e-3) / 1e9
+    # measured HBM bytes of one substep, from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this build (calibrated on a copy kernel; see the
+    # summary the file names)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_substep")
+    return {
+        "value": value, "ms_per_step": ms / steps, "wall_s_timed": wall, "real_cells": nreal,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"one RK substep ({' + '.join(sub)}; flip kernels every other substep)",
+                     "bytes_per_zone_substep": bpz, "substep_ms": substep_ms, "dominant_kernel": dominant},
+        "kernels_ms": {k: v[0] for k, v in kt.items()},
+    }
 
 
 def bench_batch(steps: int, warmup: int) -> dict:
@@ -167,10 +226,11 @@ def bench_c2p(nstates: int, train_steps: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="kepler2d_nr256", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mhd3d_nr128_nz64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-c2p", action="store_true")
     ap.add_argument("--c2p-states", type=int, default=1 << 24)
@@ -194,52 +254,43 @@ def main() -> int:
                                    "(no sharding, no collectives); run with --gpus 1"}))
         return 2
 
-    import disco_amd as D
-
-    base, over = WORKLOADS[args.workload]
-    dom = D.Domain(D.parse_par(par_text(base, **over)))
-    nreal = sum(i.nreal for i in dom.info)
-    dom.step(args.warmup)
-    dom.sync()
-    t0 = time.perf_counter()
-    ms = dom.time_steps(args.steps)
-    dom.sync()
-    wall = time.perf_counter() - t0
-    value = nreal * args.steps / (ms * 1e-3)
-
-    # per-kernel durations (HIP events around eager launches on the same stream)
-    kt = dom.kernel_times(10)
-    sub = ["k_stage_a", "k_plm_r", "k_flux", "k_update"]
-    substep_ms = sum(kt[k][0] for k in sub if k in kt)
-    dominant = max(sub, key=lambda k: kt.get(k, (0, 0))[0])
-    achieved = BYTES_PER_ZONE_SUBSTEP_HYDRO * nreal / (substep_ms * 1e-3) / 1e9
+    wl = args.workload
+    r = bench_fv(wl, args.steps, args.warmup)
+    hydro = WORKLOADS[wl][2]
+    cfg = {"workload": wl, "real_cells": r["real_cells"], "integrator": "RK2", "plm": 1.5,
+           "reference_convention_megazones_per_s": 2 * r["value"] / 1e6}
+    if hydro == "mhd":
+        cfg.update({"physics": "ideal MHD (mhd.c)", "riemann": "HLLD", "ct": "CT_MODE 2 (3D)", "Nr": 128, "Nz": 64,
+                    "z": "periodic", "mesh_motion": "Keplerian, rigid per annulus"})
+    else:
+        cfg.update({"physics": "Newtonian hydro (euler.c)", "riemann": "HLLC"})
     line = {
         "metric": "zone-updates/s (FP64) 1 GPU",
-        "value": value,
+        "value": r["value"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms / args.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Keplerian disk, identical to the reference's initial data)",
-        "config": {"workload": args.workload, "real_cells": nreal, "riemann": "HLLC", "plm": 1.5,
-                   "integrator": "RK2", "reference_convention_megazones_per_s": 2 * value / 1e6},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one RK substep (k_stage_a + k_plm_r + k_flux + k_update)",
-                     "bytes_per_zone_substep": BYTES_PER_ZONE_SUBSTEP_HYDRO,
-                     "substep_ms": substep_ms, "dominant_kernel": dominant},
-        "kernels_ms": {k: v[0] for k, v in kt.items()},
-        "wall_s_timed": wall,
+        "data": "synthetic (seeded disk, identical to the reference's initial data)",
+        "config": cfg,
+        "roofline": r["roofline"],
+        "kernels_ms": r["kernels_ms"],
+        "wall_s_timed": r["wall_s_timed"],
     }
-    dom.close()
-    if not args.no_batch:
+    if not args.no_extra and wl != "kepler2d_nr256":
+        k = bench_fv("kepler2d_nr256", 200, 20)
+        k["workload"] = "kepler2d_nr256"
+        if not args.no_cpu:
+            k["cpu_baseline"] = cpu_baseline("kepler2d_nr256")
+        line["kepler2d"] = k
+    if not args.no_batch and not args.no_extra:
         line["batch"] = bench_batch(args.steps, args.warmup)
-    if not args.no_c2p:
+    if not args.no_c2p and not args.no_extra:
         line["c2p"] = bench_c2p(args.c2p_states, args.c2p_train_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
