@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -41,6 +42,8 @@ struct disco_gpu {
     std::vector<RunDev> hruns;
     std::vector<AnnDev> hann;
     std::vector<int> run_cell0, run_ann0;
+    std::vector<std::vector<int>> cell_perm;  // per run: device position of each cell (reference order)
+    int ktile = 8;                            // z-layers per annulus tile in HBM
     int nruns = 0, ncell = 0, nann = 0, nq = 5;
     AnnDev* ann = nullptr;
     int* cell_ann = nullptr;
@@ -49,7 +52,7 @@ struct disco_gpu {
     double *prim = nullptr, *cons = nullptr, *rkcons = nullptr, *gradp = nullptr, *gradr = nullptr,
            *gradz = nullptr;
     double *piph[2] = {nullptr, nullptr}, *dphi[2] = {nullptr, nullptr}, *wiph = nullptr;
-    int *lbo = nullptr, *ubi = nullptr, *lbu = nullptr, *ubd = nullptr;
+    int *lbo = nullptr, *ubi = nullptr, *lbu = nullptr, *ubd = nullptr, *pdg = nullptr;
     int* rot[2] = {nullptr, nullptr};
     double *fphi = nullptr, *fout = nullptr, *fin = nullptr, *fup = nullptr, *fdn = nullptr;
     // constrained transport (MHD)
@@ -98,6 +101,7 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.ubi = h->ubi;
     S.lbu = h->lbu;
     S.ubd = h->ubd;
+    S.pdg = h->pdg;
     S.fphi = h->fphi;
     S.fout = h->fout;
     S.fin = h->fin;
@@ -149,32 +153,59 @@ static int dalloc(T** p, size_t n) {
 }
 
 static void build_tables(disco_gpu* h) {
-    build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape);
+    build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape, &h->cell_perm,
+                         h->ktile);
     h->ncell = 0;
     for (const HostRun& R : h->runs) h->ncell += R.ncell;
     h->nann = (int)h->hann.size();
 }
 
+// Host arrays are in the reference's (jk, i) order, the device holds each
+// run's annuli in tile order: copy nf fields of run r through the permutation.
+static int put_fields(disco_gpu* h, int r, double* dev, const double* host, int nf) {
+    const HostRun& R = h->runs[r];
+    const std::vector<int>& perm = h->cell_perm[r];
+    size_t c0 = h->run_cell0[r], N = h->ncell, n = R.ncell;
+    std::vector<double> tmp(n);
+    for (int f = 0; f < nf; ++f) {
+        const double* src = host + (size_t)f * n;
+        for (size_t c = 0; c < n; ++c) tmp[perm[c]] = src[c];
+        HIPCHK(hipMemcpy(dev + f * N + c0, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+static int get_fields(disco_gpu* h, int r, double* host, const double* dev, int nf) {
+    const HostRun& R = h->runs[r];
+    const std::vector<int>& perm = h->cell_perm[r];
+    size_t c0 = h->run_cell0[r], N = h->ncell, n = R.ncell;
+    std::vector<double> tmp(n);
+    for (int f = 0; f < nf; ++f) {
+        HIPCHK(hipMemcpy(tmp.data(), dev + f * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+        double* dst = host + (size_t)f * n;
+        for (size_t c = 0; c < n; ++c) dst[c] = tmp[perm[c]];
+    }
+    return 0;
+}
+
+// rotation points in device annulus order
+static int put_rot(disco_gpu* h, int r, int* dev) {
+    const HostRun& R = h->runs[r];
+    int a0 = h->run_ann0[r], na = R.Nr * R.Nz;
+    std::vector<int> tmp(na);
+    for (int a = 0; a < na; ++a) tmp[a] = R.rot[h->hann[a0 + a].j + R.Nr * h->hann[a0 + a].k];
+    HIPCHK(hipMemcpy(dev + a0, tmp.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
 static int upload_run_state(disco_gpu* h, int r) {
     const HostRun& R = h->runs[r];
-    size_t c0 = h->run_cell0[r];
-    size_t N = h->ncell;
-    for (int q = 0; q < h->nq; ++q) {
-        HIPCHK(hipMemcpy(h->prim + q * N + c0, R.prim.data() + (size_t)q * R.ncell, R.ncell * sizeof(double),
-                         hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->cons + q * N + c0, R.cons.data() + (size_t)q * R.ncell, R.ncell * sizeof(double),
-                         hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMemcpy(h->piph[h->parity] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->dphi[h->parity] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->wiph + c0, R.wiph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->rot[h->parity] + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
-    if (h->Phi && !R.Phi.empty()) {
-        for (int f = 0; f < 5; ++f)
-            HIPCHK(hipMemcpy(h->Phi + f * N + c0, R.Phi.data() + (size_t)f * R.ncell, R.ncell * sizeof(double),
-                             hipMemcpyHostToDevice));
-    }
+    if (put_fields(h, r, h->prim, R.prim.data(), h->nq) || put_fields(h, r, h->cons, R.cons.data(), h->nq) ||
+        put_fields(h, r, h->piph[h->parity], R.piph.data(), 1) ||
+        put_fields(h, r, h->dphi[h->parity], R.dphi.data(), 1) || put_fields(h, r, h->wiph, R.wiph.data(), 1) ||
+        put_rot(h, r, h->rot[h->parity]))
+        return -1;
+    if (h->Phi && !R.Phi.empty() && put_fields(h, r, h->Phi, R.Phi.data(), 5)) return -1;
     double pp[DISCO_MAX_PLANETS] = {0.0, 0.0};
     for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) pp[p] = R.planets[p].phi;
     HIPCHK(hipMemcpy(h->planet_phi + r * DISCO_MAX_PLANETS, pp, sizeof(pp), hipMemcpyHostToDevice));
@@ -236,7 +267,7 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot[0],  h->rot[1],     h->fphi,
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
-                    h->efin,  h->fflags,   h->err};
+                    h->efin,  h->fflags,   h->err,   h->pdg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -250,6 +281,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail("no HIP device available");
     disco_gpu* h = new disco_gpu();
     h->nruns = nruns;
+    if (const char* kt = std::getenv("DISCO_ANNULUS_TILE")) h->ktile = std::atoi(kt);
     h->runs.resize(nruns);
     for (int r = 0; r < nruns; ++r) {
         std::string err;
@@ -301,12 +333,14 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         rc |= dalloc(&h->es1, 8 * N);
         rc |= dalloc(&h->efin, 4 * N);
         rc |= dalloc(&h->fflags, N);
+        rc |= dalloc(&h->pdg, 4 * N);
     }
-    rc |= dalloc(&h->fphi, Q);
-    rc |= dalloc(&h->fout, Q);
-    rc |= dalloc(&h->fin, Q);
-    rc |= dalloc(&h->fup, Q);
-    rc |= dalloc(&h->fdn, Q);
+    const size_t QF = 5 * N;  // stored face-increment components (NF)
+    rc |= dalloc(&h->fphi, QF);
+    rc |= dalloc(&h->fout, QF);
+    rc |= dalloc(&h->fin, QF);
+    rc |= dalloc(&h->fup, QF);
+    rc |= dalloc(&h->fdn, QF);
     rc |= dalloc(&h->dtbits, nruns);
     rc |= dalloc(&h->dt, nruns);
     rc |= dalloc(&h->planet_phi, (size_t)nruns * DISCO_MAX_PLANETS);
@@ -326,9 +360,12 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         hipMemcpy(h->druns, h->hruns.data(), nruns * sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup("upload of tables failed");
     // zero the scratch fields once (eager, outside any graph)
-    double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz, h->fphi, h->fout, h->fin, h->fup, h->fdn};
+    double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz};
     for (double* p : zq)
         if (hipMemset(p, 0, Q * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
+    double* zf[] = {h->fphi, h->fout, h->fin, h->fup, h->fdn};
+    for (double* p : zf)
+        if (hipMemset(p, 0, QF * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
     int* zi[] = {h->lbo, h->ubi, h->lbu, h->ubd};
     for (int* p : zi)
         if (hipMemset(p, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
@@ -343,16 +380,14 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         for (auto& z : zc)
             if (hipMemset(z.p, 0, z.n * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
         if (hipMemset(h->fflags, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
+        if (hipMemset(h->pdg, 0, 4 * N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
     }
     for (int r = 0; r < nruns; ++r) {
         if (upload_run_state(h, r)) return cleanup(g_err);
         // the other mesh buffer starts equal (substep 2 reads it)
-        size_t c0 = h->run_cell0[r];
         const HostRun& R = h->runs[r];
-        if (hipMemcpy(h->piph[1] + c0, R.piph.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->dphi[1] + c0, R.dphi.data(), R.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->rot[1] + h->run_ann0[r], R.rot.data(), R.rot.size() * sizeof(int), hipMemcpyHostToDevice) !=
-                hipSuccess)
+        if (put_fields(h, r, h->piph[1], R.piph.data(), 1) || put_fields(h, r, h->dphi[1], R.dphi.data(), 1) ||
+            put_rot(h, r, h->rot[1]))
             return cleanup("upload");
     }
     if (capture_graphs(h)) return cleanup(g_err);
@@ -456,15 +491,11 @@ int disco_gpu_download(disco_gpu_t* h, int run, double* prim, double* cons, doub
                        double* wiph) {
     if (!h || run < 0 || run >= h->nruns) return fail("disco_gpu_download: bad arguments");
     HIPCHK(hipStreamSynchronize(h->stream));
-    const HostRun& R = h->runs[run];
-    size_t c0 = h->run_cell0[run], N = h->ncell, n = R.ncell;
-    for (int q = 0; q < h->nq; ++q) {
-        if (prim) HIPCHK(hipMemcpy(prim + q * n, h->prim + q * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
-        if (cons) HIPCHK(hipMemcpy(cons + q * n, h->cons + q * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
-    }
-    if (piph) HIPCHK(hipMemcpy(piph, h->piph[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (dphi) HIPCHK(hipMemcpy(dphi, h->dphi[h->parity] + c0, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (wiph) HIPCHK(hipMemcpy(wiph, h->wiph + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (prim && get_fields(h, run, prim, h->prim, h->nq)) return -1;
+    if (cons && get_fields(h, run, cons, h->cons, h->nq)) return -1;
+    if (piph && get_fields(h, run, piph, h->piph[h->parity], 1)) return -1;
+    if (dphi && get_fields(h, run, dphi, h->dphi[h->parity], 1)) return -1;
+    if (wiph && get_fields(h, run, wiph, h->wiph, 1)) return -1;
     return check_err(h);
 }
 
@@ -472,10 +503,7 @@ int disco_gpu_download_faces(disco_gpu_t* h, int run, double* Phi) {
     if (!h || run < 0 || run >= h->nruns || !Phi) return fail("disco_gpu_download_faces: bad arguments");
     if (!h->Phi) return fail("disco_gpu_download_faces: the run has no face fluxes (not MHD)");
     HIPCHK(hipStreamSynchronize(h->stream));
-    const HostRun& R = h->runs[run];
-    size_t c0 = h->run_cell0[run], N = h->ncell, n = R.ncell;
-    for (int f = 0; f < 5; ++f)
-        HIPCHK(hipMemcpy(Phi + f * n, h->Phi + f * N + c0, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (get_fields(h, run, Phi, h->Phi, 5)) return -1;
     return check_err(h);
 }
 
@@ -486,10 +514,7 @@ int disco_gpu_upload_faces(disco_gpu_t* h, int run, const double* Phi) {
     HostRun& R = h->runs[run];
     size_t n = R.ncell;
     std::memcpy(R.Phi.data(), Phi, 5 * n * sizeof(double));
-    size_t c0 = h->run_cell0[run], N = h->ncell;
-    for (int f = 0; f < 5; ++f)
-        HIPCHK(hipMemcpy(h->Phi + f * N + c0, Phi + f * n, n * sizeof(double), hipMemcpyHostToDevice));
-    return 0;
+    return put_fields(h, run, h->Phi, R.Phi.data(), 5);
 }
 
 int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* cons, const double* piph,

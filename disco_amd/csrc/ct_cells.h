@@ -344,11 +344,14 @@ DHD double edge_segment(const DevState& S, const int* cur, int w4, double r, dou
     return E * dl * dt;
 }
 
-// Number of fronts of annulus `a` (rank qr in the corner) that precede the
-// front value v of rank vr in the 4-way order (ties: lower rank first).
-DHD int corner_count(const DevState& S, int a, int qr, double v, int vr) {
-    const AnnDev& Q = S.ann[a];
-    return front_count(S.piph, Q.off, Q.np, S.rot[a], v, qr > vr);
+// Partner of cell x in the annulus at offset (dj, dk) from x's own: the
+// first cell of that annulus whose front does not precede x's front in the
+// (k, j) order of the merges (stage A's binary searches).
+DHD int nbr_part(const DevState& S, int x, int dj, int dk) {
+    if (dk == 0) return dj > 0 ? S.lbo[x] : S.ubi[x];
+    if (dj == 0) return dk > 0 ? S.lbu[x] : S.ubd[x];
+    int d = (dk > 0 ? 0 : 2) + (dj > 0 ? 0 : 1);
+    return S.pdg[(size_t)d * S.ncell + x];
 }
 
 DHD int corner_cell(const DevState& S, int a, int pos) {
@@ -358,10 +361,11 @@ DHD int corner_cell(const DevState& S, int a, int pos) {
 
 // Sum of E dl dt over the segments of corner K inside face F, whose two
 // cells are pc0 (rank p0) and pc1 (rank p1), which ends at the front of its
-// owner (rank orank) and starts at the front fprev of rank prank.  The other
-// two annuli (ranks q0 < q1) contribute the fronts that split the face.
-DHD double edge_face_sum(const DevState& S, const Corner& K, int p0, int p1, int pc0, int pc1, int orank,
-                         double fown, int prank, double fprev, double dt, double Pmax) {
+// owner X (rank orank) and starts at the front of cell P (rank prank).  The
+// other two annuli (ranks q0 < q1) contribute the fronts that split the
+// face; their positions come from X's and P's partners in those annuli.
+DHD double edge_face_sum(const DevState& S, const Corner& K, int p0, int p1, int pc0, int pc1, int X, int orank,
+                         int Pc, int prank, double dt, double Pmax) {
     int q0, q1;
     if (p0 == 0 && p1 == 1) { q0 = 2; q1 = 3; }
     else if (p0 == 2 && p1 == 3) { q0 = 0; q1 = 1; }
@@ -369,10 +373,12 @@ DHD double edge_face_sum(const DevState& S, const Corner& K, int p0, int p1, int
     else { q0 = 0; q1 = 2; }
     const int aq0 = K.ann[q0], aq1 = K.ann[q1];
     const int n0 = S.ann[aq0].np, n1 = S.ann[aq1].np;
-    int a0 = corner_count(S, aq0, q0, fprev, prank);
-    int b0 = corner_count(S, aq0, q0, fown, orank);
-    int a1 = corner_count(S, aq1, q1, fprev, prank);
-    int b1 = corner_count(S, aq1, q1, fown, orank);
+    const int r0 = S.rot[aq0], r1 = S.rot[aq1];
+    // rank r sits at (dj, dk) = (r & 1, r >> 1) in the corner
+    int a0 = wrap_idx(nbr_part(S, Pc, (q0 & 1) - (prank & 1), (q0 >> 1) - (prank >> 1)) - r0, n0);
+    int b0 = wrap_idx(nbr_part(S, X, (q0 & 1) - (orank & 1), (q0 >> 1) - (orank >> 1)) - r0, n0);
+    int a1 = wrap_idx(nbr_part(S, Pc, (q1 & 1) - (prank & 1), (q1 >> 1) - (prank >> 1)) - r1, n1);
+    int b1 = wrap_idx(nbr_part(S, X, (q1 & 1) - (orank & 1), (q1 >> 1) - (orank >> 1)) - r1, n1);
     int m0 = wrap_idx(b0 - a0, n0);
     int m1 = wrap_idx(b1 - a1, n1);
     int cur[4];
@@ -436,8 +442,6 @@ DHD double ct_face_update(const DevState& S, const AnnDev& A, int c, int side, d
     int ucell = side == 0 ? other : c;
     const AnnDev& L = S.ann[la];
     const AnnDev& U = S.ann[ua];
-    double fown = S.piph[c];
-    double fprev = S.piph[Pv.cell];
     // ranks of the owner / predecessor within the track pair (0 = lower annulus)
     int o_up = side;                     // owner in the upper annulus of the track?
     int p_up = Pv.side;
@@ -447,22 +451,22 @@ DHD double ct_face_update(const DevState& S, const AnnDev& A, int c, int side, d
         // bottom edge corner (j,k-1) [U]
         if (L.up >= 0 && U.up >= 0) {
             Corner K{{la, ua, L.up, U.up}, L.rp};
-            sum += edge_face_sum(S, K, 0, 1, lcell, ucell, o_up, fown, p_up, fprev, dt, Pmax);
+            sum += edge_face_sum(S, K, 0, 1, lcell, ucell, c, o_up, Pv.cell, p_up, dt, Pmax);
         }
         if (L.down >= 0 && U.down >= 0) {
             Corner K{{L.down, U.down, la, ua}, L.rp};
-            sum -= edge_face_sum(S, K, 2, 3, lcell, ucell, 2 + o_up, fown, 2 + p_up, fprev, dt, Pmax);
+            sum -= edge_face_sum(S, K, 2, 3, lcell, ucell, c, 2 + o_up, Pv.cell, 2 + p_up, dt, Pmax);
         }
     } else {
         // z face between L=(j,k) and U=(j,k+1): edge at r_jph[j] is corner
         // (j,k) [L], edge at r_jph[j-1] is corner (j-1,k) [R]
         if (L.outer >= 0 && U.outer >= 0) {
             Corner K{{la, L.outer, ua, U.outer}, L.rp};
-            sum -= edge_face_sum(S, K, 0, 2, lcell, ucell, 2 * o_up, fown, 2 * p_up, fprev, dt, Pmax);
+            sum -= edge_face_sum(S, K, 0, 2, lcell, ucell, c, 2 * o_up, Pv.cell, 2 * p_up, dt, Pmax);
         }
         if (L.inner >= 0 && U.inner >= 0) {
             Corner K{{L.inner, la, U.inner, ua}, L.rm};
-            sum += edge_face_sum(S, K, 1, 3, lcell, ucell, 1 + 2 * o_up, fown, 1 + 2 * p_up, fprev, dt, Pmax);
+            sum += edge_face_sum(S, K, 1, 3, lcell, ucell, c, 1 + 2 * o_up, Pv.cell, 1 + 2 * p_up, dt, Pmax);
         }
     }
     return delta + sum;

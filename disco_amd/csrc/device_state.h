@@ -40,6 +40,7 @@ struct AnnDev {
     int up;     // global annulus index of (j, k+1), -1 if none
     int down;   // global annulus index of (j, k-1), -1 if none
     int zsrc;   // periodic-z ghosts: the real annulus they copy, else -1
+    int diag[4];  // annuli (j+1,k+1), (j-1,k+1), (j+1,k-1), (j-1,k-1), -1 if none
     int flags;
     double rm, rp, zm, zp;  // r_jph[j-1], r_jph[j], z_kph[k-1], z_kph[k]
     double rc, zc;          // r and z centroids (geometry_functions.c:5-10)
@@ -81,6 +82,7 @@ struct DevState {
     int* ubi;            // per cell: partner in the inner annulus (array index)
     int* lbu;            // per cell: partner in the up annulus
     int* ubd;            // per cell: partner in the down annulus
+    int* pdg;            // [4][ncell] MHD with CT: partners in the diagonal annuli (order of AnnDev::diag)
     const int* rot;      // per annulus: array index of the smallest front (mesh at the start of the substep)
     int* rot_n;          // per annulus: the same after move_cells (written by the first substep)
     double* fphi;        // per cell: increment of its phi face (i | i+1)

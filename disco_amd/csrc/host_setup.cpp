@@ -746,8 +746,10 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
 // Annulus and run tables of a batch (device_state.h): neighbours, static
 // geometry, and the flags that mark which sweeps touch each annulus.
 void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
-                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape) {
+                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape,
+                          std::vector<std::vector<int>>* cell_perm, int ktile) {
     hann->clear();
+    if (cell_perm) cell_perm->clear();
     hruns->clear();
     run_cell0->clear();
     run_ann0->clear();
@@ -794,6 +796,29 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
             pkmin = R.NgZa == 0 ? 0 : R.NgZa - 1;
             pkmax = R.NgZb == 0 ? R.Nz : R.Nz - R.NgZb + 1;
         }
+        // device order of the annuli: tiles of ktile z-layers, r inside a
+        // tile, z fastest, so both the r- and the z-neighbours of a cell lie
+        // a few annuli away in HBM (ktile <= 1: the reference's jk order)
+        std::vector<int> pos(R.Nr * R.Nz), doff(R.Nr * R.Nz);
+        {
+            int kt = ktile > 1 ? ktile : 1;
+            int p = 0, o = 0;
+            for (int kb = 0; kb < R.Nz; kb += kt)
+                for (int j = 0; j < R.Nr; ++j)
+                    for (int k = kb; k < kb + kt && k < R.Nz; ++k) {
+                        int jk = j + R.Nr * k;
+                        pos[jk] = p++;
+                        doff[jk] = o;
+                        o += R.Np[jk];
+                    }
+        }
+        if (cell_perm) {
+            std::vector<int> perm(R.ncell);
+            for (int jk = 0; jk < R.Nr * R.Nz; ++jk)
+                for (int i = 0; i < R.Np[jk]; ++i) perm[R.off[jk] + i] = doff[jk] + i;
+            cell_perm->push_back(std::move(perm));
+        }
+        std::vector<AnnDev> runann(R.Nr * R.Nz);
         bool zper = R.Nz > 1 && R.par.Z_Periodic;
         int Ng = R.NgZa;
         for (int k = 0; k < R.Nz; ++k)
@@ -804,12 +829,19 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                 A.j = j;
                 A.k = k;
                 A.np = R.Np[jk];
-                A.off = cell0 + R.off[jk];
-                A.outer = (j + 1 < R.Nr) ? ann0 + jk + 1 : -1;
-                A.inner = (j > 0) ? ann0 + jk - 1 : -1;
-                A.up = (k + 1 < R.Nz) ? ann0 + jk + R.Nr : -1;
-                A.down = (k > 0) ? ann0 + jk - R.Nr : -1;
+                A.off = cell0 + doff[jk];
+                A.outer = (j + 1 < R.Nr) ? ann0 + pos[jk + 1] : -1;
+                A.inner = (j > 0) ? ann0 + pos[jk - 1] : -1;
+                A.up = (k + 1 < R.Nz) ? ann0 + pos[jk + R.Nr] : -1;
+                A.down = (k > 0) ? ann0 + pos[jk - R.Nr] : -1;
                 A.zsrc = -1;
+                {
+                    const int dj[4] = {1, -1, 1, -1}, dk[4] = {1, 1, -1, -1};
+                    for (int d = 0; d < 4; ++d) {
+                        int jj = j + dj[d], kk = k + dk[d];
+                        A.diag[d] = (jj >= 0 && jj < R.Nr && kk >= 0 && kk < R.Nz) ? ann0 + pos[jj + R.Nr * kk] : -1;
+                    }
+                }
                 int f = 0;
                 bool kin = k >= kmin && k < kmax;
                 if (R.real_annulus(j, k)) f |= ANN_REAL;
@@ -827,7 +859,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                 // Nz-2Ng+k, ghost k >= Nz-Ng copies layer k-(Nz-2Ng)
                 if (zper && (k < Ng || k >= R.Nz - Ng)) {
                     int ks = k < Ng ? k + R.Nz - 2 * Ng : k - (R.Nz - 2 * Ng);
-                    A.zsrc = ann0 + j + R.Nr * ks;
+                    A.zsrc = ann0 + pos[j + R.Nr * ks];
                     f |= ANN_ZCOPY;
                 }
                 A.flags = f;
@@ -857,8 +889,9 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                     A.rc_b = A.rc;
                     A.zc_b = A.zc;
                 }
-                hann->push_back(A);
+                runann[pos[jk]] = A;
             }
+        hann->insert(hann->end(), runann.begin(), runann.end());
         cell0 += R.ncell;
         ann0 += R.Nr * R.Nz;
     }

@@ -57,8 +57,12 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic);
 // boundary_trans for the fixed / fixed3d modules on the host (ghost prims).
 void host_boundary(HostRun* run);
 
-// Annulus/run tables of a batch and the launch shape of its substeps.
+// Annulus/run tables of a batch and the launch shape of its substeps.  The
+// device stores annuli in tiles of `ktile` z-layers; cell_perm[r][c] is the
+// device position (within run r) of the run's cell c in the reference's
+// (jk, i) order.
 void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
-                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape);
+                          std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape,
+                          std::vector<std::vector<int>>* cell_perm, int ktile);
 
 }  // namespace disco

@@ -8,7 +8,7 @@
 // EMF averaging and the Stokes update between the fluxes and the update, and
 // splits the update around the flux flips of the moving mesh:
 //   k_stage_a -> k_plm_trans -> k_flux -> k_emf_sum -> k_emf_avg1 ->
-//   k_emf_face -> k_ct_update -> k_update_a -> [k_flip_flag -> k_flip_chain]
+//   k_emf_face -> k_ct_update_a -> [k_flip_flag -> k_flip_chain]
 //   -> k_update_b (+ k_zcopy, k_planets)
 // The CFL reduction of the next step is fused into the last substep's update.
 #include <hip/hip_runtime.h>
@@ -18,6 +18,21 @@
 namespace disco {
 
 #define DEV __device__ static inline __attribute__((always_inline))
+
+// Minimum waves per SIMD requested for the register-heavy kernels (0: the
+// compiler's choice).  The FP64 Riemann solvers and slope gathers need
+// 130-180 VGPRs, i.e. 2-3 waves per SIMD, which leaves little latency hiding
+// for their neighbour gathers; the build may trade registers for occupancy.
+#ifndef DISCO_OCC_FLUX
+#define DISCO_OCC_FLUX 3
+#endif
+#ifndef DISCO_OCC_PLM
+#define DISCO_OCC_PLM 0
+#endif
+#ifndef DISCO_OCC_CT
+#define DISCO_OCC_CT 4
+#endif
+#define DISCO_OCC(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, 8)))
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Remap so that XCD x works on one contiguous band of cells: partner cells in
@@ -109,13 +124,13 @@ __global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int firs
 }
 
 template <class Ph>
-__global__ __launch_bounds__(256) void k_plm_trans(DevState S, int has_z) {
+__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_PLM) void k_plm_trans(DevState S, int has_z) {
     int c = cell_index();
     if (c < S.ncell) plm_trans_cell<Ph>(S, c, has_z);
 }
 
 template <class Ph>
-__global__ __launch_bounds__(256) void k_flux(DevState S, double dtfac) {
+__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_FLUX) void k_flux(DevState S, double dtfac) {
     int c = cell_index();
     if (c < S.ncell) flux_cell<Ph>(S, c, dtfac);
 }
@@ -134,15 +149,6 @@ __global__ __launch_bounds__(256) void k_update(DevState S, double dtfac, int fi
         dtc = update_b_cell<Ph>(S, c, first, want_dt, cons);
     }
     if (want_dt) dt_reduce(S, run, dtc);
-}
-
-template <class Ph>
-__global__ __launch_bounds__(256) void k_update_a(DevState S, double dtfac, int first) {
-    int c = cell_index();
-    if (c < S.ncell) {
-        double cons[Ph::NQ];
-        update_a_cell<Ph>(S, c, dtfac, first, cons);
-    }
 }
 
 template <class Ph>
@@ -183,9 +189,18 @@ __global__ __launch_bounds__(256) void k_emf_face(DevState S, int has_z) {
     if (c < S.ncell) emf_face_cell(S, c, has_z);
 }
 
-__global__ __launch_bounds__(256) void k_ct_update(DevState S, double dtfac, int has_z) {
+// update_B_fluxes + make_edge_adjust (face fluxes) fused with the first half
+// of the cell update (conserved state, sources, mesh move): independent work
+// on the same cell.
+template <class Ph>
+__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_CT) void k_ct_update_a(DevState S, double dtfac, int first,
+                                                                           int has_z) {
     int c = cell_index();
-    if (c < S.ncell) ct_update_cell(S, c, dtfac, has_z);
+    if (c < S.ncell) {
+        ct_update_cell(S, c, dtfac, has_z);
+        double cons[Ph::NQ];
+        update_a_cell<Ph>(S, c, dtfac, first, cons);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_flip_flag(DevState S, int has_z) {
@@ -238,8 +253,7 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, g, dim3(B), st, S, sh.has_z);
             DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, g, dim3(B), st, S);
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, g, dim3(B), st, S, sh.has_z);
-            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, g, dim3(B), st, S, dtfac, sh.has_z);
-            DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, g, dim3(B), st, S, dtfac, first);
+            DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, g, dim3(B), st, S, dtfac, first, sh.has_z);
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, g, dim3(B), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, g, dim3(B), st, S, sh.has_z);

@@ -43,6 +43,7 @@ namespace disco {
 // The Hydro module's per-face and per-cell leaves (hydro.h:6-35) as policies.
 struct PhysEuler {
     static constexpr int NQ = 5;
+    static constexpr int NF = 5;  // stored face-increment components
     static constexpr bool MHD = false;
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
@@ -64,6 +65,10 @@ struct PhysEuler {
 
 struct PhysMHD {
     static constexpr int NQ = 8;
+    // with constrained transport the cell field is rebuilt from the face
+    // fluxes (B_faces_to_cells), so the face increments of cons[BRR..BZZ]
+    // (riemann.c:314-317) are never used: only the first five are stored
+    static constexpr int NF = 5;
     static constexpr bool MHD = true;
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
@@ -310,6 +315,16 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
         const AnnDev& W = S.ann[A.down];
         S.ubd[c] = front_search(S.piph, W.off, W.np, S.rot[A.down], front, false);
     }
+    if (Ph::MHD && S.pdg) {
+        // diagonal annuli of the azimuthal-edge merges; a front of an annulus
+        // higher in (k, j) order counts after an equal front of this one
+        _Pragma("unroll") for (int d = 0; d < 4; ++d) {
+            int a = A.diag[d];
+            if (a < 0) continue;
+            const AnnDev& Q = S.ann[a];
+            S.pdg[(size_t)d * N + c] = front_search(S.piph, Q.off, Q.np, S.rot[a], front, d < 2);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ plm_trans
@@ -496,7 +511,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
         FaceEMF e = t_face_flux<Ph, D>(S, P, c, cb, F, A, dt, dU);
         double* f = Dr::f_hi(S);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) f[q * N + c] = dU[q];
         if (Ph::MHD && S.emf_t) {
             S.emf_t[(Dr::EMF_HI + 0) * N + c] = e.E1;
             S.emf_t[(Dr::EMF_HI + 1) * N + c] = e.B1;
@@ -509,7 +524,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
         FaceEMF e = t_face_flux<Ph, D>(S, P, ca, c, F, W, dt, dU);
         double* f = Dr::f_lo(S);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) f[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) f[q * N + c] = dU[q];
         if (Ph::MHD && S.emf_t) {
             S.emf_t[(Dr::EMF_LO + 0) * N + c] = e.E1;
             S.emf_t[(Dr::EMF_LO + 1) * N + c] = e.B1;
@@ -549,7 +564,7 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
         double dA = (A.rp - A.rm) * (A.zp - A.zm);  // get_dA(xp, xm, 0)
         double w = A.rc * S.wiph[c];                // get_scale_factor(x, 0) * wiph
         FaceEMF e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.fphi[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) S.fphi[q * N + c] = dU[q];
         if (Ph::MHD && S.emf_phi) {
             S.emf_phi[0 * N + c] = e.E1;  // E_z
             S.emf_phi[1 * N + c] = e.B1;  // B_r
@@ -628,17 +643,18 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
             prim[q] = S.prim[q * N + c];
         }
         // phi faces i-1 (this cell is R) and i (this cell is L)
+        constexpr int NF = Ph::NF;
         if (A.flags & ANN_PHIFLUX) {
             if (i == 0) {
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] += S.fphi[q * N + cm];
             } else {
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += S.fphi[q * N + cm];
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] -= S.fphi[q * N + c];
             }
         }
-        gather_trans<NQ, DIR_R>(S, A, c, cm, cons);
-        gather_trans<NQ, DIR_Z>(S, A, c, cm, cons);
+        gather_trans<NF, DIR_R>(S, A, c, cm, cons);
+        gather_trans<NF, DIR_Z>(S, A, c, cm, cons);
         // add_source with the pre-move geometry and the substep's prims
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
