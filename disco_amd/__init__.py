@@ -259,12 +259,12 @@ class Domain:
                                         _dptr(wiph)), "disco_gpu_download")
         out = dict(prim=prim.reshape(I.nq, I.ncell), cons=cons.reshape(I.nq, I.ncell), piph=piph, dphi=dphi,
                    wiph=wiph)
-        if self.params[run].hydro == HYDRO["mhd"]:
+        if self.params[run].hydro in (HYDRO["mhd"], HYDRO["grmhd"]):
             out["Phi"] = self.faces(run)
         return out
 
     def faces(self, run: int = 0) -> np.ndarray:
-        """Magnetic face fluxes Phi[5, ncell] (MHD with constrained transport)."""
+        """Magnetic face fluxes Phi[5, ncell] (MHD or GRMHD with constrained transport)."""
         I = self.info[run]
         Phi = np.zeros(5 * I.ncell)
         _check(lib().disco_gpu_download_faces(self._h, run, _dptr(Phi)), "disco_gpu_download_faces")

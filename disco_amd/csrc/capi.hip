@@ -295,7 +295,8 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         }
     }
     h->nq = h->runs[0].nq;
-    const bool mhd = h->runs[0].par.hydro == DISCO_HYDRO_MHD;
+    // constrained-transport buffers: ideal MHD and GRMHD
+    const bool mhd = h->runs[0].par.hydro == DISCO_HYDRO_MHD || h->runs[0].par.hydro == DISCO_HYDRO_GRMHD;
     build_tables(h);
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;

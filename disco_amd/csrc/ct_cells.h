@@ -539,12 +539,18 @@ DHD void flip_one(const DevState& S, FaceId F, bool next_toggled) {
 // least three faces apart are independent.  The thread of the first flip of
 // each chain replays the chain; a chain that runs across the start of the
 // face list is replayed by the thread of its first face after the start, its
-// part before the end of the list last.  Adjacent flipped faces cannot occur
-// when every annulus rotates rigidly and no front overtakes two others in a
-// step; they are reported in S.err instead of being replayed.
+// part before the end of the list last.  When two neighbouring annuli hold
+// the same number of cells, rigid rotation moves every front of one past the
+// fronts of the other in the same step: every other face of the track flips
+// and the chain closes on itself.  Its head (the first flip after the start)
+// then replays the whole track in list order, the last flip reading the face
+// after it (the head's predecessor) through the pointer the head's flip
+// toggled.  Adjacent flipped faces cannot occur when every annulus rotates
+// rigidly and no front overtakes two others in a step; they are reported in
+// S.err instead of being replayed.
 template <int D>
 DHD void flip_chain_face(const DevState& S, FaceId F) {
-    const int MAXCHAIN = 64;
+    const int MAXCHAIN = 1 << 16;
     FaceId z0 = face_zero<D>(S, F);
     FaceId n1 = face_next<D>(S, F);
     if (face_flag<D>(S, n1)) {
@@ -560,6 +566,7 @@ DHD void flip_chain_face(const DevState& S, FaceId F) {
     // forward part of the chain
     FaceId M = F;
     int len = 1;
+    bool closed = false;
     for (;;) {
         FaceId m1 = face_next<D>(S, M);
         FaceId m2 = face_next<D>(S, m1);
@@ -567,8 +574,10 @@ DHD void flip_chain_face(const DevState& S, FaceId F) {
         bool cross = same_face(m1, z0) || same_face(m2, z0);
         if (cross) {
             if (!wrap_head) return;  // continues into the list start: replayed by that chain's head
-            err_or(S.err, 1);      // a chain all around the track
-            return;
+            // m2 is this head (a flagged face at the start that is not the
+            // head would sit next to it): the chain goes all around the track
+            closed = true;
+            break;
         }
         M = m2;
         if (++len > MAXCHAIN) {
@@ -579,10 +588,10 @@ DHD void flip_chain_face(const DevState& S, FaceId F) {
     // replay: forward part in order
     M = F;
     for (int k = 0; k < len; ++k) {
-        flip_one<D>(S, M, false);
+        flip_one<D>(S, M, closed && k == len - 1);
         M = face_next<D>(S, face_next<D>(S, M));
     }
-    if (!wrap_head) return;
+    if (!wrap_head || closed) return;
     // the part before the end of the list: from its first flip up to p2
     FaceId T = p2;
     int tl = 1;

@@ -30,7 +30,7 @@ enum { DISCO_BC_FIXED = 0, DISCO_BC_FIXED3D = 1 };
  * equal-mass binary modelled on Planets/cmA.c */
 enum { DISCO_PLANETS_NONE = 0, DISCO_PLANETS_SUN = 1, DISCO_PLANETS_BINARY = 2 };
 /* Initial module (INITIAL slot); the set is this project's own config ICs */
-enum { DISCO_IC_KEPLER_NOISE = 0, DISCO_IC_ISO_BINARY_DISK = 1, DISCO_IC_MHD_DISK = 2 };
+enum { DISCO_IC_KEPLER_NOISE = 0, DISCO_IC_ISO_BINARY_DISK = 1, DISCO_IC_MHD_DISK = 2, DISCO_IC_GR_TORUS = 3 };
 /* Riemann solvers, riemann.c:1 */
 enum { DISCO_HLL = 0, DISCO_HLLC = 1, DISCO_HLLD = 2 };
 

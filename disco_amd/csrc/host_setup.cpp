@@ -9,6 +9,7 @@
 
 #include "phys_euler.h"
 #include "phys_mhd.h"
+#include "phys_grmhd_fv.h"
 
 namespace disco {
 
@@ -140,6 +141,7 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     if (!read_var(text, "Initial", 2, s)) {
         if (!std::strcmp(s, "iso_binary_disk")) p->initial = DISCO_IC_ISO_BINARY_DISK;
         else if (!std::strcmp(s, "mhd_disk")) p->initial = DISCO_IC_MHD_DISK;
+        else if (!std::strcmp(s, "gr_torus")) p->initial = DISCO_IC_GR_TORUS;
         else if (std::strcmp(s, "kepler_noise")) { ++e; missing += " Initial=" + std::string(s); }
     }
     p->num_c = p->hydro == DISCO_HYDRO_EULER ? 5 : 8;
@@ -183,23 +185,30 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->grav2D = par.grav2D;
     ph->num_c = par.num_c;
     ph->num_n = par.num_n;
+    ph->gr_M = par.metricPar2;
+    ph->gr_a = par.metricPar3;
     disco_ic_from_params(&par, ic);
 }
 
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
-    if (p.hydro == DISCO_HYDRO_GRMHD) m += " Hydro=grmhd (the FV step is built for euler and mhd)";
     if (p.num_n != 0) m += " Num_N>0";
     if (p.hydro == DISCO_HYDRO_EULER && p.num_c != 5) m += " Num_C!=5 for euler";
-    if (p.hydro == DISCO_HYDRO_MHD) {
-        if (p.num_c != 8) m += " Num_C!=8 for mhd";
+    if (p.hydro == DISCO_HYDRO_MHD || p.hydro == DISCO_HYDRO_GRMHD) {
+        if (p.num_c != 8) m += " Num_C!=8 for mhd/grmhd";
         // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
-        if (!p.CT) m += " mhd needs Constrained_Transport 1";
-        if (p.Num_Z < 2) m += " mhd with CT_MODE 2 needs Num_Z > 1";
+        if (!p.CT) m += " mhd/grmhd need Constrained_Transport 1";
+        if (p.Num_Z < 2) m += " mhd/grmhd with CT_MODE 2 need Num_Z > 1";
         if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
-            m += " mhd needs rigidly rotating annuli (Mesh_Motion 0 or 4) for its flux flips";
-    } else if (p.Riemann_Solver == DISCO_HLLD) {
+            m += " mhd/grmhd need rigidly rotating annuli (Mesh_Motion 0 or 4) for their flux flips";
+    }
+    if (p.hydro == DISCO_HYDRO_GRMHD) {
+        // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one
+        if (p.Riemann_Solver == DISCO_HLLD) m += " Riemann_Solver=HLLD with grmhd";
+        if (p.isothermal_flag) m += " isothermal grmhd (cons2prim_solve_isothermal is a stub)";
+        if (p.metricPar2 <= 0.0 || std::fabs(p.metricPar3) >= p.metricPar2) m += " Kerr needs M > 0 and |a| < M";
+    } else if (p.hydro == DISCO_HYDRO_EULER && p.Riemann_Solver == DISCO_HLLD) {
         m += " Riemann_Solver=HLLD needs mhd (riemann.c:19-22)";
     }
     if (p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) m += " NoBC_*";
@@ -674,6 +683,29 @@ static void host_set_B_fields(HostRun& R) {
     host_b_faces_to_cells(R, f1, f2, 1);
 }
 
+// The Hydro module's prim2cons / cons2prim for one cell (hydro.h:6-12).
+static void host_prim2cons(const HostRun& R, const double* prim, double* cons, const double* x, double dV) {
+    if (R.par.hydro == DISCO_HYDRO_MHD) {
+        mhd::prim2cons(R.ph, prim, cons, x, dV);
+    } else if (R.par.hydro == DISCO_HYDRO_GRMHD) {
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
+        grmhd::prim2cons(R.ph.gamma_law, g, prim, cons, x, dV);
+    } else {
+        euler::prim2cons<5>(R.ph, prim, cons, x, dV);
+    }
+}
+
+// prim holds the cell's current prims on entry (the GRMHD Newton guess).
+static void host_cons2prim(const HostRun& R, const double* cons, double* prim, const double* x, double dV) {
+    if (R.par.hydro == DISCO_HYDRO_MHD)
+        mhd::cons2prim(R.ph, cons, prim, x, dV);
+    else if (R.par.hydro == DISCO_HYDRO_GRMHD)
+        grmhd::cons2prim(R.ph, cons, prim, x, dV);
+    else
+        euler::cons2prim<5>(R.ph, cons, prim, x, dV);
+}
+
 int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     HostRun& R = *Rp;
     R = HostRun();
@@ -688,7 +720,7 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     R.prim.assign((size_t)R.nq * R.ncell, 0.0);
     R.cons.assign((size_t)R.nq * R.ncell, 0.0);
     R.wiph.assign(R.ncell, 0.0);
-    const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD;
+    const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD || par.hydro == DISCO_HYDRO_GRMHD;
     // domain.c:163-209: initial data and prim2cons on real cells
     double prim[8], cons[8];
     for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
@@ -702,10 +734,7 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 disco_initial(&R.ic, prim, x);
                 if (par.RotFrame) prim[UPP] -= par.RotOmega;
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
-                if (mhd_ct)
-                    mhd::prim2cons(R.ph, prim, cons, x, dV);
-                else
-                    euler::prim2cons<5>(R.ph, prim, cons, x, dV);
+                host_prim2cons(R, prim, cons, x, dV);
                 for (int q = 0; q < R.nq; ++q) R.cons[(size_t)q * R.ncell + c] = cons[q];
                 R.nreal += 1;
             }
@@ -725,10 +754,9 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 cell_x(R, j, k, c, x, xp, xm);
                 double dV = get_dV(xp, xm, R.par.phimax);
                 for (int q = 0; q < R.nq; ++q) cons[q] = R.cons[(size_t)q * R.ncell + c];
-                if (mhd_ct)
-                    mhd::cons2prim(R.ph, cons, prim, x, dV);
-                else
-                    euler::cons2prim<5>(R.ph, cons, prim, x, dV);
+                // calc_prim: the GRMHD solve starts from the cell's current prims
+                for (int q = 0; q < R.nq; ++q) prim[q] = R.prim[(size_t)q * R.ncell + c];
+                host_cons2prim(R, cons, prim, x, dV);
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
             }
         }
@@ -774,7 +802,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         // phi_flux and trans_flux ranges (misc.c:393-407, 459-500): r-tracks
         // of real z-layers, z-tracks of real r-columns; with constrained
         // transport every range grows by one ghost layer across the track
-        bool ct = R.par.hydro == DISCO_HYDRO_MHD && R.par.CT;
+        bool ct = (R.par.hydro == DISCO_HYDRO_MHD || R.par.hydro == DISCO_HYDRO_GRMHD) && R.par.CT;
         int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
         int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
         int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
@@ -898,7 +926,9 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
     shape->has_z = 0;
     shape->zcopy = 0;
-    shape->ct = (!runs.empty() && runs[0].par.hydro == DISCO_HYDRO_MHD) ? 1 : 0;
+    shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
+                    ? 1
+                    : 0;
     for (const HostRun& R : runs) {
         if (R.Nz > 1) shape->has_z = 1;
         if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;

@@ -275,6 +275,8 @@ void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook) 
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook) {
     if (sh.hydro == DISCO_HYDRO_MHD)
         Launch<PhysMHD>::mindt(S, st, hook);
+    else if (sh.hydro == DISCO_HYDRO_GRMHD)
+        Launch<PhysGRMHD>::mindt(S, st, hook);
     else
         Launch<PhysEuler>::mindt(S, st, hook);
 }
@@ -283,6 +285,8 @@ void launch_substep(const StepShape& sh, const DevState& S, double RK, double dt
                     hipStream_t st, const LaunchHook* hook) {
     if (sh.hydro == DISCO_HYDRO_MHD)
         Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
+    else if (sh.hydro == DISCO_HYDRO_GRMHD)
+        Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
     else
         Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
 }

@@ -34,6 +34,7 @@ struct PhysParams {
     int grav2D;
     int npl;
     int num_c, num_n;
+    double gr_M, gr_a;  // Kerr-Schild mass and spin (Metric_Par2, Metric_Par3)
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face
@@ -47,6 +48,17 @@ struct Planet {
     int type;
     int pad;
 };
+
+// a + b * c with the product rounded on its own, as a compiler that does not
+// contract to a fused multiply-add evaluates it (the reference's host build):
+// keeps the mesh positions bit-identical to the reference's.
+DHD double add_mul_rn(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __dadd_rn(a, __dmul_rn(b, c));
+#else
+    return a + b * c;
+#endif
+}
 
 // ---------------------------------------------------------------- geometry
 // cylindrical.c:10-15

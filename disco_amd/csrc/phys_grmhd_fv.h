@@ -1,0 +1,587 @@
+// GRMHD module, finite-volume leaves: flux, wave speeds, HLLC star state,
+// geometric sources, CFL step and EMFs, host/device.
+//
+// Follows Hydro/grmhd.c:157-226 (getUstar), :228-245
+// (cons2prim), :247-337 (flux), :339-454 (source), :463-488 (flux_to_E),
+// :489-646 (vel), :648-721 (mindt); the metric derivatives of
+// Hydro/Metric/schw_ks_rpz.c:86-189 (metric_der_g, metric_der_lapse,
+// metric_der_shift, metric_killing) and Hydro/Frame/normal.c:21-42
+// (frame_der_U).  The metric is Kerr-Schild in cylindrical coordinates
+// (phys_grmhd.h); at spin a = 0 the derivatives are the reference's
+// closed-form expressions, for a != 0 (no reference exists) they come from
+// forward-mode dual numbers through the same Kerr-Schild formulas.
+// Each face or cell evaluates the metric once and hands it to every leaf;
+// the reference re-evaluates it per call, with the same values.
+#pragma once
+
+#include "phys_common.h"
+#include "phys_grmhd.h"
+
+namespace disco {
+namespace grmhd {
+
+// ------------------------------------------------------------------ duals
+struct Dual {
+    double v, d;
+};
+DHD Dual operator+(Dual a, Dual b) { return Dual{a.v + b.v, a.d + b.d}; }
+DHD Dual operator-(Dual a, Dual b) { return Dual{a.v - b.v, a.d - b.d}; }
+DHD Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.d * b.v + a.v * b.d}; }
+DHD Dual operator/(Dual a, Dual b) { return Dual{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+DHD Dual operator+(double a, Dual b) { return Dual{a + b.v, b.d}; }
+DHD Dual operator-(double a, Dual b) { return Dual{a - b.v, -b.d}; }
+DHD Dual operator*(double a, Dual b) { return Dual{a * b.v, a * b.d}; }
+DHD Dual operator/(double a, Dual b) { return Dual{a / b.v, -a * b.d / (b.v * b.v)}; }
+DHD Dual dsqrt(Dual a) {
+    double s = sqrt(a.v);
+    return Dual{s, a.d / (2.0 * s)};
+}
+
+// Lapse, shift^i and the 4-metric g_{mu nu} (t, R, phi, z) of Kerr-Schild
+// at (R, z) with derivative seeds in R and z: g = eta + H l l, l_mu =
+// (1, rs R/(rs^2+a^2), -a R^2/(rs^2+a^2), z/rs), alpha = 1/sqrt(1+H),
+// beta^i = H l^i / (1+H).
+DHD void kerr_parts_dual(Dual R, Dual z, double M, double a, Dual* al, Dual* be, Dual* g4) {
+    Dual rho2 = R * R + z * z;
+    Dual b = 0.5 * (rho2 - Dual{a * a, 0.0});
+    Dual rs = dsqrt(b + dsqrt(b * b + (a * a) * (z * z)));
+    Dual rs2 = rs * rs;
+    Dual H = (2.0 * M) * rs2 * rs / (rs2 * rs2 + (a * a) * (z * z));
+    Dual den = rs2 + Dual{a * a, 0.0};
+    Dual l[4] = {Dual{1.0, 0.0}, rs * R / den, (-a) * R * R / den, z / rs};
+    Dual eta[4] = {Dual{-1.0, 0.0}, Dual{1.0, 0.0}, R * R, Dual{1.0, 0.0}};
+    for (int m = 0; m < 4; ++m)
+        for (int n = 0; n < 4; ++n) g4[4 * m + n] = (m == n ? eta[m] : Dual{0.0, 0.0}) + H * l[m] * l[n];
+    Dual lu[3] = {l[1], l[2] / (R * R), l[3]};
+    Dual f = H / (1.0 + H);
+    *al = 1.0 / dsqrt(1.0 + H);
+    for (int i = 0; i < 3; ++i) be[i] = f * lu[i];
+}
+
+// Derivatives of the metric at x: dg[i][16] = d_i g_{mu nu} for i = r, z
+// (metric_der_g with i = 1, 3), da[4] = d_mu alpha, db[12] = d_mu beta^j
+// (t and phi are Killing directions: zero).
+struct MetricDer {
+    double dg_r[16], dg_z[16];
+    double da[4];
+    double db[12];
+};
+
+DHD void metric_der(const double* x, double M, double a, MetricDer* D) {
+    for (int k = 0; k < 16; ++k) D->dg_r[k] = D->dg_z[k] = 0.0;
+    for (int k = 0; k < 4; ++k) D->da[k] = 0.0;
+    for (int k = 0; k < 12; ++k) D->db[k] = 0.0;
+    double r = x[0];
+    double z = x[2];
+    if (a == 0.0) {
+        // schw_ks_rpz.c:86-182
+        double R = sqrt(r * r + z * z);
+        double st = r / R;
+        double ct = z / R;
+        double MoR2 = M / (R * R);
+        double MoR3 = M / (R * R * R);
+        double MoR4 = M / (R * R * R * R);
+        double* dg = D->dg_r;
+        dg[0] = -2 * MoR2 * st;
+        dg[1] = 2 * MoR2 * (1.0 - 2 * st * st);
+        dg[3] = -4 * MoR3 * z * st;
+        dg[5] = 2 * MoR3 * r * (2.0 - 3.0 * st * st);
+        dg[7] = 2 * MoR3 * z * (1.0 - 3.0 * st * st);
+        dg[10] = 2 * r;
+        dg[15] = -6 * MoR4 * z * z * st;
+        dg[4] = dg[1];
+        dg[12] = dg[3];
+        dg[13] = dg[7];
+        dg = D->dg_z;
+        dg[0] = -2 * MoR2 * ct;
+        dg[1] = -4 * MoR3 * r * ct;
+        dg[3] = 2 * MoR2 * (1.0 - 2 * ct * ct);
+        dg[5] = -6 * MoR4 * r * r * ct;
+        dg[7] = 2 * MoR3 * r * (1.0 - 3.0 * ct * ct);
+        dg[15] = 2 * MoR3 * z * (2.0 - 3.0 * ct * ct);
+        dg[4] = dg[1];
+        dg[12] = dg[3];
+        dg[13] = dg[7];
+        double al = 1.0 / sqrt(1.0 + 2 * M / R);
+        D->da[1] = al * al * al * MoR2 * st;
+        D->da[3] = al * al * al * MoR2 * ct;
+        double denom = 1.0 / (R * R + 2 * M * R);
+        double d_denom = -denom * denom * 2 * (R + M);
+        D->db[3] = 2 * M * denom + 2 * M * r * d_denom * st;
+        D->db[4] = 0.0;
+        D->db[5] = 2 * M * z * d_denom * st;
+        D->db[9] = 2 * M * r * d_denom * ct;
+        D->db[10] = 0.0;
+        D->db[11] = 2 * M * denom + 2 * M * z * d_denom * ct;
+        return;
+    }
+    for (int dir = 0; dir < 2; ++dir) {
+        Dual R{r, dir == 0 ? 1.0 : 0.0};
+        Dual Z{z, dir == 1 ? 1.0 : 0.0};
+        Dual al, be[3], g4[16];
+        kerr_parts_dual(R, Z, M, a, &al, be, g4);
+        int mu = dir == 0 ? 1 : 3;
+        double* dg = dir == 0 ? D->dg_r : D->dg_z;
+        for (int k = 0; k < 16; ++k) dg[k] = g4[k].d;
+        D->da[mu] = al.d;
+        for (int j = 0; j < 3; ++j) D->db[3 * mu + j] = be[j].d;
+    }
+}
+
+// frame_der_U (Frame/normal.c:21-42): dU[4 mu + nu] = d_mu U^nu
+DHD void frame_der_U(const Metric& g, const MetricDer& D, double* dU) {
+    double ia = 1.0 / g.al;
+    for (int i = 0; i < 16; i++) dU[i] = 0.0;
+    for (int i = 1; i < 4; i += 2) {  // non-Killing: r, z
+        dU[4 * i] = -ia * ia * D.da[i];
+        for (int j = 0; j < 3; j++) dU[4 * i + j + 1] = g.be[j] * ia * ia * D.da[i] - D.db[3 * i + j] * ia;
+    }
+}
+
+// ------------------------------------------------------------------ leaves
+// grmhd.c:247-337
+DHD void flux(double gamma_law, const Metric& g, const double* prim, double* F, const double* x, const double* n) {
+    double r = x[0];
+    const double al = g.al;
+    const double* be = g.be;
+    const double* gam = g.gam;
+    const double* igam = g.igam;
+    const double* U = g.U;
+    double jac = g.jac;
+    double sqrtgam = jac / al;
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
+    double igaml[3];
+    for (int i = 0; i < 3; i++) {
+        igaml[i] = 0.0;
+        for (int j = 0; j < 3; j++) igaml[i] += igam[3 * i + j] * l[j];
+    }
+    double u2 = l[0] * igaml[0] + l[1] * igaml[1] + l[2] * igaml[2];
+    double w = sqrt(1.0 + u2);
+    double u0 = w / al;
+    double u[3], v[3];
+    for (int i = 0; i < 3; i++) u[i] = igaml[i] - be[i] * u0;
+    for (int i = 0; i < 3; i++) v[i] = u[i] / u0;
+    double l0 = -al * w + be[0] * l[0] + be[1] * l[1] + be[2] * l[2];
+    double uU = U[0] * l0 + U[1] * l[0] + U[2] * l[1] + U[3] * l[2];
+    double uB = l[0] * B[0] + l[1] * B[1] + l[2] * B[2];
+    double bd[3];
+    bd[0] = (gam[0] * B[0] + gam[1] * B[1] + gam[2] * B[2] + l[0] * uB) / w;
+    bd[1] = (gam[3] * B[0] + gam[4] * B[1] + gam[5] * B[2] + l[1] * uB) / w;
+    bd[2] = (gam[6] * B[0] + gam[7] * B[1] + gam[8] * B[2] + l[2] * uB) / w;
+    double B2 = 0.0, UB = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            B2 += gam[3 * i + j] * B[i] * B[j];
+            UB += gam[3 * i + j] * B[i] * (U[0] * be[j] + U[j + 1]);
+        }
+    double b2 = (B2 + uB * uB) / (w * w);
+    double Ub = (UB + uU * uB) / w;
+    double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    double Bn = B[0] * n[0] + B[1] * n[1] + B[2] * n[2];
+    double Un = U[1] * n[0] + U[2] * n[1] + U[3] * n[2];
+    double vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+    double hn = n[0] + r * n[1] + n[2];
+    double bn = (Bn + uB * un) / w;
+    double rhoh = rho + gamma_law / (gamma_law - 1.0) * Pp + b2;
+    double rhoe = Pp / (gamma_law - 1.0);
+    double Pt = Pp + 0.5 * b2;
+    F[DDD] = jac * hn * rho * un;
+    F[SRR] = jac * hn * (rhoh * un * l[0] + Pt * n[0] - bd[0] * bn);
+    F[LLL] = jac * hn * (rhoh * un * l[1] + Pt * n[1] - bd[1] * bn);
+    F[SZZ] = jac * hn * (rhoh * un * l[2] + Pt * n[2] - bd[2] * bn);
+    F[TAU] = jac * hn * (-(rhoe + 0.5 * b2) * uU * un - Pt * (uU * un + Un) - rho * (uU + 1.0) * un + Ub * bn);
+    F[BRR] = sqrtgam * hn * (B[0] * vn - Bn * v[0]) / r;
+    F[BPP] = sqrtgam * hn * (B[1] * vn - Bn * v[1]);
+    F[BZZ] = sqrtgam * hn * (B[2] * vn - Bn * v[2]);
+}
+
+// grmhd.c:157-226 (HLLC star state of the fluid part; Bpack unused)
+DHD void getUstar(double gamma_law, const Metric& g, const double* prim, double* Ustar, const double* x, double Sk,
+                  double Ss, const double* n) {
+    double r = x[0];
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
+    const double al = g.al;
+    const double* be = g.be;
+    const double* igam = g.igam;
+    const double* U = g.U;
+    double jac = g.jac;
+    double bn = n[0] * be[0] + n[1] * be[1] + n[2] * be[2];
+    double ign = n[0] * igam[0] + n[1] * igam[4] + n[2] * igam[8];
+    double Un = n[0] * U[1] + n[1] * U[2] + n[2] * U[3];
+    double hn = n[0] + r * n[1] + n[2];
+    double ss = Ss / hn;
+    double sk = Sk / hn;
+    double uS[3];
+    double u2 = 0.0;
+    for (int i = 0; i < 3; i++) {
+        uS[i] = 0.0;
+        for (int j = 0; j < 3; j++) uS[i] += igam[3 * i + j] * l[j];
+        u2 += l[i] * uS[i];
+    }
+    double w = sqrt(1 + u2);
+    double u0 = w / al;
+    double l0 = -al * w + be[0] * l[0] + be[1] * l[1] + be[2] * l[2];
+    double vn = al * (n[0] * uS[0] + n[1] * uS[1] + n[2] * uS[2]) / w - bn;
+    double uU = l0 * U[0] + l[0] * U[1] + l[1] * U[2] + l[2] * U[3];
+    double rhoh = rho + gamma_law / (gamma_law - 1.0) * Pp;
+    double qE = rhoh * w * w * (vn - sk) + Pp * (sk + bn);
+    double mn = rhoh * w * (n[0] * uS[0] + n[1] * uS[1] + n[2] * uS[2]);
+    double qMn = mn * (vn - sk) + al * Pp * ign;
+    double ssS = (ss + bn) / al;
+    double skS = (sk + bn) / al;
+    double Pstar = (qMn - ssS * qE) / (al * (ign - ssS * skS));
+    double kappa = (vn - sk) / (ss - sk);
+    double alpha1 = (Pp - Pstar) / (ss - sk);
+    double alpha2 = (vn * Pp - ss * Pstar) / (ss - sk);
+    double rhoe = Pp / (gamma_law - 1.0);
+    double tau = -rhoe * uU * u0 - Pp * (u0 * uU + U[0]) - rho * (uU + 1.0) * u0;
+    Ustar[DDD] = jac * rho * u0 * kappa;
+    Ustar[SRR] = jac * (rhoh * u0 * l[0] * kappa + alpha1 * n[0]);
+    Ustar[LLL] = jac * (rhoh * u0 * l[1] * kappa + alpha1 * n[1]);
+    Ustar[SZZ] = jac * (rhoh * u0 * l[2] * kappa + alpha1 * n[2]);
+    Ustar[TAU] = jac * (tau * kappa - Un * alpha1 + U[0] * alpha2);
+    Ustar[BRR] = Ustar[BPP] = Ustar[BZZ] = 0.0;
+}
+
+// grmhd.c:489-646: fast-magnetosonic bounds and the HLLC contact speed
+DHD void vel(double gamma_law, const Metric& g, const double* prim1, const double* prim2, double* Sl, double* Sr,
+             double* Ss, const double* n, const double* x) {
+    double r = x[0];
+    const double al = g.al;
+    const double* be = g.be;
+    const double* gam = g.gam;
+    const double* igam = g.igam;
+    double sqrtgam = x[0] / (al * r);  // metric_jacobian / (al r), as grmhd.c:499 rounds it
+    double rho1 = prim1[RHO];
+    double P1 = prim1[PPP];
+    double l1[3] = {prim1[URR], r * r * prim1[UPP], prim1[UZZ]};
+    double B1[3] = {prim1[BRR] / sqrtgam, prim1[BPP] / (r * sqrtgam), prim1[BZZ] / sqrtgam};
+    double cs21 = gamma_law * P1 / (rho1 + gamma_law / (gamma_law - 1.0) * P1);
+    double rho2 = prim2[RHO];
+    double P2 = prim2[PPP];
+    double l2[3] = {prim2[URR], r * r * prim2[UPP], prim2[UZZ]};
+    double B2[3] = {prim2[BRR] / sqrtgam, prim2[BPP] / (r * sqrtgam), prim2[BZZ] / sqrtgam};
+    double cs22 = gamma_law * P2 / (rho2 + gamma_law / (gamma_law - 1.0) * P2);
+    double u21 = 0.0, u22 = 0.0;
+    double uS1[3], uS2[3];
+    for (int i = 0; i < 3; i++) {
+        uS1[i] = 0.0;
+        uS2[i] = 0.0;
+        for (int j = 0; j < 3; j++) {
+            uS1[i] += igam[3 * i + j] * l1[j];
+            uS2[i] += igam[3 * i + j] * l2[j];
+        }
+        u21 += uS1[i] * l1[i];
+        u22 += uS2[i] * l2[i];
+    }
+    double w1 = sqrt(1.0 + u21);
+    double w2 = sqrt(1.0 + u22);
+    double v21 = u21 / (w1 * w1);
+    double v22 = u22 / (w2 * w2);
+    double uB1 = l1[0] * B1[0] + l1[1] * B1[1] + l1[2] * B1[2];
+    double uB2 = l2[0] * B2[0] + l2[1] * B2[1] + l2[2] * B2[2];
+    double B21 = 0.0, B22 = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            B21 += gam[3 * i + j] * B1[i] * B1[j];
+            B22 += gam[3 * i + j] * B2[i] * B2[j];
+        }
+    double b21 = (B21 + uB1 * uB1) / (w1 * w1);
+    double b22 = (B22 + uB2 * uB2) / (w2 * w2);
+    double va21 = b21 / (rho1 + gamma_law / (gamma_law - 1.0) * P1 + b21);
+    double va22 = b22 / (rho2 + gamma_law / (gamma_law - 1.0) * P2 + b22);
+    double cf21 = cs21 + va21 - cs21 * va21;
+    double cf22 = cs22 + va22 - cs22 * va22;
+    double vSn1 = (uS1[0] * n[0] + uS1[1] * n[1] + uS1[2] * n[2]) / w1;
+    double vSn2 = (uS2[0] * n[0] + uS2[1] * n[1] + uS2[2] * n[2]) / w2;
+    double bn = (be[0] * n[0] + be[1] * n[1] + be[2] * n[2]);
+    double ign = igam[3 * 0 + 0] * n[0] + igam[3 * 1 + 1] * n[1] + igam[3 * 2 + 2] * n[2];
+    double dv1 = sqrt(cf21 * (ign - vSn1 * vSn1 - cf21 * (ign * v21 - vSn1 * vSn1))) / w1;
+    double dv2 = sqrt(cf22 * (ign - vSn2 * vSn2 - cf22 * (ign * v22 - vSn2 * vSn2))) / w2;
+    double hn = n[0] + r * n[1] + n[2];
+    double sl1 = hn * (al * (vSn1 * (1.0 - cf21) - dv1) / (1.0 - v21 * cf21) - bn);
+    double sr1 = hn * (al * (vSn1 * (1.0 - cf21) + dv1) / (1.0 - v21 * cf21) - bn);
+    double sl2 = hn * (al * (vSn2 * (1.0 - cf22) - dv2) / (1.0 - v22 * cf22) - bn);
+    double sr2 = hn * (al * (vSn2 * (1.0 - cf22) + dv2) / (1.0 - v22 * cf22) - bn);
+    *Sr = sr1 > sr2 ? sr1 : sr2;
+    *Sl = sl1 < sl2 ? sl1 : sl2;
+    double sL = *Sl / hn;
+    double sR = *Sr / hn;
+    double rhohL = rho1 + gamma_law / (gamma_law - 1) * P1;
+    double rhohR = rho2 + gamma_law / (gamma_law - 1) * P2;
+    double vnL = al * vSn1 - bn;
+    double vnR = al * vSn2 - bn;
+    double ML[3] = {rhohL * w1 * l1[0], rhohL * w1 * l1[1], rhohL * w1 * l1[2]};
+    double MR[3] = {rhohR * w2 * l2[0], rhohR * w2 * l2[1], rhohR * w2 * l2[2]};
+    double EL = rhohL * w1 * w1 - P1;
+    double ER = rhohR * w2 * w2 - P2;
+    double FML[3] = {ML[0] * vnL + n[0] * al * P1, ML[1] * vnL + n[1] * al * P1, ML[2] * vnL + n[2] * al * P1};
+    double FMR[3] = {MR[0] * vnR + n[0] * al * P2, MR[1] * vnR + n[1] * al * P2, MR[2] * vnR + n[2] * al * P2};
+    double FEL = EL * vnL + P1 * (vnL + bn);
+    double FER = ER * vnR + P2 * (vnR + bn);
+    double UE = (sR * ER - sL * EL - FER + FEL) / (sR - sL);
+    double FE = ((sR * FEL - sL * FER + sL * sR * (ER - EL)) / (sR - sL) - bn * UE) / al;
+    double UM = 0.0, FM = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double UMh = (sR * MR[i] - sL * ML[i] - FMR[i] + FML[i]) / (sR - sL);
+        double FMh = ((sR * FML[i] - sL * FMR[i] + sL * sR * (MR[i] - ML[i])) / (sR - sL) - bn * UMh) / al;
+        double igi = n[0] * igam[3 * i] + n[1] * igam[3 * i + 1] + n[2] * igam[3 * i + 2];
+        UM += igi * UMh;
+        FM += igi * FMh;
+    }
+    double A = FE;
+    double Bq = -FM - ign * UE;
+    double C = ign * UM;
+    double sS;
+    if (fabs(4 * A * C / (Bq * Bq)) < 1.0e-7)
+        sS = -C / Bq * (1.0 + A * C / (Bq * Bq) + 2 * A * A * C * C / (Bq * Bq * Bq * Bq));
+    else
+        sS = (-Bq - sqrt(Bq * Bq - 4 * A * C)) / (2 * A);
+    *Ss = hn * (al * sS - bn);
+}
+
+// grmhd.c:463-488
+DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, int dim) {
+    double r = x[0];
+    FaceEMF e;
+    if (dim == 0) {
+        e.E1 = Flux[BRR] * r;      // E_z
+        e.B1 = Ustr[BRR] * r * r;  // r B_r
+        e.E2 = Flux[BZZ];          // E_r
+        e.B2 = Ustr[BZZ] * r;      // -r B_z
+    } else if (dim == 1) {
+        e.E1 = -Flux[BPP] * r;     // E_z
+        e.B1 = Ustr[BRR] * r * r;  // r B_r
+        e.E2 = 1.0 * Flux[BZZ];    // E_phi
+        e.B2 = 0.0;
+    } else {
+        e.E1 = -Flux[BPP] * r;        // E_r
+        e.B1 = Ustr[BZZ] * r;         // -r B_z
+        e.E2 = 1.0 * -Flux[BRR] * r;  // E_phi
+        e.B2 = 0.0;
+    }
+    return e;
+}
+
+// solve_riemann (riemann.c:216-320) for GRMHD with HLL or HLLC (HLLD needs
+// grmhd2.c's calc_msfast and is not part of this module); one metric
+// evaluation at the face centroid serves every leaf.
+DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
+                    const double* n, double w, double dAdt, int dim, double* dU) {
+    const double gl = P.gamma_law;
+    Metric g;
+    kerr_metric(x, P.gr_M, P.gr_a, &g);
+    double Flux[NQ], Ustr[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        Flux[q] = 0.0;
+        Ustr[q] = 0.0;
+    }
+    double Sl, Sr, Ss;
+    vel(gl, g, primL, primR, &Sl, &Sr, &Ss, n, x);
+    if (w < Sl) {
+        flux(gl, g, primL, Flux, x, n);
+        prim2cons(gl, g, primL, Ustr, x, 1.0);
+    } else if (w > Sr) {
+        flux(gl, g, primR, Flux, x, n);
+        prim2cons(gl, g, primR, Ustr, x, 1.0);
+    } else if (P.riemann == DISCO_HLL) {
+        double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
+        double aL = Sr;
+        double aR = -Sl;
+        prim2cons(gl, g, primL, Ul, x, 1.0);
+        prim2cons(gl, g, primR, Ur, x, 1.0);
+        flux(gl, g, primL, Fl, x, n);
+        flux(gl, g, primR, Fr, x, n);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+            Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
+            Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+        }
+    } else {
+        double Uk[NQ], Fk[NQ];
+        if (w < Ss) {
+            prim2cons(gl, g, primL, Uk, x, 1.0);
+            getUstar(gl, g, primL, Ustr, x, Sl, Ss, n);
+            flux(gl, g, primL, Fk, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
+        } else {
+            prim2cons(gl, g, primR, Uk, x, 1.0);
+            getUstar(gl, g, primR, Ustr, x, Sr, Ss, n);
+            flux(gl, g, primR, Fk, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
+        }
+    }
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+    return flux_to_E(Flux, Ustr, x, dim);
+}
+
+// grmhd.c:339-454: geometric source terms at the cell's coordinate centre
+DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                double dVdt) {
+    const double gamma_law = P.gamma_law;
+    double x[3] = {0.5 * (xm[0] + xp[0]), 0.5 * (xm[1] + xp[1]), 0.5 * (xm[2] + xp[2])};
+    double r = x[0];
+    Metric g;
+    kerr_metric(x, P.gr_M, P.gr_a, &g);
+    MetricDer Dm;
+    metric_der(x, P.gr_M, P.gr_a, &Dm);
+    double dU[16];
+    frame_der_U(g, Dm, dU);
+    const double al = g.al;
+    const double* be = g.be;
+    const double* gam = g.gam;
+    const double* igam = g.igam;
+    const double* U = g.U;
+    double jac = g.jac;
+    double sqrtgam = jac / al;
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double l[4] = {0.0, prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
+    double ia2 = 1.0 / (al * al);
+    double ig[16];
+    ig[0] = -ia2;
+    for (int mu = 0; mu < 3; mu++) {
+        ig[mu + 1] = be[mu] * ia2;
+        ig[4 * (mu + 1)] = ig[mu + 1];
+        for (int nu = 0; nu < 3; nu++) ig[4 * (mu + 1) + nu + 1] = igam[3 * mu + nu] - be[mu] * be[nu] * ia2;
+    }
+    double igaml[3];
+    igaml[0] = igam[0] * l[1] + igam[1] * l[2] + igam[2] * l[3];
+    igaml[1] = igam[3] * l[1] + igam[4] * l[2] + igam[5] * l[3];
+    igaml[2] = igam[6] * l[1] + igam[7] * l[2] + igam[8] * l[3];
+    double u2 = l[1] * igaml[0] + l[2] * igaml[1] + l[3] * igaml[2];
+    double w = sqrt(1.0 + u2);
+    double u[4];
+    u[0] = w / al;
+    for (int i = 0; i < 3; i++) u[i + 1] = igaml[i] - be[i] * u[0];
+    l[0] = -al * w + be[0] * l[1] + be[1] * l[2] + be[2] * l[3];
+    double uB = l[1] * B[0] + l[2] * B[1] + l[3] * B[2];
+    double b[4], bd[4];
+    b[0] = uB / al;
+    b[1] = (B[0] + u[1] * uB) / w;
+    b[2] = (B[1] + u[2] * uB) / w;
+    b[3] = (B[2] + u[3] * uB) / w;
+    bd[1] = (gam[0] * B[0] + gam[1] * B[1] + gam[2] * B[2] + l[1] * uB) / w;
+    bd[2] = (gam[3] * B[0] + gam[4] * B[1] + gam[5] * B[2] + l[2] * uB) / w;
+    bd[3] = (gam[6] * B[0] + gam[7] * B[1] + gam[8] * B[2] + l[3] * uB) / w;
+    double B2 = 0.0, beB = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            beB += gam[3 * i + j] * be[i] * B[j];
+            B2 += gam[3 * i + j] * B[i] * B[j];
+        }
+    bd[0] = (beB + l[0] * uB) / w;
+    double b2 = (B2 + uB * uB) / (w * w);
+    double rhoh = rho + gamma_law / (gamma_law - 1.0) * Pp + b2;
+    double Pt = Pp + 0.5 * b2;
+    double rp = xp[0];
+    double rm = xm[0];
+    double r2_3 = (rp * rp + rp * rm + rm * rm) / 3.0;
+    double dphi = get_dp(xp[1], xm[1], P.phi_max);
+    double rcorr = r2_3 / (r * r);
+    double pcorr = sin(0.5 * dphi) / (0.5 * dphi);
+    double Sk[3];
+    for (int i = 0; i < 3; i++) {
+        Sk[i] = 0.0;
+        if (i == 1) continue;  // metric_killing(phi)
+        const double* dg = i == 0 ? Dm.dg_r : Dm.dg_z;
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = 0; nu < 4; nu++)
+                Sk[i] += (rhoh * u[mu] * u[nu] * rcorr * pcorr + ig[4 * mu + nu] * Pt - b[mu] * b[nu]) * dg[4 * mu + nu];
+        Sk[i] *= 0.5;
+    }
+    double S0 = -U[1] * Sk[0] - U[2] * Sk[1] - U[3] * Sk[2];
+    for (int mu = 0; mu <= 3; mu++)
+        for (int nu = 0; nu <= 3; nu++) {
+            if (mu == nu)
+                S0 += -(rhoh * u[mu] * l[nu] + Pt - b[mu] * bd[nu]) * dU[4 * mu + nu];
+            else
+                S0 += -(rhoh * u[mu] * l[nu] - b[mu] * bd[nu]) * dU[4 * mu + nu];
+        }
+    cons[SRR] += jac * Sk[0] * dVdt;
+    cons[LLL] += jac * Sk[1] * dVdt;
+    cons[SZZ] += jac * Sk[2] * dVdt;
+    cons[TAU] += jac * S0 * dVdt;
+}
+
+// grmhd.c:648-721
+DHD double mindt(const PhysParams& P, const double* prim, double wc, const double* xp, const double* xm) {
+    const double gamma_law = P.gamma_law;
+    double x[3] = {0.5 * (xm[0] + xp[0]), 0.5 * (xm[1] + xp[1]), 0.5 * (xm[2] + xp[2])};
+    double r = x[0];
+    Metric g;
+    kerr_metric(x, P.gr_M, P.gr_a, &g);
+    const double al = g.al;
+    const double* be = g.be;
+    const double* gam = g.gam;
+    const double* igam = g.igam;
+    double sqrtgam = x[0] / (r * al);  // grmhd.c:655
+    double rho = prim[RHO];
+    double Pp = prim[PPP];
+    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
+    double cs = sqrt(gamma_law * Pp / (rho + gamma_law / (gamma_law - 1) * Pp));
+    double uS[3];
+    double u2 = 0.0;
+    for (int i = 0; i < 3; i++) {
+        uS[i] = 0.0;
+        for (int j = 0; j < 3; j++) uS[i] += igam[3 * i + j] * l[j];
+        u2 += uS[i] * l[i];
+    }
+    double w = sqrt(1.0 + u2);
+    double vS[3];
+    for (int i = 0; i < 3; i++) vS[i] = uS[i] / w;
+    double v2 = u2 / (w * w);
+    double uB = l[0] * B[0] + l[1] * B[1] + l[2] * B[2];
+    double B2 = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) B2 += gam[3 * i + j] * B[i] * B[j];
+    double b2 = (B2 + uB * uB) / (w * w);
+    double va2 = b2 / (rho + gamma_law / (gamma_law - 1.0) * Pp + b2);
+    double cf = sqrt(cs * cs + va2 - cs * cs * va2);
+    double sig = 1 - cf * cf;
+    double dvr = cf * sqrt(igam[0] * (1 - cf * cf * v2) - sig * vS[0] * vS[0]) / w;
+    double vrl = fabs(al * (vS[0] * sig - dvr) / (1 - v2 * cf * cf) - be[0]);
+    double vrr = fabs(al * (vS[0] * sig + dvr) / (1 - v2 * cf * cf) - be[0]);
+    double dvp = cf * sqrt(igam[4] * (1 - cf * cf * v2) - sig * vS[1] * vS[1]) / w;
+    double vpl = fabs(r * (al * (vS[1] * sig - dvp) / (1 - v2 * cf * cf) - be[1] - wc));
+    double vpr = fabs(r * (al * (vS[1] * sig + dvp) / (1 - v2 * cf * cf) - be[1] - wc));
+    double dvz = cf * sqrt(igam[8] * (1 - cf * cf * v2) - sig * vS[2] * vS[2]) / w;
+    double vzl = fabs(al * (vS[2] * sig - dvz) / (1 - v2 * cf * cf) - be[2]);
+    double vzr = fabs(al * (vS[2] * sig + dvz) / (1 - v2 * cf * cf) - be[2]);
+    double maxvr = vrr > vrl ? vrr : vrl;
+    double maxvp = vpr > vpl ? vpr : vpl;
+    double maxvz = vzr > vzl ? vzr : vzl;
+    double dtr = get_dL(xp, xm, 1, P.phi_max) / maxvr;
+    double dtp = get_dL(xp, xm, 0, P.phi_max) / maxvp;
+    double dtz = get_dL(xp, xm, 2, P.phi_max) / maxvz;
+    double dt = dtr;
+    dt = dt < dtp ? dt : dtp;
+    dt = dt < dtz ? dt : dtz;
+    return dt;
+}
+
+// grmhd.c:228-245 with cons2prim_solve_adiabatic_noble2d (:813-1029): the
+// Newton solve starts from the cell's previous primitives (prim on entry).
+DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+    double cons1[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; q++) cons1[q] = cons[q] / dV;
+    Metric g;
+    kerr_metric(x, P.gr_M, P.gr_a, &g);
+    C2PInvariants v;
+    c2p_invariants(g, cons1, x, &v);
+    double v20, eta0;
+    c2p_guess_from_prim(P.gamma_law, g, prim, x, &v20, &eta0);
+    double v2, eta;
+    int ok;
+    c2p_newton(P.gamma_law, v, v20, eta0, 1.0e-12, 30, 10, &v2, &eta, &ok);
+    c2p_recover(P.gamma_law, P.rho_floor, P.pre_floor, g, v, v2, eta, x, prim);
+}
+
+}  // namespace grmhd
+}  // namespace disco

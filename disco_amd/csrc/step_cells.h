@@ -36,6 +36,7 @@
 #include "device_state.h"
 #include "phys_euler.h"
 #include "phys_mhd.h"
+#include "phys_grmhd_fv.h"
 
 namespace disco {
 
@@ -45,6 +46,7 @@ struct PhysEuler {
     static constexpr int NQ = 5;
     static constexpr int NF = 5;  // stored face-increment components
     static constexpr bool MHD = false;
+    static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         (void)dim;
@@ -70,6 +72,7 @@ struct PhysMHD {
     // (riemann.c:314-317) are never used: only the first five are stored
     static constexpr int NF = 5;
     static constexpr bool MHD = true;
+    static constexpr bool C2P_GUESS = false;
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         return mhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
@@ -83,6 +86,30 @@ struct PhysMHD {
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return mhd::mindt(P, prim, w, xp, xm);
+    }
+};
+
+// GRMHD (grmhd.c) in the Kerr-Schild metric: the same CT layout as ideal MHD;
+// its Noble 2D Newton cons2prim starts from the cell's previous primitives,
+// so update_b_cell hands them in through prim[].
+struct PhysGRMHD {
+    static constexpr int NQ = 8;
+    static constexpr int NF = 5;
+    static constexpr bool MHD = true;
+    static constexpr bool C2P_GUESS = true;
+    DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
+                        double w, double dAdt, int dim, double* dU) {
+        return grmhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
+    }
+    DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                    double dVdt) {
+        grmhd::source(P, prim, cons, xp, xm, dVdt);
+    }
+    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+        grmhd::cons2prim(P, cons, prim, x, dV);
+    }
+    DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+        return grmhd::mindt(P, prim, w, xp, xm);
     }
 };
 
@@ -670,10 +697,10 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
 
     // moving mesh (first substep): every annulus, ghosts included
     if (first) {
-        double pn = phip + S.wiph[c] * dt;
+        double pn = add_mul_rn(phip, S.wiph[c], dt);  // misc.c:275
         while (pn > Pmax) pn -= Pmax;
         while (pn < 0.0) pn += Pmax;
-        double pm = S.piph[cm] + S.wiph[cm] * dt;
+        double pm = add_mul_rn(S.piph[cm], S.wiph[cm], dt);
         while (pm > Pmax) pm -= Pmax;
         while (pm < 0.0) pm += Pmax;
         S.piph_n[c] = pn;
@@ -711,6 +738,9 @@ DHD double update_b_cell(const DevState& S, int c, int first, int want_dt, doubl
         double dV = get_dV(xp, xm, Pmax);
         double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
         double prim[NQ];
+        if (Ph::C2P_GUESS) {  // calc_prim hands cons2prim the cell's current prims (misc.c:300-335)
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[q * N + c];
+        }
         Ph::cons2prim(P, cons, prim, x, dV);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
         if (want_dt) {

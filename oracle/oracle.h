@@ -27,6 +27,7 @@ void oracle_timestep(oracle_run* o, double dt);
 void oracle_get_state(oracle_run* o, double* prim, double* cons, double* piph, double* dphi, double* wiph,
                       double* planet_phi, double* t);
 void oracle_destroy(oracle_run* o);
+void oracle_initial(const disco_params* par, const double* x, long n, double* prim);
 
 /* Overlap faces of one annulus pair as the reference builds them
  * (faces_alt.c:36-179): writes, per face in merge order, the array indices

@@ -722,6 +722,15 @@ void oracle_get_state(oracle_run* o, double* prim, double* cons, double* piph, d
     if (t) *t = o->t;
 }
 
+/* The config initial condition (disco_initial.h, shared with the GPU library
+ * and the reference harness) at n points x[3n] -> prim[8n]. */
+void oracle_initial(const disco_params* par, const double* x, long n, double* prim) {
+    disco_ic ic;
+    long k;
+    disco_ic_from_params(par, &ic);
+    for (k = 0; k < n; ++k) disco_initial(&ic, prim + 8 * k, x + 3 * k);
+}
+
 void oracle_destroy(oracle_run* o) {
     int jk;
     if (!o) return;

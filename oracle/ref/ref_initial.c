@@ -17,6 +17,8 @@ void setICparams(struct domain* theDomain) {
     the_ic.num_n = NUM_N;
     the_ic.rmin = p->rmin;
     the_ic.rmax = p->rmax;
+    the_ic.zmin = p->zmin;
+    the_ic.zmax = p->zmax;
     the_ic.gamma_law = p->Adiabatic_Index;
     the_ic.noise_amp = p->initPar1;
     the_ic.p_over_rho = p->initPar2;
@@ -27,6 +29,9 @@ void setICparams(struct domain* theDomain) {
     the_ic.bz = p->initPar3;
     the_ic.bloop = p->initPar4;
     the_ic.vnoise = p->initPar5;
+    the_ic.tor_rin = p->initPar6;
+    the_ic.tor_rmax = p->initPar7;
+    the_ic.rho_atm = p->Density_Floor;
 }
 
 void initial(double* prim, double* x) {

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--3d | --mhd | --mhd-full]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--3d | --mhd | --mhd-full | --grmhd]   (only the named set)
 """
 from __future__ import annotations
 
@@ -118,7 +118,19 @@ def fixtures_mhd_full() -> None:
              ref_seconds_2_steps=np.array(secs))
 
 
+def fixtures_grmhd() -> None:
+    # small 3D GRMHD torus at spin a = 0 (the reference's schw_ks_rpz.c):
+    # grmhd, HLL, CT_MODE 2, fixed r and z boundaries, Keplerian mesh
+    txt = par_with("c5_grmhd_torus.par", Num_R=16, Num_Z=12, Metric_Par3=0.0)
+    states, _ = run_ref("grmhd_torus", txt, 20, dumps=(1, 5))
+    save_states("c5_grmhd_small.npz", states)
+
+
 def main() -> None:
+    if "--grmhd" in sys.argv:
+        fixtures_grmhd()
+        print("fixtures written")
+        return
     if "--3d" in sys.argv:
         fixtures_3d()
         print("fixtures written")

@@ -1,0 +1,132 @@
+"""GPU parity of the 3D GRMHD step (grmhd.c in the Kerr-Schild metric, HLL,
+constrained transport CT_MODE 2, fixed r and z boundaries, Keplerian mesh)
+against the reference CPU build.
+
+The fixture holds the reference's own states (tests/golden/make_fixtures.py
+runs the reference compiled from its sources, with Hydro/Metric/schw_ks_rpz.c
+as the metric) at 0, 1, 5 and 20 steps of a small version of the torus at
+spin a = 0, started from identical seeded data.  The HIP path must match per
+field (helpers.state_errors; magnetic components on the field-magnitude
+scale) to 1e-12 after 1 step, 1e-10 after 5 and 1e-9 after 20, the face
+fluxes Phi[5] to the same tolerances on the scale of the largest face flux,
+and the mesh in radians.  The log-spaced annuli all hold the same number of
+cells, so every face of a radial track flips in the steps where neighbouring
+annuli slip by one cell: the track-wide flip chains are exercised.
+
+div B: the reference sets the face fluxes from the initial field sampled at
+face centroids (bfields.c:12-87), which leaves a small discrete divergence;
+constrained transport keeps every cell's net face flux at its initial value
+to round-off, which is what is checked.
+
+Spin a = 0.9 has no reference (schw_ks_rpz.c is Schwarzschild): the
+full-size torus runs two steps and must stay finite with every cell's div B
+preserved and no flip-chain error.
+"""
+import numpy as np
+import pytest
+
+import disco_amd as D
+from dumpio import load_fixture
+from helpers import cell_radius, div_b, params, phi_err, real_mask, state_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {1: 1e-12, 5: 1e-10, 20: 1e-9}
+SMALL = dict(Num_R=16, Num_Z=12, Metric_Par3=0.0)
+
+
+def _phi_errors(st, ref, real):
+    scale = np.abs(ref["Phi"][:, real]).max()
+    return np.abs(st["Phi"][:, real] - ref["Phi"][:, real]).max(axis=1) / scale
+
+
+def test_grmhd_matches_reference():
+    ref = load_fixture("c5_grmhd_small.npz")
+    dom = D.Domain(params("c5_grmhd_torus.par", **SMALL))
+    st0 = dom.state(0)
+    real = ref[0]["real"] == 1
+    assert np.array_equal(st0["piph"], ref[0]["piph"])
+    e0 = state_errors(st0, ref[0], cell_radius(ref[0]["Nr"], ref[0]["Nz"], ref[0]["Np"], ref[0]["r_jph"]), real)
+    assert max(e0["prim"].max(), e0["cons"].max()) < 1e-14, e0
+    assert _phi_errors(st0, ref[0], real).max() < 1e-14
+    div0, _ = div_b(st0["Phi"], st0["piph"], ref[0]["Np"], ref[0]["Nr"], ref[0]["Nz"])
+    done = 0
+    for s in sorted(ref):
+        if s == 0:
+            continue
+        dom.step(s - done)
+        done = s
+        dom.sync()
+        st = dom.state(0)
+        r = ref[s]
+        tm = dom.time()
+        assert abs(tm["dt"][0] - r["dt"]) <= 1e-12 * r["dt"], (tm["dt"][0], r["dt"])
+        rc = cell_radius(r["Nr"], r["Nz"], r["Np"], r["r_jph"])
+        e = state_errors(st, r, rc, real)
+        ep = _phi_errors(st, r, real)
+        epiph = phi_err(st["piph"], r["piph"])
+        print(f"step {s}: prim {np.array2string(e['prim'], precision=1)} cons {np.array2string(e['cons'], precision=1)}"
+              f" Phi {np.array2string(ep, precision=1)} piph {epiph:.1e}", flush=True)
+        tol = TOL[s]
+        assert e["prim"].max() < tol, (s, e["prim"])
+        assert e["cons"].max() < tol, (s, e["cons"])
+        assert ep.max() < tol, (s, ep)
+        assert epiph < tol
+        div, sc = div_b(st["Phi"], st["piph"], r["Np"], r["Nr"], r["Nz"])
+        # cells without field have sc = 0 and must keep div exactly
+        assert (np.abs(div - div0)[real] <= 1e-13 * sc[real]).all()
+    dom.close()
+
+
+def test_grmhd_graph_step_equals_eager_timestep():
+    """The captured hipGraph step and an eager timestep with the same dt give
+    the same bits."""
+    p = params("c5_grmhd_torus.par", **SMALL)
+    a = D.Domain(p)
+    b = D.Domain(p)
+    dt = a.getmindt()
+    a.timestep(dt)
+    b.step(1)
+    sa, sb = a.state(0), b.state(0)
+    for k in ("prim", "cons", "piph", "Phi"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
+def test_grmhd_kerr_full_size_runs():
+    """Full benchmark torus (Nr = Nz = 192, 11.7 M real cells) at a = 0.9:
+    two steps stay finite, the density floor holds, and constrained
+    transport keeps every real cell's net face flux on a slab of annuli at
+    its initial value."""
+    dom = D.Domain(params("c5_grmhd_torus.par"))
+    I = dom.info[0]
+    g = dom.grid(0)
+    assert I.nreal == 192 * 192 * 317
+    real = real_mask(I.Nr, I.Nz, I.NgRa, I.NgRb, I.NgZa, I.NgZb, g["Np"])
+    st0 = dom.state(0)
+    # div B on the annuli of 4 z-layers around the midplane (k0-1 .. k0+2)
+    Np = g["Np"]
+    k0 = I.Nz // 2 - 1
+    lo = int(np.sum(Np[: I.Nr * k0]))
+    hi = int(np.sum(Np[: I.Nr * (k0 + 4)]))
+
+    def slab_div(st):
+        Phi = st["Phi"][:, lo:hi]
+        div, sc = div_b(Phi, st["piph"][lo:hi], Np[I.Nr * k0: I.Nr * (k0 + 4)], I.Nr, 4)
+        return div, sc
+
+    d0, _ = slab_div(st0)
+    dom.step(2)
+    dom.sync()
+    st = dom.state(0)
+    assert np.isfinite(st["prim"][:, real]).all()
+    assert np.isfinite(st["Phi"][:, real]).all()
+    assert st["prim"][0, real].min() >= 1e-6 * (1 - 1e-12)
+    d2, sc = slab_div(st)
+    # the slab's top and bottom layers lack their outer z-neighbours' faces
+    inner = np.zeros(hi - lo, bool)
+    inner[int(np.sum(Np[I.Nr * k0: I.Nr * (k0 + 1)])): int(np.sum(Np[I.Nr * k0: I.Nr * (k0 + 3)]))] = True
+    rr = real[lo:hi] & inner
+    assert (np.abs(d2 - d0)[rr] <= 1e-13 * sc[rr]).all()
+    dom.close()

@@ -1,0 +1,138 @@
+"""CPU pins of the GRMHD finite-volume step's pieces that the reference's
+full-step fixture (spin a = 0) does not reach:
+
+* the metric derivatives for a != 0 (phys_grmhd_fv.h, dual numbers through
+  the Kerr-Schild formulas): the source itself is compiled by the host
+  compiler into a small probe (tests/probes/metric_der_probe.cpp) and checked
+  against 4th-order central differences of the metric, and at a = 0 against
+  the reference's closed forms (Hydro/Metric/schw_ks_rpz.c:86-182), which
+  the a = 0 branch restates and the full-step fixture pins;
+* the torus initial condition (disco_initial.h, evaluated through the
+  oracle library): density maximum 1 at the pressure maximum, inner edge,
+  atmosphere, plasma beta 100, and a divergence-free densitised field.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import disco_amd as D
+import oracle_py as O
+from helpers import params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("probe") / "metric_der_probe")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", os.path.join(ROOT, "disco_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "probes", "metric_der_probe.cpp"), "-o", exe], check=True,
+                   timeout=120)
+
+    def run(rows):
+        text = "".join(f"{M:.17g} {a:.17g} {R:.17g} {z:.17g} {mode}\n" for M, a, R, z, mode in rows)
+        out = subprocess.run([exe], input=text, capture_output=True, text=True, check=True, timeout=60).stdout
+        return np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
+
+    return run
+
+
+def _g4(m):
+    """4-metric from (alpha, beta^i, gamma_ij)."""
+    al, be, gam = m[0], m[1:4], m[4:13].reshape(3, 3)
+    bl = gam @ be
+    g = np.zeros((4, 4))
+    g[0, 0] = -al * al + bl @ be
+    g[0, 1:] = g[1:, 0] = bl
+    g[1:, 1:] = gam
+    return g
+
+
+def _points(n, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(1.3, 60.0, n), rng.uniform(-30.0, 30.0, n)], axis=1)
+
+
+def test_dual_derivatives_equal_closed_form_at_a0(probe):
+    pts = _points(200, 3)
+    closed = probe([(1.0, 0.0, R, z, 0) for R, z in pts])
+    dual = probe([(1.0, 0.0, R, z, 1) for R, z in pts])
+    scale = np.abs(closed).max(axis=1, keepdims=True)
+    assert np.abs(dual - closed).max() <= 1e-13 * scale.max()
+    assert (np.abs(dual - closed) <= 1e-13 * scale).all()
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9])
+def test_metric_derivatives_equal_finite_differences(probe, a):
+    pts = _points(120, 4)
+    der = probe([(1.0, a, R, z, 0) for R, z in pts])
+    for (R, z), d in zip(pts, der):
+        dg = {1: d[0:16].reshape(4, 4), 3: d[16:32].reshape(4, 4)}
+        da, db = d[32:36], d[36:48].reshape(4, 3)
+        for mu, (eR, ez) in ((1, (1.0, 0.0)), (3, (0.0, 1.0))):
+            h = 1e-3 * max(1.0, abs(R) * eR + abs(z) * ez)
+            rows = [(1.0, a, R + k * h * eR, z + k * h * ez, 2) for k in (-2, -1, 1, 2)]
+            m = probe(rows)
+            fd = (m[0] - 8 * m[1] + 8 * m[2] - m[3]) / (12 * h)
+            g = [_g4(mm) for mm in m]
+            fdg = (g[0] - 8 * g[1] + 8 * g[2] - g[3]) / (12 * h)
+            # truncation O(h^4) plus round-off of the differenced entries (g_phiphi = R^2 is large)
+            tol = 1e-7 * np.abs(fdg).max() + 1e-14 * np.abs(g[1]).max() / h
+            assert np.abs(dg[mu] - fdg).max() <= tol, (a, R, z, mu)
+            assert abs(da[mu] - fd[0]) <= 1e-7 * max(abs(fd[0]), 1e-6), (a, R, z, mu)
+            assert np.abs(db[mu] - fd[1:4]).max() <= 1e-7 * max(np.abs(fd[1:4]).max(), 1e-6), (a, R, z, mu)
+        # t and phi are Killing directions
+        assert not da[0] and not da[2] and not db[0].any() and not db[2].any()
+
+
+@pytest.fixture(scope="module")
+def ic():
+    L = O.lib()
+    dp = C.POINTER(C.c_double)
+    L.oracle_initial.argtypes = [C.POINTER(D.Params), dp, C.c_long, dp]
+    p = params("c5_grmhd_torus.par")
+
+    def at(R, z):
+        R, z = np.broadcast_arrays(np.asarray(R, float), np.asarray(z, float))
+        x = np.ascontiguousarray(np.stack([R.ravel(), np.zeros(R.size), z.ravel()], axis=1))
+        out = np.zeros((R.size, 8))
+        L.oracle_initial(C.byref(p), x.ctypes.data_as(dp), R.size, out.ctypes.data_as(dp))
+        return out.reshape(R.shape + (8,))
+
+    return at, p
+
+
+def test_torus_density_pressure_and_edges(ic):
+    at, p = ic
+    # rho_max = 1 at the pressure maximum (R = 12 on the midplane), P = K rho^Gamma
+    R = np.linspace(11.0, 13.0, 2001)
+    s = at(R, 0.0)
+    # the seeded 2% perturbation acts on P only
+    assert abs(s[:, 0].max() - 1.0) < 1e-6 and abs(R[s[:, 0].argmax()] - 12.0) < 2e-3
+    # inner edge at R = 6: atmosphere inside, torus just outside
+    assert at(5.99, 0.0)[0] == p.Density_Floor and at(6.05, 0.0)[0] > p.Density_Floor
+    atm = at(3.0, 5.0)
+    assert atm[0] == p.Density_Floor and atm[1] == p.initPar2 * p.Density_Floor
+    assert not atm[2:].any()  # the normal observer (u_i = 0), no field
+
+
+def test_torus_field_beta_and_divergence(ic):
+    at, p = ic
+    R = np.linspace(6.0, 40.0, 1201)
+    z = np.linspace(-15.0, 15.0, 1001)
+    RR, ZZ = np.meshgrid(R, z, indexing="ij")
+    s = at(RR, ZZ)
+    pmag = 0.5 * (s[..., 5] ** 2 + s[..., 7] ** 2)
+    beta = s[..., 1].max() / pmag.max()
+    assert 90.0 < beta < 110.0, beta
+    assert not s[..., 6].any()  # purely poloidal
+    # d(R B^R)/dR + d(R B^z)/dz = 0 (densitised, grmhd.c's prim[B] = sqrt(gamma) B^i)
+    fR = RR * s[..., 5]
+    fz = RR * s[..., 7]
+    div = np.gradient(fR, R, axis=0) + np.gradient(fz, z, axis=1)
+    inside = s[..., 0] > 0.3  # away from the A = 0 cut, where the field has a kink
+    grad = np.abs(np.gradient(fR, R, axis=0)) + np.abs(np.gradient(fz, z, axis=1))
+    assert np.abs(div[inside]).max() < 2e-3 * grad[inside].max()
