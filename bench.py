@@ -8,7 +8,8 @@ zone-updates/s = real cells x K / wall time of the K graph-launched steps
 (HIP events on the library's stream, plus a device sync on both sides).
 The headline workload is the 3D ideal-MHD disk (HLLD, constrained
 transport, Nr=128 x Nz=64, 5,512,000 real cells); the 2D hydro disk, the
-48-disk batch and the standalone GRMHD cons2prim are reported beside it.
+3D GRMHD torus (Kerr-Schild a=0.9, 11.7 M cells), the 48-disk batch and the
+standalone GRMHD cons2prim are reported beside it.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -33,7 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # MHD with 3D CT (NUM_Q=8, NUM_FACES=5): read prim, cons, RKcons (24) + Phi,
 # RK_Phi (10) + piph, dphi (2); write cons, prim (16) + Phi (5) + piph, dphi
 # (2) = 59 doubles = 472 B.
-BYTES_PER_ZONE_SUBSTEP = {"euler": 232, "mhd": 472}
+# GRMHD with 3D CT moves the same 59 doubles as MHD.
+BYTES_PER_ZONE_SUBSTEP = {"euler": 232, "mhd": 472, "grmhd": 472}
 
 WORKLOADS = {
     # 3D cylindrical ideal MHD disk, HLLD + constrained transport, Nr=128 in
@@ -41,6 +43,10 @@ WORKLOADS = {
     "mhd3d_nr128_nz64": ("c2_mhd3d.par", {}, "mhd", "mhd_sun"),
     # 2D cylindrical Newtonian hydro Keplerian disk, Nr=256, r in [0.5, 3]
     "kepler2d_nr256": ("c1_kepler2d.par", {}, "euler", "euler_sun"),
+    # 3D GRMHD torus in Kerr-Schild (a = 0.9), HLL + constrained transport,
+    # Nr=192 log-spaced in [1.3, 60], Nz=192 in [-30, 30], fixed boundaries
+    # (11,685,888 real cells)
+    "grmhd_torus_nr192_nz192": ("c5_grmhd_torus.par", {}, "grmhd", "grmhd_torus"),
 }
 # bounded CPU sample of each workload for the reference build: the same grid
 # spacing and physics on a slab of the domain, a few steps
@@ -48,6 +54,10 @@ CPU_SAMPLES = {
     "mhd3d_nr128_nz64": ({"Num_Z": 8, "Z_Min": -0.0625, "Z_Max": 0.0625}, 3,
                          "8 of the 64 z-layers (same dz, periodic), 689,000 real cells, 3 RK2 steps"),
     "kepler2d_nr256": ({}, 20, "full 288,917-cell grid, 20 RK2 steps"),
+    # the reference's metric is Schwarzschild (schw_ks_rpz.c): spin 0
+    "grmhd_torus_nr192_nz192": ({"Num_Z": 8, "Z_Min": -1.25, "Z_Max": 1.25, "Metric_Par3": 0.0}, 1,
+                                "8 midplane z-layers of the 192 (same dz, fixed z boundaries), spin a = 0 "
+                                "(the reference's metric is Schwarzschild), 486,912 real cells, 1 RK2 step"),
 }
 
 
@@ -119,6 +129,53 @@ def bench_fv(workload: str, steps: int, warmup: int) -> dict:
                      "bytes_per_zone_substep": bpz, "substep_ms": substep_ms, "dominant_kernel": dominant},
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }
+
+
+def bench_grmhd_learned(steps: int, warmup: int, train_steps: int = 1500) -> dict:
+    """The GRMHD torus with the learned cons2prim start in the loop: the guess
+    network is trained on the GPU on (cons, prim) pairs of the run's own cells
+    after its first two steps, then the same run is timed with the plain start
+    (previous prims, as the reference) and with the learned one; Newton
+    iterations per solve and the fallback rate are counted over one step of
+    each."""
+    import disco_amd as D
+
+    base, over, _, _ = WORKLOADS["grmhd_torus_nr192_nz192"]
+    p = D.parse_par(par_text(base, **over))
+    dom = D.Domain(p)
+    nreal = dom.info[0].nreal
+    dom.step(2)
+    x, dV, real, st = dom.cell_positions()
+    cons = st["cons"][:, real] / dV
+    net = D.C2P(capacity=cons.shape[1], gamma=p.Adiabatic_Index, M=p.metricPar2, a=p.metricPar3)
+    net.upload(cons, x)
+    t0 = time.perf_counter()
+    loss = net.train_resident(seed=7, steps=train_steps, batch=4096, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    w, nrm = net.get_mlp()
+    net.close()
+    del cons, x, dV, st
+    out = {"train_pairs": int(real.sum()), "train_steps": train_steps, "train_seconds": train_s,
+           "train_loss_first_last": [float(loss[:20].mean()), float(loss[-20:].mean())]}
+    for mode in ("plain", "learned"):
+        dom.set_c2p_net(w, nrm) if mode == "learned" else dom.set_c2p_net(None)
+        dom.c2p_count(False)
+        dom.step(warmup)
+        dom.sync()
+        ms = dom.time_steps(steps)
+        kt = dom.kernel_times(2)
+        dom.c2p_count(True)
+        dom.c2p_stats(reset=True)
+        dom.step(1)
+        c = dom.c2p_stats(reset=True)
+        upd = "k_update_b_learned" if mode == "learned" else "k_update_b"
+        out[mode] = {"value": nreal * steps / (ms * 1e-3), "ms_per_step": ms / steps, "update_b_ms": kt[upd][0],
+                     "iterations_per_solve": c["iterations"] / c["solves"],
+                     "fallback_rate": c["fallbacks"] / c["solves"], "unconverged": c["unconverged"]}
+    out["speedup_learned_vs_plain"] = out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"]
+    out["update_b_speedup"] = out["plain"]["update_b_ms"] / out["learned"]["update_b_ms"]
+    dom.close()
+    return out
 
 
 def bench_batch(steps: int, warmup: int) -> dict:
@@ -259,7 +316,10 @@ def main() -> int:
     hydro = WORKLOADS[wl][2]
     cfg = {"workload": wl, "real_cells": r["real_cells"], "integrator": "RK2", "plm": 1.5,
            "reference_convention_megazones_per_s": 2 * r["value"] / 1e6}
-    if hydro == "mhd":
+    if hydro == "grmhd":
+        cfg.update({"physics": "GRMHD (grmhd.c), Kerr-Schild a=0.9", "riemann": "HLL", "ct": "CT_MODE 2 (3D)",
+                    "Nr": 192, "Nz": 192, "z": "fixed", "mesh_motion": "Keplerian, rigid per annulus"})
+    elif hydro == "mhd":
         cfg.update({"physics": "ideal MHD (mhd.c)", "riemann": "HLLD", "ct": "CT_MODE 2 (3D)", "Nr": 128, "Nz": 64,
                     "z": "periodic", "mesh_motion": "Keplerian, rigid per annulus"})
     else:
@@ -288,6 +348,13 @@ def main() -> int:
         if not args.no_cpu:
             k["cpu_baseline"] = cpu_baseline("kepler2d_nr256")
         line["kepler2d"] = k
+    if not args.no_extra and wl != "grmhd_torus_nr192_nz192":
+        g = bench_fv("grmhd_torus_nr192_nz192", 10, 2)
+        g["workload"] = "grmhd_torus_nr192_nz192"
+        g["learned_c2p"] = bench_grmhd_learned(10, 2)
+        if not args.no_cpu:
+            g["cpu_baseline"] = cpu_baseline("grmhd_torus_nr192_nz192")
+        line["grmhd_torus"] = g
     if not args.no_batch and not args.no_extra:
         line["batch"] = bench_batch(args.steps, args.warmup)
     if not args.no_c2p and not args.no_extra:

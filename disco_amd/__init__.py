@@ -110,6 +110,10 @@ def lib() -> C.CDLL:
     L.disco_gpu_time.argtypes = [vp, dp, dp, dp]
     L.disco_gpu_kernel_times.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, dp, ip]
     L.disco_gpu_time_steps.argtypes = [vp, C.c_int, dp]
+    fp = C.POINTER(C.c_float)
+    L.disco_gpu_set_c2p_net.argtypes = [vp, fp, fp]
+    L.disco_gpu_c2p_count.argtypes = [vp, C.c_int]
+    L.disco_gpu_c2p_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
     L.disco_gpu_destroy.argtypes = [vp]
     L.disco_gpu_destroy.restype = None
     L.disco_gpu_last_error.restype = C.c_char_p
@@ -295,6 +299,46 @@ class Domain:
             out[nm] = (float(avg[i]), int(cnt[i]))
         return out
 
+    def cell_positions(self, run: int = 0):
+        """Real cells of a run (host order): their cons2prim position x[3, n]
+        (centroid r, mid phi, centroid z; misc.c:314-331) and volume dV[n]."""
+        I = self.info[run]
+        g = self.grid(run)
+        st = self.state(run)
+        Np = g["Np"]
+        jk = np.repeat(np.arange(I.Nr * I.Nz), Np)
+        j, k = jk % I.Nr, jk // I.Nr
+        rp, rm = g["r_jph"][j + 1], g["r_jph"][j]
+        zp, zm = g["z_kph"][k + 1], g["z_kph"][k]
+        real = (j >= I.NgRa) & (j < I.Nr - I.NgRb) & (k >= I.NgZa) & (k < I.Nz - I.NgZb)
+        phip = st["piph"]
+        dphi = st["dphi"]
+        x = np.stack([2.0 * (rp * rp + rp * rm + rm * rm) / (3.0 * (rp + rm)), phip - 0.5 * dphi, 0.5 * (zp + zm)])
+        dV = 0.5 * (rp + rm) * (rp - rm) * dphi * (zp - zm)
+        return x[:, real], dV[real], real, st
+
+    def set_c2p_net(self, params=None, norm=None) -> None:
+        """GRMHD: start the in-loop cons2prim from a trained guess network
+        (params, norm as C2P.get_mlp returns them); None returns to the
+        reference's start (the cell's previous primitives)."""
+        if params is None:
+            _check(lib().disco_gpu_set_c2p_net(self._h, None, None), "disco_gpu_set_c2p_net")
+            return
+        w = np.ascontiguousarray(params, np.float32)
+        n = np.ascontiguousarray(norm, np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(lib().disco_gpu_set_c2p_net(self._h, w.ctypes.data_as(fp), n.ctypes.data_as(fp)),
+               "disco_gpu_set_c2p_net")
+
+    def c2p_count(self, enable: bool = True) -> None:
+        """GRMHD: count the in-loop cons2prim solves (see c2p_stats)."""
+        _check(lib().disco_gpu_c2p_count(self._h, int(bool(enable))), "disco_gpu_c2p_count")
+
+    def c2p_stats(self, reset: bool = False) -> dict:
+        out = (C.c_longlong * 4)()
+        _check(lib().disco_gpu_c2p_stats(self._h, out, int(bool(reset))), "disco_gpu_c2p_stats")
+        return dict(iterations=out[0], solves=out[1], fallbacks=out[2], unconverged=out[3])
+
     def time_steps(self, nsteps: int) -> float:
         ms = C.c_double()
         _check(lib().disco_gpu_time_steps(self._h, int(nsteps), C.byref(ms)), "disco_gpu_time_steps")
@@ -345,6 +389,7 @@ def _c2p_lib():
     L.disco_c2p_download.argtypes = [vp, dp, ip, ip, C.c_long]
     L.disco_c2p_errors.argtypes = [vp, C.c_ulonglong, C.c_long, fp]
     L.disco_mlp_train.argtypes = [vp, C.c_ulonglong, C.c_long, C.c_int, C.c_int, C.c_float, fp]
+    L.disco_mlp_train_resident.argtypes = [vp, C.c_ulonglong, C.c_int, C.c_int, C.c_float, fp]
     L.disco_mlp_get.argtypes = [vp, fp, fp]
     L.disco_mlp_set.argtypes = [vp, fp, fp]
     L.disco_mlp_grad.argtypes = [vp, fp, fp, C.c_int, fp, fp]
@@ -413,6 +458,13 @@ class C2P:
     def train(self, seed: int, ntrain: int, steps: int, batch: int, lr: float) -> np.ndarray:
         loss = np.zeros(steps, np.float32)
         self._chk(_c2p_lib().disco_mlp_train(self._h, seed, ntrain, steps, batch, lr, _fptr(loss)), "disco_mlp_train")
+        return loss
+
+    def train_resident(self, seed: int, steps: int, batch: int, lr: float) -> np.ndarray:
+        """Train on the uploaded states (e.g. a run's own cells)."""
+        loss = np.zeros(steps, np.float32)
+        self._chk(_c2p_lib().disco_mlp_train_resident(self._h, seed, steps, batch, lr, _fptr(loss)),
+                  "disco_mlp_train_resident")
         return loss
 
     def get_mlp(self):

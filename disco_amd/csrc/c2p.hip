@@ -15,9 +15,11 @@
 
 #include "../../include/disco_gpu.h"
 #include "disco_c4_states.h"
+#include "mlp_infer.h"
 #include "phys_grmhd.h"
 
 using namespace disco;
+using namespace disco::mlp;
 
 namespace {
 
@@ -34,25 +36,6 @@ int c2p_fail(const std::string& m) {
 
 #define DEV __device__ static inline __attribute__((always_inline))
 
-// ------------------------------------------------------------------ MLP layout
-constexpr int NIN = 8, NH = 64, NOUT = 2;
-constexpr int OFF_W1 = 0;                      // [NIN][NH]
-constexpr int OFF_B1 = OFF_W1 + NIN * NH;      // [NH]
-constexpr int OFF_W2 = OFF_B1 + NH;            // [NH][NH]
-constexpr int OFF_B2 = OFF_W2 + NH * NH;
-constexpr int OFF_W3 = OFF_B2 + NH;
-constexpr int OFF_B3 = OFF_W3 + NH * NH;
-constexpr int OFF_W4 = OFF_B3 + NH;
-constexpr int OFF_B4 = OFF_W4 + NH * NH;
-constexpr int OFF_W5 = OFF_B4 + NH;            // [NH][NOUT]
-constexpr int OFF_B5 = OFF_W5 + NH * NOUT;
-constexpr int NPARAM = OFF_B5 + NOUT;          // 13186
-static_assert(NPARAM == 13186, "MLP parameter count");
-__host__ __device__ constexpr int w_off(int l) { return l == 0 ? OFF_W1 : l == 1 ? OFF_W2 : l == 2 ? OFF_W3 : l == 3 ? OFF_W4 : OFF_W5; }
-__host__ __device__ constexpr int b_off(int l) { return l == 0 ? OFF_B1 : l == 1 ? OFF_B2 : l == 2 ? OFF_B3 : l == 3 ? OFF_B4 : OFF_B5; }
-__host__ __device__ constexpr int l_in(int l) { return l == 0 ? NIN : NH; }
-__host__ __device__ constexpr int l_out(int l) { return l == 4 ? NOUT : NH; }
-
 struct C2PConst {
     double gamma_law, M, a, prec;
     int max_iter, nextra;
@@ -64,23 +47,6 @@ DEV float silu(float z) { return z / (1.0f + __expf(-z)); }
 DEV float dsilu(float z) {
     float s = 1.0f / (1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
-}
-
-// Scale-free features of the Noble 2D system (it depends on s2, e, Q, psi
-// and Gamma only).  psi <= Q s2 by Cauchy-Schwarz, so f6 is a cos^2.
-DEV void c2p_features(const grmhd::C2PInvariants& v, float* f) {
-    // the network is BF16: features in FP32 arithmetic from the FP64 invariants
-    const float e = (float)v.e, s2 = (float)v.s2, Q = (float)v.Q, psi = (float)v.psi;
-    const float eq = e + Q;
-    f[0] = __logf(e);
-    f[1] = log1pf(s2);
-    f[2] = s2 / (eq * eq);
-    f[3] = log1pf(Q);
-    f[4] = Q / eq;
-    f[5] = log1pf(psi);
-    const float qs = Q * s2;
-    f[6] = qs > 0.0f ? fminf(psi / qs, 1.0f) : 0.0f;
-    f[7] = __logf(fmaxf((float)(v.e - 1.0), 1e-12f));
 }
 
 // Targets: t0 = ln W = -ln(1 - v2)/2, t1 = ln(eta / e).
@@ -305,149 +271,6 @@ __global__ void k_zero_f(float* p, int n) {
     if (i < n) p[i] = 0.0f;
 }
 
-// ------------------------------------------------------------------ MFMA inference
-// BF16 weights, transposed per layer and zero-padded to 16-multiples:
-// Wt_l[j][k] (out j, in k), K = 16 for layer 0 (8 inputs), N = 16 for the
-// output layer (2 outputs).  Activations live in LDS as bf16 rows [s][k].
-constexpr int K0P = 16, NOP = 16;
-constexpr int WT_L0 = 0;                   // [NH][K0P]
-constexpr int WT_L1 = WT_L0 + NH * K0P;    // [NH][NH]
-constexpr int WT_L2 = WT_L1 + NH * NH;
-constexpr int WT_L3 = WT_L2 + NH * NH;
-constexpr int WT_L4 = WT_L3 + NH * NH;     // [NOP][NH]
-constexpr int WT_TOTAL = WT_L4 + NOP * NH; // bf16 elements
-constexpr int NBIAS = 4 * NH + NOP;
-
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
-
-typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
-
-// float4 -> 4 x bf16 with the hardware's round-to-nearest-even convert
-// (v_cvt_pk_bf16_f32)
-DEV short4_t pack4(float4_t v) {
-    bf16x4_t b = __builtin_convertvector(v, bf16x4_t);
-    return __builtin_bit_cast(short4_t, b);
-}
-
-// SiLU for inference: v_exp_f32 + v_rcp_f32 (the network is BF16)
-DEV float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-
-DEV float4_t silu4(float4_t a) {
-    float4_t h = {silu_fast(a[0]), silu_fast(a[1]), silu_fast(a[2]), silu_fast(a[3])};
-    return h;
-}
-
-// Block-shared network parameters in LDS: transposed bf16 weights and fp32
-// biases (zero-padded), loaded once per block.
-struct MlpLds {
-    unsigned short W[WT_TOTAL];
-    float b[NBIAS];
-};
-
-DEV void mlp_load(MlpLds& L, const unsigned short* __restrict__ Wt_g, const float* __restrict__ W32) {
-    for (int i = threadIdx.x; i < WT_TOTAL / 4; i += blockDim.x)
-        reinterpret_cast<short4_t*>(L.W)[i] = reinterpret_cast<const short4_t*>(Wt_g)[i];
-    for (int i = threadIdx.x; i < NBIAS; i += blockDim.x) {
-        float b = 0.0f;
-        if (i < 4 * NH) b = W32[b_off(i / NH) + (i % NH)];
-        else if (i - 4 * NH < NOUT) b = W32[OFF_B5 + (i - 4 * NH)];
-        L.b[i] = b;
-    }
-}
-
-// The network on the 64 states of one wavefront, transposed: each layer
-// computes Y^T = W^T X^T with v_mfma_f32_16x16x16_bf16 (FP32 accumulate).
-// Operand layout of that instruction: lane l supplies A[l%16][4(l/16)+t] and
-// B[4(l/16)+t][l%16] and receives D[4(l/16)+t][l%16], t = 0..3.  With A a
-// tile of W^T (output features x input features, from LDS) and B a tile of
-// X^T (input features x states), a lane receives output features 4(l/16)+t of
-// state l%16 -- exactly the B operand of the next layer, so activations stay
-// in registers: act[ts][kt] holds features 16kt + 4(l/16) + t of state
-// 16ts + l%16.  Only the input (one row of features per lane) and the output
-// (two numbers per state) pass through LDS, to change lane ownership.
-DEV void mlp_wave(const MlpLds& L, unsigned short* sF, float* sY, const float* fstd_in, float* out2) {
-    const int lane = threadIdx.x & 63;
-    const int r = lane & 15, g = lane >> 4;
-    // lane -> row: this lane's standardised features (bf16), zero padded
-    {
-        short4_t lo = pack4((float4_t){fstd_in[0], fstd_in[1], fstd_in[2], fstd_in[3]});
-        short4_t hi = pack4((float4_t){fstd_in[4], fstd_in[5], fstd_in[6], fstd_in[7]});
-        short4_t z = {0, 0, 0, 0};
-        short4_t* row = reinterpret_cast<short4_t*>(sF + lane * K0P);
-        row[0] = lo;
-        row[1] = hi;
-        row[2] = z;
-        row[3] = z;
-    }
-    __syncthreads();
-    short4_t act[4][4], nxt[4][4];
-#pragma unroll
-    for (int ts = 0; ts < 4; ++ts) act[ts][0] = *reinterpret_cast<const short4_t*>(sF + (16 * ts + r) * K0P + 4 * g);
-    // layer 0: K = 16 (one k tile)
-#pragma unroll
-    for (int to = 0; to < 4; ++to) {
-        short4_t a = *reinterpret_cast<const short4_t*>(L.W + WT_L0 + (16 * to + r) * K0P + 4 * g);
-        const float4_t bias = {L.b[16 * to + 4 * g], L.b[16 * to + 4 * g + 1], L.b[16 * to + 4 * g + 2],
-                               L.b[16 * to + 4 * g + 3]};
-#pragma unroll
-        for (int ts = 0; ts < 4; ++ts) {
-            float4_t acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[ts][0], bias, 0, 0, 0);
-            nxt[ts][to] = pack4(silu4(acc));
-        }
-    }
-    // hidden layers 1..3: K = N = 64
-    const int woff[3] = {WT_L1, WT_L2, WT_L3};
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-#pragma unroll
-        for (int ts = 0; ts < 4; ++ts)
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) act[ts][kt] = nxt[ts][kt];
-#pragma unroll
-        for (int to = 0; to < 4; ++to) {
-            const float* bb = L.b + (l + 1) * NH + 16 * to + 4 * g;
-            float4_t acc[4];
-#pragma unroll
-            for (int ts = 0; ts < 4; ++ts) acc[ts] = (float4_t){bb[0], bb[1], bb[2], bb[3]};
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                short4_t a = *reinterpret_cast<const short4_t*>(L.W + woff[l] + (16 * to + r) * NH + 16 * kk + 4 * g);
-#pragma unroll
-                for (int ts = 0; ts < 4; ++ts)
-                    acc[ts] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[ts][kk], acc[ts], 0, 0, 0);
-            }
-#pragma unroll
-            for (int ts = 0; ts < 4; ++ts) nxt[ts][to] = pack4(silu4(acc[ts]));
-        }
-    }
-    // output layer: N = 16 (2 used); outputs 0, 1 land in lanes with g = 0
-    {
-        const float* bb = L.b + 4 * NH + 4 * g;
-        float4_t acc[4];
-#pragma unroll
-        for (int ts = 0; ts < 4; ++ts) acc[ts] = (float4_t){bb[0], bb[1], bb[2], bb[3]};
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            short4_t a = *reinterpret_cast<const short4_t*>(L.W + WT_L4 + r * NH + 16 * kk + 4 * g);
-#pragma unroll
-            for (int ts = 0; ts < 4; ++ts)
-                acc[ts] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, nxt[ts][kk], acc[ts], 0, 0, 0);
-        }
-        if (g == 0) {
-#pragma unroll
-            for (int ts = 0; ts < 4; ++ts) {
-                sY[(16 * ts + r) * 2] = acc[ts][0];
-                sY[(16 * ts + r) * 2 + 1] = acc[ts][1];
-            }
-        }
-    }
-    __syncthreads();
-    out2[0] = sY[lane * 2];
-    out2[1] = sY[lane * 2 + 1];
-}
-
 // Network outputs only (standardised), for tests and training diagnostics.
 __global__ __launch_bounds__(256) void k_mlp_guess(C2PConst C, long n, const double* __restrict__ cons,
                                                    const double* __restrict__ x,
@@ -594,21 +417,7 @@ static int refresh_bf16(disco_c2p* h) {
     std::vector<float> W(NPARAM);
     CHK(hipMemcpy(W.data(), h->W, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
     std::vector<unsigned short> T(WT_TOTAL, 0);
-    auto bf = [](float f) {
-        unsigned u;
-        std::memcpy(&u, &f, 4);
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        return (unsigned short)(u >> 16);
-    };
-    for (int j = 0; j < NH; ++j)
-        for (int kk = 0; kk < NIN; ++kk) T[WT_L0 + j * K0P + kk] = bf(W[OFF_W1 + kk * NH + j]);
-    const int offs[3] = {WT_L1, WT_L2, WT_L3};
-    const int woffs[3] = {OFF_W2, OFF_W3, OFF_W4};
-    for (int l = 0; l < 3; ++l)
-        for (int j = 0; j < NH; ++j)
-            for (int kk = 0; kk < NH; ++kk) T[offs[l] + j * NH + kk] = bf(W[woffs[l] + kk * NH + j]);
-    for (int o = 0; o < NOUT; ++o)
-        for (int kk = 0; kk < NH; ++kk) T[WT_L4 + o * NH + kk] = bf(W[OFF_W5 + kk * NOUT + o]);
+    mlp_pack_bf16(W.data(), T.data());
     CHK(hipMemcpy(h->Wt, T.data(), WT_TOTAL * sizeof(unsigned short), hipMemcpyHostToDevice));
     return 0;
 }
@@ -764,12 +573,14 @@ int disco_c2p_errors(disco_c2p_t* h, unsigned long long seed, long k0, float* er
     return 0;
 }
 
-int disco_mlp_train(disco_c2p_t* h, unsigned long long seed, long ntrain, int steps, int batch, float lr,
-                    float* loss_curve) {
-    if (!h || ntrain <= 0 || steps <= 0 || batch <= 0 || ntrain > h->cap)
-        return c2p_fail("disco_mlp_train: bad arguments");
-    // 1. training pairs: the plain solve on this code's own states
-    if (disco_c2p_generate(h, seed, 0, ntrain)) return -1;
+}  // extern "C"
+
+// Trains the network on the first ntrain resident states: (features,
+// targets) pairs from the plain solve, standardisation, Adam over seeded
+// random minibatches.
+static int train_resident(disco_c2p* h, unsigned long long seed, long ntrain, int steps, int batch, float lr,
+                          float* loss_curve) {
+    // 1. training pairs: the plain solve on the resident states
     if (h->nfeat < ntrain) {
         if (h->feat) (void)hipFree(h->feat);
         if (h->tgt) (void)hipFree(h->tgt);
@@ -841,6 +652,22 @@ int disco_mlp_train(disco_c2p_t* h, unsigned long long seed, long ntrain, int st
     CHK(hipStreamSynchronize(h->st));
     h->trained = true;
     return refresh_bf16(h);
+}
+
+extern "C" {
+
+int disco_mlp_train(disco_c2p_t* h, unsigned long long seed, long ntrain, int steps, int batch, float lr,
+                    float* loss_curve) {
+    if (!h || ntrain <= 0 || steps <= 0 || batch <= 0 || ntrain > h->cap)
+        return c2p_fail("disco_mlp_train: bad arguments");
+    if (disco_c2p_generate(h, seed, 0, ntrain)) return -1;
+    return train_resident(h, seed, ntrain, steps, batch, lr, loss_curve);
+}
+
+int disco_mlp_train_resident(disco_c2p_t* h, unsigned long long seed, int steps, int batch, float lr,
+                             float* loss_curve) {
+    if (!h || h->n <= 0 || steps <= 0 || batch <= 0) return c2p_fail("disco_mlp_train_resident: bad arguments");
+    return train_resident(h, seed, h->n, steps, batch, lr, loss_curve);
 }
 
 int disco_mlp_get(disco_c2p_t* h, float* params, float* norm) {

@@ -2,6 +2,7 @@
 // RK2 steps, upload/download and timing.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include "../../include/disco_gpu.h"
 #include "device_state.h"
 #include "host_setup.h"
+#include "mlp_infer.h"
 
 namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
@@ -65,6 +67,13 @@ struct disco_gpu {
     int parity = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // GRMHD in-loop cons2prim: learned start and counters
+    unsigned short* mlp_wt = nullptr;
+    float* mlp_w = nullptr;
+    float mlp_norm[20] = {};
+    bool learned = false;
+    unsigned long long* c2p_stats = nullptr;
+    bool counting = false;
 };
 
 static DevState make_state(disco_gpu* h, int substep) {
@@ -120,6 +129,12 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.efin = h->efin;
     S.fflags = h->fflags;
     S.err = h->err;
+    if (h->learned) {
+        S.mlp_wt = h->mlp_wt;
+        S.mlp_w = h->mlp_w;
+        std::memcpy(S.mlp_norm, h->mlp_norm, sizeof(S.mlp_norm));
+    }
+    S.c2p_stats = h->counting ? h->c2p_stats : nullptr;
     return S;
 }
 
@@ -267,7 +282,7 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot[0],  h->rot[1],     h->fphi,
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
-                    h->efin,  h->fflags,   h->err,   h->pdg};
+                    h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -604,6 +619,53 @@ int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, do
         ++k;
     }
     return k;
+}
+
+int disco_gpu_set_c2p_net(disco_gpu_t* h, const float* params, const float* norm) {
+    if (!h) return fail("disco_gpu_set_c2p_net: null handle");
+    if (h->runs[0].par.hydro != DISCO_HYDRO_GRMHD) return fail("disco_gpu_set_c2p_net: the runs are not GRMHD");
+    if (!params) {
+        h->learned = false;
+        h->shape.c2p_learned = 0;
+        return capture_graphs(h);
+    }
+    if (!norm) return fail("disco_gpu_set_c2p_net: norm is NULL");
+    for (int i = 0; i < 2 * mlp::NIN + 2 * mlp::NOUT; ++i)
+        if (!std::isfinite(norm[i]) || (((i >= mlp::NIN && i < 2 * mlp::NIN) || i >= 2 * mlp::NIN + mlp::NOUT) &&
+                                        !(norm[i] > 0.0f)))
+            return fail("disco_gpu_set_c2p_net: bad standardisation constants");
+    if (!h->mlp_wt && (dalloc(&h->mlp_wt, mlp::WT_TOTAL) || dalloc(&h->mlp_w, mlp::NPARAM))) return -1;
+    std::vector<unsigned short> T(mlp::WT_TOTAL);
+    mlp::mlp_pack_bf16(params, T.data());
+    HIPCHK(hipMemcpy(h->mlp_wt, T.data(), T.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->mlp_w, params, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    std::memcpy(h->mlp_norm, norm, sizeof(h->mlp_norm));
+    h->learned = true;
+    h->shape.c2p_learned = 1;
+    return capture_graphs(h);
+}
+
+int disco_gpu_c2p_count(disco_gpu_t* h, int enable) {
+    if (!h) return fail("disco_gpu_c2p_count: null handle");
+    if (h->runs[0].par.hydro != DISCO_HYDRO_GRMHD) return fail("disco_gpu_c2p_count: the runs are not GRMHD");
+    if (!h->c2p_stats) {
+        if (dalloc(&h->c2p_stats, 4)) return -1;
+        HIPCHK(hipMemset(h->c2p_stats, 0, 4 * sizeof(unsigned long long)));
+    }
+    h->counting = enable != 0;
+    return capture_graphs(h);
+}
+
+int disco_gpu_c2p_stats(disco_gpu_t* h, long long* out, int reset) {
+    if (!h || !out) return fail("disco_gpu_c2p_stats: bad arguments");
+    unsigned long long v[4] = {0, 0, 0, 0};
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->c2p_stats) {
+        HIPCHK(hipMemcpy(v, h->c2p_stats, sizeof(v), hipMemcpyDeviceToHost));
+        if (reset) HIPCHK(hipMemset(h->c2p_stats, 0, sizeof(v)));
+    }
+    for (int i = 0; i < 4; ++i) out[i] = (long long)v[i];
+    return 0;
 }
 
 int disco_gpu_time_steps(disco_gpu_t* h, int nsteps, double* ms) {

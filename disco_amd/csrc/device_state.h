@@ -108,6 +108,12 @@ struct DevState {
     double* efin;        // [4][ncell] final edge EMFs E0, E1, E4, E5 (face average, advective part removed)
     int* fflags;         // per cell: flip flag bits of its owned faces (1 r-out, 2 r-in, 4 z-up, 8 z-down)
     int* err;            // device error word (bit 1: unsupported flip pattern)
+    // ---- GRMHD: learned start of the in-loop cons2prim (null: the plain start) ----
+    const unsigned short* mlp_wt;   // transposed BF16 weights of the guess network
+    const float* mlp_w;             // its FP32 parameters (the biases are read from here)
+    float mlp_norm[20];             // feature mean (8), std (8); target mean (2), std (2)
+    unsigned long long* c2p_stats;  // [4] Newton iterations, solves, learned starts that fell back,
+                                    // solves unconverged at max_iter (null: not counted)
 };
 
 // What a batch needs launched each substep (uniform over its runs).
@@ -116,6 +122,7 @@ struct StepShape {
     int has_z;  // some run has Num_Z > 1: z-tracks exist
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport
+    int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

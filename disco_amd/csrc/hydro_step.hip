@@ -10,9 +10,13 @@
 //   k_stage_a -> k_plm_trans -> k_flux -> k_emf_sum -> k_emf_avg1 ->
 //   k_emf_face -> k_ct_update_a -> [k_flip_flag -> k_flip_chain]
 //   -> k_update_b (+ k_zcopy, k_planets)
+// GRMHD runs the same sequence; with a trained guess network attached,
+// k_update_b_learned replaces k_update_b (network on MFMA, then the Newton
+// solve from its guess).
 // The CFL reduction of the next step is fused into the last substep's update.
 #include <hip/hip_runtime.h>
 
+#include "mlp_infer.h"
 #include "step_cells.h"
 
 namespace disco {
@@ -33,6 +37,27 @@ namespace disco {
 #define DISCO_OCC_CT 4
 #endif
 #define DISCO_OCC(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, 8)))
+
+// Per module: the GRMHD leaves (metric, its derivatives, the relativistic
+// fluxes) need 240-280 VGPRs; capping them spills hundreds of registers to
+// scratch, whose traffic costs more than the occupancy buys.
+template <class Ph>
+struct Occ {
+    static constexpr int FLUX = DISCO_OCC_FLUX, PLM = DISCO_OCC_PLM, CT = DISCO_OCC_CT;
+};
+template <>
+struct Occ<PhysGRMHD> {
+    static constexpr int FLUX = 0, PLM = 0, CT = 0;
+};
+// launch the CT face update and the cell update separately
+template <class Ph>
+struct SplitCT {
+    static constexpr bool value = false;
+};
+template <>
+struct SplitCT<PhysGRMHD> {
+    static constexpr bool value = true;
+};
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Remap so that XCD x works on one contiguous band of cells: partner cells in
@@ -124,13 +149,13 @@ __global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int firs
 }
 
 template <class Ph>
-__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_PLM) void k_plm_trans(DevState S, int has_z) {
+__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::PLM) void k_plm_trans(DevState S, int has_z) {
     int c = cell_index();
     if (c < S.ncell) plm_trans_cell<Ph>(S, c, has_z);
 }
 
 template <class Ph>
-__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_FLUX) void k_flux(DevState S, double dtfac) {
+__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::FLUX) void k_flux(DevState S, double dtfac) {
     int c = cell_index();
     if (c < S.ncell) flux_cell<Ph>(S, c, dtfac);
 }
@@ -151,19 +176,113 @@ __global__ __launch_bounds__(256) void k_update(DevState S, double dtfac, int fi
     if (want_dt) dt_reduce(S, run, dtc);
 }
 
+// Wavefront sums of the in-loop cons2prim counters into S.c2p_stats (when
+// counting is on).  Every lane of the wavefront must call it.
+DEV void c2p_count(const DevState& S, int iters, int solves, int fallbacks, int unconverged) {
+    if (!S.c2p_stats) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        iters += __shfl_xor(iters, o);
+        solves += __shfl_xor(solves, o);
+        fallbacks += __shfl_xor(fallbacks, o);
+        unconverged += __shfl_xor(unconverged, o);
+    }
+    if ((threadIdx.x & 63) == 0 && solves > 0) {
+        atomicAdd(&S.c2p_stats[0], (unsigned long long)iters);
+        atomicAdd(&S.c2p_stats[1], (unsigned long long)solves);
+        atomicAdd(&S.c2p_stats[2], (unsigned long long)fallbacks);
+        atomicAdd(&S.c2p_stats[3], (unsigned long long)unconverged);
+    }
+}
+
 template <class Ph>
 __global__ __launch_bounds__(256) void k_update_b(DevState S, int first, int want_dt) {
     constexpr int NQ = Ph::NQ;
     int c = cell_index();
     int run = -1;
     double dtc = 1.0e300;
+    int it = -1, ok = 1;
     if (c < S.ncell) {
         const size_t N = S.ncell;
         double cons[NQ];
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] = S.cons[q * N + c];
         run = S.ann[S.cell_ann[c]].run;
-        dtc = update_b_cell<Ph>(S, c, first, want_dt, cons);
+        dtc = update_b_cell<Ph>(S, c, first, want_dt, cons, &it, &ok);
     }
+    if (Ph::C2P_GUESS) c2p_count(S, it > 0 ? it : 0, it >= 0, 0, it >= 0 && !ok);
+    if (want_dt) dt_reduce(S, run, dtc);
+}
+
+// GRMHD second half with the learned start: B faces to cells and the
+// invariants of every real cell, the guess network on MFMA for the 64 cells
+// of each wavefront (BF16, FP32 accumulate), then the reference's Newton
+// solve (same residuals, tolerance, 10 extra iterations, max_iter) from the
+// network's (W, eta).  A start that does not converge, or converges outside
+// 0 <= v^2 < 1, eta >= 1, falls back to the plain start (the cell's previous
+// prims, grmhd.c:885-893).  Non-real lanes feed the network its mean input.
+__global__ __launch_bounds__(256) void k_update_b_learned(DevState S, int first, int want_dt) {
+    __shared__ mlp::MlpLds L;
+    __shared__ unsigned short sF[4][64 * mlp::K0P];
+    __shared__ float sY[4][64 * 2];
+    mlp::mlp_load(L, S.mlp_wt, S.mlp_w);
+    const int wave = threadIdx.x >> 6;
+    const int c = cell_index();
+    const bool valid = c < S.ncell;
+    const size_t N = S.ncell;
+    int run = -1;
+    double dtc = 1.0e300;
+    bool real = false;
+    double cons[8];
+    CellGeom G;
+    if (valid) {
+        _Pragma("unroll") for (int q = 0; q < 8; ++q) cons[q] = S.cons[q * N + c];
+        run = S.ann[S.cell_ann[c]].run;
+        real = update_b_pre<PhysGRMHD>(S, c, first, cons, &G);
+    }
+    grmhd::Metric g;
+    grmhd::C2PInvariants v;
+    float f[mlp::NIN];
+    if (real) {
+        const PhysParams& P = S.runs[run].ph;
+        double cons1[8];
+        _Pragma("unroll") for (int q = 0; q < 8; ++q) cons1[q] = cons[q] / G.dV;  // grmhd.c:232-234
+        grmhd::kerr_metric(G.x, P.gr_M, P.gr_a, &g);
+        grmhd::c2p_invariants(g, cons1, G.x, &v);
+        mlp::c2p_features(v, f);
+    } else {
+        _Pragma("unroll") for (int i = 0; i < mlp::NIN; ++i) f[i] = S.mlp_norm[i];
+    }
+    _Pragma("unroll") for (int i = 0; i < mlp::NIN; ++i) f[i] = (f[i] - S.mlp_norm[i]) / S.mlp_norm[mlp::NIN + i];
+    float y[2];
+    mlp::mlp_wave(L, sF[wave], sY[wave], f, y);
+    int iters = 0, fell = 0, bad = 0;
+    if (real) {
+        const PhysParams& P = S.runs[run].ph;
+        const float* tn = S.mlp_norm + 2 * mlp::NIN;
+        float y0 = y[0] * tn[2] + tn[0];
+        float y1 = y[1] * tn[3] + tn[1];
+        double W = exp((double)fmaxf(y0, 0.0f));
+        double v20 = 1.0 - 1.0 / (W * W);
+        double eta0 = fmax(v.e * exp((double)y1), 1.0 + 1e-10);
+        double v2, eta;
+        int ok;
+        iters = grmhd::c2p_newton(P.gamma_law, v, v20, eta0, grmhd::C2P_PREC, grmhd::C2P_MAX_ITER,
+                                  grmhd::C2P_NEXTRA, &v2, &eta, &ok);
+        if (!ok || !(v2 >= 0.0 && v2 < 1.0) || !(eta >= 1.0)) {
+            double prev[8];
+            _Pragma("unroll") for (int q = 0; q < 8; ++q) prev[q] = S.prim[q * N + c];
+            grmhd::c2p_guess_from_prim(P.gamma_law, g, prev, G.x, &v20, &eta0);
+            iters += grmhd::c2p_newton(P.gamma_law, v, v20, eta0, grmhd::C2P_PREC, grmhd::C2P_MAX_ITER,
+                                       grmhd::C2P_NEXTRA, &v2, &eta, &ok);
+            fell = 1;
+        }
+        bad = !ok;
+        double prim[8];
+        grmhd::c2p_recover(P.gamma_law, P.rho_floor, P.pre_floor, g, v, v2, eta, G.x, prim);
+        dtc = update_b_post<PhysGRMHD>(S, c, prim, G, want_dt);
+    } else if (valid) {
+        update_b_ghost<PhysGRMHD>(S, c);
+    }
+    c2p_count(S, iters, real, fell, bad);
     if (want_dt) dt_reduce(S, run, dtc);
 }
 
@@ -193,11 +312,27 @@ __global__ __launch_bounds__(256) void k_emf_face(DevState S, int has_z) {
 // of the cell update (conserved state, sources, mesh move): independent work
 // on the same cell.
 template <class Ph>
-__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_CT) void k_ct_update_a(DevState S, double dtfac, int first,
+__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_ct_update_a(DevState S, double dtfac, int first,
                                                                            int has_z) {
     int c = cell_index();
     if (c < S.ncell) {
         ct_update_cell(S, c, dtfac, has_z);
+        double cons[Ph::NQ];
+        update_a_cell<Ph>(S, c, dtfac, first, cons);
+    }
+}
+
+// The same two halves as separate launches, for modules whose cell update is
+// register-heavy enough that the fused kernel drops to one wave per SIMD.
+__global__ __launch_bounds__(256) void k_ct_update(DevState S, double dtfac, int has_z) {
+    int c = cell_index();
+    if (c < S.ncell) ct_update_cell(S, c, dtfac, has_z);
+}
+
+template <class Ph>
+__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_update_a(DevState S, double dtfac, int first) {
+    int c = cell_index();
+    if (c < S.ncell) {
         double cons[Ph::NQ];
         update_a_cell<Ph>(S, c, dtfac, first, cons);
     }
@@ -253,12 +388,20 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, g, dim3(B), st, S, sh.has_z);
             DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, g, dim3(B), st, S);
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, g, dim3(B), st, S, sh.has_z);
-            DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, g, dim3(B), st, S, dtfac, first, sh.has_z);
+            if (SplitCT<Ph>::value) {
+                DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, g, dim3(B), st, S, dtfac, sh.has_z);
+                DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, g, dim3(B), st, S, dtfac, first);
+            } else {
+                DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, g, dim3(B), st, S, dtfac, first, sh.has_z);
+            }
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, g, dim3(B), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, g, dim3(B), st, S, sh.has_z);
             }
-            DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, g, dim3(B), st, S, first, want_dt);
+            if (Ph::C2P_GUESS && sh.c2p_learned)
+                DISCO_LAUNCH(hook, "k_update_b_learned", k_update_b_learned, g, dim3(B), st, S, first, want_dt);
+            else
+                DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, g, dim3(B), st, S, first, want_dt);
         } else {
             DISCO_LAUNCH(hook, "k_update", k_update<Ph>, g, dim3(B), st, S, dtfac, first, want_dt);
         }

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "disco_hd.h"
+#include "disco_params.h"
 
 namespace disco {
 namespace grmhd {

@@ -20,6 +20,11 @@
 namespace disco {
 namespace grmhd {
 
+// cons2prim_solve_adiabatic_noble2d's constants (grmhd.c:815-817)
+constexpr double C2P_PREC = 1.0e-12;
+constexpr int C2P_MAX_ITER = 30;
+constexpr int C2P_NEXTRA = 10;
+
 // ------------------------------------------------------------------ duals
 struct Dual {
     double v, d;
@@ -58,19 +63,12 @@ DHD void kerr_parts_dual(Dual R, Dual z, double M, double a, Dual* al, Dual* be,
     for (int i = 0; i < 3; ++i) be[i] = f * lu[i];
 }
 
-// Derivatives of the metric at x: dg[i][16] = d_i g_{mu nu} for i = r, z
-// (metric_der_g with i = 1, 3), da[4] = d_mu alpha, db[12] = d_mu beta^j
-// (t and phi are Killing directions: zero).
-struct MetricDer {
-    double dg_r[16], dg_z[16];
-    double da[4];
-    double db[12];
-};
-
-DHD void metric_der(const double* x, double M, double a, MetricDer* D) {
-    for (int k = 0; k < 16; ++k) D->dg_r[k] = D->dg_z[k] = 0.0;
-    for (int k = 0; k < 4; ++k) D->da[k] = 0.0;
-    for (int k = 0; k < 12; ++k) D->db[k] = 0.0;
+// Derivatives of the metric at x along one coordinate direction (dir 0: r =
+// metric index mu = 1, dir 1: z = mu = 3; t and phi are Killing directions):
+// dg[16] = d_mu g_{nu lambda} (metric_der_g), *da = d_mu alpha
+// (metric_der_lapse), db[3] = d_mu beta^j (metric_der_shift).
+DHD void metric_der_dir(const double* x, double M, double a, int dir, double* dg, double* da, double* db) {
+    for (int k = 0; k < 16; ++k) dg[k] = 0.0;
     double r = x[0];
     double z = x[2];
     if (a == 0.0) {
@@ -81,61 +79,60 @@ DHD void metric_der(const double* x, double M, double a, MetricDer* D) {
         double MoR2 = M / (R * R);
         double MoR3 = M / (R * R * R);
         double MoR4 = M / (R * R * R * R);
-        double* dg = D->dg_r;
-        dg[0] = -2 * MoR2 * st;
-        dg[1] = 2 * MoR2 * (1.0 - 2 * st * st);
-        dg[3] = -4 * MoR3 * z * st;
-        dg[5] = 2 * MoR3 * r * (2.0 - 3.0 * st * st);
-        dg[7] = 2 * MoR3 * z * (1.0 - 3.0 * st * st);
-        dg[10] = 2 * r;
-        dg[15] = -6 * MoR4 * z * z * st;
-        dg[4] = dg[1];
-        dg[12] = dg[3];
-        dg[13] = dg[7];
-        dg = D->dg_z;
-        dg[0] = -2 * MoR2 * ct;
-        dg[1] = -4 * MoR3 * r * ct;
-        dg[3] = 2 * MoR2 * (1.0 - 2 * ct * ct);
-        dg[5] = -6 * MoR4 * r * r * ct;
-        dg[7] = 2 * MoR3 * r * (1.0 - 3.0 * ct * ct);
-        dg[15] = 2 * MoR3 * z * (2.0 - 3.0 * ct * ct);
-        dg[4] = dg[1];
-        dg[12] = dg[3];
-        dg[13] = dg[7];
         double al = 1.0 / sqrt(1.0 + 2 * M / R);
-        D->da[1] = al * al * al * MoR2 * st;
-        D->da[3] = al * al * al * MoR2 * ct;
         double denom = 1.0 / (R * R + 2 * M * R);
         double d_denom = -denom * denom * 2 * (R + M);
-        D->db[3] = 2 * M * denom + 2 * M * r * d_denom * st;
-        D->db[4] = 0.0;
-        D->db[5] = 2 * M * z * d_denom * st;
-        D->db[9] = 2 * M * r * d_denom * ct;
-        D->db[10] = 0.0;
-        D->db[11] = 2 * M * denom + 2 * M * z * d_denom * ct;
+        if (dir == 0) {
+            dg[0] = -2 * MoR2 * st;
+            dg[1] = 2 * MoR2 * (1.0 - 2 * st * st);
+            dg[3] = -4 * MoR3 * z * st;
+            dg[5] = 2 * MoR3 * r * (2.0 - 3.0 * st * st);
+            dg[7] = 2 * MoR3 * z * (1.0 - 3.0 * st * st);
+            dg[10] = 2 * r;
+            dg[15] = -6 * MoR4 * z * z * st;
+            *da = al * al * al * MoR2 * st;
+            db[0] = 2 * M * denom + 2 * M * r * d_denom * st;
+            db[1] = 0.0;
+            db[2] = 2 * M * z * d_denom * st;
+        } else {
+            dg[0] = -2 * MoR2 * ct;
+            dg[1] = -4 * MoR3 * r * ct;
+            dg[3] = 2 * MoR2 * (1.0 - 2 * ct * ct);
+            dg[5] = -6 * MoR4 * r * r * ct;
+            dg[7] = 2 * MoR3 * r * (1.0 - 3.0 * ct * ct);
+            dg[15] = 2 * MoR3 * z * (2.0 - 3.0 * ct * ct);
+            *da = al * al * al * MoR2 * ct;
+            db[0] = 2 * M * r * d_denom * ct;
+            db[1] = 0.0;
+            db[2] = 2 * M * denom + 2 * M * z * d_denom * ct;
+        }
+        dg[4] = dg[1];
+        dg[12] = dg[3];
+        dg[13] = dg[7];
         return;
     }
-    for (int dir = 0; dir < 2; ++dir) {
-        Dual R{r, dir == 0 ? 1.0 : 0.0};
-        Dual Z{z, dir == 1 ? 1.0 : 0.0};
-        Dual al, be[3], g4[16];
-        kerr_parts_dual(R, Z, M, a, &al, be, g4);
-        int mu = dir == 0 ? 1 : 3;
-        double* dg = dir == 0 ? D->dg_r : D->dg_z;
-        for (int k = 0; k < 16; ++k) dg[k] = g4[k].d;
-        D->da[mu] = al.d;
-        for (int j = 0; j < 3; ++j) D->db[3 * mu + j] = be[j].d;
-    }
+    Dual R{r, dir == 0 ? 1.0 : 0.0};
+    Dual Z{z, dir == 1 ? 1.0 : 0.0};
+    Dual al, be[3], g4[16];
+    kerr_parts_dual(R, Z, M, a, &al, be, g4);
+    for (int k = 0; k < 16; ++k) dg[k] = g4[k].d;
+    *da = al.d;
+    for (int j = 0; j < 3; ++j) db[j] = be[j].d;
 }
 
-// frame_der_U (Frame/normal.c:21-42): dU[4 mu + nu] = d_mu U^nu
-DHD void frame_der_U(const Metric& g, const MetricDer& D, double* dU) {
-    double ia = 1.0 / g.al;
-    for (int i = 0; i < 16; i++) dU[i] = 0.0;
-    for (int i = 1; i < 4; i += 2) {  // non-Killing: r, z
-        dU[4 * i] = -ia * ia * D.da[i];
-        for (int j = 0; j < 3; j++) dU[4 * i + j + 1] = g.be[j] * ia * ia * D.da[i] - D.db[3 * i + j] * ia;
-    }
+// Both directions in the reference's layout: dg_r/dg_z = metric_der_g(x, 1/3),
+// da[4] = metric_der_lapse, db[12] = metric_der_shift (db[3 mu + j]).
+struct MetricDer {
+    double dg_r[16], dg_z[16];
+    double da[4];
+    double db[12];
+};
+
+DHD void metric_der(const double* x, double M, double a, MetricDer* D) {
+    for (int k = 0; k < 4; ++k) D->da[k] = 0.0;
+    for (int k = 0; k < 12; ++k) D->db[k] = 0.0;
+    metric_der_dir(x, M, a, 0, D->dg_r, &D->da[1], &D->db[3]);
+    metric_der_dir(x, M, a, 1, D->dg_z, &D->da[3], &D->db[9]);
 }
 
 // ------------------------------------------------------------------ leaves
@@ -419,7 +416,10 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
     return flux_to_E(Flux, Ustr, x, dim);
 }
 
-// grmhd.c:339-454: geometric source terms at the cell's coordinate centre
+// grmhd.c:339-454: geometric source terms at the cell's coordinate centre.
+// The metric derivatives are taken one direction at a time (frame_der_U,
+// Frame/normal.c:21-42, restated per direction) to keep the live state
+// small; the sums run in the reference's order.
 DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                 double dVdt) {
     const double gamma_law = P.gamma_law;
@@ -427,10 +427,6 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double r = x[0];
     Metric g;
     kerr_metric(x, P.gr_M, P.gr_a, &g);
-    MetricDer Dm;
-    metric_der(x, P.gr_M, P.gr_a, &Dm);
-    double dU[16];
-    frame_der_U(g, Dm, dU);
     const double al = g.al;
     const double* be = g.be;
     const double* gam = g.gam;
@@ -443,13 +439,6 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double l[4] = {0.0, prim[URR], r * r * prim[UPP], prim[UZZ]};
     double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
     double ia2 = 1.0 / (al * al);
-    double ig[16];
-    ig[0] = -ia2;
-    for (int mu = 0; mu < 3; mu++) {
-        ig[mu + 1] = be[mu] * ia2;
-        ig[4 * (mu + 1)] = ig[mu + 1];
-        for (int nu = 0; nu < 3; nu++) ig[4 * (mu + 1) + nu + 1] = igam[3 * mu + nu] - be[mu] * be[nu] * ia2;
-    }
     double igaml[3];
     igaml[0] = igam[0] * l[1] + igam[1] * l[2] + igam[2] * l[3];
     igaml[1] = igam[3] * l[1] + igam[4] * l[2] + igam[5] * l[3];
@@ -485,24 +474,44 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double dphi = get_dp(xp[1], xm[1], P.phi_max);
     double rcorr = r2_3 / (r * r);
     double pcorr = sin(0.5 * dphi) / (0.5 * dphi);
-    double Sk[3];
-    for (int i = 0; i < 3; i++) {
-        Sk[i] = 0.0;
-        if (i == 1) continue;  // metric_killing(phi)
-        const double* dg = i == 0 ? Dm.dg_r : Dm.dg_z;
-        for (int mu = 0; mu < 4; mu++)
-            for (int nu = 0; nu < 4; nu++)
-                Sk[i] += (rhoh * u[mu] * u[nu] * rcorr * pcorr + ig[4 * mu + nu] * Pt - b[mu] * b[nu]) * dg[4 * mu + nu];
-        Sk[i] *= 0.5;
+    double ia = 1.0 / al;
+    double Sk[3] = {0.0, 0.0, 0.0};  // phi (Sk[1]) is a Killing direction
+    double t[2][4];                  // the dU terms of mu = r, z
+    for (int dir = 0; dir < 2; ++dir) {
+        const int mu = 2 * dir + 1;
+        double dg[16], da, db[3];
+        metric_der_dir(x, P.gr_M, P.gr_a, dir, dg, &da, db);
+        double sk = 0.0;
+        for (int m = 0; m < 4; m++)
+            for (int n = 0; n < 4; n++) {
+                // ig^{mn} = gamma^{ij} - beta^i beta^j / alpha^2 etc. (grmhd.c:361-368)
+                double igmn;
+                if (m == 0 && n == 0)
+                    igmn = -ia2;
+                else if (m == 0)
+                    igmn = be[n - 1] * ia2;
+                else if (n == 0)
+                    igmn = be[m - 1] * ia2;
+                else
+                    igmn = igam[3 * (m - 1) + n - 1] - be[m - 1] * be[n - 1] * ia2;
+                sk += (rhoh * u[m] * u[n] * rcorr * pcorr + igmn * Pt - b[m] * b[n]) * dg[4 * m + n];
+            }
+        Sk[dir == 0 ? 0 : 2] = 0.5 * sk;
+        // d_mu U^nu (frame_der_U): nu = 0 and nu = 1..3
+        double dU0 = -ia * ia * da;
+        t[dir][0] = -(rhoh * u[mu] * l[0] - b[mu] * bd[0]) * dU0;
+        for (int j = 0; j < 3; j++) {
+            double dUj = be[j] * ia * ia * da - db[j] * ia;
+            int nu = j + 1;
+            if (mu == nu)
+                t[dir][nu] = -(rhoh * u[mu] * l[nu] + Pt - b[mu] * bd[nu]) * dUj;
+            else
+                t[dir][nu] = -(rhoh * u[mu] * l[nu] - b[mu] * bd[nu]) * dUj;
+        }
     }
     double S0 = -U[1] * Sk[0] - U[2] * Sk[1] - U[3] * Sk[2];
-    for (int mu = 0; mu <= 3; mu++)
-        for (int nu = 0; nu <= 3; nu++) {
-            if (mu == nu)
-                S0 += -(rhoh * u[mu] * l[nu] + Pt - b[mu] * bd[nu]) * dU[4 * mu + nu];
-            else
-                S0 += -(rhoh * u[mu] * l[nu] - b[mu] * bd[nu]) * dU[4 * mu + nu];
-        }
+    for (int nu = 0; nu < 4; ++nu) S0 += t[0][nu];
+    for (int nu = 0; nu < 4; ++nu) S0 += t[1][nu];
     cons[SRR] += jac * Sk[0] * dVdt;
     cons[LLL] += jac * Sk[1] * dVdt;
     cons[SZZ] += jac * Sk[2] * dVdt;
@@ -568,7 +577,9 @@ DHD double mindt(const PhysParams& P, const double* prim, double wc, const doubl
 
 // grmhd.c:228-245 with cons2prim_solve_adiabatic_noble2d (:813-1029): the
 // Newton solve starts from the cell's previous primitives (prim on entry).
-DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+// Returns the Newton iteration count; *okp (if given) = converged.
+DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
+                  int* okp = nullptr) {
     double cons1[NQ];
     _Pragma("unroll") for (int q = 0; q < NQ; q++) cons1[q] = cons[q] / dV;
     Metric g;
@@ -579,8 +590,10 @@ DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const 
     c2p_guess_from_prim(P.gamma_law, g, prim, x, &v20, &eta0);
     double v2, eta;
     int ok;
-    c2p_newton(P.gamma_law, v, v20, eta0, 1.0e-12, 30, 10, &v2, &eta, &ok);
+    int it = c2p_newton(P.gamma_law, v, v20, eta0, C2P_PREC, C2P_MAX_ITER, C2P_NEXTRA, &v2, &eta, &ok);
     c2p_recover(P.gamma_law, P.rho_floor, P.pre_floor, g, v, v2, eta, x, prim);
+    if (okp) *okp = ok;
+    return it;
 }
 
 }  // namespace grmhd

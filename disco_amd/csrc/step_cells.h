@@ -57,8 +57,11 @@ struct PhysEuler {
                     double dVdt) {
         euler::source(P, prim, cons, xp, xm, dVdt);
     }
-    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+    DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
+                      int* okp = nullptr) {
         euler::cons2prim<NQ>(P, cons, prim, x, dV);
+        if (okp) *okp = 1;
+        return 0;
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return euler::mindt(P, prim, w, xp, xm);
@@ -81,8 +84,11 @@ struct PhysMHD {
                     double dVdt) {
         mhd::source(P, prim, cons, xp, xm, dVdt);
     }
-    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
+    DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
+                      int* okp = nullptr) {
         mhd::cons2prim(P, cons, prim, x, dV);
+        if (okp) *okp = 1;
+        return 0;
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return mhd::mindt(P, prim, w, xp, xm);
@@ -105,8 +111,9 @@ struct PhysGRMHD {
                     double dVdt) {
         grmhd::source(P, prim, cons, xp, xm, dVdt);
     }
-    DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
-        grmhd::cons2prim(P, cons, prim, x, dV);
+    DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
+                      int* okp = nullptr) {
+        return grmhd::cons2prim(P, cons, prim, x, dV, okp);
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return grmhd::mindt(P, prim, w, xp, xm);
@@ -712,58 +719,100 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
 template <class Ph>
 DHD void b_faces_to_cell(const DevState& S, const AnnDev& A, int c, int first, double* cons);
 
-// Second half of the update: (MHD with CT) B_faces_to_cells (bfields.c:88-309),
-// calc_prim (misc.c:300-335) and the fixed boundaries (Boundary/fixed.c:11-60,
-// boundary_functions.c:161-266) on the mesh after the move.  The second
-// substep also produces the next step's CFL candidate from the prims it has
-// just written (misc.c:39-84 runs on exactly this state at the start of the
-// next step, with this step's mesh speeds).  cons[] holds the real cell's
-// conserved state from update_a_cell.
+// Geometry of a real cell on the moved mesh, as calc_prim builds it
+// (misc.c:314-331).
+struct CellGeom {
+    double xp[3], xm[3], x[3], dV;
+};
+
+// Second half of the update, split so that a caller can run the guess
+// network between its halves: update_b_pre does (MHD with CT)
+// B_faces_to_cells (bfields.c:88-309) and the cell geometry; the caller
+// solves cons2prim (calc_prim, misc.c:300-335); update_b_post stores the
+// prims and, in the second substep, returns the next step's CFL candidate
+// from them (misc.c:39-84 runs on exactly this state at the start of the next
+// step, with this step's mesh speeds); update_b_ghost re-initialises fixed
+// boundary cells (Boundary/fixed.c:11-60, boundary_functions.c:161-266).
+// update_b_pre returns false for cells that are not real.
 template <class Ph>
-DHD double update_b_cell(const DevState& S, int c, int first, int want_dt, double* cons) {
-    constexpr int NQ = Ph::NQ;
+DHD bool update_b_pre(const DevState& S, int c, int first, double* cons, CellGeom* G) {
     const AnnDev& A = S.ann[S.cell_ann[c]];
-    const RunDev& Rn = S.runs[A.run];
-    const PhysParams& P = Rn.ph;
+    if (!(A.flags & ANN_REAL)) return false;
+    const double Pmax = S.runs[A.run].ph.phi_max;
+    const double phip = S.piph_n[c];
+    const double phim = phip - S.dphi_n[c];
+    if (Ph::MHD && S.Phi) b_faces_to_cell<Ph>(S, A, c, first, cons);
+    G->xp[0] = A.rp;
+    G->xp[1] = phip;
+    G->xp[2] = A.zp;
+    G->xm[0] = A.rm;
+    G->xm[1] = phim;
+    G->xm[2] = A.zm;
+    G->dV = get_dV(G->xp, G->xm, Pmax);
+    G->x[0] = A.rc;
+    G->x[1] = 0.5 * (phim + phip);
+    G->x[2] = A.zc;
+    return true;
+}
+
+template <class Ph>
+DHD double update_b_post(const DevState& S, int c, const double* prim, const CellGeom& G, int want_dt) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
     const size_t N = S.ncell;
-    const double Pmax = P.phi_max;
+    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) S.prim[q * N + c] = prim[q];
+    if (!want_dt) return 1.0e300;
+    // the CFL step of the next step reads this mesh (piph_n is the current
+    // mesh after the first substep)
+    int i = c - A.off;
+    int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
+    double w = .5 * (S.wiph[cm] + S.wiph[c]);
+    return Ph::mindt(S.runs[A.run].ph, prim, w, G.xp, G.xm);
+}
+
+template <class Ph>
+DHD void update_b_ghost(const DevState& S, int c) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(A.flags & ANN_GHOST_FILL)) return;
+    const RunDev& Rn = S.runs[A.run];
+    const size_t N = S.ncell;
     const double phip = S.piph_n[c];
     const double dph = S.dphi_n[c];
+    double x[3];
+    x[0] = A.rc_b;
+    x[2] = A.zc_b;
+    if (Rn.bc == DISCO_BC_FIXED)
+        x[1] = phip - .5 * dph;
+    else
+        x[1] = 0.5 * (phip + (phip - dph));
+    double p8[8];
+    disco_initial(&Rn.ic, p8, x);
+    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) S.prim[q * N + c] = p8[q];
+}
 
-    if (A.flags & ANN_REAL) {
-        if (Ph::MHD && S.Phi) b_faces_to_cell<Ph>(S, A, c, first, cons);
-        double phim = phip - dph;
-        double xp[3] = {A.rp, phip, A.zp};
-        double xm[3] = {A.rm, phim, A.zm};
-        double dV = get_dV(xp, xm, Pmax);
-        double x[3] = {A.rc, 0.5 * (phim + phip), A.zc};
-        double prim[NQ];
-        if (Ph::C2P_GUESS) {  // calc_prim hands cons2prim the cell's current prims (misc.c:300-335)
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[q * N + c];
-        }
-        Ph::cons2prim(P, cons, prim, x, dV);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = prim[q];
-        if (want_dt) {
-            // the CFL step of the next step reads this mesh (piph_n is the
-            // current mesh after the first substep)
-            int i = c - A.off;
-            int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
-            double w = .5 * (S.wiph[cm] + S.wiph[c]);
-            return Ph::mindt(P, prim, w, xp, xm);
-        }
-    } else if (A.flags & ANN_GHOST_FILL) {
-        double x[3];
-        x[0] = A.rc_b;
-        x[2] = A.zc_b;
-        if (Rn.bc == DISCO_BC_FIXED)
-            x[1] = phip - .5 * dph;
-        else
-            x[1] = 0.5 * (phip + (phip - dph));
-        double p8[8];
-        disco_initial(&Rn.ic, p8, x);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.prim[q * N + c] = p8[q];
+// The whole second half with the module's own cons2prim.  cons[] holds the
+// real cell's conserved state from update_a_cell; *c2p_it and *c2p_ok (if
+// given) receive the Newton iteration count of a real cell's cons2prim and
+// whether it converged.
+template <class Ph>
+DHD double update_b_cell(const DevState& S, int c, int first, int want_dt, double* cons, int* c2p_it = nullptr,
+                         int* c2p_ok = nullptr) {
+    constexpr int NQ = Ph::NQ;
+    CellGeom G;
+    if (!update_b_pre<Ph>(S, c, first, cons, &G)) {
+        update_b_ghost<Ph>(S, c);
+        return 1.0e300;
     }
-    return 1.0e300;
+    const PhysParams& P = S.runs[S.ann[S.cell_ann[c]].run].ph;
+    const size_t N = S.ncell;
+    double prim[NQ];
+    if (Ph::C2P_GUESS) {  // calc_prim hands cons2prim the cell's current prims (misc.c:300-335)
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) prim[q] = S.prim[q * N + c];
+    }
+    int ok = 1;
+    int it = Ph::cons2prim(P, cons, prim, G.x, G.dV, &ok);
+    if (c2p_it) *c2p_it = it;
+    if (c2p_ok) *c2p_ok = ok;
+    return update_b_post<Ph>(S, c, prim, G, want_dt);
 }
 
 // Periodic z: what a one-rank MPI run's self-wraparound exchange does

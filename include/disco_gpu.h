@@ -102,6 +102,23 @@ int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, do
  * recorded on the handle's stream around the launches. */
 int disco_gpu_time_steps(disco_gpu_t* h, int nsteps, double* ms);
 
+/* GRMHD runs only; no counterpart in the reference, whose in-loop cons2prim
+ * (calc_prim, misc.c:300-335 -> grmhd.c:228-245) always starts its Newton
+ * solve from the cell's previous primitives (grmhd.c:885-893).
+ * disco_gpu_set_c2p_net: start the solve from a trained guess network
+ * instead -- params[13186] FP32 in the layout disco_mlp_get returns, norm[20]
+ * its standardisation (feature mean[8], std[8], target mean[2], std[2]); the
+ * network runs in BF16 on MFMA, the Newton solve and its stopping rule are
+ * the reference's, and a start that does not converge falls back to the
+ * previous primitives.  params == NULL returns to the plain start.
+ * disco_gpu_c2p_count: count (enable != 0) or stop counting the in-loop
+ * solves; disco_gpu_c2p_stats: out[4] = Newton iterations, solves, learned
+ * starts that fell back, solves unconverged at max_iter since the last reset
+ * (reset != 0 clears them after reading). */
+int disco_gpu_set_c2p_net(disco_gpu_t* h, const float* params, const float* norm);
+int disco_gpu_c2p_count(disco_gpu_t* h, int enable);
+int disco_gpu_c2p_stats(disco_gpu_t* h, long long* out, int reset);
+
 void disco_gpu_destroy(disco_gpu_t* h);
 const char* disco_gpu_last_error(void);
 
@@ -153,6 +170,10 @@ int disco_c2p_errors(disco_c2p_t* h, unsigned long long seed, long k0, float* er
  * states seed/0..ntrain-1, then `steps` Adam steps on minibatches */
 int disco_mlp_train(disco_c2p_t* h, unsigned long long seed, long ntrain, int steps, int batch, float lr,
                     float* loss_curve);
+/* the same on the states resident in the handle (disco_c2p_upload), e.g.
+ * pairs taken from a run's own early steps; `seed` drives the minibatches */
+int disco_mlp_train_resident(disco_c2p_t* h, unsigned long long seed, int steps, int batch, float lr,
+                             float* loss_curve);
 /* FP32 master parameters (13186) and normalisation (8+8 feature mean/std,
  * 2+2 target mean/std) */
 int disco_mlp_get(disco_c2p_t* h, float* params, float* norm);

@@ -130,3 +130,48 @@ def test_grmhd_kerr_full_size_runs():
     rr = real[lo:hi] & inner
     assert (np.abs(d2 - d0)[rr] <= 1e-13 * sc[rr]).all()
     dom.close()
+
+
+def test_learned_c2p_in_loop_matches_plain():
+    """The in-loop learned start (network trained on pairs from this run's own
+    first steps, BF16 MFMA inference, the reference's Newton polish, the
+    previous prims as fallback) gives the plain path's state: after 3 more
+    steps every field agrees to 1e-10 on its scale and the face fluxes to
+    1e-10, with every solve converged.  Reports iterations per solve of both
+    starts and the fallback rate."""
+    p = params("c5_grmhd_torus.par", Num_R=48, Num_Z=48)
+    a = D.Domain(p)
+    a.c2p_count(True)
+    a.step(2)
+    early = a.c2p_stats(reset=True)
+    assert early["solves"] == 2 * 2 * a.info[0].nreal and early["unconverged"] == 0
+    x, dV, real, st = a.cell_positions()
+    cons = st["cons"][:, real] / dV
+    net = D.C2P(capacity=cons.shape[1], gamma=p.Adiabatic_Index, M=p.metricPar2, a=p.metricPar3)
+    net.upload(cons, x)
+    loss = net.train_resident(seed=11, steps=400, batch=2048, lr=2e-3)
+    assert loss[-20:].mean() < 0.25 * loss[:20].mean()
+    w, nrm = net.get_mlp()
+    net.close()
+    # a second domain at the same state, stepping with the learned start
+    b = D.Domain(p)
+    b.upload(0, prim=st["prim"], cons=st["cons"], piph=st["piph"], dphi=st["dphi"], wiph=st["wiph"])
+    b.upload_faces(0, st["Phi"])
+    b.set_c2p_net(w, nrm)
+    b.c2p_count(True)
+    b.step(3)
+    a.step(3)
+    sa, sb = a.state(0), b.state(0)
+    plain, learned = a.c2p_stats(), b.c2p_stats()
+    print(f"plain: {plain['iterations'] / plain['solves']:.2f} it/solve; learned: "
+          f"{learned['iterations'] / learned['solves']:.2f} it/solve, fallback rate "
+          f"{learned['fallbacks'] / learned['solves']:.2e}", flush=True)
+    assert learned["solves"] == plain["solves"] == 3 * 2 * a.info[0].nreal
+    assert plain["unconverged"] == 0 and learned["unconverged"] == 0
+    for q in range(8):
+        sc = np.abs(sa["prim"][q, real]).max()
+        assert np.abs(sa["prim"][q, real] - sb["prim"][q, real]).max() <= 1e-10 * sc, q
+    sc = np.abs(sa["Phi"][:, real]).max()
+    assert np.abs(sa["Phi"][:, real] - sb["Phi"][:, real]).max() <= 1e-10 * sc
+    a.close()
+    b.close()
