@@ -157,6 +157,45 @@ DHD int front_search(const double* __restrict__ piph, int off, int n, int rot, d
     return idx;
 }
 
+// front_search's answer found by walking from a previous one.  The count of
+// fronts below x is monotone in the rotated position, so a walk from any
+// start ends at the same boundary (and the same array index) as the binary
+// search; between substeps a partner moves by at most a cell or two, so the
+// walk from the cell's previous partner takes a few probes instead of
+// log2(n) dependent ones.  Walks longer than 8 probes fall back to the
+// binary search.
+DHD int front_search_near(const double* __restrict__ piph, int off, int n, int rot, double x, bool strict,
+                          int hint) {
+    if (hint >= 0 && hint < n) {
+        int c = hint - rot;
+        if (c < 0) c += n;
+        for (int k = 0; k < 8; ++k) {
+            if (c > 0) {
+                int i = rot + c - 1;
+                if (i >= n) i -= n;
+                double f = piph[off + i];
+                if (!(strict ? (f < x) : (f <= x))) {
+                    --c;
+                    continue;
+                }
+            }
+            if (c < n) {
+                int i = rot + c;
+                if (i >= n) i -= n;
+                double f = piph[off + i];
+                if (strict ? (f < x) : (f <= x)) {
+                    ++c;
+                    continue;
+                }
+            }
+            int idx = rot + c;
+            if (idx >= n) idx -= n;
+            return idx;
+        }
+    }
+    return front_search(piph, off, n, rot, x, strict);
+}
+
 DHD int wrap_idx(int i, int n) {
     if (i >= n) i -= n;
     if (i < 0) i += n;
@@ -335,19 +374,19 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
     double front = S.piph[c];
     if (A.outer >= 0) {
         const AnnDev& B = S.ann[A.outer];
-        S.lbo[c] = front_search(S.piph, B.off, B.np, S.rot[A.outer], front, true);
+        S.lbo[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.outer], front, true, S.lbo[c]);
     }
     if (A.inner >= 0) {
         const AnnDev& W = S.ann[A.inner];
-        S.ubi[c] = front_search(S.piph, W.off, W.np, S.rot[A.inner], front, false);
+        S.ubi[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.inner], front, false, S.ubi[c]);
     }
     if (A.up >= 0) {
         const AnnDev& B = S.ann[A.up];
-        S.lbu[c] = front_search(S.piph, B.off, B.np, S.rot[A.up], front, true);
+        S.lbu[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.up], front, true, S.lbu[c]);
     }
     if (A.down >= 0) {
         const AnnDev& W = S.ann[A.down];
-        S.ubd[c] = front_search(S.piph, W.off, W.np, S.rot[A.down], front, false);
+        S.ubd[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.down], front, false, S.ubd[c]);
     }
     if (Ph::MHD && S.pdg) {
         // diagonal annuli of the azimuthal-edge merges; a front of an annulus
@@ -356,7 +395,8 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
             int a = A.diag[d];
             if (a < 0) continue;
             const AnnDev& Q = S.ann[a];
-            S.pdg[(size_t)d * N + c] = front_search(S.piph, Q.off, Q.np, S.rot[a], front, d < 2);
+            const size_t k = (size_t)d * N + c;
+            S.pdg[k] = front_search_near(S.piph, Q.off, Q.np, S.rot[a], front, d < 2, S.pdg[k]);
         }
     }
 }
