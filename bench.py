@@ -106,7 +106,7 @@ def bench_fv(workload: str, steps: int, warmup: int) -> dict:
     # per-kernel durations (HIP events around eager launches on the same stream)
     kt = dom.kernel_times(5)
     dom.close()
-    per_step = {"k_dt_final", "k_planets"}
+    per_step = {"k_dt_final", "k_planets", "k_diag", "k_diag_time"}
     sub = [k for k in kt if k not in per_step and not k.startswith(("k_flip", "k_mindt", "k_dt_init"))]
     # average substep: every substep kernel once, the flip kernels every other substep
     substep_ms = sum(kt[k][0] for k in sub) + 0.5 * sum(kt[k][0] for k in kt if k.startswith("k_flip"))

@@ -112,6 +112,7 @@ def lib() -> C.CDLL:
     L.disco_gpu_time_steps.argtypes = [vp, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     L.disco_gpu_set_c2p_net.argtypes = [vp, fp, fp]
+    L.disco_gpu_diagnostics.argtypes = [vp, C.c_int, dp, dp, C.c_int]
     L.disco_gpu_c2p_count.argtypes = [vp, C.c_int]
     L.disco_gpu_c2p_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
     L.disco_gpu_destroy.argtypes = [vp]
@@ -316,6 +317,16 @@ class Domain:
         x = np.stack([2.0 * (rp * rp + rp * rm + rm * rm) / (3.0 * (rp + rm)), phip - 0.5 * dphi, 0.5 * (zp + zm)])
         dV = 0.5 * (rp + rm) * (rp - rm) * dphi * (zp - zm)
         return x[:, real], dV[real], real, st
+
+    def diagnostics(self, run: int = 0, reset: bool = False) -> dict:
+        """Accumulated step diagnostics (add_diagnostics): Qrz[Nz, Nr, 8] =
+        sum over steps of dt x annulus average, and t_avg."""
+        I = self.info[run]
+        Q = np.zeros(I.Nr * I.Nz * 8)
+        t = np.zeros(1)
+        _check(lib().disco_gpu_diagnostics(self._h, run, _dptr(Q), _dptr(t), int(bool(reset))),
+               "disco_gpu_diagnostics")
+        return dict(Qrz=Q.reshape(I.Nz, I.Nr, 8), t_avg=float(t[0]))
 
     def set_c2p_net(self, params=None, norm=None) -> None:
         """GRMHD: start the in-loop cons2prim from a trained guess network

@@ -20,6 +20,7 @@
 namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
 void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook);
 }  // namespace disco
@@ -64,6 +65,7 @@ struct disco_gpu {
     int* err = nullptr;
     unsigned long long* dtbits = nullptr;
     double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
+    double *diag = nullptr, *t_avg = nullptr;
     int parity = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -120,6 +122,8 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.dt = h->dt;
     S.planet_phi = h->planet_phi;
     S.t = h->t;
+    S.diag = h->diag;
+    S.t_avg = h->t_avg;
     S.Phi = h->Phi;
     S.rkPhi = h->rkPhi;
     S.emf_phi = h->emf_phi;
@@ -148,6 +152,7 @@ static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
     if (with_dt) launch_dt_final(S0, h->stream, hook);
     launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook);
     launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook);
+    launch_diag(S1, h->nann, h->nq, h->stream, hook);
 }
 
 // Recomputes the CFL minimum over the current state (after create/upload).
@@ -282,7 +287,8 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot[0],  h->rot[1],     h->fphi,
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
-                    h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats};
+                    h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats, h->diag,
+                    h->t_avg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -361,6 +367,8 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->dt, nruns);
     rc |= dalloc(&h->planet_phi, (size_t)nruns * DISCO_MAX_PLANETS);
     rc |= dalloc(&h->t, nruns);
+    rc |= dalloc(&h->diag, (size_t)h->nann * DISCO_NDIAG);
+    rc |= dalloc(&h->t_avg, nruns);
     if (rc) {
         std::string m = g_err;
         disco_gpu_destroy(h);
@@ -386,6 +394,9 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     for (int* p : zi)
         if (hipMemset(p, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");
     if (hipMemset(h->dt, 0, nruns * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
+    if (hipMemset(h->diag, 0, (size_t)h->nann * DISCO_NDIAG * sizeof(double)) != hipSuccess ||
+        hipMemset(h->t_avg, 0, nruns * sizeof(double)) != hipSuccess)
+        return cleanup("hipMemset");
     if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) return cleanup("hipMemset");
     if (mhd) {
         struct {
@@ -619,6 +630,30 @@ int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, do
         ++k;
     }
     return k;
+}
+
+int disco_gpu_diagnostics(disco_gpu_t* h, int run, double* Qrz, double* t_avg, int reset) {
+    if (!h || run < 0 || run >= h->nruns) return fail("disco_gpu_diagnostics: bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const HostRun& R = h->runs[run];
+    const int a0 = h->run_ann0[run], na = R.Nr * R.Nz;
+    if (Qrz) {
+        std::vector<double> tmp((size_t)na * DISCO_NDIAG);
+        HIPCHK(hipMemcpy(tmp.data(), h->diag + (size_t)a0 * DISCO_NDIAG, tmp.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        // device annulus order -> the reference's jk = j + Nr*k
+        for (int a = 0; a < na; ++a) {
+            const AnnDev& A = h->hann[a0 + a];
+            std::memcpy(Qrz + (size_t)(A.j + R.Nr * A.k) * DISCO_NDIAG, tmp.data() + (size_t)a * DISCO_NDIAG,
+                        DISCO_NDIAG * sizeof(double));
+        }
+    }
+    if (t_avg) HIPCHK(hipMemcpy(t_avg, h->t_avg + run, sizeof(double), hipMemcpyDeviceToHost));
+    if (reset) {
+        HIPCHK(hipMemset(h->diag + (size_t)a0 * DISCO_NDIAG, 0, (size_t)na * DISCO_NDIAG * sizeof(double)));
+        HIPCHK(hipMemset(h->t_avg + run, 0, sizeof(double)));
+    }
+    return check_err(h);
 }
 
 int disco_gpu_set_c2p_net(disco_gpu_t* h, const float* params, const float* norm) {

@@ -16,6 +16,9 @@
 
 namespace disco {
 
+// quantities of the Analysis module's get_diagnostics (Diagnostics/diag_euler.c:9-11)
+constexpr int DISCO_NDIAG = 8;
+
 enum AnnFlags {
     ANN_REAL = 1,        // real annulus: conserved update + cons2prim
     ANN_PHIFLUX = 2,     // phi faces carry fluxes (misc.c:378-436 range)
@@ -94,6 +97,8 @@ struct DevState {
     double* dt;          // per run: current step
     double* planet_phi;  // per run x planet
     double* t;           // per run: simulation time
+    double* diag;        // [annulus][DISCO_NDIAG] time-weighted annulus averages (analysis.c theTools.Qrz)
+    double* t_avg;       // per run: their accumulated time (theTools.t_avg)
     // ---- constrained transport (MHD with CT; null otherwise) ----
     // Face fluxes (bfields.c, paul.h:196-216 Phi[NUM_FACES]): slot 0 = the
     // cell's phi face (at piph), 1 / 2 = its owned r face in the inner / outer

@@ -348,6 +348,53 @@ __global__ __launch_bounds__(256) void k_flip_chain(DevState S, int has_z) {
     if (c < S.ncell) flip_chain_cell(S, c, has_z);
 }
 
+// add_diagnostics (analysis.c:52-107), every annulus incl. ghosts: the
+// volume-weighted annulus average of get_diagnostics' quantities, times dt,
+// added to the annulus' accumulator.  One block per annulus; the block's sum
+// is a tree over its threads (the reference sums in array order).
+__global__ __launch_bounds__(256) void k_diag(DevState S, int prims_nq) {
+    __shared__ double red[DISCO_NDIAG + 1][256];
+    const int a = blockIdx.x;
+    const AnnDev& A = S.ann[a];
+    const RunDev& Rn = S.runs[A.run];
+    const double Pmax = Rn.ph.phi_max;
+    const size_t N = S.ncell;
+    double acc[DISCO_NDIAG + 1];
+    for (int q = 0; q <= DISCO_NDIAG; ++q) acc[q] = 0.0;
+    for (int i = threadIdx.x; i < A.np; i += blockDim.x) {
+        const int c = A.off + i;
+        double phip = S.piph_n[c];
+        double phim = phip - S.dphi_n[c];
+        double xp[3] = {A.rp, phip, A.zp};
+        double xm[3] = {A.rm, phim, A.zm};
+        double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
+        double dV = get_dV(xp, xm, Pmax);
+        double prim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < prims_nq; ++q) prim[q] = S.prim[q * N + c];
+        double Q[DISCO_NDIAG];
+        diag_quantities(S, Rn, A.run, x, prim, Q);
+        for (int q = 0; q < DISCO_NDIAG; ++q) acc[q] += Q[q] * dV;
+        acc[DISCO_NDIAG] += dV;
+    }
+    for (int q = 0; q <= DISCO_NDIAG; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q <= DISCO_NDIAG; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < DISCO_NDIAG) {
+        const double dt = S.dt[A.run];
+        const int q = threadIdx.x;
+        S.diag[(size_t)a * DISCO_NDIAG + q] += red[q][0] * dt / red[DISCO_NDIAG][0];
+    }
+}
+
+__global__ void k_diag_time(DevState S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < S.nruns) S.t_avg[r] += S.dt[r];
+}
+
 __global__ void k_planets(DevState S) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S.nruns) return;
@@ -409,6 +456,12 @@ struct Launch {
         if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
     }
 };
+
+// the step's add_diagnostics (timestep.c:51), after the second substep
+void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook) {
+    DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S, nq);
+    DISCO_LAUNCH(hook, "k_diag_time", k_diag_time, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+}
 
 // dt = CFL x running minimum (misc.c:79)
 void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook) {

@@ -906,6 +906,34 @@ DHD void planets_run(const DevState& S, int r) {
     S.t[r] += dt;
 }
 
+// get_diagnostics of Diagnostics/diag_euler.c:17-44 at the cell's centroid:
+// rho, 2 pi r rho v_r, 2 pi r rho v_z, the torque density of planet 1 (when
+// there are two), the advective angular-momentum flux, Omega, the angular
+// momentum per unit r and P.
+DHD void diag_quantities(const DevState& S, const RunDev& Rn, int run, const double* x, const double* prim,
+                         double* Q) {
+    double r = x[0];
+    double rho = prim[RHO];
+    double vr = prim[URR];
+    double omega = prim[UPP];
+    double vz = prim[UZZ];
+    double Pp = prim[PPP];
+    Q[0] = rho;
+    Q[1] = 2. * M_PI * r * rho * vr;
+    Q[2] = 2. * M_PI * r * rho * vz;
+    double Fr, Fp = 0.0, Fz;
+    if (Rn.ph.npl > 1) {
+        Planet pl = Rn.planets[1];
+        pl.phi = S.planet_phi[run * DISCO_MAX_PLANETS + 1];
+        planetary_force(pl, Rn.ph.grav2D, r, x[1], x[2], &Fr, &Fp, &Fz);
+    }
+    Q[3] = 2. * M_PI * r * rho * (r * Fp);
+    Q[4] = 2. * M_PI * r * rho * (r * r * omega * vr);
+    Q[5] = omega;
+    Q[6] = 2 * M_PI * r * rho * r * r * omega;
+    Q[7] = Pp;
+}
+
 }  // namespace disco
 
 #include "ct_cells.h"

@@ -87,6 +87,15 @@ int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* 
 int disco_gpu_download_faces(disco_gpu_t* h, int run, double* Phi);
 int disco_gpu_upload_faces(disco_gpu_t* h, int run, const double* Phi);
 
+/* The diagnostics every step accumulates (timestep.c:51 -> add_diagnostics,
+ * analysis.c:52-107, with Diagnostics/diag_euler.c's 8 quantities): per
+ * annulus (ghosts included) the sum over steps of dt times the
+ * volume-weighted annulus average.  Qrz[Nr*Nz*8] in the reference's
+ * theTools.Qrz order (k, j, q); t_avg = the accumulated time.  Qrz / t_avg is
+ * avg_diagnostics (analysis.c:29-50); reset != 0 clears both after reading
+ * (zero_diagnostics, analysis.c:9-27).  Either pointer may be NULL. */
+int disco_gpu_diagnostics(disco_gpu_t* h, int run, double* Qrz, double* t_avg, int reset);
+
 /* Simulation time t and last dt per run (nruns each, either may be NULL);
  * planets: phi per run x 2. */
 int disco_gpu_time(disco_gpu_t* h, double* t, double* dt, double* planet_phi);

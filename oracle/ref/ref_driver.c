@@ -6,7 +6,9 @@
  * main.c:52-61), then calls the reference's own getmindt() + timestep() a
  * fixed number of times, and writes full-precision binary dumps of every
  * annulus (ghosts included) so the build's fixtures can be taken at exact
- * step counts.
+ * step counts.  Version 3 dumps end with the time-weighted diagnostics the
+ * reference's timestep() accumulates (analysis.c:52-107): t_avg and
+ * Qrz[Nr*Nz*num_tools].
  *
  * usage: disco_ref NSTEPS OUTPREFIX [DUMPSTEP ...]
  *   reads ./in.par; writes OUTPREFIX_<step>.bin at step 0, at every DUMPSTEP
@@ -33,7 +35,7 @@ static void dump(struct domain* d, const char* prefix, int step, double dt_last)
     snprintf(name, sizeof name, "%s_%d.bin", prefix, step);
     FILE* f = fopen(name, "wb");
     if (!f) { perror(name); exit(1); }
-    int32_t hdr[12] = {0x4F435344, 2, d->Nr, d->Nz, NUM_Q, NUM_FACES, d->Npl,
+    int32_t hdr[12] = {0x4F435344, 3, d->Nr, d->Nz, NUM_Q, NUM_FACES, d->Npl,
                        d->NgRa, d->NgRb, d->NgZa, d->NgZb, step};
     fwrite(hdr, sizeof hdr, 1, f);
     double tt[2] = {d->t, dt_last};
@@ -67,6 +69,10 @@ static void dump(struct domain* d, const char* prefix, int step, double dt_last)
         double rec[5] = {pl->r, pl->phi, pl->M, pl->omega, pl->eps};
         fwrite(rec, sizeof rec, 1, f);
     }
+    int32_t nt = d->num_tools;
+    fwrite(&nt, sizeof nt, 1, f);
+    fwrite(&d->theTools.t_avg, sizeof(double), 1, f);
+    fwrite(d->theTools.Qrz, sizeof(double), (size_t)d->Nr * d->Nz * nt, f);
     fclose(f);
 }
 

@@ -12,7 +12,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def read_dump(fn: str) -> dict:
     b = open(fn, "rb").read()
     hdr = np.frombuffer(b[:48], np.int32)
-    _magic, _ver, Nr, Nz, NQ, NF, Npl, NgRa, NgRb, NgZa, NgZb, step = (int(v) for v in hdr)
+    _magic, ver, Nr, Nz, NQ, NF, Npl, NgRa, NgRb, NgZa, NgZb, step = (int(v) for v in hdr)
     o = 48
     t, dtl = np.frombuffer(b[o:o + 16], np.float64)
     o += 16
@@ -27,6 +27,7 @@ def read_dump(fn: str) -> dict:
     cells = np.frombuffer(b[o:o + 8 * rec * nc], np.float64).reshape(nc, rec)
     o += 8 * rec * nc
     pl = np.frombuffer(b[o:o + 8 * 5 * Npl], np.float64).reshape(Npl, 5)
+    o += 8 * 5 * Npl
     out = dict(Nr=Nr, Nz=Nz, NQ=NQ, NgRa=NgRa, NgRb=NgRb, NgZa=NgZa, NgZb=NgZb, step=step, t=float(t),
                dt=float(dtl), r_jph=r.copy(), z_kph=z.copy(), Np=Np.copy(), piph=cells[:, 0].copy(),
                dphi=cells[:, 1].copy(), wiph=cells[:, 2].copy(), real=cells[:, 3].copy(),
@@ -34,6 +35,12 @@ def read_dump(fn: str) -> dict:
                planets=pl.copy())
     if NF:
         out["Phi"] = cells[:, 4 + 2 * NQ:].T.copy()
+    if ver >= 3:
+        nt = int(np.frombuffer(b[o:o + 4], np.int32)[0])
+        o += 4
+        out["t_avg"] = float(np.frombuffer(b[o:o + 8], np.float64)[0])
+        o += 8
+        out["Qrz"] = np.frombuffer(b[o:o + 8 * Nr * Nz * nt], np.float64).reshape(Nz, Nr, nt).copy()
     return out
 
 

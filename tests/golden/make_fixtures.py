@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--3d | --mhd | --mhd-full | --grmhd]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --3d | --mhd | --mhd-full | --grmhd]   (only the named set)
 """
 from __future__ import annotations
 
@@ -126,6 +126,21 @@ def fixtures_grmhd() -> None:
     save_states("c5_grmhd_small.npz", states)
 
 
+def fixtures_2d() -> None:
+    # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
+    txt = par_with("c1_kepler2d.par", Num_R=16)
+    states, _ = run_ref("euler_sun", txt, 100, dumps=(1, 10))
+    save_states("c1_small.npz", states)
+    # same with HLL and analytic mesh motion (Mesh_Motion 4, Kepler)
+    txt = par_with("c1_kepler2d.par", Num_R=16, Riemann_Solver=0, Mesh_Motion=4)
+    states, _ = run_ref("euler_sun", txt, 10, dumps=(1,))
+    save_states("c1_small_hll_mm4.npz", states)
+    # small isothermal binary disk (seeded phase, softened)
+    txt = par_with("c3_binary_disk.par", Num_R=24)
+    states, _ = run_ref("euler_binary", txt, 50, dumps=(1, 10))
+    save_states("c3_small.npz", states)
+
+
 def main() -> None:
     if "--grmhd" in sys.argv:
         fixtures_grmhd()
@@ -143,18 +158,14 @@ def main() -> None:
         fixtures_mhd_full()
         print("fixtures written")
         return
-    # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
-    txt = par_with("c1_kepler2d.par", Num_R=16)
-    states, _ = run_ref("euler_sun", txt, 100, dumps=(1, 10))
-    save_states("c1_small.npz", states)
-    # same with HLL and analytic mesh motion (Mesh_Motion 4, Kepler)
-    txt = par_with("c1_kepler2d.par", Num_R=16, Riemann_Solver=0, Mesh_Motion=4)
-    states, _ = run_ref("euler_sun", txt, 10, dumps=(1,))
-    save_states("c1_small_hll_mm4.npz", states)
-    # small isothermal binary disk (seeded phase, softened)
-    txt = par_with("c3_binary_disk.par", Num_R=24)
-    states, _ = run_ref("euler_binary", txt, 50, dumps=(1, 10))
-    save_states("c3_small.npz", states)
+    if "--small" in sys.argv:
+        fixtures_2d()
+        fixtures_3d()
+        fixtures_mhd()
+        fixtures_grmhd()
+        print("fixtures written")
+        return
+    fixtures_2d()
     # full-size 2D disk: conserved totals only
     txt = par_with("c1_kepler2d.par")
     states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))

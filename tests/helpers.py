@@ -135,6 +135,33 @@ def div_b(Phi, piph, Np, Nr, Nz):
     return div, scale
 
 
+def diag_errors(dg: dict, ref: dict) -> tuple:
+    """Step diagnostics (add_diagnostics, analysis.c:52-107) against the
+    reference's theTools: per quantity max |gpu - ref| over all annuli,
+    divided by t_avg x the largest per-cell magnitude of that quantity in
+    the reference's state, with the velocity components v_r, v_z taken at
+    the flow speed scale as in state_errors (annulus averages of signed
+    quantities such as rho v_r cancel to far below their terms, so the
+    average itself is no scale); quantities the reference holds at exactly
+    zero must be exactly zero.  Also the relative error of t_avg.  The
+    per-cell magnitudes follow Diagnostics/diag_euler.c:17-44 (torque: |Qrz|
+    itself)."""
+    a, b = dg["Qrz"], ref["Qrz"]
+    rc = cell_radius(ref["Nr"], ref["Nz"], ref["Np"], ref["r_jph"])
+    p = ref["prim"]
+    rho, om, P = np.abs(p[0]), np.abs(p[3]), np.abs(p[1])
+    real = ref["real"] == 1
+    vs = np.abs(p[3][real] * rc[real]).max()  # flow speed scale
+    tw = 2 * np.pi * rc * rho
+    mag = [rho, tw * vs, tw * vs, None, tw * rc * rc * om * vs, om, tw * rc * rc * om, P]
+    e = np.zeros(a.shape[-1])
+    for q in range(a.shape[-1]):
+        d = np.abs(a[..., q] - b[..., q]).max()
+        sc = np.abs(b[..., q]).max() if mag[q] is None else ref["t_avg"] * mag[q].max()
+        e[q] = d / sc if sc > 0 else (0.0 if d == 0 else np.inf)
+    return e, abs(dg["t_avg"] - ref["t_avg"]) / max(abs(ref["t_avg"]), 1e-300)
+
+
 def real_mask(Nr, Nz, NgRa, NgRb, NgZa, NgZb, Np):
     off = np.concatenate([[0], np.cumsum(Np)])
     m = np.zeros(int(off[-1]), bool)

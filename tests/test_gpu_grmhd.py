@@ -27,7 +27,7 @@ import pytest
 
 import disco_amd as D
 from dumpio import load_fixture
-from helpers import cell_radius, div_b, params, phi_err, real_mask, state_errors
+from helpers import cell_radius, diag_errors, div_b, params, phi_err, real_mask, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +75,8 @@ def test_grmhd_matches_reference():
         div, sc = div_b(st["Phi"], st["piph"], r["Np"], r["Nr"], r["Nz"])
         # cells without field have sc = 0 and must keep div exactly
         assert (np.abs(div - div0)[real] <= 1e-13 * sc[real]).all()
+        ed, et = diag_errors(dom.diagnostics(0), r)
+        assert ed.max() < tol and et < 1e-12, (s, ed, et)
     dom.close()
 
 

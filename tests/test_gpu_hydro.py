@@ -11,7 +11,9 @@ HIP path must match them per conserved/primitive field to
   helpers.state_errors) after 1 step,
 * 1e-10 after 10 steps and 1e-9 after 100 steps (round-off growth),
 
-and the mesh (piph) to the same absolute tolerances in radians.  Summation
+and the mesh (piph) to the same absolute tolerances in radians; the
+diagnostics every step accumulates (add_diagnostics: per-annulus averages
+times dt, ghosts included) to the same tolerances per quantity.  Summation
 order differs from the reference (gather instead of in-place scatter) and
 libm/ocml pow/sin/cos differ by ulps, so bit equality is not expected.
 """
@@ -20,7 +22,7 @@ import pytest
 
 import disco_amd as D
 from dumpio import load_fixture
-from helpers import cell_radius, params, phi_err, state_errors
+from helpers import cell_radius, diag_errors, params, phi_err, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +73,8 @@ def test_hydro_matches_reference(fixture, base, over):
         assert abs(tm["dt"][0] - ref[s]["dt"]) <= 1e-12 * ref[s]["dt"], (tm["dt"][0], ref[s]["dt"])
         assert abs(tm["t"][0] - ref[s]["t"]) <= 1e-11 * max(1.0, ref[s]["t"])
         _compare(st, ref[s], TOL[s], f"{fixture} step {s}")
+        ed, et = diag_errors(dom.diagnostics(0), ref[s])
+        assert ed.max() < TOL[s] and et < 1e-12, (s, ed, et)
     dom.close()
 
 

@@ -25,7 +25,7 @@ import pytest
 
 import disco_amd as D
 from dumpio import load_fixture
-from helpers import cell_radius, div_b, params, phi_err, real_mask, state_errors
+from helpers import cell_radius, diag_errors, div_b, params, phi_err, real_mask, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -70,6 +70,8 @@ def test_mhd_matches_reference():
         assert epiph < tol
         div, sc = div_b(st["Phi"], st["piph"], r["Np"], r["Nr"], r["Nz"])
         assert np.abs(div[real] / sc[real]).max() < 1e-13
+        ed, et = diag_errors(dom.diagnostics(0), r)
+        assert ed.max() < tol and et < 1e-12, (s, ed, et)
     dom.close()
 
 
