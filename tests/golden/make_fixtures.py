@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --3d | --mhd | --mhd-full | --grmhd]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd]   (only the named set)
 """
 from __future__ import annotations
 
@@ -126,6 +126,69 @@ def fixtures_grmhd() -> None:
     save_states("c5_grmhd_small.npz", states)
 
 
+# Per-leaf vectors: (config, overrides, reference variant) per physics and
+# Riemann solver; the records are random face states in the layout of
+# oracle/ref/ref_leaf.c.
+LEAF_CASES = {
+    "euler_hllc": ("c1_kepler2d.par", {"Num_R": 16}, "euler_sun"),
+    "euler_hll": ("c1_kepler2d.par", {"Num_R": 16, "Riemann_Solver": 0}, "euler_sun"),
+    "mhd_hlld": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4}, "mhd_sun"),
+    "mhd_hllc": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 1}, "mhd_sun"),
+    "mhd_hll": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 0}, "mhd_sun"),
+    "grmhd_hll": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0}, "grmhd_torus"),
+    "grmhd_hllc": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1},
+                   "grmhd_torus"),
+}
+
+
+def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
+    """Random left/right face states, positions, normals, face speeds."""
+    rng = np.random.default_rng(seed)
+    nq = 5 if kind.startswith("euler") else 8
+    U = lambda a, b, size=None: rng.uniform(a, b, size)  # noqa: E731
+    out = []
+    for k in range(n):
+        dim = k % 3
+        if kind.startswith("grmhd"):
+            r, z = U(4.0, 20.0), U(-5.0, 5.0)
+            pL = np.array([U(0.1, 1.0), U(1e-3, 5e-2), U(-0.1, 0.1), U(0.1, 0.4) / r, U(-0.1, 0.1),
+                           U(-0.02, 0.02), U(-0.02, 0.02), U(-0.02, 0.02)])
+        else:
+            r, z = U(0.6, 2.9), U(-0.4, 0.4)
+            pL = np.array([U(0.5, 2.0), U(5e-3, 5e-2), U(-0.05, 0.05), U(0.2, 1.0), U(-0.05, 0.05),
+                           U(-0.05, 0.05), U(-0.05, 0.05), U(-0.05, 0.05)])[:nq]
+        pR = pL * (1.0 + 0.2 * U(-1, 1, nq))
+        pR[2] = pL[2] + 0.03 * U(-1, 1)
+        pR[4] = pL[4] + 0.03 * U(-1, 1)
+        if k % 17 == 0:
+            pR = pL.copy()  # identical states
+        x = np.array([r, U(0.0, 2 * np.pi), z])
+        nrm = np.zeros(3)
+        nrm[[1, 0, 2][dim]] = 1.0
+        w = r * pL[3] * U(0.8, 1.2) if dim == 0 and not kind.startswith("grmhd") else (
+            pL[3] * U(0.8, 1.2) if dim == 0 else 0.0)
+        out.append(np.concatenate([pL, pR, x, nrm, [w, U(1e-3, 1e-2), float(dim)]]))
+    return np.array(out)
+
+
+def leaf_fixtures(n: int = 192) -> None:
+    arrs = {}
+    for i, (name, (base, over, variant)) in enumerate(LEAF_CASES.items()):
+        rec = leaf_records(name, n, 1000 + i)
+        nq = (rec.shape[1] - 9) // 2
+        exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_leaf_{variant}")
+        with tempfile.TemporaryDirectory() as d:
+            open(os.path.join(d, "in.par"), "w").write(par_with(base, **over))
+            with open(os.path.join(d, "in.bin"), "wb") as f:
+                f.write(np.int32(n).tobytes())
+                f.write(rec.astype(np.float64).tobytes())
+            subprocess.run([exe, "in.bin", "out.bin"], cwd=d, check=True, timeout=120, capture_output=True)
+            out = np.fromfile(os.path.join(d, "out.bin"), np.float64).reshape(n, 4 * nq + 5)
+        arrs[f"{name}_in"] = rec
+        arrs[f"{name}_out"] = out
+    np.savez_compressed(os.path.join(HERE, "leaf_vectors.npz"), cases=np.array(list(LEAF_CASES)), **arrs)
+
+
 def fixtures_2d() -> None:
     # small 2D Keplerian disk (euler, HLLC, sun): states at 0, 1, 10, 100 steps
     txt = par_with("c1_kepler2d.par", Num_R=16)
@@ -156,6 +219,10 @@ def main() -> None:
         return
     if "--mhd-full" in sys.argv:
         fixtures_mhd_full()
+        print("fixtures written")
+        return
+    if "--leaf" in sys.argv:
+        leaf_fixtures()
         print("fixtures written")
         return
     if "--small" in sys.argv:
