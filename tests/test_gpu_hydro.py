@@ -176,3 +176,28 @@ def test_batch_matches_oracle_per_run():
         assert max(e["prim"].max(), e["cons"].max()) < 1e-10, (r, e)
         o.close()
     batch.close()
+
+
+def test_diagnostics_reset_and_accumulate():
+    """zero_diagnostics / avg_diagnostics semantics (analysis.c:9-50): a read
+    with reset returns the sums and clears them; the next step adds dt x the
+    annulus averages again; t_avg is the sum of the steps' dt."""
+    dom = D.Domain(params("c1_kepler2d.par", Num_R=16))
+    dom.step(3)
+    d3 = dom.diagnostics(0, reset=True)
+    assert d3["t_avg"] > 0 and np.isfinite(d3["Qrz"]).all()
+    z = dom.diagnostics(0)
+    assert z["t_avg"] == 0.0 and not z["Qrz"].any()
+    dom.step(1)
+    d1 = dom.diagnostics(0)
+    dt = dom.time()["dt"][0]
+    assert abs(d1["t_avg"] - dt) <= 1e-15 * dt
+    # annulus average of rho x dt: between the min and max density x dt
+    st = dom.state(0)
+    rho = st["prim"][0]
+    assert (d1["Qrz"][..., 0] >= rho.min() * dt * (1 - 1e-12)).all()
+    assert (d1["Qrz"][..., 0] <= rho.max() * dt * (1 + 1e-12)).all()
+    # the learned cons2prim start exists for GRMHD runs only
+    with pytest.raises(RuntimeError, match="not GRMHD"):
+        dom.set_c2p_net(np.zeros(13186, np.float32), np.ones(20, np.float32))
+    dom.close()

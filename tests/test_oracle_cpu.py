@@ -156,6 +156,32 @@ def test_unsupported_configuration_is_rejected():
         D.setup_run(p)
 
 
+@pytest.mark.parametrize("over,msg", [
+    (dict(Riemann_Solver=2), "HLLD with grmhd"),
+    (dict(Metric_Par3=1.2), "Kerr needs"),
+    (dict(Mesh_Motion=1), "rigidly rotating"),
+    (dict(Constrained_Transport=0), "Constrained_Transport"),
+])
+def test_unsupported_grmhd_configuration_is_rejected(over, msg):
+    """GRMHD set-ups the step does not run are refused with the key named."""
+    with pytest.raises(RuntimeError, match=msg):
+        D.setup_run(params("c5_grmhd_torus.par", Num_R=8, Num_Z=6, **over))
+
+
+def test_grmhd_setup_is_bit_exact():
+    """Host set-up of the torus (GRMHD prim2cons, set_B_fields, the Noble
+    cons2prim from the initial prims, fixed r and z ghosts) gives the
+    reference's step-0 state bit for bit."""
+    r0 = load_fixture("c5_grmhd_small.npz")[0]
+    s = D.setup_run(params("c5_grmhd_torus.par", Num_R=16, Num_Z=12, Metric_Par3=0.0))
+    real = r0["real"] == 1
+    for k in ("Np", "r_jph", "z_kph", "piph", "dphi"):
+        assert np.array_equal(s[k], r0[k]), k
+    assert np.array_equal(s["prim"][:, real], r0["prim"][:, real])
+    assert np.array_equal(s["cons"][:, real], r0["cons"][:, real])
+    assert np.array_equal(s["prim"][:, ~real], r0["prim"][:, ~real])
+
+
 def test_library_exports_every_declared_symbol():
     """The C ABI library loads and exports every function include/*.h declares."""
     L = C.CDLL(D.LIB_PATH)
