@@ -88,14 +88,17 @@ def cpu_baseline(workload: str) -> dict | None:
             "sample": f"{what}: {cells} cells x {steps} steps in {secs:.2f} s (reference C build, gcc -O3, serial)"}
 
 
-def bench_fv(workload: str, steps: int, warmup: int) -> dict:
+def bench_fv(workload: str, steps: int, warmup: int, tune: bool = True) -> dict:
     """Zone-updates/s of one workload, its per-kernel times and the roofline
-    of one RK substep (all kernels of the substep) against HBM."""
+    of one RK substep (all kernels of the substep) against HBM.  The
+    library's autotuner first picks the launch shape (annulus tile, threads
+    per block) on the run itself, outside the timed region."""
     import disco_amd as D
 
     base, over, hydro, _ = WORKLOADS[workload]
     dom = D.Domain(D.parse_par(par_text(base, **over)))
     nreal = sum(i.nreal for i in dom.info)
+    tuned = dom.autotune(2) if tune else None
     dom.step(warmup)
     dom.sync()
     t0 = time.perf_counter()
@@ -128,6 +131,7 @@ def bench_fv(workload: str, steps: int, warmup: int) -> dict:
                      "kernel": f"one RK substep ({' + '.join(sub)}; flip kernels every other substep)",
                      "bytes_per_zone_substep": bpz, "substep_ms": substep_ms, "dominant_kernel": dominant},
         "kernels_ms": {k: v[0] for k, v in kt.items()},
+        "autotune": tuned,
     }
 
 
@@ -188,6 +192,7 @@ def bench_batch(steps: int, warmup: int) -> dict:
     runs = binary_batch(48)
     dom = D.Domain(runs)
     cells = sum(i.nreal for i in dom.info)
+    tuned = dom.autotune(3)  # threads per block of the batched launches
     dom.step(warmup)
     dom.sync()
     ms = dom.time_steps(steps)
@@ -209,7 +214,7 @@ def bench_batch(steps: int, warmup: int) -> dict:
     return {"workload": "binary_batch_x48", "runs": 48, "real_cells": cells, "steps": steps,
             "value": batched, "unit": "zone-updates/s", "ms_per_step": ms / steps,
             "sequential_value": sequential, "speedup_vs_sequential": batched / sequential,
-            "kernels_ms": {k: v[0] for k, v in kt.items()}}
+            "kernels_ms": {k: v[0] for k, v in kt.items()}, "autotune": tuned}
 
 
 # FP64 operations of the plain cons2prim per state, counted by hand from
@@ -340,6 +345,7 @@ def main() -> int:
         "config": cfg,
         "roofline": r["roofline"],
         "kernels_ms": r["kernels_ms"],
+        "autotune": r["autotune"],
         "wall_s_timed": r["wall_s_timed"],
     }
     if not args.no_extra and wl != "kepler2d_nr256":

@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
     fp = C.POINTER(C.c_float)
     L.disco_gpu_set_c2p_net.argtypes = [vp, fp, fp]
     L.disco_gpu_diagnostics.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+    L.disco_gpu_autotune.argtypes = [vp, C.c_int, ip, dp]
     L.disco_gpu_c2p_count.argtypes = [vp, C.c_int]
     L.disco_gpu_c2p_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
     L.disco_gpu_destroy.argtypes = [vp]
@@ -349,6 +350,16 @@ class Domain:
         out = (C.c_longlong * 4)()
         _check(lib().disco_gpu_c2p_stats(self._h, out, int(bool(reset))), "disco_gpu_c2p_stats")
         return dict(iterations=out[0], solves=out[1], fallbacks=out[2], unconverged=out[3])
+
+    def autotune(self, nsteps: int = 3) -> dict:
+        """Pick the annulus tile and threads per block by timing (state kept)."""
+        ch = np.zeros(2, np.int32)
+        tab = np.zeros(15)
+        _check(lib().disco_gpu_autotune(self._h, int(nsteps), _iptr(ch), _dptr(tab)), "disco_gpu_autotune")
+        tiles = [1, 2, 4, 8, 16] if any(i.Nz > 1 for i in self.info) else [int(ch[0])]
+        blocks = [64, 128, 256]
+        ms = {f"{t}x{b}": float(tab[i * 3 + j]) for i, t in enumerate(tiles) for j, b in enumerate(blocks)}
+        return dict(tile=int(ch[0]), block=int(ch[1]), ms_per_step=ms)
 
     def time_steps(self, nsteps: int) -> float:
         ms = C.c_double()

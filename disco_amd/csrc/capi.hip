@@ -2,6 +2,7 @@
 // RK2 steps, upload/download and timing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -47,6 +48,7 @@ struct disco_gpu {
     std::vector<int> run_cell0, run_ann0;
     std::vector<std::vector<int>> cell_perm;  // per run: device position of each cell (reference order)
     int ktile = 8;                            // z-layers per annulus tile in HBM
+    int block = 256;                          // threads per block of the per-cell kernels
     int nruns = 0, ncell = 0, nann = 0, nq = 5;
     AnnDev* ann = nullptr;
     int* cell_ann = nullptr;
@@ -173,8 +175,11 @@ static int dalloc(T** p, size_t n) {
 }
 
 static void build_tables(disco_gpu* h) {
+    const int learned = h->shape.c2p_learned;
     build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape, &h->cell_perm,
                          h->ktile);
+    h->shape.c2p_learned = learned;
+    h->shape.block = h->block;
     h->ncell = 0;
     for (const HostRun& R : h->runs) h->ncell += R.ncell;
     h->nann = (int)h->hann.size();
@@ -254,6 +259,169 @@ static int capture_graphs(disco_gpu* h) {
     return 0;
 }
 
+// ---------------------------------------------------------------- layout
+// Tables from build_tables to the device (create and relayout).
+static int upload_tables(disco_gpu* h) {
+    std::vector<int> cell_ann(h->ncell);
+    for (int a = 0; a < h->nann; ++a)
+        for (int i = 0; i < h->hann[a].np; ++i) cell_ann[h->hann[a].off + i] = a;
+    HIPCHK(hipMemcpy(h->ann, h->hann.data(), h->nann * sizeof(AnnDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->cell_ann, cell_ann.data(), h->ncell * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->druns, h->hruns.data(), h->nruns * sizeof(RunDev), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// The array index of each annulus' smallest front (the rotation point).
+static void host_rotation(HostRun& R) {
+    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
+        int o = R.off[jk], np = R.Np[jk];
+        R.rot[jk] = 0;
+        for (int i = 0; i < np; ++i) {
+            int im = i == 0 ? np - 1 : i - 1;
+            if (R.piph[o + i] < R.piph[o + im]) R.rot[jk] = i;
+        }
+    }
+}
+
+// Every device buffer indexed by cell or by annulus: (pointer, records per
+// cell/annulus, bytes per record).  A relayout moves all of them, so no state
+// of any kind -- carried, scratch or hint -- depends on the layout.
+struct DevBuf {
+    void* p;
+    int fields;  // field-major planes of N records
+    int rec;     // bytes per record
+    bool per_annulus;
+};
+
+static std::vector<DevBuf> layout_buffers(disco_gpu* h) {
+    std::vector<DevBuf> b;
+    const int nq = h->nq;
+    auto D = [&](double* p, int f) {
+        if (p) b.push_back({p, f, 8, false});
+    };
+    auto I = [&](int* p, int f) {
+        if (p) b.push_back({p, f, 4, false});
+    };
+    D(h->prim, nq), D(h->cons, nq), D(h->rkcons, nq), D(h->gradp, nq), D(h->gradr, nq), D(h->gradz, nq);
+    D(h->piph[0], 1), D(h->piph[1], 1), D(h->dphi[0], 1), D(h->dphi[1], 1), D(h->wiph, 1);
+    D(h->fphi, 5), D(h->fout, 5), D(h->fin, 5), D(h->fup, 5), D(h->fdn, 5);
+    D(h->Phi, 5), D(h->rkPhi, 5), D(h->emf_phi, 4), D(h->emf_t, 12), D(h->esum, 8), D(h->es1, 8), D(h->efin, 4);
+    I(h->lbo, 1), I(h->ubi, 1), I(h->lbu, 1), I(h->ubd, 1), I(h->pdg, 4), I(h->fflags, 1);
+    b.push_back({h->rot[0], 1, 4, true});
+    b.push_back({h->rot[1], 1, 4, true});
+    b.push_back({h->diag, 1, 8 * DISCO_NDIAG, true});
+    return b;
+}
+
+// dst[to[c]] = src[c] for n records of `words` words of type W
+template <typename W>
+__global__ void k_move(W* __restrict__ dst, const W* __restrict__ src, const int* __restrict__ to, int n, int words) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * words) return;
+    long c = i / words, b = i - c * words;
+    dst[(long)to[c] * words + b] = src[i];
+}
+
+// Moves every buffer from the current annulus tile to `ktile`, on the device.
+static int relayout(disco_gpu* h, int ktile) {
+    if (ktile == h->ktile) return 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const std::vector<std::vector<int>> old_perm = h->cell_perm;
+    const std::vector<AnnDev> old_ann = h->hann;
+    h->ktile = ktile;
+    build_tables(h);
+    // old device position -> new device position, for cells and annuli
+    std::vector<int> to_c(h->ncell), to_a(h->nann);
+    for (int r = 0; r < h->nruns; ++r) {
+        const HostRun& R = h->runs[r];
+        const int c0 = h->run_cell0[r], a0 = h->run_ann0[r];
+        for (int c = 0; c < R.ncell; ++c) to_c[c0 + old_perm[r][c]] = c0 + h->cell_perm[r][c];
+        std::vector<int> pos(R.Nr * R.Nz);
+        for (int a = 0; a < R.Nr * R.Nz; ++a) pos[h->hann[a0 + a].j + R.Nr * h->hann[a0 + a].k] = a0 + a;
+        for (int a = 0; a < R.Nr * R.Nz; ++a) to_a[a0 + a] = pos[old_ann[a0 + a].j + R.Nr * old_ann[a0 + a].k];
+    }
+    int *dto_c = nullptr, *dto_a = nullptr;
+    char* tmp = nullptr;
+    size_t tmp_bytes = std::max((size_t)h->ncell * 8, (size_t)h->nann * 8 * DISCO_NDIAG);
+    hipError_t e = hipMalloc((void**)&dto_c, h->ncell * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&dto_a, h->nann * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&tmp, tmp_bytes);
+    if (e == hipSuccess) e = hipMemcpy(dto_c, to_c.data(), h->ncell * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dto_a, to_a.data(), h->nann * sizeof(int), hipMemcpyHostToDevice);
+    for (const DevBuf& b : layout_buffers(h)) {
+        const int n = b.per_annulus ? h->nann : h->ncell;
+        const size_t plane = (size_t)n * b.rec;
+        for (int f = 0; f < b.fields && e == hipSuccess; ++f) {
+            char* p = (char*)b.p + f * plane;
+            const int* to = b.per_annulus ? dto_a : dto_c;
+            if (b.rec % 8 == 0) {
+                const int w = b.rec / 8;
+                hipLaunchKernelGGL(k_move<unsigned long long>, dim3((unsigned)(((long)n * w + 255) / 256)), dim3(256),
+                                   0, h->stream, (unsigned long long*)tmp, (const unsigned long long*)p, to, n, w);
+            } else {
+                const int w = b.rec / 4;
+                hipLaunchKernelGGL(k_move<unsigned int>, dim3((unsigned)(((long)n * w + 255) / 256)), dim3(256), 0,
+                                   h->stream, (unsigned int*)tmp, (const unsigned int*)p, to, n, w);
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(p, tmp, plane, hipMemcpyDeviceToDevice, h->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dto_c);
+    (void)hipFree(dto_a);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(std::string("relayout: ") + hipGetErrorString(e));
+    return upload_tables(h);
+}
+
+// Annulus tile (z-layers per tile in HBM) and threads per block of the cell
+// kernels.  Neither changes any result: every kernel gathers per cell, the
+// CFL reduction is a minimum and the diagnostics sum one annulus per block,
+// whatever order the cells are stored in and however they are grouped.
+static int set_layout(disco_gpu* h, int ktile, int block) {
+    if (relayout(h, ktile)) return -1;
+    h->block = block;
+    h->shape.block = block;
+    return capture_graphs(h);
+}
+
+// A device copy of everything a step reads or writes.
+struct Snapshot {
+    std::vector<std::pair<void*, void*>> bufs;  // (live, copy)
+    std::vector<size_t> bytes;
+    ~Snapshot() {
+        for (auto& b : bufs) (void)hipFree(b.second);
+    }
+};
+
+static int snapshot(disco_gpu* h, Snapshot* S, bool restore) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!restore) {
+        std::vector<std::pair<void*, size_t>> list;
+        for (const DevBuf& b : layout_buffers(h))
+            list.push_back({b.p, (size_t)b.fields * (b.per_annulus ? h->nann : h->ncell) * b.rec});
+        const size_t n = h->nruns;
+        list.push_back({h->dtbits, n * sizeof(unsigned long long)});
+        list.push_back({h->dt, n * sizeof(double)});
+        list.push_back({h->t, n * sizeof(double)});
+        list.push_back({h->t_avg, n * sizeof(double)});
+        list.push_back({h->planet_phi, n * DISCO_MAX_PLANETS * sizeof(double)});
+        if (h->c2p_stats) list.push_back({h->c2p_stats, 4 * sizeof(unsigned long long)});
+        for (auto& l : list) {
+            void* c = nullptr;
+            HIPCHK(hipMalloc(&c, l.second));
+            S->bufs.push_back({l.first, c});
+            S->bytes.push_back(l.second);
+            HIPCHK(hipMemcpy(c, l.first, l.second, hipMemcpyDeviceToDevice));
+        }
+    } else {
+        for (size_t i = 0; i < S->bufs.size(); ++i)
+            HIPCHK(hipMemcpy(S->bufs[i].first, S->bufs[i].second, S->bytes[i], hipMemcpyDeviceToDevice));
+    }
+    return 0;
+}
+
 extern "C" {
 
 const char* disco_gpu_last_error(void) { return g_err.c_str(); }
@@ -321,9 +489,6 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     build_tables(h);
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;
-    std::vector<int> cell_ann(N);
-    for (int a = 0; a < h->nann; ++a)
-        for (int i = 0; i < h->hann[a].np; ++i) cell_ann[h->hann[a].off + i] = a;
     int rc = 0;
     rc |= dalloc(&h->ann, h->nann);
     rc |= dalloc(&h->cell_ann, N);
@@ -379,10 +544,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         return fail(m);
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
-    if (hipMemcpy(h->ann, h->hann.data(), h->nann * sizeof(AnnDev), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->cell_ann, cell_ann.data(), N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->druns, h->hruns.data(), nruns * sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup("upload of tables failed");
+    if (upload_tables(h)) return cleanup(g_err);
     // zero the scratch fields once (eager, outside any graph)
     double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz};
     for (double* p : zq)
@@ -556,14 +718,7 @@ int disco_gpu_upload(disco_gpu_t* h, int run, const double* prim, const double* 
     if (dphi) std::memcpy(R.dphi.data(), dphi, n * sizeof(double));
     if (wiph) std::memcpy(R.wiph.data(), wiph, n * sizeof(double));
     // recompute the rotation point of every annulus for the new mesh
-    for (int jk = 0; jk < R.Nr * R.Nz; ++jk) {
-        int o = R.off[jk], np = R.Np[jk];
-        R.rot[jk] = 0;
-        for (int i = 0; i < np; ++i) {
-            int im = i == 0 ? np - 1 : i - 1;
-            if (R.piph[o + i] < R.piph[o + im]) R.rot[jk] = i;
-        }
-    }
+    host_rotation(R);
     if (upload_run_state(h, run)) return -1;
     return seed_dt(h);
 }
@@ -653,6 +808,44 @@ int disco_gpu_diagnostics(disco_gpu_t* h, int run, double* Qrz, double* t_avg, i
         HIPCHK(hipMemset(h->diag + (size_t)a0 * DISCO_NDIAG, 0, (size_t)na * DISCO_NDIAG * sizeof(double)));
         HIPCHK(hipMemset(h->t_avg + run, 0, sizeof(double)));
     }
+    return check_err(h);
+}
+
+int disco_gpu_autotune(disco_gpu_t* h, int nsteps, int* choice, double* ms_table) {
+    if (!h || nsteps <= 0) return fail("disco_gpu_autotune: bad arguments");
+    // candidates: annulus tiles (3D runs only) x threads per block
+    std::vector<int> kts = {h->ktile};
+    if (h->shape.has_z) kts = {1, 2, 4, 8, 16};
+    const int bss[3] = {64, 128, 256};
+    // everything the timing steps change, in the current layout
+    const int kt0 = h->ktile, parity0 = h->parity;
+    Snapshot S;
+    if (snapshot(h, &S, false)) return -1;
+    double best = 1e300;
+    int bk = h->ktile, bb = h->block, idx = 0;
+    for (int kt : kts) {
+        for (int bs : bss) {
+            if (set_layout(h, kt, bs)) return -1;
+            double ms = 0.0;
+            if (disco_gpu_step(h, 1) || disco_gpu_time_steps(h, nsteps, &ms)) return -1;
+            if (ms_table) ms_table[idx] = ms / nsteps;
+            ++idx;
+            if (ms < best) {
+                best = ms;
+                bk = kt;
+                bb = bs;
+            }
+        }
+    }
+    // back to the snapshot's layout and state, then move it to the winner
+    if (relayout(h, kt0)) return -1;
+    h->parity = parity0;
+    if (snapshot(h, &S, true) || set_layout(h, bk, bb)) return -1;
+    if (choice) {
+        choice[0] = bk;
+        choice[1] = bb;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
     return check_err(h);
 }
 

@@ -128,6 +128,7 @@ struct StepShape {
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
+    int block;        // threads per block of the per-cell kernels (64, 128 or 256)
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

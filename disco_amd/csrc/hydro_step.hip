@@ -426,7 +426,7 @@ struct Launch {
     }
     static void substep(const DevState& S, double RK, double dtfac, int first, int want_dt, const StepShape& sh,
                         hipStream_t st, const LaunchHook* hook) {
-        const int B = 256;
+        const int B = sh.block > 0 ? sh.block : 256;
         dim3 g(nblk(S.ncell, B));
         DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, g, dim3(B), st, S, RK, first);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, g, dim3(B), st, S, sh.has_z);

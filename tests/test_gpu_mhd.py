@@ -130,3 +130,30 @@ def test_mhd_full_size_totals_and_divb():
     print(f"full-size divB after 2 steps: {worst:.2e}", flush=True)
     assert worst < 1e-12
     dom.close()
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner times every annulus tile x block size on the run itself,
+    keeps the fastest and restores the state: the tuned run then produces the
+    same bits as an untuned one, step for step."""
+    p = params("c2_mhd3d.par", Num_R=12, Num_Z=8, CFL=0.3)
+    a = D.Domain(p)
+    b = D.Domain(p)
+    a.step(1)
+    b.step(1)
+    ch = a.autotune(1)
+    assert ch["tile"] in (1, 2, 4, 8, 16) and ch["block"] in (64, 128, 256)
+    assert len(ch["ms_per_step"]) == 15
+    sa, sb = a.state(0), b.state(0)
+    for k in ("prim", "cons", "piph", "dphi", "wiph", "Phi"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.step(3)
+    b.step(3)
+    sa, sb = a.state(0), b.state(0)
+    for k in ("prim", "cons", "piph", "dphi", "wiph", "Phi"):
+        assert np.array_equal(sa[k], sb[k]), k
+    da, db = a.diagnostics(0), b.diagnostics(0)
+    assert np.array_equal(da["Qrz"], db["Qrz"]) and da["t_avg"] == db["t_avg"]
+    assert np.array_equal(a.time()["dt"], b.time()["dt"])
+    a.close()
+    b.close()
