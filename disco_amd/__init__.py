@@ -353,13 +353,14 @@ class Domain:
 
     def autotune(self, nsteps: int = 3) -> dict:
         """Pick the annulus tile and threads per block by timing (state kept)."""
-        ch = np.zeros(2, np.int32)
+        ch = np.zeros(9, np.int32)
         tab = np.zeros(15)
         _check(lib().disco_gpu_autotune(self._h, int(nsteps), _iptr(ch), _dptr(tab)), "disco_gpu_autotune")
         tiles = [1, 2, 4, 8, 16] if any(i.Nz > 1 for i in self.info) else [int(ch[0])]
         blocks = [64, 128, 256]
         ms = {f"{t}x{b}": float(tab[i * 3 + j]) for i, t in enumerate(tiles) for j, b in enumerate(blocks)}
-        return dict(tile=int(ch[0]), block=int(ch[1]), ms_per_step=ms)
+        classes = ["stage", "slopes", "flux", "emf", "ct_update", "flip", "update", "zcopy"]
+        return dict(tile=int(ch[0]), block=dict(zip(classes, (int(v) for v in ch[1:]))), ms_per_step=ms)
 
     def time_steps(self, nsteps: int) -> float:
         ms = C.c_double()

@@ -48,7 +48,7 @@ struct disco_gpu {
     std::vector<int> run_cell0, run_ann0;
     std::vector<std::vector<int>> cell_perm;  // per run: device position of each cell (reference order)
     int ktile = 8;                            // z-layers per annulus tile in HBM
-    int block = 256;                          // threads per block of the per-cell kernels
+    int blk[KC_N] = {256, 256, 256, 256, 256, 256, 256, 256};  // threads per block per kernel class
     int nruns = 0, ncell = 0, nann = 0, nq = 5;
     AnnDev* ann = nullptr;
     int* cell_ann = nullptr;
@@ -179,7 +179,7 @@ static void build_tables(disco_gpu* h) {
     build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape, &h->cell_perm,
                          h->ktile);
     h->shape.c2p_learned = learned;
-    h->shape.block = h->block;
+    for (int k = 0; k < KC_N; ++k) h->shape.blk[k] = h->blk[k];
     h->ncell = 0;
     for (const HostRun& R : h->runs) h->ncell += R.ncell;
     h->nann = (int)h->hann.size();
@@ -379,11 +379,22 @@ static int relayout(disco_gpu* h, int ktile) {
 // kernels.  Neither changes any result: every kernel gathers per cell, the
 // CFL reduction is a minimum and the diagnostics sum one annulus per block,
 // whatever order the cells are stored in and however they are grouped.
-static int set_layout(disco_gpu* h, int ktile, int block) {
+static int set_layout(disco_gpu* h, int ktile, const int* blk) {
     if (relayout(h, ktile)) return -1;
-    h->block = block;
-    h->shape.block = block;
+    for (int k = 0; k < KC_N; ++k) h->shape.blk[k] = h->blk[k] = blk[k];
     return capture_graphs(h);
+}
+
+// Kernel class of a kernel name (the launch names of hydro_step.hip).
+static int kernel_class(const std::string& n) {
+    static const std::pair<const char*, int> m[] = {
+        {"k_stage_a", KC_STAGE}, {"k_plm_trans", KC_PLM}, {"k_flux", KC_FLUX},     {"k_emf_sum", KC_EMF},
+        {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update_a", KC_CT}, {"k_ct_update", KC_CT},
+        {"k_update_a", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
+        {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}};
+    for (auto& e : m)
+        if (n == e.first) return e.second;
+    return -1;
 }
 
 // A device copy of everything a step reads or writes.
@@ -813,37 +824,68 @@ int disco_gpu_diagnostics(disco_gpu_t* h, int run, double* Qrz, double* t_avg, i
 
 int disco_gpu_autotune(disco_gpu_t* h, int nsteps, int* choice, double* ms_table) {
     if (!h || nsteps <= 0) return fail("disco_gpu_autotune: bad arguments");
-    // candidates: annulus tiles (3D runs only) x threads per block
+    // candidates: annulus tiles (3D runs only); per tile, every kernel class
+    // takes the width (64, 128, 256 threads per block) at which it ran fastest
     std::vector<int> kts = {h->ktile};
     if (h->shape.has_z) kts = {1, 2, 4, 8, 16};
     const int bss[3] = {64, 128, 256};
-    // everything the timing steps change, in the current layout
     const int kt0 = h->ktile, parity0 = h->parity;
-    Snapshot S;
+    int blk0[KC_N];
+    std::memcpy(blk0, h->blk, sizeof blk0);
+    Snapshot S;  // everything the timing steps change, in the current layout
     if (snapshot(h, &S, false)) return -1;
     double best = 1e300;
-    int bk = h->ktile, bb = h->block, idx = 0;
+    int bk = h->ktile, bblk[KC_N], idx = 0;
+    std::memcpy(bblk, h->blk, sizeof bblk);
+    const int maxk = 32;
+    std::vector<char> names(32 * maxk);
+    std::vector<double> avg(maxk);
+    std::vector<int> cnt(maxk);
     for (int kt : kts) {
-        for (int bs : bss) {
-            if (set_layout(h, kt, bs)) return -1;
-            double ms = 0.0;
-            if (disco_gpu_step(h, 1) || disco_gpu_time_steps(h, nsteps, &ms)) return -1;
-            if (ms_table) ms_table[idx] = ms / nsteps;
-            ++idx;
-            if (ms < best) {
-                best = ms;
-                bk = kt;
-                bb = bs;
+        double cls[3][KC_N] = {};
+        double other = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            int w[KC_N];
+            for (int k = 0; k < KC_N; ++k) w[k] = bss[b];
+            if (set_layout(h, kt, w)) return -1;
+            if (disco_gpu_step(h, 1)) return -1;
+            int nk = disco_gpu_kernel_times(h, nsteps, maxk, names.data(), avg.data(), cnt.data());
+            if (nk < 0) return -1;
+            double total = 0.0;
+            for (int k = 0; k < nk; ++k) {
+                double per_step = avg[k] * cnt[k] / nsteps;
+                total += per_step;
+                int c = kernel_class(std::string(names.data() + 32 * k));
+                if (c >= 0)
+                    cls[b][c] += per_step;
+                else if (b == 2)
+                    other += per_step;  // per-step kernels of fixed width
             }
+            if (ms_table) ms_table[idx] = total;
+            ++idx;
+        }
+        int w[KC_N];
+        double total = other;
+        for (int c = 0; c < KC_N; ++c) {
+            int bi = 2;
+            for (int b = 0; b < 3; ++b)
+                if (cls[b][c] < cls[bi][c]) bi = b;
+            w[c] = bss[bi];
+            total += cls[bi][c];
+        }
+        if (total < best) {
+            best = total;
+            bk = kt;
+            std::memcpy(bblk, w, sizeof w);
         }
     }
     // back to the snapshot's layout and state, then move it to the winner
-    if (relayout(h, kt0)) return -1;
+    if (set_layout(h, kt0, blk0)) return -1;
     h->parity = parity0;
-    if (snapshot(h, &S, true) || set_layout(h, bk, bb)) return -1;
+    if (snapshot(h, &S, true) || set_layout(h, bk, bblk)) return -1;
     if (choice) {
         choice[0] = bk;
-        choice[1] = bb;
+        for (int k = 0; k < KC_N; ++k) choice[1 + k] = bblk[k];
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     return check_err(h);

@@ -121,6 +121,9 @@ struct DevState {
                                     // solves unconverged at max_iter (null: not counted)
 };
 
+// Kernel classes with their own launch width (threads per block).
+enum KernelClass { KC_STAGE = 0, KC_PLM, KC_FLUX, KC_EMF, KC_CT, KC_FLIP, KC_UPDATE, KC_COPY, KC_N };
+
 // What a batch needs launched each substep (uniform over its runs).
 struct StepShape {
     int hydro;  // DISCO_HYDRO_* of every run
@@ -128,7 +131,7 @@ struct StepShape {
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
-    int block;        // threads per block of the per-cell kernels (64, 128 or 256)
+    int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

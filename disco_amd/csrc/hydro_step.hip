@@ -426,33 +426,37 @@ struct Launch {
     }
     static void substep(const DevState& S, double RK, double dtfac, int first, int want_dt, const StepShape& sh,
                         hipStream_t st, const LaunchHook* hook) {
-        const int B = sh.block > 0 ? sh.block : 256;
-        dim3 g(nblk(S.ncell, B));
-        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, g, dim3(B), st, S, RK, first);
-        DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, g, dim3(B), st, S, sh.has_z);
-        DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, g, dim3(B), st, S, dtfac);
+        auto B = [&](int k) { return sh.blk[k] > 0 ? sh.blk[k] : 256; };
+        auto G = [&](int k) { return dim3(nblk(S.ncell, B(k))); };
+        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, RK, first);
+        DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
+        DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         if (Ph::MHD && sh.ct) {
-            DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, g, dim3(B), st, S, sh.has_z);
-            DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, g, dim3(B), st, S);
-            DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, g, dim3(B), st, S, sh.has_z);
+            DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
+            DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, G(KC_EMF), dim3(B(KC_EMF)), st, S);
+            DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             if (SplitCT<Ph>::value) {
-                DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, g, dim3(B), st, S, dtfac, sh.has_z);
-                DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, g, dim3(B), st, S, dtfac, first);
+                DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, sh.has_z);
+                DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first);
             } else {
-                DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, g, dim3(B), st, S, dtfac, first, sh.has_z);
+                DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first,
+                             sh.has_z);
             }
             if (first) {
-                DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, g, dim3(B), st, S, sh.has_z);
-                DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, g, dim3(B), st, S, sh.has_z);
+                DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
+                DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
             }
             if (Ph::C2P_GUESS && sh.c2p_learned)
-                DISCO_LAUNCH(hook, "k_update_b_learned", k_update_b_learned, g, dim3(B), st, S, first, want_dt);
+                DISCO_LAUNCH(hook, "k_update_b_learned", k_update_b_learned, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S,
+                             first, want_dt);
             else
-                DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, g, dim3(B), st, S, first, want_dt);
+                DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, first,
+                             want_dt);
         } else {
-            DISCO_LAUNCH(hook, "k_update", k_update<Ph>, g, dim3(B), st, S, dtfac, first, want_dt);
+            DISCO_LAUNCH(hook, "k_update", k_update<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, dtfac, first,
+                         want_dt);
         }
-        if (sh.zcopy) DISCO_LAUNCH(hook, "k_zcopy", k_zcopy<Ph>, g, dim3(B), st, S, first);
+        if (sh.zcopy) DISCO_LAUNCH(hook, "k_zcopy", k_zcopy<Ph>, G(KC_COPY), dim3(B(KC_COPY)), st, S, first);
         if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
     }
 };

@@ -107,13 +107,15 @@ int disco_gpu_time(disco_gpu_t* h, double* t, double* dt, double* planet_phi);
  * distinct kernels. */
 int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, double* avg_ms, int* count);
 
-/* Autotuner (no counterpart in the reference): times `nsteps` steps for
- * each launch shape -- the number of z-layers per annulus tile in HBM
- * (1, 2, 4, 8, 16; 3D runs only) and the threads per block of the per-cell
- * kernels (64, 128, 256, i.e. 1, 2 or 4 wavefronts of cells) -- keeps the
- * fastest, and puts the state back as it was before the timing steps (the
- * shape changes no result).  choice[2] = {tile, threads per block}; ms_table
- * (nullable) receives the ms per step of every candidate, tiles outer. */
+/* Autotuner (no counterpart in the reference): for each annulus tile (the
+ * number of z-layers per tile in HBM: 1, 2, 4, 8, 16; 3D runs only) times
+ * `nsteps` steps per kernel at 64, 128 and 256 threads per block (1, 2 or 4
+ * wavefronts of cells per block), gives every kernel class (stage, slopes,
+ * fluxes, EMFs, CT/cell update, flips, cons2prim update, z copy) its fastest
+ * width, keeps the tile with the smallest sum, and puts the state back as it
+ * was before the timing steps (the shape changes no result).  choice[9] =
+ * {tile, width of the 8 classes}; ms_table (nullable, 15 entries) receives
+ * the ms per step of every (tile, uniform width), tiles outer. */
 int disco_gpu_autotune(disco_gpu_t* h, int nsteps, int* choice, double* ms_table);
 
 /* Wall time in ms of `nsteps` graph-launched steps, measured with HIP events

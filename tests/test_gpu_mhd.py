@@ -142,7 +142,7 @@ def test_autotune_keeps_state_and_results():
     a.step(1)
     b.step(1)
     ch = a.autotune(1)
-    assert ch["tile"] in (1, 2, 4, 8, 16) and ch["block"] in (64, 128, 256)
+    assert ch["tile"] in (1, 2, 4, 8, 16) and set(ch["block"].values()) <= {64, 128, 256}
     assert len(ch["ms_per_step"]) == 15
     sa, sb = a.state(0), b.state(0)
     for k in ("prim", "cons", "piph", "dphi", "wiph", "Phi"):
