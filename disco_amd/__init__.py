@@ -101,6 +101,8 @@ def lib() -> C.CDLL:
     L.disco_gpu_grid.argtypes = [vp, C.c_int, ip, dp, dp]
     L.disco_gpu_mindt.argtypes = [vp, dp]
     L.disco_gpu_step.argtypes = [vp, C.c_int]
+    L.disco_gpu_advance.argtypes = [vp, C.c_int, C.c_int, ip, ip]
+    L.disco_gpu_final.argtypes = [vp, ip]
     L.disco_gpu_timestep.argtypes = [vp, dp]
     L.disco_gpu_sync.argtypes = [vp]
     L.disco_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
@@ -243,6 +245,25 @@ class Domain:
 
     def step(self, nsteps: int = 1) -> None:
         _check(lib().disco_gpu_step(self._h, int(nsteps)), "disco_gpu_step")
+
+    def advance(self, max_steps: int = 1 << 30, poll_files: bool = False) -> tuple:
+        """main.c:68-75 without output: steps until every run has taken its
+        step to T_End (check_dt, domain.c:283-310) or max_steps are done.
+        Returns (steps taken, status) with status bits 1 every run final,
+        2 an "abort" file ended the loop, 4 a "latest" file was seen (removed;
+        the loop stopped before the step so the caller can write a
+        checkpoint); the files are read only with poll_files."""
+        n = C.c_int(0)
+        st = C.c_int(0)
+        _check(lib().disco_gpu_advance(self._h, int(max_steps), int(bool(poll_files)), C.byref(n), C.byref(st)),
+               "disco_gpu_advance")
+        return n.value, st.value
+
+    def final(self) -> np.ndarray:
+        """check_dt's final flag of every run."""
+        f = np.zeros(self.nruns, np.int32)
+        _check(lib().disco_gpu_final(self._h, f.ctypes.data_as(C.POINTER(C.c_int))), "disco_gpu_final")
+        return f.astype(bool)
 
     def sync(self) -> None:
         _check(lib().disco_gpu_sync(self._h), "disco_gpu_sync")

@@ -20,7 +20,7 @@
 
 namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
-void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook);
+void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook);
@@ -67,7 +67,9 @@ struct disco_gpu {
     int* err = nullptr;
     unsigned long long* dtbits = nullptr;
     double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
+    int* final_step = nullptr;  // per run: check_dt's final flag
     double *diag = nullptr, *t_avg = nullptr;
+    double* src = nullptr;  // GRMHD source increments
     int parity = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -124,6 +126,8 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.dt = h->dt;
     S.planet_phi = h->planet_phi;
     S.t = h->t;
+    S.final_step = h->final_step;
+    S.src = h->src;
     S.diag = h->diag;
     S.t_avg = h->t_avg;
     S.Phi = h->Phi;
@@ -151,7 +155,7 @@ static DevState make_state(disco_gpu* h, int substep) {
 static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
     DevState S0 = make_state(h, 0);
     DevState S1 = make_state(h, 1);
-    if (with_dt) launch_dt_final(S0, h->stream, hook);
+    if (with_dt) launch_dt_final(S0, 1, h->stream, hook);
     launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook);
     launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook);
     launch_diag(S1, h->nann, h->nq, h->stream, hook);
@@ -235,6 +239,8 @@ static int upload_run_state(disco_gpu* h, int r) {
     for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) pp[p] = R.planets[p].phi;
     HIPCHK(hipMemcpy(h->planet_phi + r * DISCO_MAX_PLANETS, pp, sizeof(pp), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->t + r, &R.t, sizeof(double), hipMemcpyHostToDevice));
+    const int not_final = 0;
+    HIPCHK(hipMemcpy(h->final_step + r, &not_final, sizeof(int), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -306,6 +312,7 @@ static std::vector<DevBuf> layout_buffers(disco_gpu* h) {
     D(h->piph[0], 1), D(h->piph[1], 1), D(h->dphi[0], 1), D(h->dphi[1], 1), D(h->wiph, 1);
     D(h->fphi, 5), D(h->fout, 5), D(h->fin, 5), D(h->fup, 5), D(h->fdn, 5);
     D(h->Phi, 5), D(h->rkPhi, 5), D(h->emf_phi, 4), D(h->emf_t, 12), D(h->esum, 8), D(h->es1, 8), D(h->efin, 4);
+    D(h->src, 4);
     I(h->lbo, 1), I(h->ubi, 1), I(h->lbu, 1), I(h->ubd, 1), I(h->pdg, 4), I(h->fflags, 1);
     b.push_back({h->rot[0], 1, 4, true});
     b.push_back({h->rot[1], 1, 4, true});
@@ -390,7 +397,7 @@ static int kernel_class(const std::string& n) {
     static const std::pair<const char*, int> m[] = {
         {"k_stage_a", KC_STAGE}, {"k_plm_trans", KC_PLM}, {"k_flux", KC_FLUX},     {"k_emf_sum", KC_EMF},
         {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update_a", KC_CT}, {"k_ct_update", KC_CT},
-        {"k_update_a", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
+        {"k_update_a", KC_CT},   {"k_source", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
         {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}};
     for (auto& e : m)
         if (n == e.first) return e.second;
@@ -416,6 +423,7 @@ static int snapshot(disco_gpu* h, Snapshot* S, bool restore) {
         list.push_back({h->dtbits, n * sizeof(unsigned long long)});
         list.push_back({h->dt, n * sizeof(double)});
         list.push_back({h->t, n * sizeof(double)});
+        list.push_back({h->final_step, n * sizeof(int)});
         list.push_back({h->t_avg, n * sizeof(double)});
         list.push_back({h->planet_phi, n * DISCO_MAX_PLANETS * sizeof(double)});
         if (h->c2p_stats) list.push_back({h->c2p_stats, 4 * sizeof(unsigned long long)});
@@ -467,7 +475,7 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
                     h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats, h->diag,
-                    h->t_avg};
+                    h->t_avg, h->src,      h->final_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -543,8 +551,10 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->dt, nruns);
     rc |= dalloc(&h->planet_phi, (size_t)nruns * DISCO_MAX_PLANETS);
     rc |= dalloc(&h->t, nruns);
+    rc |= dalloc(&h->final_step, nruns);
     rc |= dalloc(&h->diag, (size_t)h->nann * DISCO_NDIAG);
     rc |= dalloc(&h->t_avg, nruns);
+    if (h->runs[0].par.hydro == DISCO_HYDRO_GRMHD) rc |= dalloc(&h->src, 4 * N);
     if (rc) {
         std::string m = g_err;
         disco_gpu_destroy(h);
@@ -641,7 +651,7 @@ int disco_gpu_mindt(disco_gpu_t* h, double* dt_out) {
     if (!h) return fail("disco_gpu_mindt: null handle");
     DevState S = make_state(h, 0);
     launch_mindt(h->shape, S, h->stream, nullptr);
-    launch_dt_final(S, h->stream, nullptr);
+    launch_dt_final(S, 0, h->stream, nullptr);
     HIPCHK(hipGetLastError());
     if (dt_out) {
         HIPCHK(hipMemcpyAsync(dt_out, h->dt, h->nruns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -656,6 +666,87 @@ int disco_gpu_step(disco_gpu_t* h, int nsteps) {
         HIPCHK(hipGraphLaunch(h->graph[h->parity], h->stream));
         h->parity ^= 1;
     }
+    return 0;
+}
+
+static int check_err(disco_gpu* h);
+
+// main.c:68-75 over the batch: getmindt + check_dt + timestep until every
+// run has taken its step to T_End (check_dt's clip runs on the device, at
+// the start of every step).  Steps go out in runs of graph launches sized
+// from the slowest run's remaining (T_End - t) / dt, so the host reads the
+// final flags a few times per run rather than every step; a run that is
+// final holds (dt = 0) while the others finish.  poll_files: check_dt's
+// file controls in the working directory before every step: "abort" makes
+// that step the last (status bit 2), "latest" is removed and the call
+// returns before the step (status bit 4: the caller writes its checkpoint,
+// as possiblyOutput does before timestep).  Status bit 1: every run is final.
+int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_done, int* status) {
+    if (!h || max_steps < 0) return fail("disco_gpu_advance: bad arguments");
+    const int n = h->nruns;
+    std::vector<double> t(n), dt(n);
+    std::vector<int> fin(n);
+    int done = 0, st = 0;
+    bool stop = false;
+    while (!stop) {
+        HIPCHK(hipMemcpyAsync(fin.data(), h->final_step, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(t.data(), h->t, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(dt.data(), h->dt, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (check_err(h)) return -1;
+        // steps every unfinished run can take before its clipped one
+        long ahead = -1;
+        for (int r = 0; r < n; ++r) {
+            if (fin[r]) continue;
+            double left = h->runs[r].par.t_max - t[r];
+            long k = (dt[r] > 0.0 && left > 0.0) ? (long)std::min(left / dt[r], 1e9) : 0;
+            ahead = std::max(ahead, k);
+        }
+        if (ahead < 0) {
+            st |= 1;
+            break;
+        }
+        if (done >= max_steps) break;
+        // launch most of them (dt drifts between reads), at least one
+        long burst = std::max(1L, std::min(256L, (ahead * 3) / 4));
+        burst = std::min(burst, (long)(max_steps - done));
+        for (long s = 0; s < burst; ++s) {
+            if (poll_files) {
+                if (FILE* f = std::fopen("latest", "r")) {
+                    std::fclose(f);
+                    std::remove("latest");
+                    st |= 4;
+                    stop = true;
+                    break;
+                }
+                if (FILE* f = std::fopen("abort", "r")) {
+                    std::fclose(f);
+                    st |= 2;
+                    stop = true;
+                }
+            }
+            HIPCHK(hipGraphLaunch(h->graph[h->parity], h->stream));
+            h->parity ^= 1;
+            ++done;
+            if (stop) break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (check_err(h)) return -1;
+    if (stop && !(st & 1)) {
+        // the flags after the last step launched
+        HIPCHK(hipMemcpy(fin.data(), h->final_step, n * sizeof(int), hipMemcpyDeviceToHost));
+        if (std::all_of(fin.begin(), fin.end(), [](int f) { return f != 0; })) st |= 1;
+    }
+    if (steps_done) *steps_done = done;
+    if (status) *status = st;
+    return 0;
+}
+
+int disco_gpu_final(disco_gpu_t* h, int* final_step) {
+    if (!h || !final_step) return fail("disco_gpu_final: bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(final_step, h->final_step, h->nruns * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -60,6 +60,7 @@ struct RunDev {
     int cell0, ncell;
     int ann0, nann;
     int Nr, Nz;
+    double t_fin;  // T_End (domain.c:93)
     Planet planets[DISCO_MAX_PLANETS];
 };
 
@@ -97,6 +98,8 @@ struct DevState {
     double* dt;          // per run: current step
     double* planet_phi;  // per run x planet
     double* t;           // per run: simulation time
+    int* final_step;     // per run: check_dt's final flag (its step to T_End has been taken or is running)
+    double* src;         // [4][ncell] GRMHD: source increments of S_r, L, S_z, tau (null: in the update)
     double* diag;        // [annulus][DISCO_NDIAG] time-weighted annulus averages (analysis.c theTools.Qrz)
     double* t_avg;       // per run: their accumulated time (theTools.t_avg)
     // ---- constrained transport (MHD with CT; null otherwise) ----

@@ -795,6 +795,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         D.nann = R.Nr * R.Nz;
         D.Nr = R.Nr;
         D.Nz = R.Nz;
+        D.t_fin = R.par.t_max;
         for (size_t p = 0; p < R.planets.size() && p < DISCO_MAX_PLANETS; ++p) D.planets[p] = R.planets[p];
         hruns->push_back(D);
         run_cell0->push_back(cell0);

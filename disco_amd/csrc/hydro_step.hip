@@ -134,11 +134,26 @@ __global__ __launch_bounds__(256) void k_mindt(DevState S) {
     dt_reduce(S, run, dtc);
 }
 
-__global__ void k_dt_final(DevState S) {
+__global__ void k_dt_final(DevState S, int check) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S.nruns) return;
     double m = __longlong_as_double((long long)S.dtbits[r]);
-    S.dt[r] = m * S.runs[r].ph.cfl;  // misc.c:79
+    double dt = m * S.runs[r].ph.cfl;  // misc.c:79
+    if (check) {
+        // check_dt (domain.c:283-292): the step that would pass T_End ends
+        // on it and is the run's last; a run past it holds (dt = 0) while
+        // the other runs of the batch go on.
+        if (S.final_step[r]) {
+            dt = 0.0;
+        } else {
+            double t = S.t[r], tmax = S.runs[r].t_fin;
+            if (t + dt > tmax) {
+                dt = tmax - t;
+                S.final_step[r] = 1;
+            }
+        }
+    }
+    S.dt[r] = dt;
 }
 
 // ------------------------------------------------------------------ stages
@@ -322,6 +337,12 @@ __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_ct_update_a(DevS
     }
 }
 
+template <class Ph>
+__global__ __launch_bounds__(256) void k_source(DevState S, double dtfac) {
+    int c = cell_index();
+    if (c < S.ncell) source_cell<Ph>(S, c, dtfac);
+}
+
 // The same two halves as separate launches, for modules whose cell update is
 // register-heavy enough that the fused kernel drops to one wave per SIMD.
 __global__ __launch_bounds__(256) void k_ct_update(DevState S, double dtfac, int has_z) {
@@ -351,9 +372,9 @@ __global__ __launch_bounds__(256) void k_flip_chain(DevState S, int has_z) {
 // add_diagnostics (analysis.c:52-107), every annulus incl. ghosts: the
 // volume-weighted annulus average of get_diagnostics' quantities, times dt,
 // added to the annulus' accumulator.  One block per annulus; the block's sum
-// is a tree over its threads (the reference sums in array order).
-__global__ __launch_bounds__(256) void k_diag(DevState S, int prims_nq) {
-    __shared__ double red[DISCO_NDIAG + 1][256];
+// runs in a fixed order over its threads (the reference sums in array order).
+__global__ __launch_bounds__(256) void k_diag(DevState S) {
+    __shared__ double red[DISCO_NDIAG + 1][4];
     const int a = blockIdx.x;
     const AnnDev& A = S.ann[a];
     const RunDev& Rn = S.runs[A.run];
@@ -369,24 +390,29 @@ __global__ __launch_bounds__(256) void k_diag(DevState S, int prims_nq) {
         double xm[3] = {A.rm, phim, A.zm};
         double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
         double dV = get_dV(xp, xm, Pmax);
-        double prim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int q = 0; q < prims_nq; ++q) prim[q] = S.prim[q * N + c];
+        double prim[5];  // diag_euler reads rho, P, v_r, Omega, v_z only
+        for (int q = 0; q < 5; ++q) prim[q] = S.prim[q * N + c];
         double Q[DISCO_NDIAG];
         diag_quantities(S, Rn, A.run, x, prim, Q);
         for (int q = 0; q < DISCO_NDIAG; ++q) acc[q] += Q[q] * dV;
         acc[DISCO_NDIAG] += dV;
     }
-    for (int q = 0; q <= DISCO_NDIAG; ++q) red[q][threadIdx.x] = acc[q];
+    // wavefront sums (fixed butterfly order), then the block's 4 partials
+    for (int q = 0; q <= DISCO_NDIAG; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q <= DISCO_NDIAG; ++q) red[q][wave] = acc[q];
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int q = 0; q <= DISCO_NDIAG; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
-        __syncthreads();
-    }
     if (threadIdx.x < DISCO_NDIAG) {
-        const double dt = S.dt[A.run];
         const int q = threadIdx.x;
-        S.diag[(size_t)a * DISCO_NDIAG + q] += red[q][0] * dt / red[DISCO_NDIAG][0];
+        const int nw = (blockDim.x + 63) >> 6;
+        double sq = red[q][0], sv = red[DISCO_NDIAG][0];
+        for (int w = 1; w < nw; ++w) {
+            sq += red[q][w];
+            sv += red[DISCO_NDIAG][w];
+        }
+        S.diag[(size_t)a * DISCO_NDIAG + q] += sq * S.dt[A.run] / sv;
     }
 }
 
@@ -436,6 +462,8 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, G(KC_EMF), dim3(B(KC_EMF)), st, S);
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             if (SplitCT<Ph>::value) {
+                if (Ph::SOURCE_PASS)
+                    DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
                 DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, sh.has_z);
                 DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first);
             } else {
@@ -463,13 +491,14 @@ struct Launch {
 
 // the step's add_diagnostics (timestep.c:51), after the second substep
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook) {
-    DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S, nq);
+    (void)nq;
+    DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S);
     DISCO_LAUNCH(hook, "k_diag_time", k_diag_time, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
 }
 
-// dt = CFL x running minimum (misc.c:79)
-void launch_dt_final(const DevState& S, hipStream_t st, const LaunchHook* hook) {
-    DISCO_LAUNCH(hook, "k_dt_final", k_dt_final, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+// dt = CFL x running minimum (misc.c:79); check: then check_dt's clip
+void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook) {
+    DISCO_LAUNCH(hook, "k_dt_final", k_dt_final, dim3(nblk(S.nruns, 64)), dim3(64), st, S, check);
 }
 
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook) {

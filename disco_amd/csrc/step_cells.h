@@ -47,6 +47,7 @@ struct PhysEuler {
     static constexpr int NF = 5;  // stored face-increment components
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
+    static constexpr bool SOURCE_PASS = false;  // sources come from source_cell's pass
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         (void)dim;
@@ -76,6 +77,7 @@ struct PhysMHD {
     static constexpr int NF = 5;
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = false;
+    static constexpr bool SOURCE_PASS = false;
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         return mhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
@@ -103,6 +105,7 @@ struct PhysGRMHD {
     static constexpr int NF = 5;
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = true;
+    static constexpr bool SOURCE_PASS = true;
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         return grmhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
@@ -733,7 +736,15 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
         double dV = get_dV(xp, xm, Pmax);
-        Ph::source(P, prim, cons, xp, xm, dV * dt);
+        if constexpr (Ph::SOURCE_PASS) {
+            // computed by source_cell (same operands, added to zero first)
+            cons[SRR] += S.src[0 * N + c];
+            cons[LLL] += S.src[1 * N + c];
+            cons[SZZ] += S.src[2 * N + c];
+            cons[TAU] += S.src[3 * N + c];
+        } else {
+            Ph::source(P, prim, cons, xp, xm, dV * dt);
+        }
         for (int p = 0; p < P.npl; ++p) {
             Planet pl = Rn.planets[p];
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
@@ -754,6 +765,35 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
         S.dphi_n[c] = get_dp(pn, pm, Pmax);
         if (pn < pm && !(A.flags & ANN_ZCOPY)) S.rot_n[S.cell_ann[c]] = i;
     }
+}
+
+// GRMHD: add_source's geometric terms of a real cell (grmhd.c:339-454) with
+// the operands update_a_cell would use (prims of the substep, the mesh
+// before the move), as increments of S_r, L, S_z, tau.  A kernel of its own
+// because the metric derivatives need ~190 registers on top of the update's
+// gathers.
+template <class Ph>
+DHD void source_cell(const DevState& S, int c, double dtfac) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    if (!(A.flags & ANN_REAL)) return;
+    const PhysParams& P = S.runs[A.run].ph;
+    const size_t N = S.ncell;
+    const double dt = S.dt[A.run] * dtfac;
+    double prim[NQ], src[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        prim[q] = S.prim[q * N + c];
+        src[q] = 0.0;
+    }
+    double phip = S.piph[c];
+    double xp[3] = {A.rp, phip, A.zp};
+    double xm[3] = {A.rm, phip - S.dphi[c], A.zm};
+    double dV = get_dV(xp, xm, P.phi_max);
+    Ph::source(P, prim, src, xp, xm, dV * dt);
+    S.src[0 * N + c] = src[SRR];
+    S.src[1 * N + c] = src[LLL];
+    S.src[2 * N + c] = src[SZZ];
+    S.src[3 * N + c] = src[TAU];
 }
 
 template <class Ph>

@@ -57,12 +57,29 @@ int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kp
  * and leaves it on the device for the next step. */
 int disco_gpu_mindt(disco_gpu_t* h, double* dt_out);
 
-/* One iteration of the reference main loop without output: getmindt then
- * timestep (timestep.c:7-54, RK2 = onestep(0, dt, first) + onestep(1/2,
- * dt/2, last)), `nsteps` times, for every run, dt kept on the device.
- * Launched as a captured hipGraph per step; asynchronous on the handle's
- * stream (use disco_gpu_sync). */
+/* One iteration of the reference main loop without output: getmindt,
+ * check_dt's T_End clip (domain.c:283-292) and timestep (timestep.c:7-54,
+ * RK2 = onestep(0, dt, first) + onestep(1/2, dt/2, last)), `nsteps` times,
+ * for every run, dt kept on the device.  A run whose step to T_End has been
+ * taken holds (dt = 0: state, mesh and time unchanged) while the others of
+ * the batch go on.  Launched as a captured hipGraph per step; asynchronous
+ * on the handle's stream (use disco_gpu_sync). */
 int disco_gpu_step(disco_gpu_t* h, int nsteps);
+
+/* The reference main loop (main.c:68-75) without output, over the batch:
+ * steps until every run has taken its step to T_End (check_dt, domain.c:
+ * 283-310) or `max_steps` steps are done, whichever comes first.
+ * poll_files != 0: check_dt's file controls in the working directory before
+ * every step: a file "abort" makes that step the last; a file "latest" is
+ * removed and the call returns before the step, so the caller can write the
+ * checkpoint possiblyOutput would.  *steps_done: steps taken; *status:
+ * bit 1 every run is final, bit 2 "abort" seen, bit 4 "latest" seen.
+ * Either pointer may be NULL.  Synchronous. */
+int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_done, int* status);
+
+/* check_dt's final flag of every run (nruns ints; 1: its step to T_End has
+ * been taken). */
+int disco_gpu_final(disco_gpu_t* h, int* final_step);
 
 /* timestep(dom, dt) with caller-given dt per run (no CFL reduction). */
 int disco_gpu_timestep(disco_gpu_t* h, const double* dt);

@@ -634,6 +634,16 @@ void oracle_timestep(oracle_run* o, double dt) {
     o_onestep(o, 0.5, 0.5 * dt, 0);
 }
 
+/* domain.c:283-292 (check_dt without its file controls): a step that would
+ * pass T_End is cut to end on it; returns 1 for that (final) step. */
+int oracle_check_dt(oracle_run* o, double* dt) {
+    if (o->t + *dt > o->par.t_max) {
+        *dt = o->par.t_max - o->t;
+        return 1;
+    }
+    return 0;
+}
+
 /* misc.c:39-84 */
 double oracle_getmindt(oracle_run* o) {
     double dt = o->par.maxDT / o->par.CFL;

@@ -14,6 +14,10 @@
  *   reads ./in.par; writes OUTPREFIX_<step>.bin at step 0, at every DUMPSTEP
  *   and at NSTEPS; prints one line "ref steps=N cells=C seconds=S" with the
  *   wall time of the step loop alone.
+ *   NSTEPS = 0: the reference's own loop condition instead (main.c:68-75
+ *   without output): getmindt, check_dt (domain.c:283-310, the T_End clip),
+ *   timestep until check_dt marks the final step; the last dump is
+ *   OUTPREFIX_<steps taken>.bin.
  */
 #include "paul.h"
 #include <stdint.h>
@@ -27,6 +31,7 @@ void setupCells(struct domain*);
 void boundary_trans(struct domain*, int);
 void exchangeData(struct domain*, int);
 double getmindt(struct domain*);
+void check_dt(struct domain*, double*);
 void timestep(struct domain*, double);
 void freeDomain(struct domain*);
 
@@ -117,10 +122,12 @@ int main(int argc, char* argv[]) {
     double seconds = 0.0;
     double dt = 0.0;
     int n;
-    for (n = 1; n <= nsteps; ++n) {
+    const int to_end = (nsteps == 0);
+    for (n = 1; to_end ? !theDomain.final_step : n <= nsteps; ++n) {
         dt = getmindt(&theDomain);
+        if (to_end) check_dt(&theDomain, &dt);
         timestep(&theDomain, dt);
-        int want = (n == nsteps);
+        int want = to_end ? theDomain.final_step : (n == nsteps);
         int a;
         for (a = 0; a < ndump; ++a)
             if (atoi(argv[3 + a]) == n) want = 1;
@@ -131,6 +138,7 @@ int main(int argc, char* argv[]) {
         }
     }
     seconds += now() - t0;
+    if (to_end) nsteps = n - 1;
     printf("ref steps=%d cells=%ld seconds=%.6f\n", nsteps, ncells, seconds);
     freeDomain(&theDomain);
     return 0;

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end]   (only the named set)
 """
 from __future__ import annotations
 
@@ -48,8 +48,33 @@ def run_ref(variant: str, par_text: str, nsteps: int, dumps=()):
     return states, float(m.group(1))
 
 
-def save_states(name: str, states: dict) -> None:
-    arrs = {}
+def run_ref_to_end(variant: str, par_text: str):
+    """The reference's own loop to T_End (driver mode NSTEPS = 0): states at
+    step 0 and at the final step."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_{variant}")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "in.par"), "w").write(par_text)
+        out = subprocess.run([exe, "0", "out"], cwd=d, check=True, timeout=600, capture_output=True,
+                             text=True).stdout
+        n = int(re.search(r"steps=(\d+)", out).group(1))
+        return {0: read_dump(os.path.join(d, "out_0.bin")), n: read_dump(os.path.join(d, f"out_{n}.bin"))}
+
+
+def fixtures_t_end() -> None:
+    """Small 2D disk run by the reference's loop to two T_End values that
+    fall inside its 4th and 7th steps (check_dt clips the last step)."""
+    txt = par_with("c1_kepler2d.par", Num_R=16)
+    states, _ = run_ref("euler_sun", txt, 7, dumps=(3, 4, 6))
+    for name, (a, b) in (("c1_tend_a.npz", (3, 4)), ("c1_tend_b.npz", (6, 7))):
+        ta, tb = states[a]["t"], states[b]["t"]
+        t_end = ta + 0.37 * (tb - ta)
+        st = run_ref_to_end("euler_sun", par_with("c1_kepler2d.par", Num_R=16, T_End=repr(float(t_end))))
+        assert max(st) == b, (name, sorted(st))
+        save_states(name, st, t_end=np.float64(t_end))
+
+
+def save_states(name: str, states: dict, **extra) -> None:
+    arrs = dict(extra)
     for s, st in states.items():
         for k, v in st.items():
             arrs[f"s{s}_{k}"] = np.asarray(v)
@@ -221,6 +246,10 @@ def main() -> None:
         fixtures_mhd_full()
         print("fixtures written")
         return
+    if "--t-end" in sys.argv:
+        fixtures_t_end()
+        print("fixtures written")
+        return
     if "--leaf" in sys.argv:
         leaf_fixtures()
         print("fixtures written")
@@ -233,6 +262,7 @@ def main() -> None:
         print("fixtures written")
         return
     fixtures_2d()
+    fixtures_t_end()
     # full-size 2D disk: conserved totals only
     txt = par_with("c1_kepler2d.par")
     states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))

@@ -27,6 +27,8 @@ def lib():
         L.oracle_getmindt.restype = C.c_double
         L.oracle_getmindt.argtypes = [C.c_void_p]
         L.oracle_timestep.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_check_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.oracle_check_dt.restype = C.c_int
         L.oracle_get_state.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_track_faces.argtypes = [dp, dp, C.c_int, dp, dp, C.c_int, C.c_double, ip, ip, ip, dp, dp]
@@ -62,6 +64,17 @@ class Oracle:
     def step(self, n: int = 1) -> None:
         for _ in range(n):
             self.timestep(self.getmindt())
+
+    def run_to_end(self, max_steps: int = 1 << 20) -> int:
+        """main.c:68-75 without output: getmindt, check_dt, timestep until
+        the final step; returns the steps taken."""
+        for n in range(1, max_steps + 1):
+            dt = C.c_double(self.getmindt())
+            final = lib().oracle_check_dt(self._h, C.byref(dt))
+            self.timestep(dt.value)
+            if final:
+                return n
+        raise RuntimeError("T_End not reached")
 
     def state(self) -> dict:
         n = self.ncell

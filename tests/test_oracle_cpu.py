@@ -73,6 +73,35 @@ def test_oracle_detects_a_planted_sign_error():
     assert e["prim"].max() > 1e-9
 
 
+@pytest.mark.parametrize("fixture", ["c1_tend_a.npz", "c1_tend_b.npz"])
+def test_oracle_check_dt_matches_reference(fixture):
+    """check_dt (domain.c:283-292): the reference's own loop, run to a T_End
+    inside a step, takes the same number of steps as the oracle's, its last
+    step cut to end on T_End (same final time, and a state that a step of
+    any other length would miss by far more than round-off)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", fixture))
+    t_end = float(z["t_end"])
+    ref = load_fixture(fixture)
+    nref = max(ref)
+    p = params("c1_kepler2d.par", Num_R=16, T_End=repr(t_end))
+    r0 = ref[0]
+    o = Oracle(p, _grid(r0), r0, planet_phi=r0["planets"][:, 1], t=r0["t"])
+    n = o.run_to_end()
+    st = o.state()
+    assert n == nref
+    assert st["t"] == ref[nref]["t"] == t_end
+    rcell = cell_radius(r0["Nr"], r0["Nz"], r0["Np"], r0["r_jph"])
+    e = state_errors(st, ref[nref], rcell, ref[nref]["real"] == 1)
+    assert e["prim"].max() < 1e-11 and e["cons"].max() < 1e-11, e
+    o.close()
+    # a full-length last step (no clip) misses the reference's state
+    o = Oracle(p, _grid(r0), r0, planet_phi=r0["planets"][:, 1], t=r0["t"])
+    o.step(nref)
+    e = state_errors(o.state(), ref[nref], rcell, ref[nref]["real"] == 1)
+    assert e["prim"].max() > 1e-6
+    o.close()
+
+
 def _brute_overlaps(piA, dpA, piB, dpB, P=2 * np.pi):
     """All (a, b) pairs whose phi intervals overlap (positive length), by
     brute force on the circle."""
