@@ -396,7 +396,7 @@ static int set_layout(disco_gpu* h, int ktile, const int* blk) {
 static int kernel_class(const std::string& n) {
     static const std::pair<const char*, int> m[] = {
         {"k_stage_a", KC_STAGE}, {"k_plm_trans", KC_PLM}, {"k_flux", KC_FLUX},     {"k_emf_sum", KC_EMF},
-        {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update_a", KC_CT}, {"k_ct_update", KC_CT},
+        {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update", KC_CT},
         {"k_update_a", KC_CT},   {"k_source", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
         {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}};
     for (auto& e : m)

@@ -8,9 +8,10 @@
 // EMF averaging and the Stokes update between the fluxes and the update, and
 // splits the update around the flux flips of the moving mesh:
 //   k_stage_a -> k_plm_trans -> k_flux -> k_emf_sum -> k_emf_avg1 ->
-//   k_emf_face -> k_ct_update_a -> [k_flip_flag -> k_flip_chain]
+//   k_emf_face -> k_ct_update -> k_update_a -> [k_flip_flag -> k_flip_chain]
 //   -> k_update_b (+ k_zcopy, k_planets)
-// GRMHD runs the same sequence; with a trained guess network attached,
+// GRMHD runs the same sequence with its geometric sources in a kernel of
+// their own (k_source, before k_ct_update); with a trained guess network attached,
 // k_update_b_learned replaces k_update_b (network on MFMA, then the Newton
 // solve from its guess).
 // The CFL reduction of the next step is fused into the last substep's update.
@@ -48,15 +49,6 @@ struct Occ {
 template <>
 struct Occ<PhysGRMHD> {
     static constexpr int FLUX = 0, PLM = 0, CT = 0;
-};
-// launch the CT face update and the cell update separately
-template <class Ph>
-struct SplitCT {
-    static constexpr bool value = false;
-};
-template <>
-struct SplitCT<PhysGRMHD> {
-    static constexpr bool value = true;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
@@ -323,28 +315,16 @@ __global__ __launch_bounds__(256) void k_emf_face(DevState S, int has_z) {
     if (c < S.ncell) emf_face_cell(S, c, has_z);
 }
 
-// update_B_fluxes + make_edge_adjust (face fluxes) fused with the first half
-// of the cell update (conserved state, sources, mesh move): independent work
-// on the same cell.
-template <class Ph>
-__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_ct_update_a(DevState S, double dtfac, int first,
-                                                                           int has_z) {
-    int c = cell_index();
-    if (c < S.ncell) {
-        ct_update_cell(S, c, dtfac, has_z);
-        double cons[Ph::NQ];
-        update_a_cell<Ph>(S, c, dtfac, first, cons);
-    }
-}
-
 template <class Ph>
 __global__ __launch_bounds__(256) void k_source(DevState S, double dtfac) {
     int c = cell_index();
     if (c < S.ncell) source_cell<Ph>(S, c, dtfac);
 }
 
-// The same two halves as separate launches, for modules whose cell update is
-// register-heavy enough that the fused kernel drops to one wave per SIMD.
+// update_B_fluxes + make_edge_adjust (face fluxes), then the first half of
+// the cell update (conserved state, sources, mesh move) in a launch of its
+// own: fused, the two need 127 VGPRs (4 waves per SIMD) and run 19% slower
+// than apart (77 VGPRs for the Stokes update, 6 waves).
 __global__ __launch_bounds__(256) void k_ct_update(DevState S, double dtfac, int has_z) {
     int c = cell_index();
     if (c < S.ncell) ct_update_cell(S, c, dtfac, has_z);
@@ -461,15 +441,10 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, G(KC_EMF), dim3(B(KC_EMF)), st, S);
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
-            if (SplitCT<Ph>::value) {
-                if (Ph::SOURCE_PASS)
-                    DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
-                DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, sh.has_z);
-                DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first);
-            } else {
-                DISCO_LAUNCH(hook, "k_ct_update_a", k_ct_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first,
-                             sh.has_z);
-            }
+            if (Ph::SOURCE_PASS)
+                DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
+            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, sh.has_z);
+            DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first);
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
