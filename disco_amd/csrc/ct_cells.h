@@ -171,8 +171,8 @@ DHD void emf_sum_dir(const DevState& S, const AnnDev& A, const PhysParams& P, in
             sB += .5 * S.emf_t[(Dr::EMF_HI + 1) * N + ca] * fr;
         }
     }
-    S.esum[Dr::SUM_LO * N + c] = sE;
-    S.esum[(Dr::SUM_LO + 4) * N + c] = sB;
+    st_stream(&S.esum[Dr::SUM_LO * N + c], sE);
+    st_stream(&S.esum[(Dr::SUM_LO + 4) * N + c], sB);
     sE = 0.0;
     sB = 0.0;
     if (Dr::hi(A) >= 0 && (A.flags & Dr::FLAG_HI)) {  // this cell is L
@@ -193,8 +193,8 @@ DHD void emf_sum_dir(const DevState& S, const AnnDev& A, const PhysParams& P, in
             sB += .5 * S.emf_t[(Dr::EMF_LO + 1) * N + cb] * fr;
         }
     }
-    S.esum[Dr::SUM_HI * N + c] = sE;
-    S.esum[(Dr::SUM_HI + 4) * N + c] = sB;
+    st_stream(&S.esum[Dr::SUM_HI * N + c], sE);
+    st_stream(&S.esum[(Dr::SUM_HI + 4) * N + c], sB);
 }
 
 DHD void emf_sum_cell(const DevState& S, int c, int has_z) {
@@ -215,14 +215,14 @@ DHD void emf_avg1_cell(const DevState& S, int c) {
     double Ez = S.emf_phi[0 * N + c], Br = S.emf_phi[1 * N + c];
     double Er = S.emf_phi[2 * N + c], Bz = S.emf_phi[3 * N + c];
     const double* s = S.esum;
-    S.es1[0 * N + c] = .5 * ((.5 * Ez + s[0 * N + c]) + (.5 * Ez + s[0 * N + cp]));
-    S.es1[1 * N + c] = .5 * ((.5 * Ez + s[1 * N + c]) + (.5 * Ez + s[1 * N + cp]));
-    S.es1[2 * N + c] = .5 * ((.5 * Er + s[2 * N + c]) + (.5 * Er + s[2 * N + cp]));
-    S.es1[3 * N + c] = .5 * ((.5 * Er + s[3 * N + c]) + (.5 * Er + s[3 * N + cp]));
-    S.es1[4 * N + c] = .5 * ((.5 * Br + s[4 * N + c]) + (.5 * Br + s[4 * N + cp]));
-    S.es1[5 * N + c] = .5 * ((.5 * Br + s[5 * N + c]) + (.5 * Br + s[5 * N + cp]));
-    S.es1[6 * N + c] = .5 * ((.5 * Bz + s[6 * N + c]) + (.5 * Bz + s[6 * N + cp]));
-    S.es1[7 * N + c] = .5 * ((.5 * Bz + s[7 * N + c]) + (.5 * Bz + s[7 * N + cp]));
+    st_stream(&S.es1[0 * N + c], .5 * ((.5 * Ez + s[0 * N + c]) + (.5 * Ez + s[0 * N + cp])));
+    st_stream(&S.es1[1 * N + c], .5 * ((.5 * Ez + s[1 * N + c]) + (.5 * Ez + s[1 * N + cp])));
+    st_stream(&S.es1[2 * N + c], .5 * ((.5 * Er + s[2 * N + c]) + (.5 * Er + s[2 * N + cp])));
+    st_stream(&S.es1[3 * N + c], .5 * ((.5 * Er + s[3 * N + c]) + (.5 * Er + s[3 * N + cp])));
+    st_stream(&S.es1[4 * N + c], .5 * ((.5 * Br + s[4 * N + c]) + (.5 * Br + s[4 * N + cp])));
+    st_stream(&S.es1[5 * N + c], .5 * ((.5 * Br + s[5 * N + c]) + (.5 * Br + s[5 * N + cp])));
+    st_stream(&S.es1[6 * N + c], .5 * ((.5 * Bz + s[6 * N + c]) + (.5 * Bz + s[6 * N + cp])));
+    st_stream(&S.es1[7 * N + c], .5 * ((.5 * Bz + s[7 * N + c]) + (.5 * Bz + s[7 * N + cp])));
 }
 
 // avg_Efields, face loops (bfields.c:446-538): the edge at the owner's front
@@ -272,15 +272,15 @@ DHD void emf_face_cell(const DevState& S, int c, int has_z) {
     emf_face_dir<DIR_R>(S, A, c, &E1, &B1, &E0, &B0);
     E0 -= A.rm * w * B0;  // hL = r_jph[j-1]
     E1 -= A.rp * w * B1;  // hR = r_jph[j]
-    S.efin[0 * N + c] = E0;
-    S.efin[1 * N + c] = E1;
+    st_stream(&S.efin[0 * N + c], E0);
+    st_stream(&S.efin[1 * N + c], E1);
     if (has_z) {
         double E5, B5, E4, B4;
         emf_face_dir<DIR_Z>(S, A, c, &E5, &B5, &E4, &B4);
         E4 += A.rc * w * B4;  // hD = hU = r centroid
         E5 += A.rc * w * B5;
-        S.efin[2 * N + c] = E4;
-        S.efin[3 * N + c] = E5;
+        st_stream(&S.efin[2 * N + c], E4);
+        st_stream(&S.efin[3 * N + c], E5);
     }
 }
 

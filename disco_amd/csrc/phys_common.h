@@ -60,6 +60,17 @@ DHD double add_mul_rn(double a, double b, double c) {
 #endif
 }
 
+// Store of a per-cell result that only a later kernel reads: non-temporal on
+// the device, so the stream of results does not evict from L2 the neighbour
+// cells' state the kernel is still gathering.
+DHD void st_stream(double* p, double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // ---------------------------------------------------------------- geometry
 // cylindrical.c:10-15
 DHD double get_dp(double phip, double phim, double phi_max) {

@@ -176,18 +176,16 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
             Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
         }
     } else {
-        double Uk[NQ], Fk[NQ];
-        if (w < Ss) {
-            prim2cons<NQ>(P, primL, Uk, x, 1.0);
-            getUstar<NQ>(P, primL, Ustr, x, Sl, Ss, n);
-            flux<NQ>(P, primL, Fk, x, n);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
-        } else {
-            prim2cons<NQ>(P, primR, Uk, x, 1.0);
-            getUstar<NQ>(P, primR, Ustr, x, Sr, Ss, n);
-            flux<NQ>(P, primR, Fk, x, n);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
-        }
+        // riemann.c:272-290 on the side of the contact; the lanes of a
+        // wavefront straddle it, so one path on the selected side's state
+        double Uk[NQ], Fk[NQ], pk[NQ];
+        const bool left = w < Ss;
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) pk[q] = left ? primL[q] : primR[q];
+        const double Sk = left ? Sl : Sr;
+        prim2cons<NQ>(P, pk, Uk, x, 1.0);
+        getUstar<NQ>(P, pk, Ustr, x, Sk, Ss, n);
+        flux<NQ>(P, pk, Fk, x, n);
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sk * (Ustr[q] - Uk[q]);
     }
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
 }

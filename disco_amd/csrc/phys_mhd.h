@@ -477,37 +477,40 @@ DHD void solve(const PhysParams& P, double w, const double* primL, const double*
     pR[UZZ] *= hz;
     double v[5], Pt_star;
     velocities(g, pL, pR, n, v, &Pt_star);
-    if (w < v[2]) {
-        double UL[NQ];
-        get_flux(g, pL, F, n);
-        prim_to_cons(g, pL, UL);
-        if (w < v[0]) {
-            prim_to_cons(g, pL, U);
-        } else if (w < v[1]) {
-            single_star(g, pL, U, v[0], v[2], Pt_star, n);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[0] * (U[q] - UL[q]);
-        } else {
-            double UsL[NQ], UsR[NQ];
-            single_star(g, pL, UsL, v[0], v[2], Pt_star, n);
-            single_star(g, pR, UsR, v[4], v[2], Pt_star, n);
-            double_star(UsL, UsR, U, v, n, 0);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[1] * (U[q] - UsL[q]) + v[0] * (UsL[q] - UL[q]);
-        }
+    // hlld.c:325-386 branches on the side of the contact (w < S_M: the left
+    // states) and then on the region.  The two sides are mirror images, and
+    // in a disk the contact speed lies within round-off of the face speed,
+    // so the lanes of a wavefront straddle it: one side-neutral path on the
+    // selected side's states keeps every lane on it, with the same arithmetic
+    // per side as the reference's branches.
+    const bool left = w < v[2];
+    double pK[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) pK[q] = left ? pL[q] : pR[q];
+    const double SK = left ? v[0] : v[4];   // outer wave of the side
+    const double SsK = left ? v[1] : v[3];  // its Alfven wave
+    double UK[NQ];
+    get_flux(g, pK, F, n);
+    prim_to_cons(g, pK, UK);
+    if (left ? w < v[0] : w > v[4]) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) U[q] = UK[q];
     } else {
-        double UR[NQ];
-        get_flux(g, pR, F, n);
-        prim_to_cons(g, pR, UR);
-        if (w > v[4]) {
-            prim_to_cons(g, pR, U);
-        } else if (w > v[3]) {
-            single_star(g, pR, U, v[4], v[2], Pt_star, n);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[4] * (U[q] - UR[q]);
+        double UsK[NQ];
+        single_star(g, pK, UsK, SK, v[2], Pt_star, n);
+        if (left ? w < v[1] : w > v[3]) {
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+                U[q] = UsK[q];
+                F[q] += SK * (UsK[q] - UK[q]);
+            }
         } else {
-            double UsL[NQ], UsR[NQ];
-            single_star(g, pL, UsL, v[0], v[2], Pt_star, n);
-            single_star(g, pR, UsR, v[4], v[2], Pt_star, n);
-            double_star(UsL, UsR, U, v, n, 1);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += v[3] * (U[q] - UsR[q]) + v[4] * (UsR[q] - UR[q]);
+            double pO[NQ], UsO[NQ], UsL[NQ], UsR[NQ];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) pO[q] = left ? pR[q] : pL[q];
+            single_star(g, pO, UsO, left ? v[4] : v[0], v[2], Pt_star, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+                UsL[q] = left ? UsK[q] : UsO[q];
+                UsR[q] = left ? UsO[q] : UsK[q];
+            }
+            double_star(UsL, UsR, U, v, n, left ? 0 : 1);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += SsK * (U[q] - UsK[q]) + SK * (UsK[q] - UK[q]);
         }
     }
     const double brfac = bscale_r(x[0]), bpfac = 1.0, bzfac = 1.0;
@@ -586,18 +589,16 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
                 Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
             }
         } else {
-            double Uk[NQ], Fk[NQ];
-            if (w < Ss) {
-                prim2cons(P, primL, Uk, x, 1.0);
-                getUstar(P, primL, Ustr, x, Sl, Ss, n, Bpack);
-                flux(P, primL, Fk, x, n);
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
-            } else {
-                prim2cons(P, primR, Uk, x, 1.0);
-                getUstar(P, primR, Ustr, x, Sr, Ss, n, Bpack);
-                flux(P, primR, Fk, x, n);
-                _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
-            }
+            // riemann.c:272-290 on the side of the contact, one path for the
+            // lanes of both sides
+            double Uk[NQ], Fk[NQ], pk[NQ];
+            const bool left = w < Ss;
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) pk[q] = left ? primL[q] : primR[q];
+            const double Sk = left ? Sl : Sr;
+            prim2cons(P, pk, Uk, x, 1.0);
+            getUstar(P, pk, Ustr, x, Sk, Ss, n, Bpack);
+            flux(P, pk, Fk, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sk * (Ustr[q] - Uk[q]);
         }
     }
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;

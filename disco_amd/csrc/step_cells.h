@@ -344,7 +344,7 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
     }
     if (A.flags & ANN_REAL) {
         if (first) {
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.rkcons[q * N + c] = S.cons[q * N + c];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.rkcons[q * N + c], S.cons[q * N + c]);
         } else {
             _Pragma("unroll") for (int q = 0; q < NQ; ++q)
                 S.cons[q * N + c] = (1. - RK) * S.cons[q * N + c] + RK * S.rkcons[q * N + c];
@@ -354,7 +354,7 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
         // face fluxes of every cell (misc.c:239-241; timestep.c:22 at the first substep)
         _Pragma("unroll") for (int f = 0; f < 5; ++f) {
             if (first)
-                S.rkPhi[f * N + c] = S.Phi[f * N + c];
+                st_stream(&S.rkPhi[f * N + c], S.Phi[f * N + c]);
             else
                 S.Phi[f * N + c] = (1. - RK) * S.Phi[f * N + c] + RK * S.rkPhi[f * N + c];
         }
@@ -372,7 +372,7 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
         sR /= .5 * (dpR + dpC);
         double sC = pR - pL;
         sC /= .5 * (dpL + dpR) + dpC;
-        S.gradp[q * N + c] = minmod(P.plm * sL, sC, P.plm * sR);
+        st_stream(&S.gradp[q * N + c], minmod(P.plm * sL, sC, P.plm * sR));
     }
     double front = S.piph[c];
     if (A.outer >= 0) {
@@ -530,7 +530,7 @@ DHD void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P,
                 if (fabs(lim) < fabs(g)) g = lim;
             }
         }
-        grad[q * N + c] = g;
+        st_stream(&grad[q * N + c], g);
     }
 }
 
@@ -588,11 +588,11 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
         FaceEMF e = t_face_flux<Ph, D>(S, P, c, cb, F, A, dt, dU);
         double* f = Dr::f_hi(S);
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) f[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
-            S.emf_t[(Dr::EMF_HI + 0) * N + c] = e.E1;
-            S.emf_t[(Dr::EMF_HI + 1) * N + c] = e.B1;
-            S.emf_t[(Dr::EMF_HI + 2) * N + c] = e.E2;
+            st_stream(&S.emf_t[(Dr::EMF_HI + 0) * N + c], e.E1);
+            st_stream(&S.emf_t[(Dr::EMF_HI + 1) * N + c], e.B1);
+            st_stream(&S.emf_t[(Dr::EMF_HI + 2) * N + c], e.E2);
         }
     }
     if (A.flags & Dr::FLAG_LO) {
@@ -601,11 +601,11 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
         FaceEMF e = t_face_flux<Ph, D>(S, P, ca, c, F, W, dt, dU);
         double* f = Dr::f_lo(S);
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) f[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
-            S.emf_t[(Dr::EMF_LO + 0) * N + c] = e.E1;
-            S.emf_t[(Dr::EMF_LO + 1) * N + c] = e.B1;
-            S.emf_t[(Dr::EMF_LO + 2) * N + c] = e.E2;
+            st_stream(&S.emf_t[(Dr::EMF_LO + 0) * N + c], e.E1);
+            st_stream(&S.emf_t[(Dr::EMF_LO + 1) * N + c], e.B1);
+            st_stream(&S.emf_t[(Dr::EMF_LO + 2) * N + c], e.E2);
         }
     }
 }
@@ -641,12 +641,12 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
         double dA = (A.rp - A.rm) * (A.zp - A.zm);  // get_dA(xp, xm, 0)
         double w = A.rc * S.wiph[c];                // get_scale_factor(x, 0) * wiph
         FaceEMF e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) S.fphi[q * N + c] = dU[q];
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&S.fphi[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_phi) {
-            S.emf_phi[0 * N + c] = e.E1;  // E_z
-            S.emf_phi[1 * N + c] = e.B1;  // B_r
-            S.emf_phi[2 * N + c] = e.E2;  // E_r
-            S.emf_phi[3 * N + c] = e.B2;  // B_z
+            st_stream(&S.emf_phi[0 * N + c], e.E1);  // E_z
+            st_stream(&S.emf_phi[1 * N + c], e.B1);  // B_r
+            st_stream(&S.emf_phi[2 * N + c], e.E2);  // E_r
+            st_stream(&S.emf_phi[3 * N + c], e.B2);  // B_z
         }
     }
     owned_faces_flux<Ph, DIR_R>(S, A, P, c, dt);
@@ -750,7 +750,7 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
             euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);  // planet.c:138-158 (module independent)
         }
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) S.cons[q * N + c] = cons[q];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.cons[q * N + c], cons[q]);
     }
 
     // moving mesh (first substep): every annulus, ghosts included
@@ -839,7 +839,7 @@ template <class Ph>
 DHD double update_b_post(const DevState& S, int c, const double* prim, const CellGeom& G, int want_dt) {
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const size_t N = S.ncell;
-    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) S.prim[q * N + c] = prim[q];
+    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) st_stream(&S.prim[q * N + c], prim[q]);
     if (!want_dt) return 1.0e300;
     // the CFL step of the next step reads this mesh (piph_n is the current
     // mesh after the first substep)
@@ -866,7 +866,7 @@ DHD void update_b_ghost(const DevState& S, int c) {
         x[1] = 0.5 * (phip + (phip - dph));
     double p8[8];
     disco_initial(&Rn.ic, p8, x);
-    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) S.prim[q * N + c] = p8[q];
+    _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) st_stream(&S.prim[q * N + c], p8[q]);
 }
 
 // The whole second half with the module's own cons2prim.  cons[] holds the
