@@ -21,6 +21,7 @@
 namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
 void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
+void launch_partners(const DevState& S, hipStream_t st);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook);
@@ -161,9 +162,12 @@ static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
     launch_diag(S1, h->nann, h->nq, h->stream, hook);
 }
 
-// Recomputes the CFL minimum over the current state (after create/upload).
+// Recomputes the overlap-face partners and the CFL minimum over the current
+// state (after create/upload: the steps search partners only after the mesh
+// moved).
 static int seed_dt(disco_gpu* h) {
     DevState S = make_state(h, 0);
+    launch_partners(S, h->stream);
     launch_mindt(h->shape, S, h->stream, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));

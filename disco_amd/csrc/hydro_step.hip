@@ -150,9 +150,14 @@ __global__ void k_dt_final(DevState S, int check) {
 
 // ------------------------------------------------------------------ stages
 template <class Ph>
-__global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int first) {
+__global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int first, int search) {
     int c = cell_index();
-    if (c < S.ncell) stage_a_cell<Ph>(S, c, RK, first);
+    if (c < S.ncell) stage_a_cell<Ph>(S, c, RK, first, search);
+}
+
+__global__ __launch_bounds__(256) void k_partners(DevState S) {
+    int c = cell_index();
+    if (c < S.ncell) partners_cell(S, c);
 }
 
 template <class Ph>
@@ -434,7 +439,9 @@ struct Launch {
                         hipStream_t st, const LaunchHook* hook) {
         auto B = [&](int k) { return sh.blk[k] > 0 ? sh.blk[k] : 256; };
         auto G = [&](int k) { return dim3(nblk(S.ncell, B(k))); };
-        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, RK, first);
+        // partners: the first substep's are the last second substep's (the
+        // mesh has not moved since)
+        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, RK, first, !first);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
         DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         if (Ph::MHD && sh.ct) {
@@ -469,6 +476,11 @@ void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const Laun
     (void)nq;
     DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S);
     DISCO_LAUNCH(hook, "k_diag_time", k_diag_time, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+}
+
+// partners of every cell on the current mesh (after it is set from the host)
+void launch_partners(const DevState& S, hipStream_t st) {
+    hipLaunchKernelGGL(k_partners, dim3(nblk(S.ncell, 256)), dim3(256), 0, st, S);
 }
 
 // dt = CFL x running minimum (misc.c:79); check: then check_dt's clip

@@ -315,12 +315,53 @@ DHD double mindt_cell(const DevState& S, int c) {
     return cell_mindt<Ph>(S, A, S.runs[A.run].ph, c, prim);
 }
 
+// ------------------------------------------------------------------ partners
+// The partner of cell c in each neighbouring annulus (and, with CT, in the
+// diagonal ones): the first cell whose front does not precede c's front, as
+// the overlap faces of faces_alt.c:36-179 pair them, walked from the last
+// substep's partner.  The mesh only moves in the first substep, so the
+// partners found in a step's second substep are those of the next step's
+// first substep: the search runs in the second substep, and once after the
+// mesh is set (creation, upload).
+DHD void partners_cell(const DevState& S, int c) {
+    const AnnDev& A = S.ann[S.cell_ann[c]];
+    const size_t N = S.ncell;
+    double front = S.piph[c];
+    if (A.outer >= 0) {
+        const AnnDev& B = S.ann[A.outer];
+        S.lbo[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.outer], front, true, S.lbo[c]);
+    }
+    if (A.inner >= 0) {
+        const AnnDev& W = S.ann[A.inner];
+        S.ubi[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.inner], front, false, S.ubi[c]);
+    }
+    if (A.up >= 0) {
+        const AnnDev& B = S.ann[A.up];
+        S.lbu[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.up], front, true, S.lbu[c]);
+    }
+    if (A.down >= 0) {
+        const AnnDev& W = S.ann[A.down];
+        S.ubd[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.down], front, false, S.ubd[c]);
+    }
+    if (S.pdg) {
+        // diagonal annuli of the azimuthal-edge merges; a front of an annulus
+        // higher in (k, j) order counts after an equal front of this one
+        _Pragma("unroll") for (int d = 0; d < 4; ++d) {
+            int a = A.diag[d];
+            if (a < 0) continue;
+            const AnnDev& Q = S.ann[a];
+            const size_t k = (size_t)d * N + c;
+            S.pdg[k] = front_search_near(S.piph, Q.off, Q.np, S.rot[a], front, d < 2, S.pdg[k]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ stage A
 // set_wcell (misc.c:101-168, first substep), adjust_RK_cons (misc.c:226-244,
 // with timestep.c:18-24's copy folded into the first substep), plm_phi
-// (plm.c:15-51) and the partner search of the overlap faces.
+// (plm.c:15-51) and, when `search`, the partner search of the overlap faces.
 template <class Ph>
-DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
+DHD void stage_a_cell(const DevState& S, int c, double RK, int first, int search) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const PhysParams& P = S.runs[A.run].ph;
@@ -374,34 +415,7 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first) {
         sC /= .5 * (dpL + dpR) + dpC;
         st_stream(&S.gradp[q * N + c], minmod(P.plm * sL, sC, P.plm * sR));
     }
-    double front = S.piph[c];
-    if (A.outer >= 0) {
-        const AnnDev& B = S.ann[A.outer];
-        S.lbo[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.outer], front, true, S.lbo[c]);
-    }
-    if (A.inner >= 0) {
-        const AnnDev& W = S.ann[A.inner];
-        S.ubi[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.inner], front, false, S.ubi[c]);
-    }
-    if (A.up >= 0) {
-        const AnnDev& B = S.ann[A.up];
-        S.lbu[c] = front_search_near(S.piph, B.off, B.np, S.rot[A.up], front, true, S.lbu[c]);
-    }
-    if (A.down >= 0) {
-        const AnnDev& W = S.ann[A.down];
-        S.ubd[c] = front_search_near(S.piph, W.off, W.np, S.rot[A.down], front, false, S.ubd[c]);
-    }
-    if (Ph::MHD && S.pdg) {
-        // diagonal annuli of the azimuthal-edge merges; a front of an annulus
-        // higher in (k, j) order counts after an equal front of this one
-        _Pragma("unroll") for (int d = 0; d < 4; ++d) {
-            int a = A.diag[d];
-            if (a < 0) continue;
-            const AnnDev& Q = S.ann[a];
-            const size_t k = (size_t)d * N + c;
-            S.pdg[k] = front_search_near(S.piph, Q.off, Q.np, S.rot[a], front, d < 2, S.pdg[k]);
-        }
-    }
+    if (search) partners_cell(S, c);
 }
 
 // ------------------------------------------------------------------ plm_trans
