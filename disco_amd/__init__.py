@@ -69,6 +69,11 @@ class Params(C.Structure):
         ("noiseAbs", C.c_double), ("noiseRel", C.c_double),
         ("hydro", C.c_int), ("boundary", C.c_int), ("planets", C.c_int), ("initial", C.c_int),
         ("num_c", C.c_int), ("num_n", C.c_int),
+        ("sinkType", C.c_int), ("sinkNumber", C.c_int),
+        ("sinkPar1", C.c_double), ("sinkPar2", C.c_double), ("sinkPar3", C.c_double),
+        ("sinkPar4", C.c_double), ("sinkPar5", C.c_double),
+        ("coolType", C.c_int), ("nozzleType", C.c_int), ("dampInnerType", C.c_int),
+        ("dampOuterType", C.c_int), ("dampUpperType", C.c_int), ("dampLowerType", C.c_int),
     ]
 
     def copy(self) -> "Params":
@@ -103,6 +108,7 @@ def lib() -> C.CDLL:
     L.disco_gpu_step.argtypes = [vp, C.c_int]
     L.disco_gpu_advance.argtypes = [vp, C.c_int, C.c_int, ip, ip]
     L.disco_gpu_final.argtypes = [vp, ip]
+    L.disco_gpu_report.argtypes = [vp, C.c_int, dp, C.c_int, ip]
     L.disco_gpu_timestep.argtypes = [vp, dp]
     L.disco_gpu_sync.argtypes = [vp]
     L.disco_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
@@ -264,6 +270,15 @@ class Domain:
         f = np.zeros(self.nruns, np.int32)
         _check(lib().disco_gpu_final(self._h, f.ctypes.data_as(C.POINTER(C.c_int))), "disco_gpu_final")
         return f.astype(bool)
+
+    def report(self, run: int = 0) -> np.ndarray:
+        """The values of report.c's report.dat line for run `run` (see
+        include/disco_gpu.h: t, q, 8 x Npl planet accumulators, Mass, Torque,
+        Torque2, 5 x (Torque1_cut, Torque2_cut))."""
+        out = np.zeros(15 + 8 * self.info[run].npl)
+        n = C.c_int(0)
+        _check(lib().disco_gpu_report(self._h, int(run), _dptr(out), out.size, C.byref(n)), "disco_gpu_report")
+        return out[:n.value]
 
     def sync(self) -> None:
         _check(lib().disco_gpu_sync(self._h), "disco_gpu_sync")

@@ -22,6 +22,7 @@ namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
 void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
 void launch_partners(const DevState& S, hipStream_t st);
+void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook);
@@ -744,6 +745,46 @@ int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_
     }
     if (steps_done) *steps_done = done;
     if (status) *status = st;
+    return 0;
+}
+
+int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) {
+    if (!h || run < 0 || run >= h->nruns || !out) return fail("disco_gpu_report: bad arguments");
+    const HostRun& R = h->runs[run];
+    const int npl = (int)R.planets.size();
+    const int n = 15 + 8 * npl;
+    if (nout) *nout = n;
+    if (nmax < n) return fail("disco_gpu_report: out holds " + std::to_string(nmax) + " values, the line has " +
+                              std::to_string(n));
+    const int na = R.Nr * R.Nz, a0 = h->run_ann0[run];
+    std::vector<double> part((size_t)na * DISCO_NREPORT);
+    double* dpart = nullptr;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMalloc((void**)&dpart, part.size() * sizeof(double)));
+    DevState S = make_state(h, 0);
+    launch_report(S, a0, na, dpart, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(part.data(), dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                            h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dpart);
+    if (e != hipSuccess) return fail(std::string("disco_gpu_report: ") + hipGetErrorString(e));
+    // the annuli in report.c's loop order: j outer, k inner
+    std::vector<int> pos(na);
+    for (int a = 0; a < na; ++a) pos[h->hann[a0 + a].j + R.Nr * h->hann[a0 + a].k] = a;
+    double sum[DISCO_NREPORT] = {};
+    for (int j = 0; j < R.Nr; ++j)
+        for (int k = 0; k < R.Nz; ++k)
+            for (int q = 0; q < DISCO_NREPORT; ++q) sum[q] += part[(size_t)pos[j + R.Nr * k] * DISCO_NREPORT + q];
+    double t = 0.0;
+    HIPCHK(hipMemcpy(&t, h->t + run, sizeof(double), hipMemcpyDeviceToHost));
+    int o = 0;
+    out[o++] = t;
+    out[o++] = npl > 1 ? R.planets[1].M / R.planets[0].M : 0.0;
+    // the planets' accreted and exchanged quantities (sink.c): no sinks run
+    for (int p = 0; p < 8 * npl; ++p) out[o++] = 0.0;
+    for (int q = 0; q < 3; ++q) out[o++] = sum[q];
+    for (int q = 3; q < DISCO_NREPORT; ++q) out[o++] = sum[q];
     return 0;
 }
 

@@ -18,6 +18,9 @@ namespace disco {
 
 // quantities of the Analysis module's get_diagnostics (Diagnostics/diag_euler.c:9-11)
 constexpr int DISCO_NDIAG = 8;
+// report.c's gas sums per annulus: mass, the torques of planets 1 and 0, and
+// both torques outside (n + 1) x Sink_Par3 of planet 1 for n = 0..4
+constexpr int DISCO_NREPORT = 13;
 
 enum AnnFlags {
     ANN_REAL = 1,        // real annulus: conserved update + cons2prim

@@ -78,6 +78,11 @@ typedef struct disco_params {
     /* module selection (compile-time in the reference) */
     int hydro, boundary, planets, initial;
     int num_c, num_n;
+    /* sink.c's source terms (not built: a non-zero type is rejected);
+       Sink_Par3 is also report.c's torque-cut length */
+    int sinkType, sinkNumber;
+    double sinkPar1, sinkPar2, sinkPar3, sinkPar4, sinkPar5;
+    int coolType, nozzleType, dampInnerType, dampOuterType, dampUpperType, dampLowerType;
 } disco_params;
 
 #ifdef __cplusplus

@@ -118,6 +118,20 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     D("Noise_Abs", &p->noiseAbs);
     D("Noise_Rel", &p->noiseRel);
     D("Softening", &p->grav_eps);
+    // readpar.c:146-174 (sink.c's sinks, cooling, nozzle and damping zones)
+    I("Sink_Type", &p->sinkType);
+    I("Sink_Number", &p->sinkNumber);
+    D("Sink_Par1", &p->sinkPar1);
+    D("Sink_Par2", &p->sinkPar2);
+    D("Sink_Par3", &p->sinkPar3);
+    D("Sink_Par4", &p->sinkPar4);
+    D("Sink_Par5", &p->sinkPar5);
+    I("Cool_Type", &p->coolType);
+    I("Nozzle_Type", &p->nozzleType);
+    I("DampInner_Type", &p->dampInnerType);
+    I("DampOuter_Type", &p->dampOuterType);
+    I("DampUpper_Type", &p->dampUpperType);
+    I("DampLower_Type", &p->dampLowerType);
     // module slots (Makefile_opt.in in the reference); default euler/fixed/sun
     char s[256];
     p->hydro = DISCO_HYDRO_EULER;
@@ -187,6 +201,7 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->num_n = par.num_n;
     ph->gr_M = par.metricPar2;
     ph->gr_a = par.metricPar3;
+    ph->report_soft = par.sinkPar3;
     disco_ic_from_params(&par, ic);
 }
 
@@ -222,6 +237,11 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
         m += " Riemann_Solver";
     if (p.visc_flag) m += " Use_Viscosity";
+    // add_source's sink.c terms (misc.c:579-591) are not built
+    if (p.sinkType) m += " Sink_Type";
+    if (p.coolType) m += " Cool_Type";
+    if (p.nozzleType) m += " Nozzle_Type";
+    if (p.dampInnerType || p.dampOuterType || p.dampUpperType || p.dampLowerType) m += " Damp*_Type";
     if (p.RotFrame) m += " RotFrame";
     if (p.noiseType) m += " Noise_Type";
     if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";

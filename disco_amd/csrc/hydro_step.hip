@@ -401,6 +401,36 @@ __global__ __launch_bounds__(256) void k_diag(DevState S) {
     }
 }
 
+// report.c's gas sums (report_quantities) of the annuli a0, a0+1, ... of one
+// run: one block per annulus, each writing its DISCO_NREPORT sums (zero for
+// ghost annuli); the host adds the annuli in the reference's loop order.
+__global__ __launch_bounds__(256) void k_report(DevState S, int a0, double* out) {
+    __shared__ double red[DISCO_NREPORT][4];
+    const int a = a0 + blockIdx.x;
+    const AnnDev& A = S.ann[a];
+    double acc[DISCO_NREPORT];
+    for (int q = 0; q < DISCO_NREPORT; ++q) acc[q] = 0.0;
+    if (A.flags & ANN_REAL) {
+        for (int i = threadIdx.x; i < A.np; i += blockDim.x) {
+            double Q[DISCO_NREPORT];
+            report_quantities(S, A, A.off + i, Q);
+            for (int q = 0; q < DISCO_NREPORT; ++q) acc[q] += Q[q];
+        }
+    }
+    for (int q = 0; q < DISCO_NREPORT; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < DISCO_NREPORT; ++q) red[q][wave] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < DISCO_NREPORT) {
+        const int q = threadIdx.x;
+        double s = red[q][0];
+        for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[q][w];
+        out[(size_t)blockIdx.x * DISCO_NREPORT + q] = s;
+    }
+}
+
 __global__ void k_diag_time(DevState S) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < S.nruns) S.t_avg[r] += S.dt[r];
@@ -476,6 +506,10 @@ void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const Laun
     (void)nq;
     DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S);
     DISCO_LAUNCH(hook, "k_diag_time", k_diag_time, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+}
+
+void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_report, dim3(nann), dim3(256), 0, st, S, a0, out);
 }
 
 // partners of every cell on the current mesh (after it is set from the host)

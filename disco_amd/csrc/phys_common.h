@@ -35,6 +35,7 @@ struct PhysParams {
     int npl;
     int num_c, num_n;
     double gr_M, gr_a;  // Kerr-Schild mass and spin (Metric_Par2, Metric_Par3)
+    double report_soft; // Sink_Par3: report.c's torque-cut length
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face
@@ -178,10 +179,11 @@ DHD double fgrav_pointmass(double M, double r, double eps) {
     return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
 }
 
-// planet.c:160-234: force per unit mass from one planet, returned as
-// (fr, fp, fz) in the orthonormal cylindrical basis at (r, phi, z).
+// planet.c:138-187: force per unit mass from one planet, returned as
+// (fr, fp, fz) in the orthonormal cylindrical basis at (r, phi, z); mode 1
+// (report.c's torques) projects on the planet's direction instead.
 DHD void planetary_force(const Planet& pl, int grav2D, double r, double phi, double z,
-                         double* fr, double* fp, double* fz) {
+                         double* fr, double* fp, double* fz, int mode = 0) {
     double rp = pl.r;
     double pp = pl.phi;
     if (grav2D == 1) {
@@ -202,6 +204,10 @@ DHD void planetary_force(const Planet& pl, int grav2D, double r, double phi, dou
     double sina = dy / script_r_perp;
     double cosap = cosa * cosp + sina * sinp;
     double sinap = sina * cosp - cosa * sinp;
+    if (mode == 1) {
+        cosap = cosa * cos(pp) + sina * sin(pp);
+        sinap = sina * cos(pp) - cosa * sin(pp);
+    }
     if (script_r_perp <= 0.0) {
         cosap = 0.0;
         sinap = 0.0;

@@ -960,6 +960,51 @@ DHD void planets_run(const DevState& S, int r) {
     S.t[r] += dt;
 }
 
+// report.c:117-207 for one cell of a real annulus (mesh, prims and planet
+// phases of the current state): Q[0] = rho dV; with two planets or more,
+// Q[1] = -(rho - 1) r_1 f_phi,1 dV and Q[2] = -(rho - 1) r_0 f_phi,0 dV (the
+// forces of planets 1 and 0 at (r, phi, 0) projected on the planet's
+// direction, planetaryForce mode 1), and Q[3 + 2n], Q[4 + 2n] the same two
+// where the cell lies farther than (n + 1) x Sink_Par3 from planet 1.
+DHD void report_quantities(const DevState& S, const AnnDev& A, int c, double* Q) {
+    const RunDev& Rn = S.runs[A.run];
+    const PhysParams& P = Rn.ph;
+    const size_t N = S.ncell;
+    for (int q = 0; q < DISCO_NREPORT; ++q) Q[q] = 0.0;
+    const double phip = S.piph[c];
+    const double dph = S.dphi[c];
+    const double phi = phip - .5 * dph;
+    const double rho = S.prim[RHO * N + c];
+    double xp[3] = {A.rp, phip, A.zp};
+    double xm[3] = {A.rm, phip - dph, A.zm};
+    const double dV = get_dV(xp, xm, P.phi_max);
+    const double r = get_centroid(xp[0], xm[0], 1);
+    Q[0] = rho * dV;
+    if (P.npl > 1) {
+        Planet p0 = Rn.planets[0], p1 = Rn.planets[1];
+        p0.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + 0];
+        p1.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + 1];
+        double fr, fz, fp, fp2;
+        planetary_force(p0, P.grav2D, r, phi, 0.0, &fr, &fp2, &fz, 1);
+        planetary_force(p1, P.grav2D, r, phi, 0.0, &fr, &fp, &fz, 1);
+        const double rS = p0.r, rp = p1.r, pp = p1.phi;
+        double dx = r * cos(phi) - rp * cos(pp);
+        double dy = r * sin(phi) - rp * sin(pp);
+        double script_r = sqrt(dx * dx + dy * dy);
+        double t1v = (rho - 1.0) * rp * fp * dV;
+        double t2v = (rho - 1.0) * rS * fp2 * dV;
+        Q[1] = -t1v;
+        Q[2] = -t2v;
+        for (int n = 0; n < 5; ++n) {
+            double r_cut = (double)(n + 1.) * P.report_soft;
+            if (script_r > r_cut) {
+                Q[3 + 2 * n] = -t1v;
+                Q[4 + 2 * n] = -t2v;
+            }
+        }
+    }
+}
+
 // get_diagnostics of Diagnostics/diag_euler.c:17-44 at the cell's centroid:
 // rho, 2 pi r rho v_r, 2 pi r rho v_z, the torque density of planet 1 (when
 // there are two), the advective angular-momentum flux, Omega, the angular

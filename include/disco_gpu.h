@@ -81,6 +81,16 @@ int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_
  * been taken). */
 int disco_gpu_final(disco_gpu_t* h, int* final_step);
 
+/* report (report.c:6-326) of run `run` at the current state: the values of
+ * its report.dat line, in order: t, q = M_1/M_0 (0 with fewer than two
+ * planets), the planets' Lg, M_acc, La, Ls, kin, therm, xMom, yMom (Npl each;
+ * sink.c's accumulators, zero: sinks are not run), Mass, Torque, Torque2,
+ * then Torque1_cut[n], Torque2_cut[n] for n = 0..4 (cut at (n+1) x
+ * Sink_Par3).  *nout = 15 + 8 Npl; fails if nmax is smaller.  The gas sums
+ * run on the device (one block per annulus) and are added on the host in the
+ * reference's loop order.  Synchronous; changes no state. */
+int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
+
 /* timestep(dom, dt) with caller-given dt per run (no CFL reduction). */
 int disco_gpu_timestep(disco_gpu_t* h, const double* dt);
 

@@ -24,6 +24,7 @@ oracle_run* oracle_create(const disco_params* par, int Nr, int Nz, int NgRa, int
                           const double* planet_phi, double t);
 double oracle_getmindt(oracle_run* o);
 int oracle_check_dt(oracle_run* o, double* dt);
+int oracle_report(oracle_run* o, double* out);
 void oracle_timestep(oracle_run* o, double dt);
 void oracle_get_state(oracle_run* o, double* prim, double* cons, double* piph, double* dphi, double* wiph,
                       double* planet_phi, double* t);

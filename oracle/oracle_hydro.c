@@ -634,6 +634,73 @@ void oracle_timestep(oracle_run* o, double dt) {
     o_onestep(o, 0.5, 0.5 * dt, 0);
 }
 
+/* planet.c:138-187 with mode 1 (report.c's torques): the azimuthal force of
+ * planet ip at (r, phi, z = 0), projected on the planet's direction */
+static double o_force_phi_mode1(const oracle_run* o, int ip, double r, double phi) {
+    double rp = o->plr[ip], pp = o->plphi[ip], z = 0.0;
+    if (o->par.grav2D == 2) {
+        rp = r;
+        pp = phi;
+        z = 1.0;
+    }
+    double dx = r * cos(phi) - rp * cos(pp);
+    double dy = r * sin(phi) - rp * sin(pp);
+    double sr = sqrt(dx * dx + dy * dy + z * z), sperp = sqrt(dx * dx + dy * dy);
+    double f1 = -o_fgrav(o->plM[ip], sr, o->pleps[ip]);
+    double ca = dx / sperp, sa = dy / sperp;
+    double sap = sa * cos(pp) - ca * sin(pp);
+    if (sperp <= 0.0) sap = 0.0;
+    return sap * f1 * (sperp / sr);
+}
+
+/* report.c:6-326: the values of its report.dat line -- t, q, the planets'
+ * 8 x Npl sink accumulators (zero: no sinks run), Mass, Torque, Torque2 and
+ * 5 x (Torque1_cut, Torque2_cut); returns their count. */
+int oracle_report(oracle_run* o, double* out) {
+    double Mass = 0.0, T1 = 0.0, T2 = 0.0, c1[5] = {0, 0, 0, 0, 0}, c2[5] = {0, 0, 0, 0, 0};
+    int j, k, i, n, m = 0;
+    for (j = o->NgRa; j < o->Nr - o->NgRb; ++j)
+        for (k = o->NgZa; k < o->Nz - o->NgZb; ++k) {
+            int jk = j + o->Nr * k;
+            for (i = 0; i < o->Np[jk]; ++i) {
+                const ocell* c = &o->cells[jk][i];
+                double phi = c->piph - .5 * c->dphi;
+                double rho = c->prim[RHO];
+                double xp[3] = {RJ(o, j), c->piph, ZK(o, k)};
+                double xm[3] = {RJ(o, j - 1), c->piph - c->dphi, ZK(o, k - 1)};
+                double dV = o_dV(o, xp, xm);
+                double r = o_cent(xp[0], xm[0], 1);
+                Mass += rho * dV;
+                if (o->npl > 1) {
+                    double fp2 = o_force_phi_mode1(o, 0, r, phi);
+                    double fp = o_force_phi_mode1(o, 1, r, phi);
+                    double rS = o->plr[0], rp = o->plr[1], pp = o->plphi[1];
+                    double dx = r * cos(phi) - rp * cos(pp), dy = r * sin(phi) - rp * sin(pp);
+                    double sr = sqrt(dx * dx + dy * dy);
+                    double t1v = (rho - 1.0) * rp * fp * dV, t2v = (rho - 1.0) * rS * fp2 * dV;
+                    T1 -= t1v;
+                    T2 -= t2v;
+                    for (n = 0; n < 5; ++n)
+                        if (sr > (double)(n + 1.) * o->par.sinkPar3) {
+                            c1[n] -= t1v;
+                            c2[n] -= t2v;
+                        }
+                }
+            }
+        }
+    out[m++] = o->t;
+    out[m++] = o->npl > 1 ? o->plM[1] / o->plM[0] : 0.0;
+    for (n = 0; n < 8 * o->npl; ++n) out[m++] = 0.0;
+    out[m++] = Mass;
+    out[m++] = T1;
+    out[m++] = T2;
+    for (n = 0; n < 5; ++n) {
+        out[m++] = c1[n];
+        out[m++] = c2[n];
+    }
+    return m;
+}
+
 /* domain.c:283-292 (check_dt without its file controls): a step that would
  * pass T_End is cut to end on it; returns 1 for that (final) step. */
 int oracle_check_dt(oracle_run* o, double* dt) {

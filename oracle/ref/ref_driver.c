@@ -13,7 +13,8 @@
  * usage: disco_ref NSTEPS OUTPREFIX [DUMPSTEP ...]
  *   reads ./in.par; writes OUTPREFIX_<step>.bin at step 0, at every DUMPSTEP
  *   and at NSTEPS; prints one line "ref steps=N cells=C seconds=S" with the
- *   wall time of the step loop alone.
+ *   wall time of the step loop alone; after the loop the reference's own
+ *   report() (report.c) appends its line to ./report.dat.
  *   NSTEPS = 0: the reference's own loop condition instead (main.c:68-75
  *   without output): getmindt, check_dt (domain.c:283-310, the T_End clip),
  *   timestep until check_dt marks the final step; the last dump is
@@ -34,6 +35,7 @@ double getmindt(struct domain*);
 void check_dt(struct domain*, double*);
 void timestep(struct domain*, double);
 void freeDomain(struct domain*);
+void report(struct domain*);
 
 static void dump(struct domain* d, const char* prefix, int step, double dt_last) {
     char name[1024];
@@ -140,6 +142,7 @@ int main(int argc, char* argv[]) {
     seconds += now() - t0;
     if (to_end) nsteps = n - 1;
     printf("ref steps=%d cells=%ld seconds=%.6f\n", nsteps, ncells, seconds);
+    report(&theDomain);
     freeDomain(&theDomain);
     return 0;
 }

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report]   (only the named set)
 """
 from __future__ import annotations
 
@@ -58,6 +58,32 @@ def run_ref_to_end(variant: str, par_text: str):
                              text=True).stdout
         n = int(re.search(r"steps=(\d+)", out).group(1))
         return {0: read_dump(os.path.join(d, "out_0.bin")), n: read_dump(os.path.join(d, f"out_{n}.bin"))}
+
+
+def run_ref_report(variant: str, par_text: str, nsteps: int):
+    """States at steps 0 and nsteps and the values of the line the
+    reference's report() (report.c) writes after the last step.  Its format
+    prints t and q with %.7le and no separator after q, every other value
+    with %le: floats are read by their two-digit exponents."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_{variant}")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "in.par"), "w").write(par_text)
+        subprocess.run([exe, str(nsteps), "out"], cwd=d, check=True, timeout=600, capture_output=True)
+        states = {s: read_dump(os.path.join(d, f"out_{s}.bin")) for s in (0, nsteps)}
+        line = open(os.path.join(d, "report.dat")).read().strip().splitlines()[-1]
+    vals = [float(v) for v in re.findall(r"[-+]?\d+\.\d+e[-+]\d\d", line)]
+    return states, np.array(vals)
+
+
+def fixtures_report() -> None:
+    """report.c's line after 10 steps of the small binary disk (two planets:
+    torques and their cuts at (n+1) x Sink_Par3 = 0.2 (n+1)) and of the small
+    single-star disk (mass only)."""
+    for name, base, over, variant in (("c3_report.npz", "c3_binary_disk.par", dict(Num_R=24, Sink_Par3=0.2),
+                                       "euler_binary"),
+                                      ("c1_report.npz", "c1_kepler2d.par", dict(Num_R=16), "euler_sun")):
+        states, rep = run_ref_report(variant, par_with(base, **over), 10)
+        save_states(name, states, report=rep)
 
 
 def fixtures_t_end() -> None:
@@ -246,6 +272,10 @@ def main() -> None:
         fixtures_mhd_full()
         print("fixtures written")
         return
+    if "--report" in sys.argv:
+        fixtures_report()
+        print("fixtures written")
+        return
     if "--t-end" in sys.argv:
         fixtures_t_end()
         print("fixtures written")
@@ -263,6 +293,7 @@ def main() -> None:
         return
     fixtures_2d()
     fixtures_t_end()
+    fixtures_report()
     # full-size 2D disk: conserved totals only
     txt = par_with("c1_kepler2d.par")
     states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))

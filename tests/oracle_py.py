@@ -29,6 +29,8 @@ def lib():
         L.oracle_timestep.argtypes = [C.c_void_p, C.c_double]
         L.oracle_check_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.oracle_check_dt.restype = C.c_int
+        L.oracle_report.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.oracle_report.restype = C.c_int
         L.oracle_get_state.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_track_faces.argtypes = [dp, dp, C.c_int, dp, dp, C.c_int, C.c_double, ip, ip, ip, dp, dp]
@@ -75,6 +77,12 @@ class Oracle:
             if final:
                 return n
         raise RuntimeError("T_End not reached")
+
+    def report(self) -> np.ndarray:
+        """report.c's report.dat values for the current state."""
+        out = np.zeros(64)
+        m = lib().oracle_report(self._h, _d(out))
+        return out[:m]
 
     def state(self) -> dict:
         n = self.ncell

@@ -17,11 +17,13 @@ times dt, ghosts included) to the same tolerances per quantity.  Summation
 order differs from the reference (gather instead of in-place scatter) and
 libm/ocml pow/sin/cos differ by ulps, so bit equality is not expected.
 """
+import os
+
 import numpy as np
 import pytest
 
 import disco_amd as D
-from dumpio import load_fixture
+from dumpio import GOLDEN, load_fixture
 from helpers import cell_radius, diag_errors, params, phi_err, state_errors
 
 pytestmark = pytest.mark.gpu
@@ -200,4 +202,38 @@ def test_diagnostics_reset_and_accumulate():
     # the learned cons2prim start exists for GRMHD runs only
     with pytest.raises(RuntimeError, match="not GRMHD"):
         dom.set_c2p_net(np.zeros(13186, np.float32), np.ones(20, np.float32))
+    dom.close()
+
+
+REPORT_CASES = [("c3_report.npz", "c3_binary_disk.par", dict(Num_R=24, Sink_Par3=0.2)),
+                ("c1_report.npz", "c1_kepler2d.par", dict(Num_R=16))]
+
+
+@pytest.mark.parametrize("fixture,base,over", REPORT_CASES, ids=[c[0] for c in REPORT_CASES])
+def test_report_matches_reference_and_oracle(fixture, base, over):
+    """report.c's line after 10 steps: against the line the reference
+    printed (%le, 7 significant digits: 5e-7 relative; zeros exact), and
+    against the oracle's report of the GPU's own state (1e-11 relative: the
+    same sums in another order).  In a batch, the run's line is the one it
+    has alone."""
+    from oracle_py import Oracle
+
+    rep = np.load(os.path.join(GOLDEN, fixture))["report"]
+    p = params(base, **over)
+    dom = D.Domain(p)
+    dom.step(10)
+    got = dom.report(0)
+    assert got.size == rep.size
+    assert np.all(np.abs(got - rep) <= 5.01e-7 * np.abs(rep)), np.abs(got - rep) / np.maximum(np.abs(rep), 1e-300)
+    I, g, tm = dom.info[0], dom.grid(0), dom.time()
+    grid = dict(Nr=I.Nr, Nz=I.Nz, NgRa=I.NgRa, NgRb=I.NgRb, NgZa=I.NgZa, NgZb=I.NgZb, **g)
+    o = Oracle(p, grid, dom.state(0), planet_phi=tm["planet_phi"][0], t=tm["t"][0])
+    want = o.report()
+    o.close()
+    assert np.all(np.abs(got - want) <= 1e-11 * np.abs(want)), np.abs(got - want) / np.maximum(np.abs(want), 1e-300)
+    # the same run second in a batch
+    batch = D.Domain([params(base, **{**over, "Num_R": 12}), p])
+    batch.step(10)
+    assert np.array_equal(batch.report(1), got)
+    batch.close()
     dom.close()

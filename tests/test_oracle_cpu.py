@@ -102,6 +102,46 @@ def test_oracle_check_dt_matches_reference(fixture):
     o.close()
 
 
+REPORT_CASES = [("c3_report.npz", "c3_binary_disk.par", dict(Num_R=24, Sink_Par3=0.2)),
+                ("c1_report.npz", "c1_kepler2d.par", dict(Num_R=16))]
+
+
+@pytest.mark.parametrize("fixture,base,over", REPORT_CASES, ids=[c[0] for c in REPORT_CASES])
+def test_oracle_report_matches_reference(fixture, base, over):
+    """report.c on the reference's own state after 10 steps: the oracle's
+    full-precision values against the line the reference printed (%le: 7
+    significant digits, so 5e-7 relative), zeros exactly; and the mass
+    against an independent sum of rho dV."""
+    rep = np.load(os.path.join(ROOT, "tests", "golden", fixture))["report"]
+    ref = load_fixture(fixture)
+    st = ref[max(ref)]
+    pl = st["planets"][:, 1] if st["planets"].size else None
+    o = Oracle(params(base, **over), _grid(st), st, planet_phi=pl, t=st["t"])
+    got = o.report()
+    o.close()
+    assert got.size == rep.size
+    assert np.all(np.abs(got - rep) <= 5.01e-7 * np.abs(rep)), np.abs(got - rep) / np.maximum(np.abs(rep), 1e-300)
+    # the gas mass: sum of rho x rbar dr dphi dz over the real cells
+    rj, zk = st["r_jph"], st["z_kph"]
+    off = np.concatenate([[0], np.cumsum(st["Np"])])
+    mass = 0.0
+    for k in range(st["NgZa"], st["Nz"] - st["NgZb"]):
+        for j in range(st["NgRa"], st["Nr"] - st["NgRb"]):
+            c = slice(off[j + st["Nr"] * k], off[j + st["Nr"] * k + 1])
+            dV = .5 * (rj[j + 1] + rj[j]) * (rj[j + 1] - rj[j]) * st["dphi"][c] * (zk[k + 1] - zk[k])
+            mass += (st["prim"][0, c] * dV).sum()
+    npl = (rep.size - 15) // 8
+    assert abs(got[2 + 8 * npl] - mass) <= 1e-13 * mass
+    if npl > 1:
+        # with a cut length of zero every cell lies outside every cut: each
+        # cut sum is the whole torque
+        o = Oracle(params(base, **{**over, "Sink_Par3": 0.0}), _grid(st), st, planet_phi=pl, t=st["t"])
+        z = o.report()
+        o.close()
+        i = 3 + 8 * npl
+        assert np.array_equal(z[i + 2::2], np.full(5, z[i])) and np.array_equal(z[i + 3::2], np.full(5, z[i + 1]))
+
+
 def _brute_overlaps(piA, dpA, piB, dpB, P=2 * np.pi):
     """All (a, b) pairs whose phi intervals overlap (positive length), by
     brute force on the circle."""
