@@ -316,8 +316,12 @@ def main() -> int:
                                    "(no sharding, no collectives); run with --gpus 1"}))
         return 2
 
+    def note(msg: str) -> None:  # progress on stderr; the JSON line stays alone on stdout
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     wl = args.workload
     r = bench_fv(wl, args.steps, args.warmup)
+    note(f"{wl}: {r['value']:.4g} zone-updates/s")
     hydro = WORKLOADS[wl][2]
     cfg = {"workload": wl, "real_cells": r["real_cells"], "integrator": "RK2", "plm": 1.5,
            "reference_convention_megazones_per_s": 2 * r["value"] / 1e6}
@@ -354,17 +358,22 @@ def main() -> int:
         if not args.no_cpu:
             k["cpu_baseline"] = cpu_baseline("kepler2d_nr256")
         line["kepler2d"] = k
+        note(f"kepler2d_nr256: {k['value']:.4g} zone-updates/s")
     if not args.no_extra and wl != "grmhd_torus_nr192_nz192":
         g = bench_fv("grmhd_torus_nr192_nz192", 10, 2)
         g["workload"] = "grmhd_torus_nr192_nz192"
+        note(f"grmhd_torus_nr192_nz192: {g['value']:.4g} zone-updates/s")
         g["learned_c2p"] = bench_grmhd_learned(10, 2)
+        note("grmhd torus with the learned cons2prim start done")
         if not args.no_cpu:
             g["cpu_baseline"] = cpu_baseline("grmhd_torus_nr192_nz192")
         line["grmhd_torus"] = g
     if not args.no_batch and not args.no_extra:
         line["batch"] = bench_batch(args.steps, args.warmup)
+        note(f"batch: {line['batch']['value']:.4g} zone-updates/s")
     if not args.no_c2p and not args.no_extra:
         line["c2p"] = bench_c2p(args.c2p_states, args.c2p_train_steps)
+        note("cons2prim done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line))
