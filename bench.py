@@ -297,6 +297,7 @@ def main() -> int:
     ap.add_argument("--no-c2p", action="store_true")
     ap.add_argument("--c2p-states", type=int, default=1 << 24)
     ap.add_argument("--c2p-train-steps", type=int, default=2000)
+    ap.add_argument("--progress", action="store_true", help="a line on stderr after each workload")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -317,7 +318,8 @@ def main() -> int:
         return 2
 
     def note(msg: str) -> None:  # progress on stderr; the JSON line stays alone on stdout
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+        if args.progress:
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     wl = args.workload
     r = bench_fv(wl, args.steps, args.warmup)

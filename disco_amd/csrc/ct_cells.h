@@ -294,14 +294,10 @@ struct Corner {
 
 // EMF along the part of the edge that ends at the front of the cell of rank
 // `w4` (make_edge_adjust's segment, bfields.c:946-1108), given the four
-// current cells; returns E * dl * dt.
-DHD double edge_segment(const DevState& S, const int* cur, int w4, double r, double dt, double Pmax) {
+// current cells and their fronts ph and backs bk; returns E * dl * dt.
+DHD double edge_segment(const DevState& S, const int* cur, const double* ph, const double* bk, int w4, double r,
+                        double dt, double Pmax) {
     const size_t N = S.ncell;
-    double ph[4], bk[4];
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {
-        ph[q] = S.piph[cur[q]];
-        bk[q] = ph[q] - S.dphi[cur[q]];
-    }
     // get_which4 mode 0 toggles (bfields.c:834-885)
     int LR_D = get_signed_dp(ph[0], ph[1], Pmax) > 0.0;
     int LR_U = get_signed_dp(ph[2], ph[3], Pmax) > 0.0;
@@ -354,11 +350,6 @@ DHD int nbr_part(const DevState& S, int x, int dj, int dk) {
     return S.pdg[(size_t)d * S.ncell + x];
 }
 
-DHD int corner_cell(const DevState& S, int a, int pos) {
-    const AnnDev& Q = S.ann[a];
-    return Q.off + (S.rot[a] + pos) % Q.np;
-}
-
 // Sum of E dl dt over the segments of corner K inside face F, whose two
 // cells are pc0 (rank p0) and pc1 (rank p1), which ends at the front of its
 // owner X (rank orank) and starts at the front of cell P (rank prank).  The
@@ -381,32 +372,44 @@ DHD double edge_face_sum(const DevState& S, const Corner& K, int p0, int p1, int
     int b1 = wrap_idx(nbr_part(S, X, (q1 & 1) - (orank & 1), (q1 >> 1) - (orank >> 1)) - r1, n1);
     int m0 = wrap_idx(b0 - a0, n0);
     int m1 = wrap_idx(b1 - a1, n1);
+    // the face's two cells stay; the other two annuli's cells advance
+    const int o0 = S.ann[aq0].off, o1 = S.ann[aq1].off;
     int cur[4];
+    double ph[4], bk[4];
     cur[p0] = pc0;
     cur[p1] = pc1;
+    ph[p0] = S.piph[pc0];
+    bk[p0] = ph[p0] - S.dphi[pc0];
+    ph[p1] = S.piph[pc1];
+    bk[p1] = ph[p1] - S.dphi[pc1];
+    auto at = [&](int q, int o, int n, int rot, int pos) {  // corner_cell, loads kept
+        int i = rot + pos;
+        if (i >= n) i -= n;
+        if (i >= n) i -= n;
+        const int cc = o + i;
+        cur[q] = cc;
+        ph[q] = S.piph[cc];
+        bk[q] = ph[q] - S.dphi[cc];
+    };
     double sum = 0.0;
     int i0 = 0, i1 = 0;
+    at(q0, o0, n0, r0, a0);
+    at(q1, o1, n1, r1, a1);
     while (i0 < m0 || i1 < m1) {
-        int c0 = corner_cell(S, aq0, a0 + i0);
-        int c1 = corner_cell(S, aq1, a1 + i1);
         bool take0;
         if (i0 >= m0)
             take0 = false;
         else if (i1 >= m1)
             take0 = true;
         else
-            take0 = get_signed_dp(S.piph[c0], S.piph[c1], Pmax) <= 0.0;  // tie: lower rank q0 first
-        cur[q0] = c0;
-        cur[q1] = c1;
-        sum += edge_segment(S, cur, take0 ? q0 : q1, K.r, dt, Pmax);
+            take0 = get_signed_dp(ph[q0], ph[q1], Pmax) <= 0.0;  // tie: lower rank q0 first
+        sum += edge_segment(S, cur, ph, bk, take0 ? q0 : q1, K.r, dt, Pmax);
         if (take0)
-            ++i0;
+            at(q0, o0, n0, r0, a0 + (++i0));
         else
-            ++i1;
+            at(q1, o1, n1, r1, a1 + (++i1));
     }
-    cur[q0] = corner_cell(S, aq0, a0 + m0);
-    cur[q1] = corner_cell(S, aq1, a1 + m1);
-    sum += edge_segment(S, cur, orank, K.r, dt, Pmax);
+    sum += edge_segment(S, cur, ph, bk, orank, K.r, dt, Pmax);
     return sum;
 }
 

@@ -329,8 +329,9 @@ __global__ __launch_bounds__(256) void k_source(DevState S, double dtfac) {
 // update_B_fluxes + make_edge_adjust (face fluxes), then the first half of
 // the cell update (conserved state, sources, mesh move) in a launch of its
 // own: fused, the two need 127 VGPRs (4 waves per SIMD) and run 19% slower
-// than apart (77 VGPRs for the Stokes update, 6 waves).
-__global__ __launch_bounds__(256) void k_ct_update(DevState S, double dtfac, int has_z) {
+// than apart.  The Stokes update is held to 6 waves per SIMD (80 VGPRs; 5%
+// faster than its natural 83 at 5 waves).
+__global__ __launch_bounds__(256) DISCO_OCC(6) void k_ct_update(DevState S, double dtfac, int has_z) {
     int c = cell_index();
     if (c < S.ncell) ct_update_cell(S, c, dtfac, has_z);
 }
