@@ -154,6 +154,12 @@ def fixtures_mhd() -> None:
     txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3)
     states, _ = run_ref("mhd_sun", txt, 20, dumps=(1, 5))
     save_states("c2_mhd_small.npz", states)
+    # the same disk with the HLLC and HLL solvers (riemann.c:253-290): steps
+    # 1 and 5 (step 0 is the HLLD fixture's)
+    for name, rs in (("c2_mhd_small_hllc.npz", 1), ("c2_mhd_small_hll.npz", 0)):
+        txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, Riemann_Solver=rs)
+        states, _ = run_ref("mhd_sun", txt, 5, dumps=(1,))
+        save_states(name, {s: st for s, st in states.items() if s > 0})
 
 
 def fixtures_mhd_full() -> None:

@@ -3,7 +3,8 @@ CT_MODE 2, periodic z, Keplerian mesh) against the reference CPU build.
 
 The fixture holds the reference's own states (tests/golden/make_fixtures.py
 runs the reference compiled from its sources) at 0, 1, 5 and 20 steps of a
-small version of the benchmark disk started from identical seeded data.  The
+small version of the benchmark disk started from identical seeded data, and
+at 1 and 5 steps of the same disk with the HLLC and HLL solvers.  The
 HIP path must match, per field (helpers.state_errors; magnetic components on
 the field-magnitude scale),
 
@@ -13,9 +14,12 @@ the field-magnitude scale),
 * the face fluxes Phi[5] to the same tolerances on the scale of the largest
   face flux, and the mesh (piph, dphi) in radians,
 
-and keep div B (net outward face flux of every real cell, the sum
-Python/discopy/geom.py:370-520 integrates) at round-off, as the reference
-does (<= 1e-15 of each cell's face-flux scale in the fixture).  The fixture's
+and leave div B (net outward face flux of every real cell, the sum
+Python/discopy/geom.py:370-520 integrates) where the reference leaves it:
+at round-off (<= 1e-15 of each cell's face-flux scale) for HLLD and HLLC,
+and, with HLL at step 5, off by 2.3e-5 in the 12 cells where the
+reference's own flux flips do not close -- the same cells by the same
+amount to 1e-13.  The fixture's
 coarse grid makes the moving mesh flip about 1800 face fluxes per step,
 including chains of interacting flips that cross the start of a track's face
 list, so the order-dependent flip_fluxes replay is exercised every step.
@@ -37,15 +41,22 @@ def _phi_errors(st, ref, real):
     return np.abs(st["Phi"][:, real] - ref["Phi"][:, real]).max(axis=1) / scale
 
 
-def test_mhd_matches_reference():
-    ref = load_fixture("c2_mhd_small.npz")
-    dom = D.Domain(params("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3))
+SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
+           ("c2_mhd_small_hllc.npz", {"Riemann_Solver": 1}),
+           ("c2_mhd_small_hll.npz", {"Riemann_Solver": 0})]
+
+
+@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll"])
+def test_mhd_matches_reference(fixture, over):
+    ref = load_fixture(fixture)
+    ref0 = load_fixture("c2_mhd_small.npz")[0]  # the same seeded start for every solver
+    dom = D.Domain(params("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, **over))
     st0 = dom.state(0)
-    real = ref[0]["real"] == 1
-    assert np.array_equal(st0["piph"], ref[0]["piph"])
-    e0 = state_errors(st0, ref[0], cell_radius(ref[0]["Nr"], ref[0]["Nz"], ref[0]["Np"], ref[0]["r_jph"]), real)
+    real = ref0["real"] == 1
+    assert np.array_equal(st0["piph"], ref0["piph"])
+    e0 = state_errors(st0, ref0, cell_radius(ref0["Nr"], ref0["Nz"], ref0["Np"], ref0["r_jph"]), real)
     assert max(e0["prim"].max(), e0["cons"].max()) < 1e-14, e0
-    assert _phi_errors(st0, ref[0], real).max() < 1e-14
+    assert _phi_errors(st0, ref0, real).max() < 1e-14
     done = 0
     for s in sorted(ref):
         if s == 0:
@@ -68,8 +79,12 @@ def test_mhd_matches_reference():
         assert e["cons"].max() < tol, (s, e["cons"])
         assert ep.max() < tol, (s, ep)
         assert epiph < tol
+        # div B as the reference leaves it: at round-off, except where its
+        # own replay of the flips does not close (the HLL disk at step 5: 12
+        # cells at 2.3e-5 of their face-flux scale, reproduced here)
         div, sc = div_b(st["Phi"], st["piph"], r["Np"], r["Nr"], r["Nz"])
-        assert np.abs(div[real] / sc[real]).max() < 1e-13
+        div_ref, _ = div_b(r["Phi"], r["piph"], r["Np"], r["Nr"], r["Nz"])
+        assert np.abs((div - div_ref)[real] / sc[real]).max() < 1e-13
         ed, et = diag_errors(dom.diagnostics(0), r)
         assert ed.max() < tol and et < 1e-12, (s, ed, et)
     dom.close()
