@@ -475,10 +475,20 @@ DHD double ct_face_update(const DevState& S, const AnnDev& A, int c, int side, d
     return delta + sum;
 }
 
-DHD void ct_update_cell(const DevState& S, int c, double dtfac, int has_z) {
+DHD void ct_update_cell(const DevState& S, int c, double RK, double dtfac, int first, int has_z) {
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const size_t N = S.ncell;
     const double dt = S.dt[A.run] * dtfac;
+    // the face fluxes at the substep's start, every cell: timestep.c:22's copy
+    // in the first substep, adjust_RK_cons's average (misc.c:239-241) in the
+    // second; no kernel reads them between the stage and here
+    _Pragma("unroll") for (int f = 0; f < 5; ++f) {
+        const double p = S.Phi[f * N + c];
+        if (first)
+            st_stream(&S.rkPhi[f * N + c], p);
+        else
+            S.Phi[f * N + c] = (1. - RK) * p + RK * S.rkPhi[f * N + c];
+    }
     double dPhi0 = 0.0;
     if (A.outer >= 0) S.Phi[2 * N + c] += ct_face_update<DIR_R>(S, A, c, 0, dt, has_z, &dPhi0);
     if (A.inner >= 0) S.Phi[1 * N + c] += ct_face_update<DIR_R>(S, A, c, 1, dt, has_z, &dPhi0);

@@ -150,9 +150,9 @@ __global__ void k_dt_final(DevState S, int check) {
 
 // ------------------------------------------------------------------ stages
 template <class Ph>
-__global__ __launch_bounds__(256) void k_stage_a(DevState S, double RK, int first, int search) {
+__global__ __launch_bounds__(256) void k_stage_a(DevState S, int first, int search) {
     int c = cell_index();
-    if (c < S.ncell) stage_a_cell<Ph>(S, c, RK, first, search);
+    if (c < S.ncell) stage_a_cell<Ph>(S, c, first, search);
 }
 
 __global__ __launch_bounds__(256) void k_partners(DevState S) {
@@ -175,14 +175,14 @@ __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::FLUX) void k_flux(DevState 
 // Fused update for hydro: conserved state stays in registers between the
 // gather/source half and the cons2prim/boundary half.
 template <class Ph>
-__global__ __launch_bounds__(256) void k_update(DevState S, double dtfac, int first, int want_dt) {
+__global__ __launch_bounds__(256) void k_update(DevState S, double RK, double dtfac, int first, int want_dt) {
     int c = cell_index();
     int run = -1;
     double dtc = 1.0e300;
     if (c < S.ncell) {
         double cons[Ph::NQ];
         run = S.ann[S.cell_ann[c]].run;
-        update_a_cell<Ph>(S, c, dtfac, first, cons);
+        update_a_cell<Ph>(S, c, RK, dtfac, first, cons);
         dtc = update_b_cell<Ph>(S, c, first, want_dt, cons);
     }
     if (want_dt) dt_reduce(S, run, dtc);
@@ -331,17 +331,19 @@ __global__ __launch_bounds__(256) void k_source(DevState S, double dtfac) {
 // own: fused, the two need 127 VGPRs (4 waves per SIMD) and run 19% slower
 // than apart.  The Stokes update is held to 6 waves per SIMD (80 VGPRs; 5%
 // faster than its natural 83 at 5 waves).
-__global__ __launch_bounds__(256) DISCO_OCC(6) void k_ct_update(DevState S, double dtfac, int has_z) {
+__global__ __launch_bounds__(256) DISCO_OCC(6) void k_ct_update(DevState S, double RK, double dtfac, int first,
+                                                               int has_z) {
     int c = cell_index();
-    if (c < S.ncell) ct_update_cell(S, c, dtfac, has_z);
+    if (c < S.ncell) ct_update_cell(S, c, RK, dtfac, first, has_z);
 }
 
 template <class Ph>
-__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_update_a(DevState S, double dtfac, int first) {
+__global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_update_a(DevState S, double RK, double dtfac,
+                                                                        int first) {
     int c = cell_index();
     if (c < S.ncell) {
         double cons[Ph::NQ];
-        update_a_cell<Ph>(S, c, dtfac, first, cons);
+        update_a_cell<Ph>(S, c, RK, dtfac, first, cons);
     }
 }
 
@@ -472,7 +474,7 @@ struct Launch {
         auto G = [&](int k) { return dim3(nblk(S.ncell, B(k))); };
         // partners: the first substep's are the last second substep's (the
         // mesh has not moved since)
-        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, RK, first, !first);
+        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, !first);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
         DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         if (Ph::MHD && sh.ct) {
@@ -481,8 +483,9 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             if (Ph::SOURCE_PASS)
                 DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
-            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, sh.has_z);
-            DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac, first);
+            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first,
+                         sh.has_z);
+            DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first);
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
@@ -494,7 +497,7 @@ struct Launch {
                 DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, first,
                              want_dt);
         } else {
-            DISCO_LAUNCH(hook, "k_update", k_update<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, dtfac, first,
+            DISCO_LAUNCH(hook, "k_update", k_update<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK, dtfac, first,
                          want_dt);
         }
         if (sh.zcopy) DISCO_LAUNCH(hook, "k_zcopy", k_zcopy<Ph>, G(KC_COPY), dim3(B(KC_COPY)), st, S, first);

@@ -357,11 +357,13 @@ DHD void partners_cell(const DevState& S, int c) {
 }
 
 // ------------------------------------------------------------------ stage A
-// set_wcell (misc.c:101-168, first substep), adjust_RK_cons (misc.c:226-244,
-// with timestep.c:18-24's copy folded into the first substep), plm_phi
-// (plm.c:15-51) and, when `search`, the partner search of the overlap faces.
+// set_wcell (misc.c:101-168, first substep), plm_phi (plm.c:15-51) and, when
+// `search`, the partner search of the overlap faces.  adjust_RK_cons
+// (misc.c:226-244) and timestep.c:18-24's copy run where the state is next
+// written: in update_a_cell for the conserved state, in ct_update_cell for
+// the face fluxes (nothing reads either in between).
 template <class Ph>
-DHD void stage_a_cell(const DevState& S, int c, double RK, int first, int search) {
+DHD void stage_a_cell(const DevState& S, int c, int first, int search) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const PhysParams& P = S.runs[A.run].ph;
@@ -382,23 +384,6 @@ DHD void stage_a_cell(const DevState& S, int c, double RK, int first, int search
             w = mesh_om(P, x);
         }
         S.wiph[c] = w;
-    }
-    if (A.flags & ANN_REAL) {
-        if (first) {
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.rkcons[q * N + c], S.cons[q * N + c]);
-        } else {
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q)
-                S.cons[q * N + c] = (1. - RK) * S.cons[q * N + c] + RK * S.rkcons[q * N + c];
-        }
-    }
-    if (Ph::MHD && S.Phi) {
-        // face fluxes of every cell (misc.c:239-241; timestep.c:22 at the first substep)
-        _Pragma("unroll") for (int f = 0; f < 5; ++f) {
-            if (first)
-                st_stream(&S.rkPhi[f * N + c], S.Phi[f * N + c]);
-            else
-                S.Phi[f * N + c] = (1. - RK) * S.Phi[f * N + c] + RK * S.rkPhi[f * N + c];
-        }
     }
     double dpL = S.dphi[cm];
     double dpC = S.dphi[c];
@@ -708,12 +693,14 @@ DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double*
     }
 }
 
-// First half of the update: conservative update from the gathered face
-// increments (riemann.c:314-317), add_source (misc.c:550-593) and, in the
-// first substep, move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns
-// the updated conserved state of a real cell in cons[] (and stores it).
+// First half of the update: the substep's start state (timestep.c:18-24's
+// copy in the first substep, adjust_RK_cons's average in the second,
+// misc.c:226-244), the conservative update from the gathered face increments
+// (riemann.c:314-317), add_source (misc.c:550-593) and, in the first substep,
+// move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns the updated
+// conserved state of a real cell in cons[] (and stores it).
 template <class Ph>
-DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double* cons) {
+DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int first, double* cons) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
     const RunDev& Rn = S.runs[A.run];
@@ -730,7 +717,13 @@ DHD void update_a_cell(const DevState& S, int c, double dtfac, int first, double
     if (A.flags & ANN_REAL) {
         double prim[NQ];
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-            cons[q] = S.cons[q * N + c];
+            const double u = S.cons[q * N + c];
+            if (first) {
+                st_stream(&S.rkcons[q * N + c], u);
+                cons[q] = u;
+            } else {
+                cons[q] = (1. - RK) * u + RK * S.rkcons[q * N + c];
+            }
             prim[q] = S.prim[q * N + c];
         }
         // phi faces i-1 (this cell is R) and i (this cell is L)
