@@ -70,6 +70,9 @@ struct disco_gpu {
     unsigned long long* dtbits = nullptr;
     double *dt = nullptr, *planet_phi = nullptr, *t = nullptr;
     int* final_step = nullptr;  // per run: check_dt's final flag
+    std::vector<int> hbcjobs;   // zero-gradient ghost-fill jobs (StepShape::bc_off)
+    int* bcjobs = nullptr;
+    size_t bcjobs_cap = 0;
     double *diag = nullptr, *t_avg = nullptr;
     double* src = nullptr;  // GRMHD source increments
     int parity = 0;
@@ -114,6 +117,7 @@ static DevState make_state(disco_gpu* h, int substep) {
         S.rot_n = h->rot[1 - p];
     }
     S.wiph = h->wiph;
+    S.bcjobs = h->bcjobs;
     S.lbo = h->lbo;
     S.ubi = h->ubi;
     S.lbu = h->lbu;
@@ -186,7 +190,7 @@ static int dalloc(T** p, size_t n) {
 static void build_tables(disco_gpu* h) {
     const int learned = h->shape.c2p_learned;
     build_annulus_tables(h->runs, &h->hann, &h->hruns, &h->run_cell0, &h->run_ann0, &h->shape, &h->cell_perm,
-                         h->ktile);
+                         h->ktile, &h->hbcjobs);
     h->shape.c2p_learned = learned;
     for (int k = 0; k < KC_N; ++k) h->shape.blk[k] = h->blk[k];
     h->ncell = 0;
@@ -279,6 +283,15 @@ static int upload_tables(disco_gpu* h) {
     HIPCHK(hipMemcpy(h->ann, h->hann.data(), h->nann * sizeof(AnnDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->cell_ann, cell_ann.data(), h->ncell * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->druns, h->hruns.data(), h->nruns * sizeof(RunDev), hipMemcpyHostToDevice));
+    if (!h->hbcjobs.empty()) {
+        if (h->bcjobs_cap < h->hbcjobs.size()) {
+            if (h->bcjobs) HIPCHK(hipFree(h->bcjobs));
+            h->bcjobs = nullptr;
+            HIPCHK(hipMalloc((void**)&h->bcjobs, h->hbcjobs.size() * sizeof(int)));
+            h->bcjobs_cap = h->hbcjobs.size();
+        }
+        HIPCHK(hipMemcpy(h->bcjobs, h->hbcjobs.data(), h->hbcjobs.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -403,7 +416,7 @@ static int kernel_class(const std::string& n) {
         {"k_stage_a", KC_STAGE}, {"k_plm_trans", KC_PLM}, {"k_flux", KC_FLUX},     {"k_emf_sum", KC_EMF},
         {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update", KC_CT},
         {"k_update_a", KC_CT},   {"k_source", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
-        {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}};
+        {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}, {"k_bc_copy", KC_COPY}};
     for (auto& e : m)
         if (n == e.first) return e.second;
     return -1;
@@ -480,7 +493,7 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
                     h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats, h->diag,
-                    h->t_avg, h->src,      h->final_step};
+                    h->t_avg, h->src,      h->final_step, h->bcjobs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -21,6 +21,11 @@ constexpr int DISCO_NDIAG = 8;
 // report.c's gas sums per annulus: mass, the torques of planets 1 and 0, and
 // both torques outside (n + 1) x Sink_Par3 of planet 1 for n = 0..4
 constexpr int DISCO_NREPORT = 13;
+// levels of the zero-gradient ghost fill: NUM_G r-layers then NUM_G z-layers
+constexpr int DISCO_MAX_BC_LEVELS = 8;
+// one ghost-fill job: {ghost annulus, source annulus, velocity field cut by
+// the outflow diode (-1: none), sign of the cut velocity}
+constexpr int DISCO_BC_JOB = 4;
 
 enum AnnFlags {
     ANN_REAL = 1,        // real annulus: conserved update + cons2prim
@@ -91,6 +96,7 @@ struct DevState {
     int* ubd;            // per cell: partner in the down annulus
     int* pdg;            // [4][ncell] MHD with CT: partners in the diagonal annuli (order of AnnDev::diag)
     const int* rot;      // per annulus: array index of the smallest front (mesh at the start of the substep)
+    const int* bcjobs;   // [njobs][DISCO_BC_JOB] zero-gradient ghost fills (StepShape::bc_off)
     int* rot_n;          // per annulus: the same after move_cells (written by the first substep)
     double* fphi;        // per cell: increment of its phi face (i | i+1)
     double* fout;        // per cell: increment of its owned face in the outer track
@@ -138,6 +144,11 @@ struct StepShape {
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
+    // zero-gradient ghost fills (zerograd.c, outflow3d.c): DevState::bcjobs
+    // holds bc_off[bc_nlev] jobs, level l in [bc_off[l], bc_off[l+1]); the
+    // levels run in order (each reads the annuli the one before wrote)
+    int bc_nlev;
+    int bc_off[DISCO_MAX_BC_LEVELS + 1];
 };
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).

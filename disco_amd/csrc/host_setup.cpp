@@ -1,6 +1,7 @@
 // Host-side set-up of one run.  See host_setup.h.
 #include "host_setup.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -143,6 +144,8 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     p->boundary = DISCO_BC_FIXED;
     if (!read_var(text, "Boundary", 2, s)) {
         if (!std::strcmp(s, "fixed3d")) p->boundary = DISCO_BC_FIXED3D;
+        else if (!std::strcmp(s, "zerograd")) p->boundary = DISCO_BC_ZEROGRAD;
+        else if (!std::strcmp(s, "outflow3d")) p->boundary = DISCO_BC_OUTFLOW3D;
         else if (std::strcmp(s, "fixed")) { ++e; missing += " Boundary=" + std::string(s); }
     }
     p->planets = DISCO_PLANETS_SUN;
@@ -230,7 +233,11 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.R_Periodic) m += " R_Periodic";
     // a serial run fills ghost z-layers only through periodic copies or the
     // fixed3d boundary (Boundary/fixed.c leaves them unset)
-    if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary != DISCO_BC_FIXED3D) m += " Num_Z>1 needs Z_Periodic or fixed3d";
+    if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED)
+        m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
+    // the copied ghost annuli would need their face fluxes carried as well
+    if ((p.boundary == DISCO_BC_ZEROGRAD || p.boundary == DISCO_BC_OUTFLOW3D) && p.hydro != DISCO_HYDRO_EULER)
+        m += " zerograd/outflow3d boundaries with mhd/grmhd";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
@@ -405,8 +412,92 @@ static void cell_x(const HostRun& R, int j, int k, int c, double* x, double* xp,
     x[2] = get_centroid(R.zk(k), R.zk(k - 1), 2);
 }
 
+// boundary_functions.c:90-159 set_cells_copy_distant: every cell of annulus
+// jk takes the overlap-weighted mean of the prims of annulus jk1, the
+// overlaps summed in phi order from the cell that holds its back edge.
+void copy_distant(HostRun& R, int jk, int jk1) {
+    const int n = R.Np[jk], n1 = R.Np[jk1], o = R.off[jk], o1 = R.off[jk1];
+    const double Pmax = R.ph.phi_max;
+    const size_t N = R.ncell;
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < R.nq; ++q) R.prim[q * N + o + i] = 0.0;
+    int i1 = 0;
+    const double phi0 = R.piph[o] - R.dphi[o];
+    for (i1 = 0; i1 < n1; ++i1) {
+        double dphip = get_signed_dp(R.piph[o1 + i1], phi0, Pmax);
+        double dphim = dphip - R.dphi[o1 + i1];
+        if (dphip > 0 && dphim <= 0.0) break;
+    }
+    if (i1 >= n1) i1 = 0;
+    for (int i = 0; i < n; ++i) {
+        const double phip = R.piph[o + i];
+        const double phim = R.piph[o + i] - R.dphi[o + i];
+        double phip1, phim1;
+        do {
+            phip1 = R.piph[o1 + i1];
+            phim1 = R.piph[o1 + i1] - R.dphi[o1 + i1];
+            while (phip1 < phim) {
+                phip1 += Pmax;
+                phim1 += Pmax;
+            }
+            while (phim1 > phip) {
+                phip1 -= Pmax;
+                phim1 -= Pmax;
+            }
+            const double phi1 = phip < phip1 ? phip : phip1;
+            const double phi2 = phim > phim1 ? phim : phim1;
+            const double dphi = phi1 - phi2;
+            for (int q = 0; q < R.nq; ++q) R.prim[q * N + o + i] += R.prim[q * N + o1 + i1] * dphi;
+            i1 = i1 < n1 - 1 ? i1 + 1 : 0;
+        } while (phip1 < phip);
+        i1 = i1 > 0 ? i1 - 1 : n1 - 1;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < R.nq; ++q) R.prim[q * N + o + i] /= R.dphi[o + i];
+}
+
+// boundary_functions.c:269-438 (zero gradient; `diode` cuts the velocity
+// into the domain: v_r > 0 inside, < 0 outside, v_z likewise), in the order
+// Boundary/zerograd.c and outflow3d.c call them: r for every z-layer, then z
+// for every r-column (non-periodic z only).
+static void host_zerograd(HostRun& R, bool diode) {
+    const size_t N = R.ncell;
+    auto cut = [&](int jk, int q, int sgn) {
+        for (int i = 0; i < R.Np[jk]; ++i) {
+            double& v = R.prim[q * N + R.off[jk] + i];
+            if (sgn > 0 ? v > 0 : v < 0) v = 0.0;
+        }
+    };
+    for (int k = 0; k < R.Nz; ++k) {
+        for (int j = R.NgRa - 1; j >= 0; --j) {
+            copy_distant(R, j + R.Nr * k, j + 1 + R.Nr * k);
+            if (diode) cut(j + R.Nr * k, URR, +1);
+        }
+        for (int j = R.Nr - R.NgRb; j < R.Nr; ++j) {
+            copy_distant(R, j + R.Nr * k, j - 1 + R.Nr * k);
+            if (diode) cut(j + R.Nr * k, URR, -1);
+        }
+    }
+    if (R.Nz > 1 && !R.par.Z_Periodic) {
+        for (int j = 0; j < R.Nr; ++j) {
+            for (int k = R.NgZa - 1; k >= 0; --k) {
+                copy_distant(R, j + R.Nr * k, j + R.Nr * (k + 1));
+                if (diode) cut(j + R.Nr * k, UZZ, +1);
+            }
+            for (int k = R.Nz - R.NgZb; k < R.Nz; ++k) {
+                copy_distant(R, j + R.Nr * k, j + R.Nr * (k - 1));
+                if (diode) cut(j + R.Nr * k, UZZ, -1);
+            }
+        }
+    }
+}
+
 void host_boundary(HostRun* Rp) {
     HostRun& R = *Rp;
+    if (R.par.boundary == DISCO_BC_ZEROGRAD || R.par.boundary == DISCO_BC_OUTFLOW3D) {
+        host_zerograd(R, R.par.boundary == DISCO_BC_OUTFLOW3D);
+        return;
+    }
     double prim[8];
     auto fill = [&](int j, int k) {
         int jk = j + R.Nr * k;
@@ -795,7 +886,8 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
 // geometry, and the flags that mark which sweeps touch each annulus.
 void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
                           std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape,
-                          std::vector<std::vector<int>>* cell_perm, int ktile) {
+                          std::vector<std::vector<int>>* cell_perm, int ktile, std::vector<int>* bc_jobs) {
+    std::vector<std::vector<int>> lev(DISCO_MAX_BC_LEVELS);
     hann->clear();
     if (cell_perm) cell_perm->clear();
     hruns->clear();
@@ -900,7 +992,8 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                 bool jin = j >= zjmin && j < zjmax;
                 if (R.Nz > 1 && jin && k >= zkmin && k < zkmax) f |= ANN_UPFLUX;
                 if (R.Nz > 1 && jin && k - 1 >= zkmin && k - 1 < zkmax) f |= ANN_DNFLUX;
-                bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic;
+                const bool copied = R.par.boundary == DISCO_BC_ZEROGRAD || R.par.boundary == DISCO_BC_OUTFLOW3D;
+                bool rghost = (j < R.NgRa || j >= R.Nr - R.NgRb) && !R.par.R_Periodic && !copied;
                 bool zghost = (k < R.NgZa || k >= R.Nz - R.NgZb) && !R.par.Z_Periodic &&
                               R.par.boundary == DISCO_BC_FIXED3D;
                 if (!(f & ANN_REAL) && (rghost || zghost)) f |= ANN_GHOST_FILL;
@@ -941,8 +1034,44 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                 runann[pos[jk]] = A;
             }
         hann->insert(hann->end(), runann.begin(), runann.end());
+        // zero-gradient ghost fills in the order of boundary_functions.c:
+        // r-layer l (from the real region out) of every z-layer, then z-layer
+        // l of every r-column; each layer copies the one inside it
+        if (R.par.boundary == DISCO_BC_ZEROGRAD || R.par.boundary == DISCO_BC_OUTFLOW3D) {
+            const bool diode = R.par.boundary == DISCO_BC_OUTFLOW3D;
+            auto job = [&](int l, int jk, int jk1, int q, int sgn) {
+                lev[l].insert(lev[l].end(), {ann0 + pos[jk], ann0 + pos[jk1], diode ? q : -1, sgn});
+            };
+            const int ngr = std::max(R.NgRa, R.NgRb);
+            for (int l = 0; l < ngr; ++l)
+                for (int k = 0; k < R.Nz; ++k) {
+                    int j = R.NgRa - 1 - l;
+                    if (j >= 0) job(l, j + R.Nr * k, j + 1 + R.Nr * k, URR, +1);
+                    j = R.Nr - R.NgRb + l;
+                    if (l < R.NgRb) job(l, j + R.Nr * k, j - 1 + R.Nr * k, URR, -1);
+                }
+            if (R.Nz > 1 && !R.par.Z_Periodic) {
+                const int ngz = std::max(R.NgZa, R.NgZb);
+                for (int l = 0; l < ngz; ++l)
+                    for (int j = 0; j < R.Nr; ++j) {
+                        int k = R.NgZa - 1 - l;
+                        if (k >= 0) job(ngr + l, j + R.Nr * k, j + R.Nr * (k + 1), UZZ, +1);
+                        k = R.Nz - R.NgZb + l;
+                        if (l < R.NgZb) job(ngr + l, j + R.Nr * k, j + R.Nr * (k - 1), UZZ, -1);
+                    }
+            }
+        }
         cell0 += R.ncell;
         ann0 += R.Nr * R.Nz;
+    }
+    shape->bc_nlev = 0;
+    shape->bc_off[0] = 0;
+    if (bc_jobs) bc_jobs->clear();
+    for (int l = 0; l < DISCO_MAX_BC_LEVELS; ++l) {
+        if (lev[l].empty()) break;
+        if (bc_jobs) bc_jobs->insert(bc_jobs->end(), lev[l].begin(), lev[l].end());
+        shape->bc_off[l + 1] = shape->bc_off[l] + (int)lev[l].size() / DISCO_BC_JOB;
+        shape->bc_nlev = l + 1;
     }
     shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
     shape->has_z = 0;

@@ -56,13 +56,17 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic);
 
 // boundary_trans for the fixed / fixed3d modules on the host (ghost prims).
 void host_boundary(HostRun* run);
+// set_cells_copy_distant (boundary_functions.c:90-159): annulus jk's prims
+// from annulus jk1's, overlap-weighted, in the reference's loop order.
+void copy_distant(HostRun& R, int jk, int jk1);
 
 // Annulus/run tables of a batch and the launch shape of its substeps.  The
 // device stores annuli in tiles of `ktile` z-layers; cell_perm[r][c] is the
 // device position (within run r) of the run's cell c in the reference's
-// (jk, i) order.
+// (jk, i) order; bc_jobs (if given) receives the zero-gradient ghost-fill
+// jobs of shape->bc_off.
 void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>* hann, std::vector<RunDev>* hruns,
                           std::vector<int>* run_cell0, std::vector<int>* run_ann0, StepShape* shape,
-                          std::vector<std::vector<int>>* cell_perm, int ktile);
+                          std::vector<std::vector<int>>* cell_perm, int ktile, std::vector<int>* bc_jobs = nullptr);
 
 }  // namespace disco

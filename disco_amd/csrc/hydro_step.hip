@@ -434,6 +434,14 @@ __global__ __launch_bounds__(256) void k_report(DevState S, int a0, double* out)
     }
 }
 
+// Zero-gradient ghost fill, one block per job of one level (bc_copy_cell).
+template <class Ph>
+__global__ __launch_bounds__(256) void k_bc_copy(DevState S, int j0) {
+    const int* J = S.bcjobs + (size_t)(j0 + blockIdx.x) * DISCO_BC_JOB;
+    const AnnDev& G = S.ann[J[0]];
+    for (int i = threadIdx.x; i < G.np; i += blockDim.x) bc_copy_cell<Ph>(S, G, G.off + i, J[1], J[2], J[3]);
+}
+
 __global__ void k_diag_time(DevState S) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < S.nruns) S.t_avg[r] += S.dt[r];
@@ -500,6 +508,10 @@ struct Launch {
             DISCO_LAUNCH(hook, "k_update", k_update<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK, dtfac, first,
                          want_dt);
         }
+        // boundary_trans (onestep.c:166-173): zero-gradient ghosts level by level, then the periodic z copy
+        for (int l = 0; l < sh.bc_nlev; ++l)
+            DISCO_LAUNCH(hook, "k_bc_copy", k_bc_copy<Ph>, dim3(sh.bc_off[l + 1] - sh.bc_off[l]), dim3(B(KC_COPY)), st,
+                         S, sh.bc_off[l]);
         if (sh.zcopy) DISCO_LAUNCH(hook, "k_zcopy", k_zcopy<Ph>, G(KC_COPY), dim3(B(KC_COPY)), st, S, first);
         if (first) DISCO_LAUNCH(hook, "k_planets", k_planets, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
     }

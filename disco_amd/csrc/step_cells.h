@@ -876,6 +876,69 @@ DHD void update_b_ghost(const DevState& S, int c) {
     _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) st_stream(&S.prim[q * N + c], p8[q]);
 }
 
+// set_cells_copy_distant (boundary_functions.c:90-159) for cell c of a ghost
+// annulus from annulus `src` (the layer inside it), on the mesh after the
+// substep's move: the overlap-weighted mean of src's prims, the overlaps
+// summed in phi order from the src cell where the reference's loop starts
+// for this cell (array index 0: the one holding its back edge; later cells:
+// the first whose front is not behind the front of the cell before), then
+// divided by the cell's dphi; the outflow diode sets the velocity field dq
+// to zero where its sign is dsgn (boundary_functions.c:269-438).  Products
+// are rounded before the adds, as the reference's host build does.
+template <class Ph>
+DHD void bc_copy_cell(const DevState& S, const AnnDev& G, int c, int src, int dq, int dsgn) {
+    constexpr int NQ = Ph::NQ;
+    const AnnDev& B = S.ann[src];
+    const size_t N = S.ncell;
+    const double Pmax = S.runs[B.run].ph.phi_max;
+    const double phip = S.piph_n[c];
+    const double phim = phip - S.dphi_n[c];
+    const int n1 = B.np, rot = S.rot_n[src];
+    const bool head = c == G.off;
+    double key = head ? phim : S.piph_n[c - 1];
+    while (key < 0.0) key += Pmax;
+    while (key >= Pmax) key -= Pmax;
+    int lo = 0, hi = n1;  // first rank (from the rotation point) whose front is > key (head) or >= key
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        int i = rot + mid;
+        if (i >= n1) i -= n1;
+        const double f = S.piph_n[B.off + i];
+        if (head ? f > key : f >= key)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    int i1 = rot + (lo == n1 ? 0 : lo);
+    if (i1 >= n1) i1 -= n1;
+    double sum[NQ];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) sum[q] = 0.0;
+    double phip1;
+    do {
+        phip1 = S.piph_n[B.off + i1];
+        double phim1 = phip1 - S.dphi_n[B.off + i1];
+        while (phip1 < phim) {
+            phip1 += Pmax;
+            phim1 += Pmax;
+        }
+        while (phim1 > phip) {
+            phip1 -= Pmax;
+            phim1 -= Pmax;
+        }
+        const double phi1 = phip < phip1 ? phip : phip1;
+        const double phi2 = phim > phim1 ? phim : phim1;
+        const double dphi = phi1 - phi2;
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) sum[q] = add_mul_rn(sum[q], S.prim[q * N + B.off + i1], dphi);
+        i1 = i1 < n1 - 1 ? i1 + 1 : 0;
+    } while (phip1 < phip);
+    const double dph = S.dphi_n[c];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+        double v = sum[q] / dph;
+        if (q == dq && (dsgn > 0 ? v > 0 : v < 0)) v = 0.0;
+        st_stream(&S.prim[q * N + c], v);
+    }
+}
+
 // The whole second half with the module's own cons2prim.  cons[] holds the
 // real cell's conserved state from update_a_cell; *c2p_it and *c2p_ok (if
 // given) receive the Newton iteration count of a real cell's cons2prim and

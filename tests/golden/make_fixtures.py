@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report | --bc]   (only the named set)
 """
 from __future__ import annotations
 
@@ -139,6 +139,22 @@ def grmhd_leaf_vectors(n: int = 512) -> None:
         out = np.fromfile(fout, np.float64).reshape(n, 8 + 8 + 23)
     np.savez_compressed(os.path.join(HERE, "grmhd_leaf_a0.npz"), gamma=P["gamma"], M=P["M"], x=x, prim=prim,
                         guess=guess, cons=out[:, :8], prim_out=out[:, 8:16], metric=out[:, 16:])
+
+
+BC_CASES = [  # (fixture, overrides of c1_kepler2d.par, reference variant, steps, dumps)
+    ("c1_zerograd.npz", dict(Num_R=16, Boundary="zerograd"), "euler_sun_zerograd", 10, (1,)),
+    ("c1_outflow.npz", dict(Num_R=16, Boundary="outflow3d"), "euler_sun_outflow3d", 10, (1,)),
+    ("c2e_outflow.npz", dict(Num_R=12, Num_Z=6, Boundary="outflow3d"), "euler_sun_outflow3d", 5, (1,)),
+]
+
+
+def fixtures_boundaries() -> None:
+    """The small disks with zero-gradient (Boundary/zerograd.c) and outflow
+    (Boundary/outflow3d.c: zero gradient, inflow velocity cut) ghosts, in 2D
+    and in 3D with non-periodic z."""
+    for name, over, variant, n, dumps in BC_CASES:
+        states, _ = run_ref(variant, par_with("c1_kepler2d.par", **over), n, dumps=dumps)
+        save_states(name, states)
 
 
 def fixtures_3d() -> None:
@@ -282,6 +298,10 @@ def main() -> None:
         fixtures_report()
         print("fixtures written")
         return
+    if "--bc" in sys.argv:
+        fixtures_boundaries()
+        print("fixtures written")
+        return
     if "--t-end" in sys.argv:
         fixtures_t_end()
         print("fixtures written")
@@ -300,6 +320,7 @@ def main() -> None:
     fixtures_2d()
     fixtures_t_end()
     fixtures_report()
+    fixtures_boundaries()
     # full-size 2D disk: conserved totals only
     txt = par_with("c1_kepler2d.par")
     states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))

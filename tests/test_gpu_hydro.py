@@ -28,13 +28,17 @@ from helpers import cell_radius, diag_errors, params, phi_err, state_errors
 
 pytestmark = pytest.mark.gpu
 
-TOL = {1: 1e-12, 10: 1e-10, 50: 1e-9, 100: 1e-9}
+TOL = {1: 1e-12, 5: 1e-10, 10: 1e-10, 50: 1e-9, 100: 1e-9}
 
 CASES = [
     ("c1_small.npz", "c1_kepler2d.par", dict(Num_R=16)),
     ("c1_small_hll_mm4.npz", "c1_kepler2d.par", dict(Num_R=16, Riemann_Solver=0, Mesh_Motion=4)),
     ("c3_small.npz", "c3_binary_disk.par", dict(Num_R=24)),
     ("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1)),
+    # zero-gradient and outflow ghosts (Boundary/zerograd.c, outflow3d.c), 3D with non-periodic z
+    ("c1_zerograd.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="zerograd")),
+    ("c1_outflow.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="outflow3d")),
+    ("c2e_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d")),
 ]
 
 
