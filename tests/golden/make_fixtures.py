@@ -151,10 +151,13 @@ BC_CASES = [  # (fixture, overrides of c1_kepler2d.par, reference variant, steps
 def fixtures_boundaries() -> None:
     """The small disks with zero-gradient (Boundary/zerograd.c) and outflow
     (Boundary/outflow3d.c: zero gradient, inflow velocity cut) ghosts, in 2D
-    and in 3D with non-periodic z."""
+    and in 3D with non-periodic z, and the small MHD disk with outflow z."""
     for name, over, variant, n, dumps in BC_CASES:
         states, _ = run_ref(variant, par_with("c1_kepler2d.par", **over), n, dumps=dumps)
         save_states(name, states)
+    txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d")
+    states, _ = run_ref("mhd_sun_outflow3d", txt, 5, dumps=(1,))
+    save_states("c2_mhd_outflow.npz", states)
 
 
 def fixtures_3d() -> None:

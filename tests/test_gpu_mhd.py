@@ -43,13 +43,16 @@ def _phi_errors(st, ref, real):
 
 SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            ("c2_mhd_small_hllc.npz", {"Riemann_Solver": 1}),
-           ("c2_mhd_small_hll.npz", {"Riemann_Solver": 0})]
+           ("c2_mhd_small_hll.npz", {"Riemann_Solver": 0}),
+           # outflow ghosts in z (Boundary/outflow3d.c) instead of the periodic copy
+           ("c2_mhd_outflow.npz", {"Z_Periodic": 0, "Boundary": "outflow3d"})]
 
 
-@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll"])
+@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "outflow"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
-    ref0 = load_fixture("c2_mhd_small.npz")[0]  # the same seeded start for every solver
+    # the same seeded start for every solver
+    ref0 = ref[0] if 0 in ref else load_fixture("c2_mhd_small.npz")[0]
     dom = D.Domain(params("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, **over))
     st0 = dom.state(0)
     real = ref0["real"] == 1

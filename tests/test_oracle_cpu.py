@@ -184,7 +184,9 @@ def test_track_faces_equal_brute_force(seed):
 SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1)),
                        ("c1_zerograd.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="zerograd")),
                        ("c1_outflow.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="outflow3d")),
-                       ("c2e_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d"))]
+                       ("c2e_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d")),
+                       ("c2_mhd_outflow.npz", "c2_mhd3d.par",
+                        dict(Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d"))]
 
 
 @pytest.mark.parametrize("fixture,base,over", SETUP_CASES, ids=[c[0] for c in SETUP_CASES])
