@@ -235,8 +235,6 @@ static int unsupported(const disco_params& p, std::string* err) {
     // fixed3d boundary (Boundary/fixed.c leaves them unset)
     if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED)
         m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
-    if ((p.boundary == DISCO_BC_ZEROGRAD || p.boundary == DISCO_BC_OUTFLOW3D) && p.hydro == DISCO_HYDRO_GRMHD)
-        m += " zerograd/outflow3d boundaries with grmhd";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";

@@ -158,6 +158,9 @@ def fixtures_boundaries() -> None:
     txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d")
     states, _ = run_ref("mhd_sun_outflow3d", txt, 5, dumps=(1,))
     save_states("c2_mhd_outflow.npz", states)
+    txt = par_with("c5_grmhd_torus.par", Num_R=16, Num_Z=12, Metric_Par3=0.0, Boundary="outflow3d")
+    states, _ = run_ref("grmhd_torus_outflow3d", txt, 5, dumps=(1,))
+    save_states("c5_grmhd_outflow.npz", states)
 
 
 def fixtures_3d() -> None:

@@ -40,9 +40,12 @@ def _phi_errors(st, ref, real):
     return np.abs(st["Phi"][:, real] - ref["Phi"][:, real]).max(axis=1) / scale
 
 
-def test_grmhd_matches_reference():
-    ref = load_fixture("c5_grmhd_small.npz")
-    dom = D.Domain(params("c5_grmhd_torus.par", **SMALL))
+@pytest.mark.parametrize("fixture,over", [("c5_grmhd_small.npz", {}),
+                                           ("c5_grmhd_outflow.npz", {"Boundary": "outflow3d"})],
+                         ids=["fixed3d", "outflow3d"])
+def test_grmhd_matches_reference(fixture, over):
+    ref = load_fixture(fixture)
+    dom = D.Domain(params("c5_grmhd_torus.par", **SMALL, **over))
     st0 = dom.state(0)
     real = ref[0]["real"] == 1
     assert np.array_equal(st0["piph"], ref[0]["piph"])

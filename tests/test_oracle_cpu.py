@@ -186,7 +186,9 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                        ("c1_outflow.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="outflow3d")),
                        ("c2e_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d")),
                        ("c2_mhd_outflow.npz", "c2_mhd3d.par",
-                        dict(Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d"))]
+                        dict(Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d")),
+                       ("c5_grmhd_outflow.npz", "c5_grmhd_torus.par",
+                        dict(Num_R=16, Num_Z=12, Metric_Par3=0.0, Boundary="outflow3d"))]
 
 
 @pytest.mark.parametrize("fixture,base,over", SETUP_CASES, ids=[c[0] for c in SETUP_CASES])
