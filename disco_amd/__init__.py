@@ -74,6 +74,7 @@ class Params(C.Structure):
         ("sinkPar4", C.c_double), ("sinkPar5", C.c_double),
         ("coolType", C.c_int), ("nozzleType", C.c_int), ("dampInnerType", C.c_int),
         ("dampOuterType", C.c_int), ("dampUpperType", C.c_int), ("dampLowerType", C.c_int),
+        ("coolPar1", C.c_double), ("coolPar2", C.c_double), ("coolPar3", C.c_double), ("coolPar4", C.c_double),
     ]
 
     def copy(self) -> "Params":

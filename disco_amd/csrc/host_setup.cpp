@@ -128,6 +128,10 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     D("Sink_Par4", &p->sinkPar4);
     D("Sink_Par5", &p->sinkPar5);
     I("Cool_Type", &p->coolType);
+    D("Cool_Par1", &p->coolPar1);
+    D("Cool_Par2", &p->coolPar2);
+    D("Cool_Par3", &p->coolPar3);
+    D("Cool_Par4", &p->coolPar4);
     I("Nozzle_Type", &p->nozzleType);
     I("DampInner_Type", &p->dampInnerType);
     I("DampOuter_Type", &p->dampOuterType);
@@ -205,6 +209,8 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->gr_M = par.metricPar2;
     ph->gr_a = par.metricPar3;
     ph->report_soft = par.sinkPar3;
+    ph->cool_type = par.coolType;
+    ph->cool_beta = par.coolPar1;
     disco_ic_from_params(&par, ic);
 }
 
@@ -243,7 +249,10 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.visc_flag) m += " Use_Viscosity";
     // add_source's sink.c terms (misc.c:579-591) are not built
     if (p.sinkType) m += " Sink_Type";
-    if (p.coolType) m += " Cool_Type";
+    // sink.c:274-322: beta cooling (COOL_BETA, COOL_BETA_RELAX) of a Newtonian gas
+    if (p.coolType < 0 || p.coolType > 2) m += " Cool_Type";
+    if (p.coolType && p.hydro == DISCO_HYDRO_GRMHD) m += " Cool_Type with grmhd";
+    if (p.coolType && p.coolPar1 == 0.0) m += " Cool_Type needs Cool_Par1 (beta) != 0";
     if (p.nozzleType) m += " Nozzle_Type";
     if (p.dampInnerType || p.dampOuterType || p.dampUpperType || p.dampLowerType) m += " Damp*_Type";
     if (p.RotFrame) m += " RotFrame";

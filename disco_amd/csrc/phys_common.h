@@ -36,6 +36,8 @@ struct PhysParams {
     int num_c, num_n;
     double gr_M, gr_a;  // Kerr-Schild mass and spin (Metric_Par2, Metric_Par3)
     double report_soft; // Sink_Par3: report.c's torque-cut length
+    int cool_type;      // Cool_Type: 0 none, 1 beta cooling above the target energy, 2 toward it
+    double cool_beta;   // Cool_Par1: the cooling time in units of 1/omega_tot
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face

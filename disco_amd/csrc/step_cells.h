@@ -693,6 +693,38 @@ DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double*
     }
 }
 
+// sink.c:274-322 (cooling): beta cooling of the specific internal energy
+// toward get_cs2 / (gamma (gamma - 1)) on the time beta / omega_tot, with
+// omega_tot^2 the sum over the planets of |force| / distance in the plane,
+// integrated over the substep (expm1); Cool_Type 1 cools only above the
+// target, 2 relaxes toward it either way.
+DHD void cooling_cell(const DevState& S, const RunDev& Rn, int run, const double* prim, double* cons,
+                      const double* xp, const double* xm, double dV, double dt) {
+    const PhysParams& P = Rn.ph;
+    const double press = prim[PPP], rho = prim[RHO];
+    const double gm1 = P.gamma_law - 1.0;
+    const double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
+    const double enTarget = get_cs2(P, x) / (P.gamma_law * gm1);
+    const double enCurrent = press / (rho * gm1);
+    if (P.cool_type == 2 || enCurrent > enTarget) {
+        const double r = x[0], phi = x[1], z = x[2];
+        const double gx = r * cos(phi), gy = r * sin(phi);
+        double omtot = 0.0;
+        for (int p = 0; p < P.npl; ++p) {
+            Planet pl = Rn.planets[p];
+            pl.phi = S.planet_phi[run * DISCO_MAX_PLANETS + p];
+            const double dx = gx - pl.r * cos(pl.phi), dy = gy - pl.r * sin(pl.phi);
+            const double mag = sqrt(dx * dx + dy * dy);
+            double fr, fp, fz;
+            planetary_force(pl, P.grav2D, r, phi, z, &fr, &fp, &fz, 1);
+            omtot += sqrt(fr * fr + fp * fp + fz * fz) / mag;
+        }
+        omtot = sqrt(omtot);
+        const double Tm1 = expm1(-dt * omtot / P.cool_beta);
+        cons[TAU] += rho * dV * (enCurrent - enTarget) * Tm1;
+    }
+}
+
 // First half of the update: the substep's start state (timestep.c:18-24's
 // copy in the first substep, adjust_RK_cons's average in the second,
 // misc.c:226-244), the conservative update from the gathered face increments
@@ -743,6 +775,7 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
         double dV = get_dV(xp, xm, Pmax);
+        if (P.cool_type) cooling_cell(S, Rn, A.run, prim, cons, xp, xm, dV, dt);  // misc.c:579
         if constexpr (Ph::SOURCE_PASS) {
             // computed by source_cell (same operands, added to zero first)
             cons[SRR] += S.src[0 * N + c];

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report | --bc]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report | --bc | --cool]   (only the named set)
 """
 from __future__ import annotations
 
@@ -139,6 +139,18 @@ def grmhd_leaf_vectors(n: int = 512) -> None:
         out = np.fromfile(fout, np.float64).reshape(n, 8 + 8 + 23)
     np.savez_compressed(os.path.join(HERE, "grmhd_leaf_a0.npz"), gamma=P["gamma"], M=P["M"], x=x, prim=prim,
                         guess=guess, cons=out[:, :8], prim_out=out[:, 8:16], metric=out[:, 16:])
+
+
+COOL_CASES = [  # beta cooling (sink.c:274-322) of the small 2D disk toward the Cs2_Profile 4 target
+    ("c1_cool_beta.npz", dict(Num_R=16, Cool_Type=1, Cool_Par1=5.0, Cs2_Profile=4)),
+    ("c1_cool_relax.npz", dict(Num_R=16, Cool_Type=2, Cool_Par1=2.0, Cs2_Profile=4)),
+]
+
+
+def fixtures_cooling() -> None:
+    for name, over in COOL_CASES:
+        states, _ = run_ref("euler_sun", par_with("c1_kepler2d.par", **over), 10, dumps=(1,))
+        save_states(name, states)
 
 
 BC_CASES = [  # (fixture, overrides of c1_kepler2d.par, reference variant, steps, dumps)
@@ -304,6 +316,10 @@ def main() -> None:
         fixtures_report()
         print("fixtures written")
         return
+    if "--cool" in sys.argv:
+        fixtures_cooling()
+        print("fixtures written")
+        return
     if "--bc" in sys.argv:
         fixtures_boundaries()
         print("fixtures written")
@@ -327,6 +343,7 @@ def main() -> None:
     fixtures_t_end()
     fixtures_report()
     fixtures_boundaries()
+    fixtures_cooling()
     # full-size 2D disk: conserved totals only
     txt = par_with("c1_kepler2d.par")
     states, secs = run_ref("euler_sun", txt, 20, dumps=(1,))

@@ -39,6 +39,9 @@ CASES = [
     ("c1_zerograd.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="zerograd")),
     ("c1_outflow.npz", "c1_kepler2d.par", dict(Num_R=16, Boundary="outflow3d")),
     ("c2e_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d")),
+    # beta cooling (sink.c:274-322) toward the Cs2_Profile 4 energy: above the target only, and both ways
+    ("c1_cool_beta.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=1, Cool_Par1=5.0, Cs2_Profile=4)),
+    ("c1_cool_relax.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=2, Cool_Par1=2.0, Cs2_Profile=4)),
 ]
 
 

@@ -230,6 +230,13 @@ def test_unsupported_configuration_is_rejected():
     p = params("c1_kepler2d.par", Timestep=1)
     with pytest.raises(RuntimeError, match="Timestep"):
         D.setup_run(p)
+    # sink.c terms that are not built are refused, not ignored
+    for key, msg in (("Sink_Type", "Sink_Type"), ("Nozzle_Type", "Nozzle_Type"), ("DampInner_Type", "Damp"),
+                     ("Cool_Type", "Cool_Type")):
+        with pytest.raises(RuntimeError, match=msg):
+            D.setup_run(params("c1_kepler2d.par", **{key: 3}))
+    with pytest.raises(RuntimeError, match="Cool_Par1"):
+        D.setup_run(params("c1_kepler2d.par", Cool_Type=1))
 
 
 @pytest.mark.parametrize("over,msg", [
