@@ -75,6 +75,9 @@ class Params(C.Structure):
         ("coolType", C.c_int), ("nozzleType", C.c_int), ("dampInnerType", C.c_int),
         ("dampOuterType", C.c_int), ("dampUpperType", C.c_int), ("dampLowerType", C.c_int),
         ("coolPar1", C.c_double), ("coolPar2", C.c_double), ("coolPar3", C.c_double), ("coolPar4", C.c_double),
+        ("dampTimeInner", C.c_double), ("dampLenInner", C.c_double), ("dampTimeOuter", C.c_double),
+        ("dampLenOuter", C.c_double), ("dampTimeUpper", C.c_double), ("dampLenUpper", C.c_double),
+        ("dampTimeLower", C.c_double), ("dampLenLower", C.c_double),
     ]
 
     def copy(self) -> "Params":

@@ -134,9 +134,17 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     D("Cool_Par4", &p->coolPar4);
     I("Nozzle_Type", &p->nozzleType);
     I("DampInner_Type", &p->dampInnerType);
+    D("DampInner_Time", &p->dampTimeInner);
+    D("DampInner_Len", &p->dampLenInner);
     I("DampOuter_Type", &p->dampOuterType);
+    D("DampOuter_Time", &p->dampTimeOuter);
+    D("DampOuter_Len", &p->dampLenOuter);
     I("DampUpper_Type", &p->dampUpperType);
+    D("DampUpper_Time", &p->dampTimeUpper);
+    D("DampUpper_Len", &p->dampLenUpper);
     I("DampLower_Type", &p->dampLowerType);
+    D("DampLower_Time", &p->dampTimeLower);
+    D("DampLower_Len", &p->dampLenLower);
     // module slots (Makefile_opt.in in the reference); default euler/fixed/sun
     char s[256];
     p->hydro = DISCO_HYDRO_EULER;
@@ -211,6 +219,22 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->report_soft = par.sinkPar3;
     ph->cool_type = par.coolType;
     ph->cool_beta = par.coolPar1;
+    ph->damp_type[0] = par.dampInnerType;
+    ph->damp_type[1] = par.dampOuterType;
+    ph->damp_type[2] = par.dampUpperType;
+    ph->damp_type[3] = par.dampLowerType;
+    ph->damp_time[0] = par.dampTimeInner;
+    ph->damp_time[1] = par.dampTimeOuter;
+    ph->damp_time[2] = par.dampTimeUpper;
+    ph->damp_time[3] = par.dampTimeLower;
+    ph->damp_len[0] = par.dampLenInner;
+    ph->damp_len[1] = par.dampLenOuter;
+    ph->damp_len[2] = par.dampLenUpper;
+    ph->damp_len[3] = par.dampLenLower;
+    ph->rmin = par.rmin;
+    ph->rmax = par.rmax;
+    ph->zmax = par.zmax;
+    ph->two_d = par.Num_Z == 1;
     disco_ic_from_params(&par, ic);
 }
 
@@ -254,7 +278,13 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.coolType && p.hydro == DISCO_HYDRO_GRMHD) m += " Cool_Type with grmhd";
     if (p.coolType && p.coolPar1 == 0.0) m += " Cool_Type needs Cool_Par1 (beta) != 0";
     if (p.nozzleType) m += " Nozzle_Type";
-    if (p.dampInnerType || p.dampOuterType || p.dampUpperType || p.dampLowerType) m += " Damp*_Type";
+    // sink.c:329-426 damping zones: types 0-2; with only the z zones in a 2D
+    // run the reference's damping factor is never set
+    const int dt4[4] = {p.dampInnerType, p.dampOuterType, p.dampUpperType, p.dampLowerType};
+    for (int t : dt4)
+        if (t < 0 || t > 2) m += " Damp*_Type";
+    if ((dt4[2] || dt4[3]) && !dt4[0] && !dt4[1] && p.Num_Z == 1) m += " Damp{Upper,Lower}_Type alone in 2D";
+    if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && p.hydro == DISCO_HYDRO_GRMHD) m += " Damp*_Type with grmhd";
     if (p.RotFrame) m += " RotFrame";
     if (p.noiseType) m += " Noise_Type";
     if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";

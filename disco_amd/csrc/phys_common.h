@@ -38,6 +38,10 @@ struct PhysParams {
     double report_soft; // Sink_Par3: report.c's torque-cut length
     int cool_type;      // Cool_Type: 0 none, 1 beta cooling above the target energy, 2 toward it
     double cool_beta;   // Cool_Par1: the cooling time in units of 1/omega_tot
+    int damp_type[4];   // damping zones inner, outer, upper, lower: 0 off, 1 on, 2 time in units of 1/omega_tot
+    double damp_time[4], damp_len[4];
+    double rmin, rmax, zmax;  // R_Min, R_Max, Z_Max (sink.c's zone edges)
+    int two_d;                // Num_Z == 1: the z zones are off
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face

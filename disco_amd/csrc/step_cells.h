@@ -64,6 +64,10 @@ struct PhysEuler {
         if (okp) *okp = 1;
         return 0;
     }
+    static constexpr bool NEWTONIAN = true;  // sink.c's cooling and damping apply
+    DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const double* x, double dV) {
+        euler::prim2cons<NQ>(P, prim, cons, x, dV);
+    }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return euler::mindt(P, prim, w, xp, xm);
     }
@@ -92,6 +96,10 @@ struct PhysMHD {
         if (okp) *okp = 1;
         return 0;
     }
+    static constexpr bool NEWTONIAN = true;
+    DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const double* x, double dV) {
+        mhd::prim2cons(P, prim, cons, x, dV);
+    }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return mhd::mindt(P, prim, w, xp, xm);
     }
@@ -106,6 +114,7 @@ struct PhysGRMHD {
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = true;
     static constexpr bool SOURCE_PASS = true;
+    static constexpr bool NEWTONIAN = false;  // sink.c's terms are rejected at set-up
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         return grmhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
@@ -693,36 +702,81 @@ DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double*
     }
 }
 
+// omega_tot of sink.c's cooling and damping: the square root of the sum over
+// the planets of |force| / distance in the plane (planetaryForce mode 1).
+DHD double omega_tot(const DevState& S, const RunDev& Rn, int run, const double* x) {
+    const PhysParams& P = Rn.ph;
+    const double r = x[0], phi = x[1], z = x[2];
+    const double gx = r * cos(phi), gy = r * sin(phi);
+    double omtot = 0.0;
+    for (int p = 0; p < P.npl; ++p) {
+        Planet pl = Rn.planets[p];
+        pl.phi = S.planet_phi[run * DISCO_MAX_PLANETS + p];
+        const double dx = gx - pl.r * cos(pl.phi), dy = gy - pl.r * sin(pl.phi);
+        const double mag = sqrt(dx * dx + dy * dy);
+        double fr, fp, fz;
+        planetary_force(pl, P.grav2D, r, phi, z, &fr, &fp, &fz, 1);
+        omtot += sqrt(fr * fr + fp * fp + fz * fz) / mag;
+    }
+    return sqrt(omtot);
+}
+
 // sink.c:274-322 (cooling): beta cooling of the specific internal energy
 // toward get_cs2 / (gamma (gamma - 1)) on the time beta / omega_tot, with
 // omega_tot^2 the sum over the planets of |force| / distance in the plane,
 // integrated over the substep (expm1); Cool_Type 1 cools only above the
 // target, 2 relaxes toward it either way.
 DHD void cooling_cell(const DevState& S, const RunDev& Rn, int run, const double* prim, double* cons,
-                      const double* xp, const double* xm, double dV, double dt) {
+                      const double* x, double dV, double dt) {
     const PhysParams& P = Rn.ph;
     const double press = prim[PPP], rho = prim[RHO];
     const double gm1 = P.gamma_law - 1.0;
-    const double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
     const double enTarget = get_cs2(P, x) / (P.gamma_law * gm1);
     const double enCurrent = press / (rho * gm1);
     if (P.cool_type == 2 || enCurrent > enTarget) {
-        const double r = x[0], phi = x[1], z = x[2];
-        const double gx = r * cos(phi), gy = r * sin(phi);
-        double omtot = 0.0;
-        for (int p = 0; p < P.npl; ++p) {
-            Planet pl = Rn.planets[p];
-            pl.phi = S.planet_phi[run * DISCO_MAX_PLANETS + p];
-            const double dx = gx - pl.r * cos(pl.phi), dy = gy - pl.r * sin(pl.phi);
-            const double mag = sqrt(dx * dx + dy * dy);
-            double fr, fp, fz;
-            planetary_force(pl, P.grav2D, r, phi, z, &fr, &fp, &fz, 1);
-            omtot += sqrt(fr * fr + fp * fp + fz * fz) / mag;
-        }
-        omtot = sqrt(omtot);
+        const double omtot = omega_tot(S, Rn, run, x);
         const double Tm1 = expm1(-dt * omtot / P.cool_beta);
         cons[TAU] += rho * dV * (enCurrent - enTarget) * Tm1;
     }
+}
+
+// sink.c:329-426 (damping): the conserved state relaxes toward the initial
+// condition's in the zones within Damp*_Len of the inner and outer radius
+// and of the upper and lower boundary (the lower zone measured from Z_Max,
+// as sink.c has it), at the rate of the last enabled zone (sink.c averages
+// the zones' rates but applies the last zone's factor), over the substep.
+template <class Ph>
+DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double* prim, double* cons,
+                      const double* x, double dV, double dt) {
+    constexpr int NQ = Ph::NQ;
+    const PhysParams& P = Rn.ph;
+    double omtot = 1.0;
+    if (P.damp_type[0] == 2 || P.damp_type[1] == 2 || P.damp_type[2] == 2 || P.damp_type[3] == 2)
+        omtot = omega_tot(S, Rn, run, x);
+    double dampFactor = 0.0;
+    for (int z = 0; z < 4; ++z) {
+        if (P.damp_type[z] <= 0 || (z >= 2 && P.two_d)) continue;
+        double dampTime = P.damp_time[z];
+        if (P.damp_type[z] == 2) dampTime = dampTime / omtot;
+        (void)dampTime;  // sink.c folds it into a rate it does not apply
+        const double theta = (z == 0 ? x[0] - P.rmin : z == 1 ? P.rmax - x[0] : P.zmax - x[2]) / P.damp_len[z];
+        if (theta > 1.0)
+            dampFactor = 0.0;
+        else if (z == 0 || z == 3)
+            dampFactor = 1.0 - pow(1.0 - pow(theta, 2.0), 2.0);
+        else
+            dampFactor = pow(1.0 - pow(theta, 2.0), 2.0);
+    }
+    dampFactor = expm1(-dt * dampFactor);
+    double p0[8], c0[NQ], c1[NQ];
+    disco_initial(&Rn.ic, p0, x);
+    Ph::prim2cons(P, p0, c0, x, dV);
+    Ph::prim2cons(P, prim, c1, x, dV);
+    cons[DDD] += (c1[DDD] - c0[DDD]) * dampFactor;
+    cons[SRR] += (c1[SRR] - c0[SRR]) * dampFactor;
+    cons[LLL] += (c1[LLL] - c0[LLL]) * dampFactor;
+    cons[SZZ] += (c1[SZZ] - c0[SZZ]) * dampFactor;
+    cons[TAU] += (c1[TAU] - c0[TAU]) * dampFactor;
 }
 
 // First half of the update: the substep's start state (timestep.c:18-24's
@@ -775,7 +829,15 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
         double dV = get_dV(xp, xm, Pmax);
-        if (P.cool_type) cooling_cell(S, Rn, A.run, prim, cons, xp, xm, dV, dt);  // misc.c:579
+        if constexpr (Ph::NEWTONIAN) {  // misc.c:579-580
+            // get_centroid_arr (geometry_functions.c:5-10) with the host's
+            // r and z centroids: the initial condition damping relaxes to
+            // hashes these bits for its noise
+            const double xc[3] = {A.rc, get_centroid(xp[1], xm[1], 0), A.zc};
+            if (P.cool_type) cooling_cell(S, Rn, A.run, prim, cons, xc, dV, dt);
+            if (P.damp_type[0] | P.damp_type[1] | P.damp_type[2] | P.damp_type[3])
+                damping_cell<Ph>(S, Rn, A.run, prim, cons, xc, dV, dt);
+        }
         if constexpr (Ph::SOURCE_PASS) {
             // computed by source_cell (same operands, added to zero first)
             cons[SRR] += S.src[0 * N + c];

@@ -147,9 +147,24 @@ COOL_CASES = [  # beta cooling (sink.c:274-322) of the small 2D disk toward the 
 ]
 
 
+# damping zones (sink.c:329-426) on the binary disk without density noise
+# (damping relaxes toward the initial condition evaluated at the moving cell
+# centres, and the seeded noise is keyed on their exact bits): inner + outer
+# (outer time in 1/omega_tot) in 2D, upper + lower in 3D
+DAMP_CASES = [
+    ("c3_damp.npz", dict(Num_R=24, Init_Par1=0.0, DampInner_Type=1, DampInner_Time=1.0, DampInner_Len=0.5,
+                         DampOuter_Type=2, DampOuter_Time=1.0, DampOuter_Len=0.8), 10),
+    ("c3z_damp.npz", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Init_Par1=0.0, DampUpper_Type=1, DampUpper_Time=1.0,
+                          DampUpper_Len=0.3, DampLower_Type=1, DampLower_Time=1.0, DampLower_Len=0.3), 5),
+]
+
+
 def fixtures_cooling() -> None:
     for name, over in COOL_CASES:
         states, _ = run_ref("euler_sun", par_with("c1_kepler2d.par", **over), 10, dumps=(1,))
+        save_states(name, states)
+    for name, over, n in DAMP_CASES:
+        states, _ = run_ref("euler_binary", par_with("c3_binary_disk.par", **over), n, dumps=(1,))
         save_states(name, states)
 
 

@@ -42,6 +42,15 @@ CASES = [
     # beta cooling (sink.c:274-322) toward the Cs2_Profile 4 energy: above the target only, and both ways
     ("c1_cool_beta.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=1, Cool_Par1=5.0, Cs2_Profile=4)),
     ("c1_cool_relax.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=2, Cool_Par1=2.0, Cs2_Profile=4)),
+    # damping zones (sink.c:329-426) on the binary disk without density noise (the zones relax toward the
+    # initial condition at the moving cell centres): inner + outer (outer time in 1/omega_tot) in 2D,
+    # upper + lower in 3D
+    ("c3_damp.npz", "c3_binary_disk.par", dict(Num_R=24, Init_Par1=0.0, DampInner_Type=1, DampInner_Time=1.0,
+                                                DampInner_Len=0.5, DampOuter_Type=2, DampOuter_Time=1.0,
+                                                DampOuter_Len=0.8)),
+    ("c3z_damp.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Init_Par1=0.0,
+                                                 DampUpper_Type=1, DampUpper_Time=1.0, DampUpper_Len=0.3,
+                                                 DampLower_Type=1, DampLower_Time=1.0, DampLower_Len=0.3)),
 ]
 
 

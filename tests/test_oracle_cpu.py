@@ -237,6 +237,8 @@ def test_unsupported_configuration_is_rejected():
             D.setup_run(params("c1_kepler2d.par", **{key: 3}))
     with pytest.raises(RuntimeError, match="Cool_Par1"):
         D.setup_run(params("c1_kepler2d.par", Cool_Type=1))
+    with pytest.raises(RuntimeError, match="alone in 2D"):
+        D.setup_run(params("c1_kepler2d.par", DampUpper_Type=1))
 
 
 @pytest.mark.parametrize("over,msg", [
