@@ -143,6 +143,7 @@ struct StepShape {
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
+    int sink;         // some run cools or has damping zones: the update kernels with sink.c's terms run
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
     // zero-gradient ghost fills (zerograd.c, outflow3d.c): DevState::bcjobs
     // holds bc_off[bc_nlev] jobs, level l in [bc_off[l], bc_off[l+1]); the

@@ -1112,12 +1112,15 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
     shape->has_z = 0;
     shape->zcopy = 0;
+    shape->sink = 0;
     shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
                     ? 1
                     : 0;
     for (const HostRun& R : runs) {
         if (R.Nz > 1) shape->has_z = 1;
         if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;
+        const auto& q = R.par;
+        if (q.coolType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType) shape->sink = 1;
     }
 }
 

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::FLUX) void k_flux(DevState 
 
 // Fused update for hydro: conserved state stays in registers between the
 // gather/source half and the cons2prim/boundary half.
-template <class Ph>
+template <class Ph, bool SINK>
 __global__ __launch_bounds__(256) void k_update(DevState S, double RK, double dtfac, int first, int want_dt) {
     int c = cell_index();
     int run = -1;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_update(DevState S, double RK, double dt
     if (c < S.ncell) {
         double cons[Ph::NQ];
         run = S.ann[S.cell_ann[c]].run;
-        update_a_cell<Ph>(S, c, RK, dtfac, first, cons);
+        update_a_cell<Ph, SINK>(S, c, RK, dtfac, first, cons);
         dtc = update_b_cell<Ph>(S, c, first, want_dt, cons);
     }
     if (want_dt) dt_reduce(S, run, dtc);
@@ -337,13 +337,13 @@ __global__ __launch_bounds__(256) DISCO_OCC(6) void k_ct_update(DevState S, doub
     if (c < S.ncell) ct_update_cell(S, c, RK, dtfac, first, has_z);
 }
 
-template <class Ph>
+template <class Ph, bool SINK>
 __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::CT) void k_update_a(DevState S, double RK, double dtfac,
                                                                         int first) {
     int c = cell_index();
     if (c < S.ncell) {
         double cons[Ph::NQ];
-        update_a_cell<Ph>(S, c, RK, dtfac, first, cons);
+        update_a_cell<Ph, SINK>(S, c, RK, dtfac, first, cons);
     }
 }
 
@@ -493,7 +493,12 @@ struct Launch {
                 DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
             DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first,
                          sh.has_z);
-            DISCO_LAUNCH(hook, "k_update_a", k_update_a<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first);
+            if (Ph::NEWTONIAN && sh.sink)
+                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, true>), G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac,
+                             first);
+            else
+                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, false>), G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac,
+                             first);
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
@@ -505,8 +510,12 @@ struct Launch {
                 DISCO_LAUNCH(hook, "k_update_b", k_update_b<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, first,
                              want_dt);
         } else {
-            DISCO_LAUNCH(hook, "k_update", k_update<Ph>, G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK, dtfac, first,
-                         want_dt);
+            if (Ph::NEWTONIAN && sh.sink)
+                DISCO_LAUNCH(hook, "k_update", (k_update<Ph, true>), G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK,
+                             dtfac, first, want_dt);
+            else
+                DISCO_LAUNCH(hook, "k_update", (k_update<Ph, false>), G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK,
+                             dtfac, first, want_dt);
         }
         // boundary_trans (onestep.c:166-173): zero-gradient ghosts level by level, then the periodic z copy
         for (int l = 0; l < sh.bc_nlev; ++l)

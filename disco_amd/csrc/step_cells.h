@@ -784,8 +784,10 @@ DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double
 // misc.c:226-244), the conservative update from the gathered face increments
 // (riemann.c:314-317), add_source (misc.c:550-593) and, in the first substep,
 // move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns the updated
-// conserved state of a real cell in cons[] (and stores it).
-template <class Ph>
+// conserved state of a real cell in cons[] (and stores it).  SINK compiles in
+// sink.c's cooling and damping (a separate instantiation, so that the update
+// without them keeps its registers).
+template <class Ph, bool SINK = false>
 DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int first, double* cons) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
@@ -829,7 +831,7 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
         double dV = get_dV(xp, xm, Pmax);
-        if constexpr (Ph::NEWTONIAN) {  // misc.c:579-580
+        if constexpr (Ph::NEWTONIAN && SINK) {  // misc.c:579-580
             // get_centroid_arr (geometry_functions.c:5-10) with the host's
             // r and z centroids: the initial condition damping relaxes to
             // hashes these bits for its noise
