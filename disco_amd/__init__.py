@@ -78,6 +78,8 @@ class Params(C.Structure):
         ("dampTimeInner", C.c_double), ("dampLenInner", C.c_double), ("dampTimeOuter", C.c_double),
         ("dampLenOuter", C.c_double), ("dampTimeUpper", C.c_double), ("dampLenUpper", C.c_double),
         ("dampTimeLower", C.c_double), ("dampLenLower", C.c_double),
+        ("nozzlePar1", C.c_double), ("nozzlePar2", C.c_double), ("nozzlePar3", C.c_double),
+        ("nozzlePar4", C.c_double),
     ]
 
     def copy(self) -> "Params":

@@ -79,9 +79,9 @@ typedef struct disco_params {
     /* module selection (compile-time in the reference) */
     int hydro, boundary, planets, initial;
     int num_c, num_n;
-    /* sink.c's source terms: beta cooling (Cool_Type 1, 2) and the damping
-       zones (Damp*_Type 1, 2) are built, the sinks and the nozzle are not
-       (a non-zero type is rejected);
+    /* sink.c's source terms: beta cooling (Cool_Type 1, 2), the damping
+       zones (Damp*_Type 1, 2) and the mass nozzle (Nozzle_Type 1) are built,
+       the sinks are not (a non-zero type is rejected);
        Sink_Par3 is also report.c's torque-cut length */
     int sinkType, sinkNumber;
     double sinkPar1, sinkPar2, sinkPar3, sinkPar4, sinkPar5;
@@ -89,6 +89,7 @@ typedef struct disco_params {
     double coolPar1, coolPar2, coolPar3, coolPar4;
     double dampTimeInner, dampLenInner, dampTimeOuter, dampLenOuter;
     double dampTimeUpper, dampLenUpper, dampTimeLower, dampLenLower;
+    double nozzlePar1, nozzlePar2, nozzlePar3, nozzlePar4;
 } disco_params;
 
 #ifdef __cplusplus

@@ -133,6 +133,10 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     D("Cool_Par3", &p->coolPar3);
     D("Cool_Par4", &p->coolPar4);
     I("Nozzle_Type", &p->nozzleType);
+    D("Nozzle_Par1", &p->nozzlePar1);
+    D("Nozzle_Par2", &p->nozzlePar2);
+    D("Nozzle_Par3", &p->nozzlePar3);
+    D("Nozzle_Par4", &p->nozzlePar4);
     I("DampInner_Type", &p->dampInnerType);
     D("DampInner_Time", &p->dampTimeInner);
     D("DampInner_Len", &p->dampLenInner);
@@ -231,6 +235,11 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->damp_len[1] = par.dampLenOuter;
     ph->damp_len[2] = par.dampLenUpper;
     ph->damp_len[3] = par.dampLenLower;
+    ph->nozzle_type = par.nozzleType;
+    ph->nozzle_par[0] = par.nozzlePar1;
+    ph->nozzle_par[1] = par.nozzlePar2;
+    ph->nozzle_par[2] = par.nozzlePar3;
+    ph->nozzle_par[3] = par.nozzlePar4;
     ph->rmin = par.rmin;
     ph->rmax = par.rmax;
     ph->zmax = par.zmax;
@@ -277,7 +286,10 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.coolType < 0 || p.coolType > 2) m += " Cool_Type";
     if (p.coolType && p.hydro == DISCO_HYDRO_GRMHD) m += " Cool_Type with grmhd";
     if (p.coolType && p.coolPar1 == 0.0) m += " Cool_Type needs Cool_Par1 (beta) != 0";
-    if (p.nozzleType) m += " Nozzle_Type";
+    // sink.c:107-139: the nozzle (type 1), a Newtonian source
+    if (p.nozzleType < 0 || p.nozzleType > 1) m += " Nozzle_Type";
+    if (p.nozzleType && p.hydro == DISCO_HYDRO_GRMHD) m += " Nozzle_Type with grmhd";
+    if (p.nozzleType && (p.nozzlePar2 <= 0.0 || p.nozzlePar4 == 0.0)) m += " Nozzle_Par2 > 0 and Nozzle_Par4 != 0";
     // sink.c:329-426 damping zones: types 0-2; with only the z zones in a 2D
     // run the reference's damping factor is never set
     const int dt4[4] = {p.dampInnerType, p.dampOuterType, p.dampUpperType, p.dampLowerType};
@@ -1120,7 +1132,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         if (R.Nz > 1) shape->has_z = 1;
         if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;
         const auto& q = R.par;
-        if (q.coolType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType) shape->sink = 1;
+        if (q.coolType || q.nozzleType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType) shape->sink = 1;
     }
 }
 

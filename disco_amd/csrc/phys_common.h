@@ -41,6 +41,8 @@ struct PhysParams {
     int damp_type[4];   // damping zones inner, outer, upper, lower: 0 off, 1 on, 2 time in units of 1/omega_tot
     double damp_time[4], damp_len[4];
     double rmin, rmax, zmax;  // R_Min, R_Max, Z_Max (sink.c's zone edges)
+    int nozzle_type;          // Nozzle_Type: 0 none, 1 a mass source in a ball
+    double nozzle_par[4];     // Nozzle_Par1-4: ball centre radius, ball radius, injection speed, its Mach number
     int two_d;                // Num_Z == 1: the z zones are off
 };
 

@@ -779,14 +779,41 @@ DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double
     cons[TAU] += (c1[TAU] - c0[TAU]) * dampFactor;
 }
 
+// sink.c:107-139 (nozzle): mass injected at the rate 3 / (4 pi dR^3) per unit
+// volume (times the chord 2 sqrt(dR^2 - d^2) in 2D) inside the ball of radius
+// dR about (r, phi, z) = (R0, 0, 0), d measured with the midpoint
+// coordinates and phi folded into (-pi, pi], moving at azimuthal speed v with
+// the thermal energy of sound speed v / mach, its kinetic energy counted
+// relative to the frame speed r get_om.  The clip compares the injected mass
+// with the cell's mass and then sets the rate to cons[DDD] / dV * dt, as the
+// reference writes it.
+DHD void nozzle_cell(const PhysParams& P, double* cons, const double* xp, const double* xm, double dV, double dt) {
+    const double R0 = P.nozzle_par[0], dR = P.nozzle_par[1], v = P.nozzle_par[2], mach = P.nozzle_par[3];
+    const double r = 0.5 * (xp[0] + xm[0]);
+    double phi = 0.5 * (xp[1] + xm[1]);
+    const double z = 0.5 * (xp[2] + xm[2]);
+    if (phi > M_PI) phi -= 2 * M_PI;
+    if ((r - R0) * (r - R0) + r * r * phi * phi + z * z > dR * dR) return;
+    double rhodot = 1.0 * 3.0 / (4.0 * M_PI * dR * dR * dR);
+    if (P.two_d) rhodot *= 2 * sqrt(dR * dR - (r - R0) * (r - R0) - r * r * phi * phi);
+    if (rhodot * dV * dt > cons[DDD]) rhodot = cons[DDD] / dV * dt;
+    const double x[3] = {r, phi, z};
+    const double om = get_om(P, x);
+    const double cs2 = v * v / (mach * mach);
+    const double eps = cs2 / (P.gamma_law * (P.gamma_law - 1));
+    cons[DDD] += rhodot * dV * dt;
+    cons[LLL] += rhodot * r * v * dV * dt;
+    cons[TAU] += (0.5 * rhodot * (v - r * om) * (v - r * om) + rhodot * eps) * dV * dt;
+}
+
 // First half of the update: the substep's start state (timestep.c:18-24's
 // copy in the first substep, adjust_RK_cons's average in the second,
 // misc.c:226-244), the conservative update from the gathered face increments
 // (riemann.c:314-317), add_source (misc.c:550-593) and, in the first substep,
 // move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns the updated
 // conserved state of a real cell in cons[] (and stores it).  SINK compiles in
-// sink.c's cooling and damping (a separate instantiation, so that the update
-// without them keeps its registers).
+// sink.c's cooling, damping and nozzle (a separate instantiation, so that the
+// update without them keeps its registers).
 template <class Ph, bool SINK = false>
 DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int first, double* cons) {
     constexpr int NQ = Ph::NQ;
@@ -854,6 +881,8 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
             euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);  // planet.c:138-158 (module independent)
         }
+        if constexpr (Ph::NEWTONIAN && SINK)  // misc.c:589, after the planets
+            if (P.nozzle_type) nozzle_cell(P, cons, xp, xm, dV, dt);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.cons[q * N + c], cons[q]);
     }
 

@@ -159,7 +159,21 @@ DAMP_CASES = [
 ]
 
 
+# the mass nozzle (sink.c:107-139): a ball of radius 0.4 at r = 1.5 on the
+# small single-star disk, injecting at near-Keplerian speed, in 2D (the chord
+# factor) and in 3D
+NOZZLE_CASES = [
+    ("c1_nozzle.npz", dict(Num_R=16, Nozzle_Type=1, Nozzle_Par1=1.5, Nozzle_Par2=0.4, Nozzle_Par3=0.8,
+                           Nozzle_Par4=10.0), 10),
+    ("c2e_nozzle.npz", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Nozzle_Type=1, Nozzle_Par1=1.5, Nozzle_Par2=0.4,
+                            Nozzle_Par3=0.8, Nozzle_Par4=10.0), 5),
+]
+
+
 def fixtures_cooling() -> None:
+    for name, over, n in NOZZLE_CASES:
+        states, _ = run_ref("euler_sun", par_with("c1_kepler2d.par", **over), n, dumps=(1,))
+        save_states(name, states)
     for name, over in COOL_CASES:
         states, _ = run_ref("euler_sun", par_with("c1_kepler2d.par", **over), 10, dumps=(1,))
         save_states(name, states)

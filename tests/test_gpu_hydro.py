@@ -42,6 +42,11 @@ CASES = [
     # beta cooling (sink.c:274-322) toward the Cs2_Profile 4 energy: above the target only, and both ways
     ("c1_cool_beta.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=1, Cool_Par1=5.0, Cs2_Profile=4)),
     ("c1_cool_relax.npz", "c1_kepler2d.par", dict(Num_R=16, Cool_Type=2, Cool_Par1=2.0, Cs2_Profile=4)),
+    # the mass nozzle (sink.c:107-139): a ball of radius 0.4 at r = 1.5, 2D (chord factor) and 3D
+    ("c1_nozzle.npz", "c1_kepler2d.par", dict(Num_R=16, Nozzle_Type=1, Nozzle_Par1=1.5, Nozzle_Par2=0.4,
+                                               Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
+    ("c2e_nozzle.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Nozzle_Type=1, Nozzle_Par1=1.5,
+                                                Nozzle_Par2=0.4, Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
     # damping zones (sink.c:329-426) on the binary disk without density noise (the zones relax toward the
     # initial condition at the moving cell centres): inner + outer (outer time in 1/omega_tot) in 2D,
     # upper + lower in 3D

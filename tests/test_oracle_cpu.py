@@ -239,6 +239,8 @@ def test_unsupported_configuration_is_rejected():
         D.setup_run(params("c1_kepler2d.par", Cool_Type=1))
     with pytest.raises(RuntimeError, match="alone in 2D"):
         D.setup_run(params("c1_kepler2d.par", DampUpper_Type=1))
+    with pytest.raises(RuntimeError, match="Nozzle_Par2"):
+        D.setup_run(params("c1_kepler2d.par", Nozzle_Type=1))
 
 
 @pytest.mark.parametrize("over,msg", [
