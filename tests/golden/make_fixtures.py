@@ -170,7 +170,19 @@ NOZZLE_CASES = [
 ]
 
 
+# the small MHD disk with all three of sink.c's built terms at once: relax
+# cooling toward get_cs2 = 1 (beta 50), an outer damping zone (the target's
+# prim2cons carries the initial field) and a nozzle ball at r = 2; velocity
+# noise off, since the damping target evaluates the initial condition at the
+# moving cell centre and the noise is keyed on that position's bits
+MHD_SINK_OVER = dict(Num_R=12, Num_Z=4, CFL=0.3, Init_Par5=0.0, Cool_Type=2, Cool_Par1=50.0, DampOuter_Type=1, DampOuter_Time=1.0,
+                     DampOuter_Len=0.8, Nozzle_Type=1, Nozzle_Par1=2.0, Nozzle_Par2=0.4, Nozzle_Par3=0.7,
+                     Nozzle_Par4=20.0)
+
+
 def fixtures_cooling() -> None:
+    states, _ = run_ref("mhd_sun", par_with("c2_mhd3d.par", **MHD_SINK_OVER), 5, dumps=(1,))
+    save_states("c2_mhd_sinkterms.npz", states)
     for name, over, n in NOZZLE_CASES:
         states, _ = run_ref("euler_sun", par_with("c1_kepler2d.par", **over), n, dumps=(1,))
         save_states(name, states)

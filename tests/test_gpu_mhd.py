@@ -45,10 +45,15 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            ("c2_mhd_small_hllc.npz", {"Riemann_Solver": 1}),
            ("c2_mhd_small_hll.npz", {"Riemann_Solver": 0}),
            # outflow ghosts in z (Boundary/outflow3d.c) instead of the periodic copy
-           ("c2_mhd_outflow.npz", {"Z_Periodic": 0, "Boundary": "outflow3d"})]
+           ("c2_mhd_outflow.npz", {"Z_Periodic": 0, "Boundary": "outflow3d"}),
+           # sink.c's cooling (relax, beta 50), outer damping zone and nozzle ball at r = 2 together (velocity
+           # noise off: the damping target is the initial condition at the moving centre)
+           ("c2_mhd_sinkterms.npz", dict(Init_Par5=0.0, Cool_Type=2, Cool_Par1=50.0, DampOuter_Type=1, DampOuter_Time=1.0,
+                                         DampOuter_Len=0.8, Nozzle_Type=1, Nozzle_Par1=2.0, Nozzle_Par2=0.4,
+                                         Nozzle_Par3=0.7, Nozzle_Par4=20.0))]
 
 
-@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "outflow"])
+@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "outflow", "sinkterms"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver
