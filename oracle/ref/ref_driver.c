@@ -19,6 +19,9 @@
  *   without output): getmindt, check_dt (domain.c:283-310, the T_End clip),
  *   timestep until check_dt marks the final step; the last dump is
  *   OUTPREFIX_<steps taken>.bin.
+ *   With the environment variable DISCO_REF_TOTALS set, every step (and
+ *   step 0) also prints "tot STEP DT Q0 .. Q(NUM_Q-1)": the conserved totals
+ *   over the real cells (jk order, then array order) with 17 digits.
  */
 #include "paul.h"
 #include <stdint.h>
@@ -83,6 +86,21 @@ static void dump(struct domain* d, const char* prefix, int step, double dt_last)
     fclose(f);
 }
 
+static void print_totals(struct domain* d, int step, double dt) {
+    double tot[NUM_Q] = {0};
+    int j, k, i, q;
+    for (k = d->NgZa; k < d->Nz - d->NgZb; ++k)
+        for (j = d->NgRa; j < d->Nr - d->NgRb; ++j) {
+            int jk = j + d->Nr * k;
+            for (i = 0; i < d->Np[jk]; ++i)
+                for (q = 0; q < NUM_Q; ++q) tot[q] += d->theCells[jk][i].cons[q];
+        }
+    printf("tot %d %.17e", step, dt);
+    for (q = 0; q < NUM_Q; ++q) printf(" %.17e", tot[q]);
+    printf("\n");
+    fflush(stdout);
+}
+
 static double now(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -113,6 +131,8 @@ int main(int argc, char* argv[]) {
         if (!theDomain.theParList.Z_Periodic) boundary_trans(&theDomain, 2);
     }
     if (strcmp(prefix, "-") != 0) dump(&theDomain, prefix, 0, 0.0);
+    const int want_totals = getenv("DISCO_REF_TOTALS") != NULL;
+    if (want_totals) print_totals(&theDomain, 0, 0.0);
 
     long ncells = 0;
     int j, k;
@@ -129,6 +149,11 @@ int main(int argc, char* argv[]) {
         dt = getmindt(&theDomain);
         if (to_end) check_dt(&theDomain, &dt);
         timestep(&theDomain, dt);
+        if (want_totals) {
+            seconds += now() - t0;
+            print_totals(&theDomain, n, dt);
+            t0 = now();
+        }
         int want = to_end ? theDomain.final_step : (n == nsteps);
         int a;
         for (a = 0; a < ndump; ++a)

@@ -173,6 +173,33 @@ def test_batch_equals_single_runs():
     batch.close()
 
 
+def test_full_48_disk_batch_equals_single_runs():
+    """The benchmark batch at its real size (48 binary disks, seeded Nr in
+    [32, 256], 7.19 M cells) for 3 steps: every run bit for bit what it gives
+    advanced alone, with its own dt, time and binary phase."""
+    from disco_amd.workloads import binary_batch
+
+    runs = binary_batch(48)
+    batch = D.Domain(runs)
+    assert sum(i.nreal for i in batch.info) > 7_000_000
+    batch.step(3)
+    batch.sync()
+    tb = batch.time()
+    for r, p in enumerate(runs):
+        one = D.Domain(p)
+        one.step(3)
+        one.sync()
+        sa, sb = batch.state(r), one.state(0)
+        for k in ("prim", "cons", "piph", "dphi", "wiph"):
+            assert np.array_equal(sa[k], sb[k]), (r, k)
+        t1 = one.time()
+        assert tb["dt"][r] == t1["dt"][0] and tb["t"][r] == t1["t"][0], r
+        assert np.array_equal(tb["planet_phi"][r], t1["planet_phi"][0]), r
+        one.close()
+    print(f"48 runs, {sum(i.nreal for i in batch.info)} real cells: batch == singles after 3 steps", flush=True)
+    batch.close()
+
+
 def test_batch_matches_oracle_per_run():
     """Each run of a small batch against the CPU oracle stepped alone."""
     from disco_amd.workloads import binary_batch
