@@ -1,0 +1,70 @@
+// Test probe (built by tests/test_grmhd_spin_cpu.py with the host compiler):
+// the library's GRMHD leaves -- the same phys_grmhd.h / phys_grmhd_fv.h the
+// GPU kernels compile -- at any spin, on records read from stdin, one record
+// per line:
+//   M a gamma riemann  prim1[8] prim2[8]  R phi z  dir
+// and per record one output line of 62 numbers:
+//   alpha, beta[3], gamma_ij[9], gamma^ij[9]                  (22)
+//   prim2cons(prim1, dV = 1)                                  (8)
+//   flux(prim1) in direction dir (0 r, 1 phi, 2 z)            (8)
+//   vel(prim1, prim2) in direction dir: Sl, Sr, Ss            (3)
+//   riemann(prim1, prim2, w = 0, dAdt = 1): dU[8], E1, B1, E2, B2   (12)
+//   source(prim1) over the cell x +- (0.05, 0.02, 0.03), dVdt = 0.7 (8)
+//   mindt(prim1, wc = 0.1) over the same cell                 (1)
+#include <cstdio>
+#include <iostream>
+#include <sstream>
+#include <string>
+
+#include "step_cells.h"
+
+using namespace disco;
+
+int main() {
+    std::string line;
+    while (std::getline(std::cin, line)) {
+        std::istringstream is(line);
+        double M, a, gam_law;
+        int rs;
+        double p1[8], p2[8], x[3];
+        int dir;
+        if (!(is >> M >> a >> gam_law >> rs)) continue;
+        for (double& v : p1) is >> v;
+        for (double& v : p2) is >> v;
+        is >> x[0] >> x[1] >> x[2] >> dir;
+        PhysParams P{};
+        P.gamma_law = gam_law;
+        P.gr_M = M;
+        P.gr_a = a;
+        P.riemann = rs;
+        P.phi_max = 2.0 * M_PI;
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, M, a, &g);
+        std::printf("%.17g", g.al);
+        for (double v : g.be) std::printf(" %.17g", v);
+        for (double v : g.gam) std::printf(" %.17g", v);
+        for (double v : g.igam) std::printf(" %.17g", v);
+        double cons[8];
+        grmhd::prim2cons(gam_law, g, p1, cons, x, 1.0);
+        for (double v : cons) std::printf(" %.17g", v);
+        const double n[3] = {dir == 0 ? 1.0 : 0.0, dir == 1 ? 1.0 : 0.0, dir == 2 ? 1.0 : 0.0};
+        double F[8];
+        grmhd::flux(gam_law, g, p1, F, x, n);
+        for (double v : F) std::printf(" %.17g", v);
+        double Sl, Sr, Ss;
+        grmhd::vel(gam_law, g, p1, p2, &Sl, &Sr, &Ss, n, x);
+        std::printf(" %.17g %.17g %.17g", Sl, Sr, Ss);
+        double dU[8];
+        const int dim = dir == 0 ? 1 : (dir == 1 ? 0 : 2);  // the reference's dim: 0 phi, 1 r, 2 z
+        FaceEMF e = PhysGRMHD::riemann(P, p1, p2, x, n, 0.0, 1.0, dim, dU);
+        for (double v : dU) std::printf(" %.17g", v);
+        std::printf(" %.17g %.17g %.17g %.17g", e.E1, e.B1, e.E2, e.B2);
+        double xp[3] = {x[0] + 0.05, x[1] + 0.02, x[2] + 0.03};
+        double xm[3] = {x[0] - 0.05, x[1] - 0.02, x[2] - 0.03};
+        double src[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        PhysGRMHD::source(P, p1, src, xp, xm, 0.7);
+        for (double v : src) std::printf(" %.17g", v);
+        std::printf(" %.17g\n", PhysGRMHD::mindt(P, p1, 0.1, xp, xm));
+    }
+    return 0;
+}
