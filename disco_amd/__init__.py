@@ -28,7 +28,7 @@ NQ_OF = {0: 5, 1: 8, 2: 8}
 
 
 class Params(C.Structure):
-    """Mirror of ``disco_params`` (disco_amd/csrc/disco_params.h)."""
+    """Mirror of ``disco_params`` (include/disco_params.h)."""
 
     _fields_ = [
         ("t_min", C.c_double), ("t_max", C.c_double),

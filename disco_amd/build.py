@@ -32,6 +32,7 @@ def hip_flags() -> list[str]:
         "-Wno-unused-function",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", CSRC,
+        "-I", os.path.join(os.path.dirname(HERE), "include"),
     ]
 
 

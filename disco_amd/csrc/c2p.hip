@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/disco_gpu.h"
+#include "disco_gpu.h"
 #include "disco_c4_states.h"
 #include "mlp_infer.h"
 #include "phys_grmhd.h"

@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/disco_gpu.h"
+#include "disco_gpu.h"
 #include "device_state.h"
 #include "host_setup.h"
 #include "mlp_infer.h"
@@ -25,7 +25,7 @@ void launch_partners(const DevState& S, hipStream_t st);
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
-                    hipStream_t st, const LaunchHook* hook);
+                    hipStream_t st, const LaunchHook* hook, const Fork* fk);
 }  // namespace disco
 
 using namespace disco;
@@ -64,7 +64,7 @@ struct disco_gpu {
     double *fphi = nullptr, *fout = nullptr, *fin = nullptr, *fup = nullptr, *fdn = nullptr;
     // constrained transport (MHD)
     double *Phi = nullptr, *rkPhi = nullptr, *emf_phi = nullptr, *emf_t = nullptr, *esum = nullptr,
-           *es1 = nullptr, *efin = nullptr;
+           *efin = nullptr;
     int* fflags = nullptr;
     int* err = nullptr;
     unsigned long long* dtbits = nullptr;
@@ -77,6 +77,7 @@ struct disco_gpu {
     double* src = nullptr;  // GRMHD source increments
     int parity = 0;
     hipStream_t stream = nullptr;
+    Fork fork{nullptr, {nullptr, nullptr, nullptr}};  // second stream of the captured steps
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // GRMHD in-loop cons2prim: learned start and counters
     unsigned short* mlp_wt = nullptr;
@@ -141,7 +142,6 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.emf_phi = h->emf_phi;
     S.emf_t = h->emf_t;
     S.esum = h->esum;
-    S.es1 = h->es1;
     S.efin = h->efin;
     S.fflags = h->fflags;
     S.err = h->err;
@@ -158,12 +158,14 @@ static DevState make_state(disco_gpu* h, int substep) {
 // onestep(0, dt, first) + onestep(1/2, dt/2, last) (timestep.c:45-48).
 // Invariant between steps: dtbits holds the CFL minimum over the current
 // state (the last substep's update computes it; create/upload seed it).
-static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook) {
+// fk: run the independent branches of each substep on two streams (graph
+// capture); null: every kernel in order on the handle's stream.
+static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook, const Fork* fk = nullptr) {
     DevState S0 = make_state(h, 0);
     DevState S1 = make_state(h, 1);
     if (with_dt) launch_dt_final(S0, 1, h->stream, hook);
-    launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook);
-    launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook);
+    launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook, fk);
+    launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook, fk);
     launch_diag(S1, h->nann, h->nq, h->stream, hook);
 }
 
@@ -265,7 +267,7 @@ static int capture_graphs(disco_gpu* h) {
         h->parity = p;
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(h, true, nullptr);
+        enqueue_step(h, true, nullptr, &h->fork);
         HIPCHK(hipStreamEndCapture(h->stream, &g));
         HIPCHK(hipGraphInstantiate(&h->graph[p], g, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(g));
@@ -329,7 +331,7 @@ static std::vector<DevBuf> layout_buffers(disco_gpu* h) {
     D(h->prim, nq), D(h->cons, nq), D(h->rkcons, nq), D(h->gradp, nq), D(h->gradr, nq), D(h->gradz, nq);
     D(h->piph[0], 1), D(h->piph[1], 1), D(h->dphi[0], 1), D(h->dphi[1], 1), D(h->wiph, 1);
     D(h->fphi, 5), D(h->fout, 5), D(h->fin, 5), D(h->fup, 5), D(h->fdn, 5);
-    D(h->Phi, 5), D(h->rkPhi, 5), D(h->emf_phi, 4), D(h->emf_t, 12), D(h->esum, 8), D(h->es1, 8), D(h->efin, 4);
+    D(h->Phi, 5), D(h->rkPhi, 5), D(h->emf_phi, 4), D(h->emf_t, 12), D(h->esum, 8), D(h->efin, 4);
     D(h->src, 4);
     I(h->lbo, 1), I(h->ubi, 1), I(h->lbu, 1), I(h->ubd, 1), I(h->pdg, 4), I(h->fflags, 1);
     b.push_back({h->rot[0], 1, 4, true});
@@ -414,7 +416,7 @@ static int set_layout(disco_gpu* h, int ktile, const int* blk) {
 static int kernel_class(const std::string& n) {
     static const std::pair<const char*, int> m[] = {
         {"k_stage_a", KC_STAGE}, {"k_plm_trans", KC_PLM}, {"k_flux", KC_FLUX},     {"k_emf_sum", KC_EMF},
-        {"k_emf_avg1", KC_EMF},  {"k_emf_face", KC_EMF},  {"k_ct_update", KC_CT},
+        {"k_emf_face", KC_EMF},  {"k_ct_update", KC_CT},
         {"k_update_a", KC_CT},   {"k_source", KC_CT},   {"k_flip_flag", KC_FLIP}, {"k_flip_chain", KC_FLIP}, {"k_update_b", KC_UPDATE},
         {"k_update_b_learned", KC_UPDATE}, {"k_update", KC_UPDATE}, {"k_zcopy", KC_COPY}, {"k_bc_copy", KC_COPY}};
     for (auto& e : m)
@@ -491,12 +493,15 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->gradr, h->gradz,    h->piph[0], h->piph[1], h->dphi[0], h->dphi[1], h->wiph,
                     h->lbo,   h->ubi,      h->lbu,   h->ubd,    h->rot[0],  h->rot[1],     h->fphi,
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
-                    h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,       h->es1,
+                    h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,
                     h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats, h->diag,
                     h->t_avg, h->src,      h->final_step, h->bcjobs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->fork.st2) (void)hipStreamDestroy(h->fork.st2);
+    for (hipEvent_t e : h->fork.ev)
+        if (e) (void)hipEventDestroy(e);
     delete h;
 }
 
@@ -554,7 +559,6 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         rc |= dalloc(&h->emf_phi, 4 * N);
         rc |= dalloc(&h->emf_t, 12 * N);
         rc |= dalloc(&h->esum, 8 * N);
-        rc |= dalloc(&h->es1, 8 * N);
         rc |= dalloc(&h->efin, 4 * N);
         rc |= dalloc(&h->fflags, N);
         rc |= dalloc(&h->pdg, 4 * N);
@@ -583,6 +587,9 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         return fail(m);
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+    if (hipStreamCreateWithFlags(&h->fork.st2, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+    for (hipEvent_t& e : h->fork.ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return cleanup("event");
     if (upload_tables(h)) return cleanup(g_err);
     // zero the scratch fields once (eager, outside any graph)
     double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz};
@@ -604,7 +611,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
             double* p;
             size_t n;
         } zc[] = {{h->Phi, 5 * N},  {h->rkPhi, 5 * N}, {h->emf_phi, 4 * N}, {h->emf_t, 12 * N},
-                  {h->esum, 8 * N}, {h->es1, 8 * N},   {h->efin, 4 * N}};
+                  {h->esum, 8 * N}, {h->efin, 4 * N}};
         for (auto& z : zc)
             if (hipMemset(z.p, 0, z.n * sizeof(double)) != hipSuccess) return cleanup("hipMemset");
         if (hipMemset(h->fflags, 0, N * sizeof(int)) != hipSuccess) return cleanup("hipMemset");

@@ -7,9 +7,9 @@
 //
 //   emf_sum_cell    riemann_trans's EMF accumulation (riemann.c:172-212):
 //                   sums of .5*E*frac over the faces of each track a cell touches
-//   emf_avg1_cell   avg_Efields, phi-neighbour averages (bfields.c:400-444)
-//   emf_face_cell   avg_Efields, per-face average across the sheared
-//                   neighbour (bfields.c:446-538) and
+//   emf_face_cell   avg_Efields: phi-neighbour averages (bfields.c:400-444,
+//                   evaluated where needed), per-face average across the
+//                   sheared neighbour (bfields.c:446-538) and
 //                   subtract_advective_B_fluxes (bfields.c:646-690)
 //   ct_update_cell  update_B_fluxes (bfields.c:313-382) and make_edge_adjust
 //                   (bfields.c:888-1127): each owned face sums the EMFs of its
@@ -205,24 +205,18 @@ DHD void emf_sum_cell(const DevState& S, int c, int has_z) {
     if (has_z) emf_sum_dir<DIR_Z>(S, A, P, c, cm);
 }
 
-// avg_Efields, first loop (bfields.c:400-444): the edge values at the cell's
-// front are the mean of the cell's and its phi-neighbour's edge slots; each
-// slot holds .5 x the phi face's EMF plus the track sums.
-DHD void emf_avg1_cell(const DevState& S, int c) {
-    const AnnDev& A = S.ann[S.cell_ann[c]];
+// avg_Efields, first loop (bfields.c:400-444): edge slot k at the front of
+// cell c (whose phi-successor is cp) is the mean of the cell's and its
+// successor's slots, each holding .5 x the cell's phi-face EMF plus the track
+// sums.  Slots: E0, E1, E4, E5 (phi-face E_z, E_z, E_r, E_r), then B0, B1, B4,
+// B5 (phi-face B_r, B_r, B_z, B_z).  Evaluated where the face stage needs it
+// instead of being stored.
+DHD double edge_avg1(const DevState& S, int k, int c, int cp) {
     const size_t N = S.ncell;
-    int cp = cell_succ(A, c);
-    double Ez = S.emf_phi[0 * N + c], Br = S.emf_phi[1 * N + c];
-    double Er = S.emf_phi[2 * N + c], Bz = S.emf_phi[3 * N + c];
+    const int src = k < 4 ? (k < 2 ? 0 : 2) : (k < 6 ? 1 : 3);
+    const double X = S.emf_phi[src * N + c];
     const double* s = S.esum;
-    st_stream(&S.es1[0 * N + c], .5 * ((.5 * Ez + s[0 * N + c]) + (.5 * Ez + s[0 * N + cp])));
-    st_stream(&S.es1[1 * N + c], .5 * ((.5 * Ez + s[1 * N + c]) + (.5 * Ez + s[1 * N + cp])));
-    st_stream(&S.es1[2 * N + c], .5 * ((.5 * Er + s[2 * N + c]) + (.5 * Er + s[2 * N + cp])));
-    st_stream(&S.es1[3 * N + c], .5 * ((.5 * Er + s[3 * N + c]) + (.5 * Er + s[3 * N + cp])));
-    st_stream(&S.es1[4 * N + c], .5 * ((.5 * Br + s[4 * N + c]) + (.5 * Br + s[4 * N + cp])));
-    st_stream(&S.es1[5 * N + c], .5 * ((.5 * Br + s[5 * N + c]) + (.5 * Br + s[5 * N + cp])));
-    st_stream(&S.es1[6 * N + c], .5 * ((.5 * Bz + s[6 * N + c]) + (.5 * Bz + s[6 * N + cp])));
-    st_stream(&S.es1[7 * N + c], .5 * ((.5 * Bz + s[7 * N + c]) + (.5 * Bz + s[7 * N + cp])));
+    return .5 * ((.5 * X + s[k * N + c]) + (.5 * X + s[k * N + cp]));
 }
 
 // avg_Efields, face loops (bfields.c:446-538): the edge at the owner's front
@@ -233,34 +227,37 @@ template <int D>
 DHD void emf_face_dir(const DevState& S, const AnnDev& A, int c, double* Ehi, double* Bhi, double* Elo,
                       double* Blo) {
     using Dr = Dir<D>;
-    const size_t N = S.ncell;
     const double Pmax = S.runs[A.run].ph.phi_max;
-    const double* e = S.es1;
-    *Ehi = e[Dr::E_HI * N + c];
-    *Bhi = e[(Dr::E_HI + 4) * N + c];
-    *Elo = e[Dr::E_LO * N + c];
-    *Blo = e[(Dr::E_LO + 4) * N + c];
+    const int cp = cell_succ(A, c);
+    *Ehi = edge_avg1(S, Dr::E_HI, c, cp);
+    *Bhi = edge_avg1(S, Dr::E_HI + 4, c, cp);
+    *Elo = edge_avg1(S, Dr::E_LO, c, cp);
+    *Blo = edge_avg1(S, Dr::E_LO + 4, c, cp);
     if (Dr::hi(A) >= 0) {  // owned face, LRtype 0: other cell c2 = R
         const AnnDev& B = S.ann[Dr::hi(A)];
         int c2 = B.off + Dr::part_hi(S)[c];
         int c2m = cell_pred(B, c2);
+        int c2p = cell_succ(B, c2);
         double dp1 = get_dp(S.piph[c2], S.piph[c], Pmax);
         double dp2 = S.dphi[c2] - dp1;
-        double Eavg = (dp2 * e[Dr::E_LO * N + c2] + dp1 * e[Dr::E_LO * N + c2m]) / (dp1 + dp2);
-        double Bavg = (dp2 * e[(Dr::E_LO + 4) * N + c2] + dp1 * e[(Dr::E_LO + 4) * N + c2m]) / (dp1 + dp2);
-        *Ehi = .5 * (e[Dr::E_HI * N + c] + Eavg);
-        *Bhi = .5 * (e[(Dr::E_HI + 4) * N + c] + Bavg);
+        double Eavg = (dp2 * edge_avg1(S, Dr::E_LO, c2, c2p) + dp1 * edge_avg1(S, Dr::E_LO, c2m, c2)) / (dp1 + dp2);
+        double Bavg =
+            (dp2 * edge_avg1(S, Dr::E_LO + 4, c2, c2p) + dp1 * edge_avg1(S, Dr::E_LO + 4, c2m, c2)) / (dp1 + dp2);
+        *Ehi = .5 * (*Ehi + Eavg);
+        *Bhi = .5 * (*Bhi + Bavg);
     }
     if (Dr::lo(A) >= 0) {  // owned face, LRtype 1: other cell c2 = L
         const AnnDev& W = S.ann[Dr::lo(A)];
         int c2 = W.off + Dr::part_lo(S)[c];
         int c2m = cell_pred(W, c2);
+        int c2p = cell_succ(W, c2);
         double dp1 = get_dp(S.piph[c2], S.piph[c], Pmax);
         double dp2 = S.dphi[c2] - dp1;
-        double Eavg = (dp2 * e[Dr::E_HI * N + c2] + dp1 * e[Dr::E_HI * N + c2m]) / (dp1 + dp2);
-        double Bavg = (dp2 * e[(Dr::E_HI + 4) * N + c2] + dp1 * e[(Dr::E_HI + 4) * N + c2m]) / (dp1 + dp2);
-        *Elo = .5 * (e[Dr::E_LO * N + c] + Eavg);
-        *Blo = .5 * (e[(Dr::E_LO + 4) * N + c] + Bavg);
+        double Eavg = (dp2 * edge_avg1(S, Dr::E_HI, c2, c2p) + dp1 * edge_avg1(S, Dr::E_HI, c2m, c2)) / (dp1 + dp2);
+        double Bavg =
+            (dp2 * edge_avg1(S, Dr::E_HI + 4, c2, c2p) + dp1 * edge_avg1(S, Dr::E_HI + 4, c2m, c2)) / (dp1 + dp2);
+        *Elo = .5 * (*Elo + Eavg);
+        *Blo = .5 * (*Blo + Bavg);
     }
 }
 

@@ -121,7 +121,6 @@ struct DevState {
     double* emf_phi;     // [4][ncell] the cell's phi face: E_z, B_r, E_r, B_z
     double* emf_t;       // [12][ncell] owned faces: (E1, B1, E_phi) for r-out, r-in, z-up, z-down
     double* esum;        // [8][ncell] sums of .5*E1*frac over the faces of a track: E r-in, r-out, z-down, z-up; B likewise
-    double* es1;         // [8][ncell] edge values after the phi average: E0, E1, E4, E5, B0, B1, B4, B5
     double* efin;        // [4][ncell] final edge EMFs E0, E1, E4, E5 (face average, advective part removed)
     int* fflags;         // per cell: flip flag bits of its owned faces (1 r-out, 2 r-in, 4 z-up, 8 z-down)
     int* err;            // device error word (bit 1: unsupported flip pattern)
@@ -151,6 +150,15 @@ struct StepShape {
     int bc_nlev;
     int bc_off[DISCO_MAX_BC_LEVELS + 1];
 };
+
+#if defined(__HIPCC__)
+// A second stream and events for the branches of a substep that may run
+// concurrently (used while a step is captured as a graph).
+struct Fork {
+    hipStream_t st2;
+    hipEvent_t ev[3];
+};
+#endif
 
 // Optional callbacks around every kernel launch (used for per-kernel timing).
 struct LaunchHook {

@@ -7,8 +7,8 @@
 // Ideal MHD with constrained transport (mhd.c, hlld.c, bfields.c) adds the
 // EMF averaging and the Stokes update between the fluxes and the update, and
 // splits the update around the flux flips of the moving mesh:
-//   k_stage_a -> k_plm_trans -> k_flux -> k_emf_sum -> k_emf_avg1 ->
-//   k_emf_face -> k_ct_update -> k_update_a -> [k_flip_flag -> k_flip_chain]
+//   k_stage_a -> k_plm_trans -> k_flux -> k_emf_sum -> k_emf_face ->
+//   k_ct_update -> k_update_a -> [k_flip_flag -> k_flip_chain]
 //   -> k_update_b (+ k_zcopy, k_planets)
 // GRMHD runs the same sequence with its geometric sources in a kernel of
 // their own (k_source, before k_ct_update); with a trained guess network attached,
@@ -310,11 +310,6 @@ __global__ __launch_bounds__(256) void k_emf_sum(DevState S, int has_z) {
     if (c < S.ncell) emf_sum_cell(S, c, has_z);
 }
 
-__global__ __launch_bounds__(256) void k_emf_avg1(DevState S) {
-    int c = cell_index();
-    if (c < S.ncell) emf_avg1_cell(S, c);
-}
-
 __global__ __launch_bounds__(256) void k_emf_face(DevState S, int has_z) {
     int c = cell_index();
     if (c < S.ncell) emf_face_cell(S, c, has_z);
@@ -476,29 +471,50 @@ struct Launch {
         DISCO_LAUNCH(hook, "k_dt_init", k_dt_init, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
         DISCO_LAUNCH(hook, "k_mindt", k_mindt<Ph>, dim3(nblk(S.ncell, B)), dim3(B), st, S);
     }
+    // With a Fork (graph capture), the cell update of the conserved state
+    // (and GRMHD's geometric sources) runs on a second stream beside the EMF
+    // averaging and the Stokes update: after the fluxes the two branches
+    // touch disjoint arrays (cons, rkcons and the moved mesh on one; EMF
+    // sums, edge EMFs and face fluxes on the other), and the flip and
+    // face-to-cell kernels wait for both.  Without a Fork every kernel runs in
+    // order on `st`.
     static void substep(const DevState& S, double RK, double dtfac, int first, int want_dt, const StepShape& sh,
-                        hipStream_t st, const LaunchHook* hook) {
+                        hipStream_t st, const LaunchHook* hook, const Fork* fk) {
         auto B = [&](int k) { return sh.blk[k] > 0 ? sh.blk[k] : 256; };
         auto G = [&](int k) { return dim3(nblk(S.ncell, B(k))); };
+        const bool fork = fk != nullptr && Ph::MHD && sh.ct;
+        hipStream_t su = fork ? fk->st2 : st;  // stream of the cell-update branch
+        if (fork) {
+            (void)hipEventRecord(fk->ev[0], st);
+            (void)hipStreamWaitEvent(su, fk->ev[0], 0);
+        }
+        // GRMHD's sources read only the substep's prims and mesh
+        if (Ph::MHD && sh.ct && Ph::SOURCE_PASS)
+            DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), su, S, dtfac);
         // partners: the first substep's are the last second substep's (the
         // mesh has not moved since)
         DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, !first);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
         DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         if (Ph::MHD && sh.ct) {
-            DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
-            DISCO_LAUNCH(hook, "k_emf_avg1", k_emf_avg1, G(KC_EMF), dim3(B(KC_EMF)), st, S);
-            DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
-            if (Ph::SOURCE_PASS)
-                DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), st, S, dtfac);
-            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first,
-                         sh.has_z);
+            if (fork) {
+                (void)hipEventRecord(fk->ev[1], st);
+                (void)hipStreamWaitEvent(su, fk->ev[1], 0);
+            }
             if (Ph::NEWTONIAN && sh.sink)
-                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, true>), G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac,
+                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, true>), G(KC_CT), dim3(B(KC_CT)), su, S, RK, dtfac,
                              first);
             else
-                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, false>), G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac,
+                DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, false>), G(KC_CT), dim3(B(KC_CT)), su, S, RK, dtfac,
                              first);
+            DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
+            DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
+            DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first,
+                         sh.has_z);
+            if (fork) {
+                (void)hipEventRecord(fk->ev[2], su);
+                (void)hipStreamWaitEvent(st, fk->ev[2], 0);
+            }
             if (first) {
                 DISCO_LAUNCH(hook, "k_flip_flag", k_flip_flag, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
                 DISCO_LAUNCH(hook, "k_flip_chain", k_flip_chain, G(KC_FLIP), dim3(B(KC_FLIP)), st, S, sh.has_z);
@@ -557,13 +573,13 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
 }
 
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
-                    hipStream_t st, const LaunchHook* hook) {
+                    hipStream_t st, const LaunchHook* hook, const Fork* fk) {
     if (sh.hydro == DISCO_HYDRO_MHD)
-        Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
+        Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
-        Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
+        Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else
-        Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook);
+        Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
 }
 
 }  // namespace disco

@@ -236,7 +236,7 @@ struct Dir<DIR_R> {
     static constexpr int PHI_HI = 2, PHI_LO = 1;     // Phi slots of the owned faces
     static constexpr int EMF_HI = 0, EMF_LO = 3;     // emf_t base slots
     static constexpr int SUM_HI = 1, SUM_LO = 0;     // esum slots (E; +4 for B)
-    static constexpr int E_HI = 1, E_LO = 0;         // es1 / efin slots (E1, E0)
+    static constexpr int E_HI = 1, E_LO = 0;         // edge slots (E1, E0) of edge_avg1 / efin
     static constexpr int FL_HI = 1, FL_LO = 2;       // fflags bits
     static constexpr double SGN = 1.0;               // update_B_fluxes orientation (bfields.c:321-346)
     DHD double dl(const AnnDev& A) { return A.zp - A.zm; }  // face dl = get_dL(.., 2) (faces_alt.c:26-27)

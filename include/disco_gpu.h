@@ -18,7 +18,7 @@
 #ifndef DISCO_GPU_H
 #define DISCO_GPU_H
 
-#include "../disco_amd/csrc/disco_params.h"
+#include "disco_params.h"
 
 #ifdef __cplusplus
 extern "C" {

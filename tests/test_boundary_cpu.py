@@ -16,7 +16,7 @@ CSRC = os.path.join(ROOT, "disco_amd", "csrc")
 
 def test_per_cell_fill_equals_sequential_loop(tmp_path):
     exe = str(tmp_path / "bc_probe")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, os.path.join(ROOT, "tests", "probes", "bc_probe.cpp"),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "probes", "bc_probe.cpp"),
                     os.path.join(CSRC, "host_setup.cpp"), "-o", exe], check=True, timeout=300)
     out = subprocess.run([exe, "11", "600"], capture_output=True, text=True, timeout=120)
     rows = [list(map(int, line.split())) for line in out.stdout.strip().splitlines()]

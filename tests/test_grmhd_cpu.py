@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.fixture(scope="module")
 def probe(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("probe") / "metric_der_probe")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", os.path.join(ROOT, "disco_amd", "csrc"),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", os.path.join(ROOT, "disco_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "probes", "metric_der_probe.cpp"), "-o", exe], check=True,
                    timeout=120)
 

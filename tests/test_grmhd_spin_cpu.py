@@ -50,7 +50,7 @@ GC = _load_oracle()
 @pytest.fixture(scope="module")
 def probe(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("spin") / "grmhd_spin_probe")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC,
+    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "probes", "grmhd_spin_probe.cpp"), "-o", exe], check=True, timeout=300)
 
     def run(a, recs):

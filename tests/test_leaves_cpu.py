@@ -40,7 +40,7 @@ CASES = {
 @pytest.fixture(scope="module")
 def probe(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("leaf") / "leaf_probe")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, os.path.join(ROOT, "tests", "probes", "leaf_probe.cpp"),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "probes", "leaf_probe.cpp"),
                     os.path.join(CSRC, "host_setup.cpp"), "-o", exe], check=True, timeout=300)
     return exe
 
