@@ -115,6 +115,8 @@ def lib() -> C.CDLL:
     L.disco_gpu_advance.argtypes = [vp, C.c_int, C.c_int, ip, ip]
     L.disco_gpu_final.argtypes = [vp, ip]
     L.disco_gpu_report.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+    if hasattr(L, "disco_gpu_report_noalloc"):
+        L.disco_gpu_report_noalloc.argtypes = [vp, C.c_int, dp, C.c_int, ip]
     L.disco_gpu_timestep.argtypes = [vp, dp]
     L.disco_gpu_sync.argtypes = [vp]
     L.disco_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
@@ -284,6 +286,16 @@ class Domain:
         out = np.zeros(15 + 8 * self.info[run].npl)
         n = C.c_int(0)
         _check(lib().disco_gpu_report(self._h, int(run), _dptr(out), out.size, C.byref(n)), "disco_gpu_report")
+        return out[:n.value]
+
+    def report_noalloc(self, run: int = 0) -> np.ndarray:
+        """The same line as report(), from the entry point that allocates
+        nothing (a per-GPU static scratch; at most 65536 annuli per run;
+        calls that overlap on one GPU race)."""
+        out = np.zeros(15 + 8 * self.info[run].npl)
+        n = C.c_int(0)
+        _check(lib().disco_gpu_report_noalloc(self._h, int(run), _dptr(out), out.size, C.byref(n)),
+               "disco_gpu_report_noalloc")
         return out[:n.value]
 
     def sync(self) -> None:

@@ -23,6 +23,7 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
 void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
 void launch_partners(const DevState& S, hipStream_t st);
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st);
+void launch_report_sum(const double* part, int Nr, int Nz, double* sums, hipStream_t st);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook, const Fork* fk);
@@ -30,11 +31,39 @@ void launch_substep(const StepShape& sh, const DevState& S, double RK, double dt
 
 using namespace disco;
 
-static thread_local std::string g_err;
+// last error message: a fixed buffer, so that setting it allocates nothing
+static thread_local char g_err[512];
 
-static int fail(const std::string& m) {
-    g_err = m;
+static int fail_c(const char* m) {
+    std::snprintf(g_err, sizeof g_err, "%s", m);
     return -1;
+}
+static int fail(const std::string& m) { return fail_c(m.c_str()); }
+
+// Device scratch of disco_gpu_report_noalloc: the per-annulus report sums of
+// one run (slot j + Nr k, DISCO_NREPORT each) and, after them, their total.
+// One copy per GPU for the life of the process; every call on that GPU
+// shares it.  In host code `report_scratch` names a host-side stand-in, not
+// device memory: host code uses only the address hipGetSymbolAddress gives
+// for the current device (report_scratch_dev).
+constexpr int DISCO_REPORT_MAX_ANN = 65536;
+constexpr size_t DISCO_REPORT_SCRATCH = (size_t)DISCO_REPORT_MAX_ANN * DISCO_NREPORT + DISCO_NREPORT;
+static __device__ __align__(256) double report_scratch[DISCO_REPORT_SCRATCH];
+constexpr int DISCO_MAX_DEVICES = 64;
+static double* report_scratch_dev[DISCO_MAX_DEVICES];
+
+// Looks up this device's copy of the scratch once (with the device current);
+// called by disco_gpu_create.
+static int init_report_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= DISCO_MAX_DEVICES)
+        return fail_c("report scratch: no current device");
+    if (report_scratch_dev[dev]) return 0;
+    double* p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(report_scratch));
+    if (e != hipSuccess || !p) return fail_c("report scratch: hipGetSymbolAddress failed");
+    report_scratch_dev[dev] = p;
+    return 0;
 }
 
 #define HIPCHK(x)                                                                              \
@@ -463,7 +492,7 @@ static int snapshot(disco_gpu* h, Snapshot* S, bool restore) {
 
 extern "C" {
 
-const char* disco_gpu_last_error(void) { return g_err.c_str(); }
+const char* disco_gpu_last_error(void) { return g_err; }
 
 int disco_parse_par(const char* text, disco_params* out, char* errbuf, int errlen) {
     std::string err;
@@ -590,6 +619,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     if (hipStreamCreateWithFlags(&h->fork.st2, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
     for (hipEvent_t& e : h->fork.ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return cleanup("event");
+    if (init_report_scratch()) return cleanup(g_err);
     if (upload_tables(h)) return cleanup(g_err);
     // zero the scratch fields once (eager, outside any graph)
     double* zq[] = {h->rkcons, h->gradp, h->gradr, h->gradz};
@@ -789,13 +819,11 @@ int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) 
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(dpart);
     if (e != hipSuccess) return fail(std::string("disco_gpu_report: ") + hipGetErrorString(e));
-    // the annuli in report.c's loop order: j outer, k inner
-    std::vector<int> pos(na);
-    for (int a = 0; a < na; ++a) pos[h->hann[a0 + a].j + R.Nr * h->hann[a0 + a].k] = a;
+    // the annuli in report.c's loop order: j outer, k inner (slot j + Nr k)
     double sum[DISCO_NREPORT] = {};
     for (int j = 0; j < R.Nr; ++j)
         for (int k = 0; k < R.Nz; ++k)
-            for (int q = 0; q < DISCO_NREPORT; ++q) sum[q] += part[(size_t)pos[j + R.Nr * k] * DISCO_NREPORT + q];
+            for (int q = 0; q < DISCO_NREPORT; ++q) sum[q] += part[((size_t)j + (size_t)R.Nr * k) * DISCO_NREPORT + q];
     double t = 0.0;
     HIPCHK(hipMemcpy(&t, h->t + run, sizeof(double), hipMemcpyDeviceToHost));
     int o = 0;
@@ -805,6 +833,42 @@ int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) 
     for (int p = 0; p < 8 * npl; ++p) out[o++] = 0.0;
     for (int q = 0; q < 3; ++q) out[o++] = sum[q];
     for (int q = 3; q < DISCO_NREPORT; ++q) out[o++] = sum[q];
+    return 0;
+}
+
+// disco_gpu_report without allocating: the annulus sums go to this GPU's
+// report_scratch and are added on the device in the same order as the host
+// adds them in disco_gpu_report (so both return the same bits); only the
+// handle's stream, kernel launches and copies into the caller's `out`.
+int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int* nout) {
+    if (!h || run < 0 || run >= h->nruns || !out) return fail_c("disco_gpu_report_noalloc: bad arguments");
+    const HostRun& R = h->runs[run];
+    const int npl = (int)R.planets.size();
+    const int n = 15 + 8 * npl;
+    if (nout) *nout = n;
+    if (nmax < n) return fail_c("disco_gpu_report_noalloc: out holds fewer values than the report line");
+    const int na = R.Nr * R.Nz, a0 = h->run_ann0[run];
+    if (na > DISCO_REPORT_MAX_ANN)
+        return fail_c("disco_gpu_report_noalloc: the run has more annuli than the scratch holds (65536)");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= DISCO_MAX_DEVICES || !report_scratch_dev[dev])
+        return fail_c("disco_gpu_report_noalloc: no scratch on the current device (create a handle on it first)");
+    double* part = report_scratch_dev[dev];
+    double* sums = part + (size_t)DISCO_REPORT_MAX_ANN * DISCO_NREPORT;
+    // every slot of the run's Nr x Nz annuli is written by k_report (zeros
+    // for ghost annuli) and every total by k_report_sum: nothing left from an
+    // earlier call is read
+    DevState S = make_state(h, 0);
+    launch_report(S, a0, na, part, h->stream);
+    launch_report_sum(part, R.Nr, R.Nz, sums, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, h->t + run, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out + 2 + 8 * npl, sums, DISCO_NREPORT * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_c("disco_gpu_report_noalloc: a kernel or copy failed");
+    out[1] = npl > 1 ? R.planets[1].M / R.planets[0].M : 0.0;
+    for (int p = 0; p < 8 * npl; ++p) out[2 + p] = 0.0;
     return 0;
 }
 

@@ -401,7 +401,8 @@ __global__ __launch_bounds__(256) void k_diag(DevState S) {
 
 // report.c's gas sums (report_quantities) of the annuli a0, a0+1, ... of one
 // run: one block per annulus, each writing its DISCO_NREPORT sums (zero for
-// ghost annuli); the host adds the annuli in the reference's loop order.
+// ghost annuli) to the annulus' slot j + Nr k of `out`; the annuli are then
+// added in the reference's loop order (host, or k_report_sum).
 __global__ __launch_bounds__(256) void k_report(DevState S, int a0, double* out) {
     __shared__ double red[DISCO_NREPORT][4];
     const int a = a0 + blockIdx.x;
@@ -425,8 +426,20 @@ __global__ __launch_bounds__(256) void k_report(DevState S, int a0, double* out)
         const int q = threadIdx.x;
         double s = red[q][0];
         for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[q][w];
-        out[(size_t)blockIdx.x * DISCO_NREPORT + q] = s;
+        const size_t slot = (size_t)A.j + (size_t)S.runs[A.run].Nr * A.k;
+        out[slot * DISCO_NREPORT + q] = s;
     }
+}
+
+// The annulus sums of k_report added in report.c's loop order (j outer, k
+// inner), one thread per quantity: sums[q] = sum over the Nr x Nz slots.
+__global__ void k_report_sum(const double* part, int Nr, int Nz, double* sums) {
+    const int q = threadIdx.x;
+    if (q >= DISCO_NREPORT) return;
+    double s = 0.0;
+    for (int j = 0; j < Nr; ++j)
+        for (int k = 0; k < Nz; ++k) s += part[((size_t)j + (size_t)Nr * k) * DISCO_NREPORT + q];
+    sums[q] = s;
 }
 
 // Zero-gradient ghost fill, one block per job of one level (bc_copy_cell).
@@ -551,6 +564,10 @@ void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const Laun
 
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_report, dim3(nann), dim3(256), 0, st, S, a0, out);
+}
+
+void launch_report_sum(const double* part, int Nr, int Nz, double* sums, hipStream_t st) {
+    hipLaunchKernelGGL(k_report_sum, dim3(1), dim3(64), 0, st, part, Nr, Nz, sums);
 }
 
 // partners of every cell on the current mesh (after it is set from the host)

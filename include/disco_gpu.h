@@ -91,6 +91,17 @@ int disco_gpu_final(disco_gpu_t* h, int* final_step);
  * reference's loop order.  Synchronous; changes no state. */
 int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
 
+/* No counterpart in the reference.  For callers that cannot allocate (HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same report line as disco_gpu_report, bit for bit, with its annulus sums
+ * kept in a static per-GPU device scratch instead of a buffer allocated per
+ * call; it calls no hipMalloc/hipFree and allocates no host memory.  Limit:
+ * at most 65536 annuli (Nr x Nz, ghosts included) per run, larger runs fail
+ * with a message.  The scratch is set up by disco_gpu_create on the current
+ * device and shared by every call on that GPU: calls that overlap on one GPU
+ * (from several threads or handles) race. */
+int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
+
 /* timestep(dom, dt) with caller-given dt per run (no CFL reduction). */
 int disco_gpu_timestep(disco_gpu_t* h, const double* dt);
 
