@@ -115,6 +115,8 @@ def lib() -> C.CDLL:
     L.disco_gpu_advance.argtypes = [vp, C.c_int, C.c_int, ip, ip]
     L.disco_gpu_final.argtypes = [vp, ip]
     L.disco_gpu_report.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+    L.disco_gpu_write_checkpoint.argtypes = [vp, C.c_int, C.c_char_p]
+    L.disco_gpu_restart.argtypes = [vp, C.c_int, C.c_char_p]
     if hasattr(L, "disco_gpu_report_noalloc"):
         L.disco_gpu_report_noalloc.argtypes = [vp, C.c_int, dp, C.c_int, ip]
     L.disco_gpu_timestep.argtypes = [vp, dp]
@@ -287,6 +289,17 @@ class Domain:
         n = C.c_int(0)
         _check(lib().disco_gpu_report(self._h, int(run), _dptr(out), out.size, C.byref(n)), "disco_gpu_report")
         return out[:n.value]
+
+    def write_checkpoint(self, path: str, run: int = 0) -> None:
+        """output() of the HDF5 module: the reference's checkpoint datasets
+        (plus the conserved state and mesh speeds) in this library's
+        container; read it with disco_amd.checkpoint.read.  Zeroes the
+        accumulated diagnostics, as output() does."""
+        _check(lib().disco_gpu_write_checkpoint(self._h, int(run), os.fsencode(path)), "disco_gpu_write_checkpoint")
+
+    def restart(self, path: str, run: int = 0) -> None:
+        """restart() from a checkpoint of a run with the same parameters."""
+        _check(lib().disco_gpu_restart(self._h, int(run), os.fsencode(path)), "disco_gpu_restart")
 
     def report_noalloc(self, run: int = 0) -> np.ndarray:
         """The same line as report(), from the entry point that allocates

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "libdisco_gpu.so"
 
-SOURCES = ["host_setup.cpp", "hydro_step.hip", "capi.hip", "c2p.hip"]
+SOURCES = ["host_setup.cpp", "checkpoint.cpp", "hydro_step.hip", "capi.hip", "c2p.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

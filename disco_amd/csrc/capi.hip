@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "checkpoint.h"
 #include "disco_gpu.h"
 #include "device_state.h"
 #include "host_setup.h"
@@ -870,6 +871,195 @@ int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int
     out[1] = npl > 1 ? R.planets[1].M / R.planets[0].M : 0.0;
     for (int p = 0; p < 8 * npl; ++p) out[2 + p] = 0.0;
     return 0;
+}
+
+// ---------------------------------------------------------------- checkpoints
+static const char* hydro_name(int v) { return v == DISCO_HYDRO_MHD ? "mhd" : v == DISCO_HYDRO_GRMHD ? "grmhd" : "euler"; }
+static const char* boundary_name(int v) {
+    static const char* n[] = {"fixed", "fixed3d", "zerograd", "outflow3d"};
+    return v >= 0 && v < 4 ? n[v] : "?";
+}
+static const char* planets_name(int v) {
+    static const char* n[] = {"none", "sun", "binary"};
+    return v >= 0 && v < 3 ? n[v] : "?";
+}
+static const char* initial_name(int v) {
+    static const char* n[] = {"kepler_noise", "iso_binary_disk", "mhd_disk", "gr_torus"};
+    return v >= 0 && v < 4 ? n[v] : "?";
+}
+
+// Output/h5out.c:366-622 for run `run`, in the container of checkpoint.h:
+// Grid/{T, r_jph, z_kph, Index, Np, Id_phi0}, Data/Cells[Ntot][NUM_Q +
+// NUM_FACES + 1] (prim, Phi, piph per cell in theCells order), Data/Planets
+// [Npl][7], Data/Diagnostics[Nz][Nr][8] (avg_diagnostics, then the
+// accumulators are zeroed as output() does), Pars/<in.par key>, Opts/<module
+// slot>; and, beyond the reference's layout, Data/Cons[Ntot][NUM_Q] and
+// Data/Wiph[Ntot], which let disco_gpu_restart resume the run bit for bit.
+int disco_gpu_write_checkpoint(disco_gpu_t* h, int run, const char* path) {
+    if (!h || run < 0 || run >= h->nruns || !path) return fail("disco_gpu_write_checkpoint: bad arguments");
+    const HostRun& R = h->runs[run];
+    const int nq = h->nq, nf = h->Phi ? 5 : 0, nc = R.ncell, na = R.Nr * R.Nz;
+    std::vector<double> prim((size_t)nq * nc), cons((size_t)nq * nc), piph(nc), dphi(nc), wiph(nc), Phi((size_t)nf * nc);
+    if (disco_gpu_download(h, run, prim.data(), cons.data(), piph.data(), dphi.data(), wiph.data())) return -1;
+    if (nf && disco_gpu_download_faces(h, run, Phi.data())) return -1;
+    double t = 0.0, pph[DISCO_MAX_PLANETS] = {0.0, 0.0};
+    HIPCHK(hipMemcpy(&t, h->t + run, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pph, h->planet_phi + run * DISCO_MAX_PLANETS, sizeof pph, hipMemcpyDeviceToHost));
+    std::vector<double> Qrz((size_t)na * DISCO_NDIAG);
+    double t_avg = 0.0;
+    if (disco_gpu_diagnostics(h, run, Qrz.data(), &t_avg, 1)) return -1;
+    for (double& v : Qrz) v /= t_avg;  // avg_diagnostics (analysis.c:29-50)
+
+    const int nd = nq + nf + 1;
+    std::vector<double> cells((size_t)nc * nd), consT((size_t)nc * nq);
+    for (int c = 0; c < nc; ++c) {
+        for (int q = 0; q < nq; ++q) cells[(size_t)c * nd + q] = prim[(size_t)q * nc + c];
+        for (int f = 0; f < nf; ++f) cells[(size_t)c * nd + nq + f] = Phi[(size_t)f * nc + c];
+        cells[(size_t)c * nd + nq + nf] = piph[c];
+        for (int q = 0; q < nq; ++q) consT[(size_t)c * nq + q] = cons[(size_t)q * nc + c];
+    }
+    // Index, Np, Id_phi0 per (k, j): h5out.c:526-547
+    std::vector<int32_t> index(na), np(na), id0(na);
+    for (int k = 0; k < R.Nz; ++k)
+        for (int j = 0; j < R.Nr; ++j) {
+            const int jk = j + R.Nr * k, o = R.off[jk];
+            index[jk] = o;
+            np[jk] = R.Np[jk];
+            double phi0 = M_PI;
+            int id = 0;
+            for (int i = 0; i < R.Np[jk]; ++i) {
+                const double phi = piph[o + i] - .5 * dphi[o + i];
+                if (std::cos(phi0) < std::cos(phi)) {
+                    phi0 = phi;
+                    id = o + i;
+                }
+            }
+            id0[jk] = id;
+        }
+    const int npl = (int)R.planets.size();
+    std::vector<double> pl((size_t)npl * 7);
+    for (int p = 0; p < npl; ++p) {
+        const Planet& P = R.planets[p];
+        const double row[7] = {P.M, 0.0, P.omega, P.r, p < DISCO_MAX_PLANETS ? pph[p] : P.phi, P.eps, (double)P.type};
+        std::memcpy(&pl[(size_t)p * 7], row, sizeof row);
+    }
+    using namespace ckpt;
+    std::vector<Dataset> d;
+    const uint64_t Nr = R.Nr, Nz = R.Nz;
+    d.push_back(make_f64("Grid/T", {1}, &t));
+    d.push_back(make_f64("Grid/r_jph", {Nr + 1}, R.r_jph.data()));
+    d.push_back(make_f64("Grid/z_kph", {Nz + 1}, R.z_kph.data()));
+    d.push_back(make_i32("Grid/Index", {Nz, Nr}, index.data()));
+    d.push_back(make_i32("Grid/Np", {Nz, Nr}, np.data()));
+    d.push_back(make_i32("Grid/Id_phi0", {Nz, Nr}, id0.data()));
+    d.push_back(make_f64("Data/Cells", {(uint64_t)nc, (uint64_t)nd}, cells.data()));
+    d.push_back(make_f64("Data/Planets", {(uint64_t)npl, 7}, pl.data()));
+    d.push_back(make_f64("Data/Diagnostics", {Nz, Nr, DISCO_NDIAG}, Qrz.data()));
+    d.push_back(make_f64("Data/Cons", {(uint64_t)nc, (uint64_t)nq}, consT.data()));
+    d.push_back(make_f64("Data/Wiph", {(uint64_t)nc}, wiph.data()));
+    const disco_params& P = R.par;
+    auto pi = [&](const char* k, int v) { d.push_back(make_i32(std::string("Pars/") + k, {1}, &v)); };
+    auto pd = [&](const char* k, double v) { d.push_back(make_f64(std::string("Pars/") + k, {1}, &v)); };
+    // writePars (h5out.c:164-364), the keys this build reads
+    pi("Num_R", P.Num_R); pi("Num_Z", P.Num_Z); pd("aspect", P.aspect); pd("T_Start", P.t_min);
+    pd("T_End", P.t_max); pd("R_Min", P.rmin); pd("R_Max", P.rmax); pd("Z_Min", P.zmin); pd("Z_Max", P.zmax);
+    pd("Phi_Max", P.phimax); pi("Log_Zoning", P.LogZoning); pi("R_Periodic", P.R_Periodic);
+    pi("Z_Periodic", P.Z_Periodic); pi("NoBC_Rmin", P.NoBC_Rmin); pi("NoBC_Rmax", P.NoBC_Rmax);
+    pi("NoBC_Zmin", P.NoBC_Zmin); pi("NoBC_Zmax", P.NoBC_Zmax); pd("Log_Radius", P.LogRadius); pd("CFL", P.CFL);
+    pd("PLM", P.PLM); pd("Max_DT", P.maxDT); pd("Adiabatic_Index", P.Adiabatic_Index);
+    pi("Isothermal", P.isothermal_flag); pi("Cs2_Profile", P.Cs2_Profile); pd("Cs2_Par", P.Cs2_Par);
+    pd("Density_Floor", P.Density_Floor); pd("Pressure_Floor", P.Pressure_Floor); pi("Mesh_Motion", P.Mesh_Motion);
+    pi("Riemann_Solver", P.Riemann_Solver); pi("Exact_Mesh_Omega", P.Exact_Mesh_Omega);
+    pd("Exact_Mesh_Omega_Par", P.Exact_Mesh_Omega_Par); pi("Energy_Omega", P.Energy_Omega);
+    pd("Energy_Omega_Par", P.Energy_Omega_Par); pi("RotFrame", P.RotFrame); pd("RotOmega", P.RotOmega);
+    pd("RotD", P.RotD); pi("Use_Viscosity", P.visc_flag); pd("Mach_Number", P.Disk_Mach);
+    pd("Mass_Ratio", P.Mass_Ratio); pi("Constrained_Transport", P.CT); pi("Metric_Par0", P.metricPar0);
+    pd("Metric_Par1", P.metricPar1); pd("Metric_Par2", P.metricPar2); pd("Metric_Par3", P.metricPar3);
+    pd("Metric_Par4", P.metricPar4); pi("Init_Par0", P.initPar0); pd("Init_Par1", P.initPar1);
+    pd("Init_Par2", P.initPar2); pd("Init_Par3", P.initPar3); pd("Init_Par4", P.initPar4); pd("Init_Par5", P.initPar5);
+    pd("Init_Par6", P.initPar6); pd("Init_Par7", P.initPar7); pd("Init_Par8", P.initPar8);
+    pi("Noise_Type", P.noiseType); pd("Noise_Abs", P.noiseAbs); pd("Noise_Rel", P.noiseRel);
+    pi("Sink_Type", P.sinkType); pd("Sink_Par1", P.sinkPar1); pd("Sink_Par2", P.sinkPar2);
+    pd("Sink_Par3", P.sinkPar3); pd("Sink_Par4", P.sinkPar4); pd("Sink_Par5", P.sinkPar5);
+    pi("Nozzle_Type", P.nozzleType); pd("Nozzle_Par1", P.nozzlePar1); pd("Nozzle_Par2", P.nozzlePar2);
+    pd("Nozzle_Par3", P.nozzlePar3); pd("Nozzle_Par4", P.nozzlePar4); pi("Cool_Type", P.coolType);
+    pd("Cool_Par1", P.coolPar1); pd("Cool_Par2", P.coolPar2); pd("Cool_Par3", P.coolPar3); pd("Cool_Par4", P.coolPar4);
+    pd("Softening", P.grav_eps);
+    // writeOpts (h5out.c:96-162): the module slots
+    d.push_back(make_str("Opts/HYDRO", hydro_name(P.hydro)));
+    d.push_back(make_str("Opts/BOUNDARY", boundary_name(P.boundary)));
+    d.push_back(make_str("Opts/PLANETS", planets_name(P.planets)));
+    d.push_back(make_str("Opts/INITIAL", initial_name(P.initial)));
+    d.push_back(make_str("Opts/GEOMETRY", "cylindrical"));
+    pi("NUM_C", P.num_c);
+    d.back().name = "Opts/NUM_C";
+    pi("NUM_N", P.num_n);
+    d.back().name = "Opts/NUM_N";
+    pi("CT_MODE", nf ? 2 : 0);
+    d.back().name = "Opts/CT_MODE";
+    std::string err;
+    if (write_file(path, d, &err)) return fail("disco_gpu_write_checkpoint: " + err);
+    return 0;
+}
+
+// Restart/h5in.c:101-368 into run `run` of a handle created from the same
+// parameters: the grid of the file must be the run's (Nr, Nz, Np, r_jph,
+// z_kph); prims, face fluxes and fronts come from Data/Cells, the time from
+// Grid/T, the planet phases from Data/Planets.  With Data/Cons and Data/Wiph
+// in the file (this library's checkpoints) the run continues bit for bit as
+// the run that wrote it; otherwise the conserved state, ghosts and mesh
+// speeds are rebuilt as the reference's restart does.  The diagnostics start
+// from zero.
+int disco_gpu_restart(disco_gpu_t* h, int run, const char* path) {
+    if (!h || run < 0 || run >= h->nruns || !path) return fail("disco_gpu_restart: bad arguments");
+    using namespace ckpt;
+    std::vector<Dataset> d;
+    std::string err;
+    if (read_file(path, &d, &err)) return fail("disco_gpu_restart: " + err);
+    HostRun& R = h->runs[run];
+    const int nq = h->nq, nf = h->Phi ? 5 : 0, nc = R.ncell, na = R.Nr * R.Nz;
+    const Dataset *T = find(d, "Grid/T"), *Ix = find(d, "Grid/Index"), *Np = find(d, "Grid/Np"),
+                  *rj = find(d, "Grid/r_jph"), *zk = find(d, "Grid/z_kph"), *C = find(d, "Data/Cells"),
+                  *Pl = find(d, "Data/Planets"), *Cs = find(d, "Data/Cons"), *Wi = find(d, "Data/Wiph");
+    if (!T || !Ix || !Np || !rj || !zk || !C) return fail("disco_gpu_restart: missing Grid/ or Data/Cells datasets");
+    if (Ix->type != I32 || Np->type != I32 || Ix->dims.size() != 2 || Ix->dims[0] != (uint64_t)R.Nz ||
+        Ix->dims[1] != (uint64_t)R.Nr || Np->count() != (size_t)na)
+        return fail("disco_gpu_restart: the file's grid is " + std::to_string(Ix->dims.size() == 2 ? Ix->dims[1] : 0) +
+                    " x " + std::to_string(Ix->dims.size() == 2 ? Ix->dims[0] : 0) + " annuli, the run's " +
+                    std::to_string(R.Nr) + " x " + std::to_string(R.Nz));
+    for (int jk = 0; jk < na; ++jk)
+        if (Np->i32()[jk] != R.Np[jk]) return fail("disco_gpu_restart: Np differs from the run's at annulus " + std::to_string(jk));
+    if (rj->count() != R.r_jph.size() || zk->count() != R.z_kph.size() ||
+        std::memcmp(rj->f64(), R.r_jph.data(), rj->data.size()) || std::memcmp(zk->f64(), R.z_kph.data(), zk->data.size()))
+        return fail("disco_gpu_restart: r_jph / z_kph differ from the run's");
+    const int nd = nq + nf + 1;
+    if (C->type != F64 || C->dims.size() != 2 || C->dims[0] != (uint64_t)nc || C->dims[1] != (uint64_t)nd)
+        return fail("disco_gpu_restart: Data/Cells is not [" + std::to_string(nc) + "][" + std::to_string(nd) +
+                    "] (NUM_Q + NUM_FACES + 1 of this run)");
+    const bool have_cons = Cs && Cs->type == F64 && Cs->count() == (size_t)nc * nq;
+    const bool have_wiph = Wi && Wi->type == F64 && Wi->count() == (size_t)nc;
+    for (int jk = 0; jk < na; ++jk) {
+        const int o = R.off[jk], src = Ix->i32()[jk];
+        if (src < 0 || src + R.Np[jk] > nc) return fail("disco_gpu_restart: Grid/Index out of range");
+        for (int i = 0; i < R.Np[jk]; ++i) {
+            const double* rec = C->f64() + (size_t)(src + i) * nd;
+            const int c = o + i;
+            for (int q = 0; q < nq; ++q) R.prim[(size_t)q * nc + c] = rec[q];
+            for (int f = 0; f < nf; ++f) R.Phi[(size_t)f * nc + c] = rec[nq + f];
+            R.piph[c] = rec[nq + nf];
+            if (have_cons)
+                for (int q = 0; q < nq; ++q) R.cons[(size_t)q * nc + c] = Cs->f64()[(size_t)(src + i) * nq + q];
+            if (have_wiph) R.wiph[c] = Wi->f64()[src + i];
+        }
+    }
+    host_restart_fill(&R, have_cons, have_wiph);
+    R.t = T->f64()[0];
+    if (Pl && Pl->dims.size() == 2 && Pl->dims[1] >= 5)
+        for (size_t p = 0; p < R.planets.size() && p < Pl->dims[0]; ++p) R.planets[p].phi = Pl->f64()[p * Pl->dims[1] + 4];
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (upload_run_state(h, run)) return -1;
+    if (disco_gpu_diagnostics(h, run, nullptr, nullptr, 1)) return -1;
+    return seed_dt(h);
 }
 
 int disco_gpu_final(disco_gpu_t* h, int* final_step) {

@@ -929,6 +929,39 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     return 0;
 }
 
+// Restart (Restart/h5in.c:101-368 then domain.c:163-241 and main.c:52-61)
+// of a run whose prims, face fluxes and mesh fronts were read from a
+// checkpoint: dphi from the fronts (calc_dp); without a stored conserved
+// state, prim2cons and cons2prim on the real cells, the first ghost fill and
+// set_wcell as the reference's restart does; with it (and the mesh speeds),
+// the stored state as it was.
+void host_restart_fill(HostRun* Rp, bool have_cons, bool have_wiph) {
+    HostRun& R = *Rp;
+    calc_dp(R);
+    if (!have_cons) {
+        double prim[8], cons[8];
+        for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
+            for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
+                const int jk = j + R.Nr * k;
+                for (int i = 0; i < R.Np[jk]; ++i) {
+                    const int c = R.off[jk] + i;
+                    double x[3], xp[3], xm[3];
+                    cell_x(R, j, k, c, x, xp, xm);
+                    const double dV = get_dV(xp, xm, R.par.phimax);
+                    for (int q = 0; q < R.nq; ++q) prim[q] = R.prim[(size_t)q * R.ncell + c];
+                    host_prim2cons(R, prim, cons, x, dV);
+                    for (int q = 0; q < R.nq; ++q) R.cons[(size_t)q * R.ncell + c] = cons[q];
+                    host_cons2prim(R, cons, prim, x, dV);
+                    for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
+                }
+            }
+        host_boundary(&R);
+    }
+    if (!have_wiph) set_wcell(R);
+    if (!have_cons) host_zcopy(R, true);
+    find_rotation(R);
+}
+
 // ---------------------------------------------------------------- tables
 // Annulus and run tables of a batch (device_state.h): neighbours, static
 // geometry, and the flags that mark which sweeps touch each annulus.

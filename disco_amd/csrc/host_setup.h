@@ -60,6 +60,9 @@ void host_boundary(HostRun* run);
 // from annulus jk1's, overlap-weighted, in the reference's loop order.
 void copy_distant(HostRun& R, int jk, int jk1);
 
+// Derived state of a run restarted from a checkpoint (see host_setup.cpp).
+void host_restart_fill(HostRun* R, bool have_cons, bool have_wiph);
+
 // Annulus/run tables of a batch and the launch shape of its substeps.  The
 // device stores annuli in tiles of `ktile` z-layers; cell_perm[r][c] is the
 // device position (within run r) of the run's cell c in the reference's

@@ -81,6 +81,28 @@ int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_
  * been taken). */
 int disco_gpu_final(disco_gpu_t* h, int* final_step);
 
+/* Replaces output() with the HDF5 module (Output/h5out.c:366-622) for run
+ * `run`: writes `path` with the reference's checkpoint datasets --
+ * Grid/T, Grid/r_jph[Nr+1], Grid/z_kph[Nz+1], Grid/Index, Grid/Np,
+ * Grid/Id_phi0 [Nz][Nr] (ghost annuli included), Data/Cells[Ntot][NUM_Q +
+ * NUM_FACES + 1] (prim, Phi, piph of every cell in theCells order),
+ * Data/Planets[Npl][7], Data/Diagnostics[Nz][Nr][8] (the averaged
+ * diagnostics; the accumulators are then zeroed, as output() does),
+ * Pars/<in.par key>, Opts/<module slot> -- plus Data/Cons[Ntot][NUM_Q] and
+ * Data/Wiph[Ntot], which make a restart exact.  The container is this
+ * library's (a flat list of named arrays, disco_amd/checkpoint.py reads it),
+ * not HDF5, which this build does not link. */
+int disco_gpu_write_checkpoint(disco_gpu_t* h, int run, const char* path);
+
+/* Replaces restart() (Restart/h5in.c:101-368) for run `run` of a handle
+ * created from the same in.par: the file's grid must equal the run's; prims,
+ * face fluxes, fronts, time and planet phases come from the file.  With
+ * Data/Cons and Data/Wiph present the run then continues bit for bit as the
+ * one that wrote the checkpoint; without them the conserved state, ghost
+ * cells and mesh speeds are rebuilt as the reference's restart does
+ * (prim2cons + cons2prim on real cells, boundary fill, set_wcell). */
+int disco_gpu_restart(disco_gpu_t* h, int run, const char* path);
+
 /* report (report.c:6-326) of run `run` at the current state: the values of
  * its report.dat line, in order: t, q = M_1/M_0 (0 with fewer than two
  * planets), the planets' Lg, M_acc, La, Ls, kin, therm, xMom, yMom (Npl each;
