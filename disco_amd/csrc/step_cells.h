@@ -47,7 +47,7 @@ struct PhysEuler {
     static constexpr int NF = 5;  // stored face-increment components
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
-    static constexpr bool SOURCE_PASS = false;  // sources come from source_cell's pass
+    static constexpr bool SOURCE_PASS = false;  // sources are added inline in update_a_cell
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         (void)dim;
@@ -743,22 +743,21 @@ DHD void cooling_cell(const DevState& S, const RunDev& Rn, int run, const double
 // sink.c:329-426 (damping): the conserved state relaxes toward the initial
 // condition's in the zones within Damp*_Len of the inner and outer radius
 // and of the upper and lower boundary (the lower zone measured from Z_Max,
-// as sink.c has it), at the rate of the last enabled zone (sink.c averages
-// the zones' rates but applies the last zone's factor), over the substep.
+// as sink.c has it), with the ramp of the last enabled zone, over the
+// substep.  sink.c also turns each zone's Damp*_Time (type 1: code units,
+// type 2: units of 1/omega_tot) into a rate that it never applies: the
+// factor is expm1(-dt x ramp) whatever the time and the type, and so it is
+// here.
 template <class Ph>
 DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double* prim, double* cons,
                       const double* x, double dV, double dt) {
     constexpr int NQ = Ph::NQ;
+    (void)S;
+    (void)run;
     const PhysParams& P = Rn.ph;
-    double omtot = 1.0;
-    if (P.damp_type[0] == 2 || P.damp_type[1] == 2 || P.damp_type[2] == 2 || P.damp_type[3] == 2)
-        omtot = omega_tot(S, Rn, run, x);
     double dampFactor = 0.0;
     for (int z = 0; z < 4; ++z) {
         if (P.damp_type[z] <= 0 || (z >= 2 && P.two_d)) continue;
-        double dampTime = P.damp_time[z];
-        if (P.damp_type[z] == 2) dampTime = dampTime / omtot;
-        (void)dampTime;  // sink.c folds it into a rate it does not apply
         const double theta = (z == 0 ? x[0] - P.rmin : z == 1 ? P.rmax - x[0] : P.zmax - x[2]) / P.damp_len[z];
         if (theta > 1.0)
             dampFactor = 0.0;

@@ -48,8 +48,9 @@ CASES = [
     ("c2e_nozzle.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Nozzle_Type=1, Nozzle_Par1=1.5,
                                                 Nozzle_Par2=0.4, Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
     # damping zones (sink.c:329-426) on the binary disk without density noise (the zones relax toward the
-    # initial condition at the moving cell centres): inner + outer (outer time in 1/omega_tot) in 2D,
-    # upper + lower in 3D
+    # initial condition at the moving cell centres): inner + outer in 2D, upper + lower in 3D.  The outer
+    # zone's type 2 and both Damp*_Time values change nothing: sink.c computes the zones' rate from them
+    # but applies only the ramp (the factor is expm1(-dt x ramp))
     ("c3_damp.npz", "c3_binary_disk.par", dict(Num_R=24, Init_Par1=0.0, DampInner_Type=1, DampInner_Time=1.0,
                                                 DampInner_Len=0.5, DampOuter_Type=2, DampOuter_Time=1.0,
                                                 DampOuter_Len=0.8)),
