@@ -75,6 +75,15 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// Two 4-element bf16 fragments as the 8-element operand of
+// v_mfma_f32_16x16x32_bf16 (slots 0-3 from lo, 4-7 from hi).
+MLPDEV bf16x8_t cat8(short4_t lo, short4_t hi) {
+    typedef short short8_t __attribute__((ext_vector_type(8)));
+    short8_t v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(bf16x8_t, v);
+}
 
 // float4 -> 4 x bf16 with the hardware's round-to-nearest-even convert
 // (v_cvt_pk_bf16_f32)
@@ -110,15 +119,22 @@ MLPDEV void mlp_load(MlpLds& L, const unsigned short* __restrict__ Wt_g, const f
 }
 
 // The network on the 64 states of one wavefront, transposed: each layer
-// computes Y^T = W^T X^T with v_mfma_f32_16x16x16_bf16 (FP32 accumulate).
-// Operand layout of that instruction: lane l supplies A[l%16][4(l/16)+t] and
-// B[4(l/16)+t][l%16] and receives D[4(l/16)+t][l%16], t = 0..3.  With A a
+// computes Y^T = W^T X^T on the matrix cores (FP32 accumulate).  Output
+// layout of the MFMA: lane l receives D[4(l/16)+t][l%16], t = 0..3; with A a
 // tile of W^T (output features x input features, from LDS) and B a tile of
-// X^T (input features x states), a lane receives output features 4(l/16)+t of
-// state l%16 -- exactly the B operand of the next layer, so activations stay
-// in registers: act[ts][kt] holds features 16kt + 4(l/16) + t of state
-// 16ts + l%16.  Only the input (one row of features per lane) and the output
-// (two numbers per state) pass through LDS, to change lane ownership.
+// X^T (input features x states), a lane receives output features 4(l/16)+t
+// of state l%16, and act[ts][kt] holds features 16kt + 4(l/16) + t of state
+// 16ts + l%16.  The input layer (8 features, padded to 16) runs
+// v_mfma_f32_16x16x16_bf16 with that fragment as its B operand.  The hidden
+// and output layers run gfx950's v_mfma_f32_16x16x32_bf16, whose A and B
+// operands carry 8 bf16 values per lane for the same K positions: the B
+// operand of K-tile kt is (act[ts][2kt], act[ts][2kt+1]) as the lane already
+// holds it, i.e. K positions paired with features 32kt + 4(l/16) + t and
+// 32kt + 16 + 4(l/16) + t, and the A operand reads the weights of exactly
+// those features (two 4-element LDS reads), so the contraction runs over the
+// right pairs whatever order the instruction assigns to K positions and no
+// activation crosses lanes.  Only the input (one row of features per lane)
+// and the output (two numbers per state) pass through LDS.
 MLPDEV void mlp_wave(const MlpLds& L, unsigned short* sF, float* sY, const float* fstd_in, float* out2) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
@@ -149,7 +165,7 @@ MLPDEV void mlp_wave(const MlpLds& L, unsigned short* sF, float* sY, const float
             nxt[ts][to] = pack4(silu4(acc));
         }
     }
-    // hidden layers 1..3: K = N = 64
+    // hidden layers 1..3: K = N = 64, two K-tiles of 32
     const int woff[3] = {WT_L1, WT_L2, WT_L3};
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
@@ -164,11 +180,14 @@ MLPDEV void mlp_wave(const MlpLds& L, unsigned short* sF, float* sY, const float
 #pragma unroll
             for (int ts = 0; ts < 4; ++ts) acc[ts] = (float4_t){bb[0], bb[1], bb[2], bb[3]};
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                short4_t a = *reinterpret_cast<const short4_t*>(L.W + woff[l] + (16 * to + r) * NH + 16 * kk + 4 * g);
+            for (int kt = 0; kt < 2; ++kt) {
+                const unsigned short* wr = L.W + woff[l] + (16 * to + r) * NH + 32 * kt + 4 * g;
+                const bf16x8_t a = cat8(*reinterpret_cast<const short4_t*>(wr),
+                                        *reinterpret_cast<const short4_t*>(wr + 16));
 #pragma unroll
                 for (int ts = 0; ts < 4; ++ts)
-                    acc[ts] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[ts][kk], acc[ts], 0, 0, 0);
+                    acc[ts] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cat8(act[ts][2 * kt], act[ts][2 * kt + 1]),
+                                                                      acc[ts], 0, 0, 0);
             }
 #pragma unroll
             for (int ts = 0; ts < 4; ++ts) nxt[ts][to] = pack4(silu4(acc[ts]));
@@ -181,11 +200,13 @@ MLPDEV void mlp_wave(const MlpLds& L, unsigned short* sF, float* sY, const float
 #pragma unroll
         for (int ts = 0; ts < 4; ++ts) acc[ts] = (float4_t){bb[0], bb[1], bb[2], bb[3]};
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            short4_t a = *reinterpret_cast<const short4_t*>(L.W + WT_L4 + r * NH + 16 * kk + 4 * g);
+        for (int kt = 0; kt < 2; ++kt) {
+            const unsigned short* wr = L.W + WT_L4 + r * NH + 32 * kt + 4 * g;
+            const bf16x8_t a = cat8(*reinterpret_cast<const short4_t*>(wr), *reinterpret_cast<const short4_t*>(wr + 16));
 #pragma unroll
             for (int ts = 0; ts < 4; ++ts)
-                acc[ts] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, nxt[ts][kk], acc[ts], 0, 0, 0);
+                acc[ts] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cat8(nxt[ts][2 * kt], nxt[ts][2 * kt + 1]),
+                                                                  acc[ts], 0, 0, 0);
         }
         if (g == 0) {
 #pragma unroll
