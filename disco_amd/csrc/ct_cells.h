@@ -522,14 +522,19 @@ DHD void flip_flag_cell(const DevState& S, int c, int has_z) {
     S.fflags[c] = f;
 }
 
-// One flip at face F (bfields.c:781-808), with the next face's slot taken
-// through its toggled pointer when a later face of the chain was already
-// flipped (the chain crossed the start of the track's face list).
+// One flip at face F (bfields.c:781-808).  tm, tf, tp: whether the faces
+// before F, F itself and the face after F have had their LRtype toggled by
+// the flips replayed so far; a toggled face's flux is read and written
+// through the slot of its other cell (get_phi_pointer after the toggle,
+// bfields.c:738-758).  The loads and stores keep the reference's order, so
+// slots that coincide through toggles behave as its pointers do.
 template <int D>
-DHD void flip_one(const DevState& S, FaceId F, bool next_toggled) {
+DHD void flip_one(const DevState& S, FaceId F, bool tm, bool tf, bool tp) {
     FaceId Fm = face_prev<D>(S, F);
     FaceId Fp = face_next<D>(S, F);
-    if (next_toggled) Fp = face_other<D>(S, Fp);
+    if (tm) Fm = face_other<D>(S, Fm);
+    if (tp) Fp = face_other<D>(S, Fp);
+    if (tf) F = face_other<D>(S, F);
     size_t sP = face_slot<D>(S, F), sPm = face_slot<D>(S, Fm), sPp = face_slot<D>(S, Fp);
     double* Ph = S.Phi;
     double* Rk = S.rkPhi;
@@ -543,84 +548,129 @@ DHD void flip_one(const DevState& S, FaceId F, bool next_toggled) {
     Rk[sPp] = Rk[sPp] + RK_Phi;
 }
 
+// Replays the flips of one chain part in list order, from its first flagged
+// face `first` to its last one, and returns the number of flips.  A flip
+// toggles its face and the face before it, so when flip n runs, the faces
+// toggled so far around it are: the face before n (if it was flagged, and n
+// is not the list's first face), and -- only at the end of the list, after
+// the start of the list was replayed -- the first two faces of the list.
+template <int D>
+DHD int flip_replay(const DevState& S, FaceId first, FaceId z0, int maxlen) {
+    FaceId M = first;
+    int len = 0;
+    for (;;) {
+        const FaceId m0 = face_prev<D>(S, M);
+        const FaceId p1 = face_next<D>(S, M);
+        const FaceId p2 = face_next<D>(S, p1);
+        const bool f_z0 = face_flag<D>(S, z0);
+        const bool done1 = same_face(p1, z0) && f_z0;  // the face after M: replayed already
+        const bool done2 = (same_face(p2, z0) && f_z0) || (same_face(p1, z0) && face_flag<D>(S, face_next<D>(S, z0)));
+        const bool tm = !same_face(M, z0) && face_flag<D>(S, m0);
+        flip_one<D>(S, M, tm, done1, done1 != done2);
+        if (++len > maxlen) {
+            err_or(S.err, 1);
+            return len;
+        }
+        // the next flagged face within two, not past the end of the list
+        if (same_face(p1, z0)) break;
+        if (face_flag<D>(S, p1)) {
+            M = p1;
+            continue;
+        }
+        if (same_face(p2, z0) || !face_flag<D>(S, p2)) break;
+        M = p2;
+    }
+    return len;
+}
+
 // flip_fluxes (bfields.c:760-814).  A flip reads and writes the fluxes of the
-// faces before and after it, so flips two faces apart interact and must run
-// in the reference's loop order (face list from its first face); flips at
-// least three faces apart are independent.  The thread of the first flip of
-// each chain replays the chain; a chain that runs across the start of the
-// face list is replayed by the thread of its first face after the start, its
-// part before the end of the list last.  When two neighbouring annuli hold
-// the same number of cells, rigid rotation moves every front of one past the
-// fronts of the other in the same step: every other face of the track flips
-// and the chain closes on itself.  Its head (the first flip after the start)
-// then replays the whole track in list order, the last flip reading the face
-// after it (the head's predecessor) through the pointer the head's flip
-// toggled.  Adjacent flipped faces cannot occur when every annulus rotates
-// rigidly and no front overtakes two others in a step; they are reported in
-// S.err instead of being replayed.
+// faces before and after it and toggles the LRtype of itself and the face
+// before it, so flips one or two faces apart interact and must run in the
+// reference's loop order (face list from its first face, the quadrant-I
+// start of faces_alt.c:57-79, i.e. the lower annulus' smallest front); flips
+// at least three faces apart are independent.  Each chain of interacting
+// flips is replayed by one thread: the thread of its first flagged face
+// counting from the start of the list.  A chain that runs over the end of
+// the list into its start belongs to the chain at the start: that thread
+// replays its own part, then the part at the end of the list (which sees the
+// start's toggles).  A chain that closes on itself around the whole track is
+// replayed from its first face to the end of the list.  This handles any
+// pattern of flags, adjacent ones included (per-cell mesh motion).
 template <int D>
 DHD void flip_chain_face(const DevState& S, FaceId F) {
-    const int MAXCHAIN = 1 << 16;
-    FaceId z0 = face_zero<D>(S, F);
-    FaceId n1 = face_next<D>(S, F);
-    if (face_flag<D>(S, n1)) {
-        err_or(S.err, 1);
-        return;
-    }
-    FaceId p1 = face_prev<D>(S, F);
-    FaceId p2 = face_prev<D>(S, p1);
-    bool cross_in = same_face(F, z0) || same_face(p1, z0);
-    bool pred = face_flag<D>(S, p2);
-    if (pred && !cross_in) return;  // not the first flip of its chain
-    bool wrap_head = pred && cross_in;
-    // forward part of the chain
+    const AnnDev& A = S.ann[S.cell_ann[F.cell]];
+    const AnnDev& B = S.ann[F.side == 0 ? Dir<D>::hi(A) : Dir<D>::lo(A)];
+    const int maxlen = A.np + B.np + 2;  // faces in the track
+    const FaceId z0 = face_zero<D>(S, F);
+    const FaceId b1 = face_prev<D>(S, F);
+    const FaceId b2 = face_prev<D>(S, b1);
+    // not the first flagged face of its chain from the start of the list
+    if (!same_face(F, z0) && face_flag<D>(S, b1)) return;
+    if (!same_face(F, z0) && !same_face(b1, z0) && face_flag<D>(S, b2)) return;
+    // the chain's last face before the end of the list, and whether the chain
+    // continues over the end into the start
     FaceId M = F;
-    int len = 1;
-    bool closed = false;
+    int len = 0;
     for (;;) {
-        FaceId m1 = face_next<D>(S, M);
-        FaceId m2 = face_next<D>(S, m1);
-        if (!face_flag<D>(S, m2)) break;
-        bool cross = same_face(m1, z0) || same_face(m2, z0);
-        if (cross) {
-            if (!wrap_head) return;  // continues into the list start: replayed by that chain's head
-            // m2 is this head (a flagged face at the start that is not the
-            // head would sit next to it): the chain goes all around the track
-            closed = true;
+        const FaceId m1 = face_next<D>(S, M), m2 = face_next<D>(S, m1);
+        if (same_face(m1, z0)) break;
+        if (face_flag<D>(S, m1)) {
+            M = m1;
+        } else if (!same_face(m2, z0) && face_flag<D>(S, m2)) {
+            M = m2;
+        } else {
             break;
         }
-        M = m2;
-        if (++len > MAXCHAIN) {
+        if (++len > maxlen) {
             err_or(S.err, 1);
             return;
         }
     }
-    // replay: forward part in order
-    M = F;
-    for (int k = 0; k < len; ++k) {
-        flip_one<D>(S, M, closed && k == len - 1);
-        M = face_next<D>(S, face_next<D>(S, M));
+    const FaceId m1 = face_next<D>(S, M), m2 = face_next<D>(S, m1);
+    FaceId c;  // the first flagged face of the list's start that M's chain reaches
+    bool over_end = false;
+    if (same_face(m1, z0)) {
+        if (face_flag<D>(S, z0)) {
+            c = z0;
+            over_end = true;
+        } else if (face_flag<D>(S, m2)) {
+            c = m2;
+            over_end = true;
+        }
+    } else if (same_face(m2, z0) && face_flag<D>(S, z0)) {
+        c = z0;
+        over_end = true;
     }
-    if (!wrap_head || closed) return;
-    // the part before the end of the list: from its first flip up to p2
-    FaceId T = p2;
-    int tl = 1;
+    if (over_end && !same_face(c, F)) return;  // the end part of a chain that starts at the list's start
+    flip_replay<D>(S, F, z0, maxlen);
+    if (over_end) return;  // closed around the track: replayed to the end of the list
+    // the end of the list holds the rest of this chain when its last faces
+    // reach F (which is then the list's first or second face)
+    const FaceId e1 = face_prev<D>(S, z0), e2 = face_prev<D>(S, e1);
+    const bool f_first = same_face(F, z0), f_second = same_face(b1, z0);
+    FaceId E;
+    if (face_flag<D>(S, e1) && (f_first || f_second))
+        E = e1;
+    else if (face_flag<D>(S, e2) && f_first)
+        E = e2;
+    else
+        return;
+    // back to the first face of that end part
+    len = 0;
     for (;;) {
-        FaceId q1 = face_prev<D>(S, T);
-        FaceId q2 = face_prev<D>(S, q1);
-        if (!face_flag<D>(S, q2) || same_face(T, z0) || same_face(q1, z0)) break;
-        T = q2;
-        if (++tl > MAXCHAIN) {
+        const FaceId q1 = face_prev<D>(S, E), q2 = face_prev<D>(S, q1);
+        if (face_flag<D>(S, q1))
+            E = q1;
+        else if (face_flag<D>(S, q2))
+            E = q2;
+        else
+            break;
+        if (++len > maxlen) {
             err_or(S.err, 1);
             return;
         }
     }
-    for (int k = 0; k < tl; ++k) {
-        // the last one (p2) reads the face after it through the pointer that
-        // the head's flip toggled
-        flip_one<D>(S, T, k == tl - 1);
-        T = face_next<D>(S, face_next<D>(S, T));
-    }
+    flip_replay<D>(S, E, z0, maxlen);
 }
 
 template <int D>

@@ -257,8 +257,8 @@ static int unsupported(const disco_params& p, std::string* err) {
         // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
         if (!p.CT) m += " mhd/grmhd need Constrained_Transport 1";
         if (p.Num_Z < 2) m += " mhd/grmhd with CT_MODE 2 need Num_Z > 1";
-        if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
-            m += " mhd/grmhd need rigidly rotating annuli (Mesh_Motion 0 or 4) for their flux flips";
+        if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4 && !(p.hydro == DISCO_HYDRO_MHD && p.Mesh_Motion == 1))
+            m += " Mesh_Motion (mhd: 0, 1 or 4; grmhd: 0 or 4)";
     }
     if (p.hydro == DISCO_HYDRO_GRMHD) {
         // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one

@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-full | --grmhd | --t-end | --report | --bc | --cool]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-mm1 | --mhd-50 | --mhd-full | --grmhd | --t-end | --report | --bc | --cool]   (only the named set)
 """
 from __future__ import annotations
 
@@ -237,6 +237,33 @@ def fixtures_mhd() -> None:
         save_states(name, {s: st for s, st in states.items() if s > 0})
 
 
+def fixtures_mhd_mm1() -> None:
+    # the small MHD disk with per-face mesh speeds (Mesh_Motion 1, misc.c:
+    # 101-136): the fronts of one annulus no longer move rigidly
+    txt = par_with("c2_mhd3d.par", Num_R=12, Num_Z=4, CFL=0.3, Mesh_Motion=1)
+    states, _ = run_ref("mhd_sun", txt, 20, dumps=(1, 5))
+    save_states("c2_mhd_small_mm1.npz", states)
+
+
+def fixtures_mhd_50() -> None:
+    # full-size 3D MHD disk for the configuration's 50 steps: conserved totals
+    # and dt after every step (the driver's DISCO_REF_TOTALS lines; no dumps)
+    exe = os.path.join(ROOT, "oracle", "_ref", "disco_ref_mhd_sun")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "in.par"), "w").write(par_with("c2_mhd3d.par"))
+        out = subprocess.run([exe, "50", "-"], cwd=d, check=True, capture_output=True, text=True, timeout=7200,
+                             env={**os.environ, "DISCO_REF_TOTALS": "1"}).stdout
+    save_totals_50(out)
+
+
+def save_totals_50(out: str) -> None:
+    rows = [ln.split()[1:] for ln in out.splitlines() if ln.startswith("tot ")]
+    steps = np.array([int(r[0]) for r in rows])
+    assert steps.tolist() == list(range(51)), steps
+    np.savez(os.path.join(HERE, "c2_full_totals_50.npz"), steps=steps,
+             dt=np.array([float(r[1]) for r in rows]), totals=np.array([[float(v) for v in r[2:]] for r in rows]))
+
+
 def fixtures_mhd_full() -> None:
     # full-size 3D MHD disk (5.5 M cells): conserved totals and dt after 1 and
     # 2 steps (about 90 s of reference CPU time and 3.6 GB of temporary dumps)
@@ -347,6 +374,14 @@ def main() -> None:
         return
     if "--mhd" in sys.argv:
         fixtures_mhd()
+        print("fixtures written")
+        return
+    if "--mhd-mm1" in sys.argv:
+        fixtures_mhd_mm1()
+        print("fixtures written")
+        return
+    if "--mhd-50" in sys.argv:
+        fixtures_mhd_50()
         print("fixtures written")
         return
     if "--mhd-full" in sys.argv:

@@ -44,6 +44,9 @@ def _phi_errors(st, ref, real):
 SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            ("c2_mhd_small_hllc.npz", {"Riemann_Solver": 1}),
            ("c2_mhd_small_hll.npz", {"Riemann_Solver": 0}),
+           # per-face mesh speeds (Mesh_Motion 1): fronts of one annulus move apart, so flux flips
+           # can sit next to each other in a track
+           ("c2_mhd_small_mm1.npz", {"Mesh_Motion": 1}),
            # outflow ghosts in z (Boundary/outflow3d.c) instead of the periodic copy
            ("c2_mhd_outflow.npz", {"Z_Periodic": 0, "Boundary": "outflow3d"}),
            # sink.c's cooling (relax, beta 50), outer damping zone and nozzle ball at r = 2 together (velocity
@@ -53,7 +56,7 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
                                          Nozzle_Par3=0.7, Nozzle_Par4=20.0))]
 
 
-@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "outflow", "sinkterms"])
+@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver

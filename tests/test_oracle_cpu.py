@@ -246,7 +246,7 @@ def test_unsupported_configuration_is_rejected():
 @pytest.mark.parametrize("over,msg", [
     (dict(Riemann_Solver=2), "HLLD with grmhd"),
     (dict(Metric_Par3=1.2), "Kerr needs"),
-    (dict(Mesh_Motion=1), "rigidly rotating"),
+    (dict(Mesh_Motion=1), "grmhd: 0 or 4"),
     (dict(Constrained_Transport=0), "Constrained_Transport"),
 ])
 def test_unsupported_grmhd_configuration_is_rejected(over, msg):
