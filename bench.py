@@ -119,15 +119,17 @@ def bench_fv(workload: str, steps: int, warmup: int, tune: bool = True) -> dict:
     # measured HBM bytes of one substep, from the rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs of this build (calibrated on a copy kernel; see the
     # summary the file names)
-    traffic = None
+    traffic = traffic_src = None
     tf = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_substep")
+            t = json.load(f)
+        traffic = t.get("bytes_per_substep")
+        traffic_src = f"{t.get('source')} (counters of the build at commit {t.get('commit', '?')})"
     return {
         "value": value, "ms_per_step": ms / steps, "wall_s_timed": wall, "real_cells": nreal,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"one RK substep ({' + '.join(sub)}; flip kernels every other substep)",
                      "bytes_per_zone_substep": bpz, "substep_ms": substep_ms, "dominant_kernel": dominant},
         "kernels_ms": {k: v[0] for k, v in kt.items()},
@@ -211,9 +213,15 @@ def bench_batch(steps: int, warmup: int) -> dict:
     for d in singles:
         d.close()
     sequential = cells * steps / seq_s
+    per_step = {"k_dt_final", "k_planets", "k_diag", "k_diag_time"}
+    substep_ms = sum(v[0] for k, v in kt.items() if k not in per_step and not k.startswith(("k_mindt", "k_dt_init")))
+    achieved = BYTES_PER_ZONE_SUBSTEP["euler"] * cells / (substep_ms * 1e-3) / 1e9
     return {"workload": "binary_batch_x48", "runs": 48, "real_cells": cells, "steps": steps,
             "value": batched, "unit": "zone-updates/s", "ms_per_step": ms / steps,
             "sequential_value": sequential, "speedup_vs_sequential": batched / sequential,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_zone_substep": BYTES_PER_ZONE_SUBSTEP["euler"],
+                         "substep_ms": substep_ms},
             "kernels_ms": {k: v[0] for k, v in kt.items()}, "autotune": tuned}
 
 
@@ -298,6 +306,8 @@ def main() -> int:
     ap.add_argument("--c2p-states", type=int, default=1 << 24)
     ap.add_argument("--c2p-train-steps", type=int, default=2000)
     ap.add_argument("--progress", action="store_true", help="a line on stderr after each workload")
+    ap.add_argument("--only", choices=["batch", "c2p"], default=None,
+                    help="run only the 48-disk batch or the standalone cons2prim leg (profiling)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -321,6 +331,12 @@ def main() -> int:
         if args.progress:
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.only == "batch":
+        print(json.dumps({"batch": bench_batch(args.steps, args.warmup)}))
+        return 0
+    if args.only == "c2p":
+        print(json.dumps({"c2p": bench_c2p(args.c2p_states, args.c2p_train_steps)}))
+        return 0
     wl = args.workload
     r = bench_fv(wl, args.steps, args.warmup)
     note(f"{wl}: {r['value']:.4g} zone-updates/s")
