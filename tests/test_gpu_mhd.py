@@ -158,6 +158,54 @@ def test_mhd_full_size_totals_and_divb():
     dom.close()
 
 
+def test_mhd_full_size_50_steps_totals_dt_and_divb():
+    """The benchmark configuration's 50 steps at full size (Nr=128, Nz=64,
+    5.5 M cells) against the reference's own full-size run
+    (tests/golden/c2_full_totals_50.npz, its conserved totals and dt after
+    every step): dt to 1e-12 relative every step; the totals of mass, energy
+    and angular momentum to 1e-12 relative every step, the near-zero totals
+    (radial and vertical momentum, field) to 1e-12 of the momentum / field
+    scale (measured: 5e-15 at worst over the 50 steps); and every real
+    cell's net face flux at round-off after the 50th step."""
+    import ctypes as C
+    import os
+
+    import dumpio
+
+    z = np.load(os.path.join(dumpio.GOLDEN, "c2_full_totals_50.npz"))
+    dom = D.Domain(params("c2_mhd3d.par"))
+    I = dom.info[0]
+    g = dom.grid(0)
+    real = real_mask(I.Nr, I.Nz, I.NgRa, I.NgRb, I.NgZa, I.NgZb, g["Np"])
+    cons = np.zeros(I.nq * I.ncell)
+    worst = 0.0
+    for s in range(51):
+        if s:
+            dom.step(1)
+            dom.sync()
+            dt = dom.time()["dt"][0]
+            assert abs(dt - z["dt"][s]) <= 1e-12 * z["dt"][s], (s, dt, z["dt"][s])
+        # the conserved state only (download's second array)
+        assert D.lib().disco_gpu_download(dom._h, 0, None, cons.ctypes.data_as(C.POINTER(C.c_double)), None, None,
+                                          None) == 0
+        tot = cons.reshape(I.nq, I.ncell)[:, real].sum(axis=1)
+        ref = z["totals"][s]
+        mscale = abs(ref[3]) / 2.5
+        bscale = abs(ref[6]) + abs(ref[7])
+        tol = 1e-12
+        e = max(max(abs(tot[q] - ref[q]) / abs(ref[q]) for q in (0, 1, 3)),
+                max(abs(tot[q] - ref[q]) / mscale for q in (2, 4)),
+                max(abs(tot[q] - ref[q]) / bscale for q in (5, 6, 7)))
+        worst = max(worst, e)
+        assert e <= tol, (s, e, tot, ref)
+    st = dom.state(0)
+    div, sc = div_b(st["Phi"], st["piph"], g["Np"], I.Nr, I.Nz)
+    dworst = np.abs(div[real] / sc[real]).max()
+    print(f"50 full-size steps: worst totals error {worst:.2e}, div B {dworst:.2e}", flush=True)
+    assert dworst < 1e-12
+    dom.close()
+
+
 def test_autotune_keeps_state_and_results():
     """The autotuner times every annulus tile x block size on the run itself,
     keeps the fastest and restores the state: the tuned run then produces the
