@@ -6,7 +6,7 @@ versions of the benchmark disks and stores full-precision states at fixed
 step counts, plus conserved totals of the full-size 2D disk.  Needs the
 reference sources (this container only); the fixtures are committed.
 
-usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-mm1 | --mhd-50 | --mhd-full | --grmhd | --t-end | --report | --bc | --cool]   (only the named set)
+usage: python tests/golden/make_fixtures.py [--small | --leaf | --3d | --mhd | --mhd-mm1 | --mhd-50 | --sink-z | --mhd-full | --grmhd | --t-end | --report | --bc | --cool]   (only the named set)
 """
 from __future__ import annotations
 
@@ -201,6 +201,25 @@ BC_CASES = [  # (fixture, overrides of c1_kepler2d.par, reference variant, steps
 ]
 
 
+# the damping zones and the nozzle in 3D with outflow z-ghosts instead of the
+# periodic copy: there the z-ghosts come from the reference's own boundary
+# module (outflow3d.c), not from the harness's serial exchange
+SINK_Z_CASES = [
+    ("c3z_damp_outflow.npz", "euler_binary_outflow3d", "c3_binary_disk.par",
+     dict(Num_R=16, Num_Z=6, Boundary="outflow3d", Init_Par1=0.0, DampUpper_Type=1, DampUpper_Time=1.0,
+          DampUpper_Len=0.3, DampLower_Type=1, DampLower_Time=1.0, DampLower_Len=0.3), 5),
+    ("c2e_nozzle_outflow.npz", "euler_sun_outflow3d", "c1_kepler2d.par",
+     dict(Num_R=12, Num_Z=6, Boundary="outflow3d", Nozzle_Type=1, Nozzle_Par1=1.5, Nozzle_Par2=0.4,
+          Nozzle_Par3=0.8, Nozzle_Par4=10.0), 5),
+]
+
+
+def fixtures_sink_z() -> None:
+    for name, variant, base, over, n in SINK_Z_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=(1,))
+        save_states(name, states)
+
+
 def fixtures_boundaries() -> None:
     """The small disks with zero-gradient (Boundary/zerograd.c) and outflow
     (Boundary/outflow3d.c: zero gradient, inflow velocity cut) ghosts, in 2D
@@ -374,6 +393,10 @@ def main() -> None:
         return
     if "--mhd" in sys.argv:
         fixtures_mhd()
+        print("fixtures written")
+        return
+    if "--sink-z" in sys.argv:
+        fixtures_sink_z()
         print("fixtures written")
         return
     if "--mhd-mm1" in sys.argv:

@@ -57,6 +57,14 @@ CASES = [
     ("c3z_damp.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Init_Par1=0.0,
                                                  DampUpper_Type=1, DampUpper_Time=1.0, DampUpper_Len=0.3,
                                                  DampLower_Type=1, DampLower_Time=1.0, DampLower_Len=0.3)),
+    # the same z zones and the 3D nozzle with outflow z-ghosts: the reference's own boundary module fills
+    # the ghost layers there (no serial exchange involved)
+    ("c3z_damp_outflow.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Boundary="outflow3d", Init_Par1=0.0,
+                                                         DampUpper_Type=1, DampUpper_Time=1.0, DampUpper_Len=0.3,
+                                                         DampLower_Type=1, DampLower_Time=1.0, DampLower_Len=0.3)),
+    ("c2e_nozzle_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d", Nozzle_Type=1,
+                                                        Nozzle_Par1=1.5, Nozzle_Par2=0.4, Nozzle_Par3=0.8,
+                                                        Nozzle_Par4=10.0)),
 ]
 
 
