@@ -15,6 +15,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import re
@@ -59,6 +60,19 @@ CPU_SAMPLES = {
                                 "8 midplane z-layers of the 192 (same dz, fixed z boundaries), spin a = 0 "
                                 "(the reference's metric is Schwarzschild), 486,912 real cells, 1 RK2 step"),
 }
+
+
+def kernel_sources_digest() -> str:
+    """SHA-256 over the device and C-ABI sources (disco_amd/csrc, include):
+    the build a set of rocprofv3 counters belongs to."""
+    h = hashlib.sha256()
+    for d in ("disco_amd/csrc", "include"):
+        for name in sorted(os.listdir(os.path.join(ROOT, d))):
+            if name.endswith((".hip", ".h", ".cpp")):
+                h.update(name.encode())
+                with open(os.path.join(ROOT, d, name), "rb") as f:
+                    h.update(f.read())
+    return h.hexdigest()
 
 
 def par_text(base: str, **kv) -> str:
@@ -117,15 +131,19 @@ def bench_fv(workload: str, steps: int, warmup: int, tune: bool = True) -> dict:
     bpz = BYTES_PER_ZONE_SUBSTEP[hydro]
     achieved = bpz * nreal / (substep_ms * 1e-3) / 1e9
     # measured HBM bytes of one substep, from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs of this build (calibrated on a copy kernel; see the
-    # summary the file names)
+    # WRITE_SIZE runs (calibrated on a copy kernel; the summary the file names
+    # holds the commands).  Reported only while the kernel sources are the
+    # ones those counters were taken on: any change to them drops the figure.
     traffic = traffic_src = None
     tf = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        traffic = t.get("bytes_per_substep")
-        traffic_src = f"{t.get('source')} (counters of the build at commit {t.get('commit', '?')})"
+        if t.get("sources_sha256") == kernel_sources_digest():
+            traffic = t.get("bytes_per_substep")
+            traffic_src = f"{t.get('source')} (counters of this build's sources, commit {t.get('commit', '?')})"
+        else:
+            traffic_src = f"stale: {t.get('source')} holds counters of other kernel sources (commit {t.get('commit', '?')})"
     return {
         "value": value, "ms_per_step": ms / steps, "wall_s_timed": wall, "real_cells": nreal,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
