@@ -118,7 +118,10 @@ struct disco_gpu {
     bool counting = false;
 };
 
-static DevState make_state(disco_gpu* h, int substep) {
+// The kernels' view of the state with the mesh of buffer `cur` at the start
+// of the substep; a substep that moves the mesh writes it into buffer `nxt`
+// (nxt == cur: the mesh stays where it is).
+static DevState make_state(disco_gpu* h, int cur, int nxt) {
     DevState S{};
     S.ann = h->ann;
     S.cell_ann = h->cell_ann;
@@ -131,22 +134,12 @@ static DevState make_state(disco_gpu* h, int substep) {
     S.gradp = h->gradp;
     S.gradr = h->gradr;
     S.gradz = h->gradz;
-    int p = h->parity;
-    if (substep == 0) {
-        S.piph = h->piph[p];
-        S.dphi = h->dphi[p];
-        S.piph_n = h->piph[1 - p];
-        S.dphi_n = h->dphi[1 - p];
-        S.rot = h->rot[p];
-        S.rot_n = h->rot[1 - p];
-    } else {
-        S.piph = h->piph[1 - p];
-        S.dphi = h->dphi[1 - p];
-        S.piph_n = h->piph[1 - p];
-        S.dphi_n = h->dphi[1 - p];
-        S.rot = h->rot[1 - p];
-        S.rot_n = h->rot[1 - p];
-    }
+    S.piph = h->piph[cur];
+    S.dphi = h->dphi[cur];
+    S.piph_n = h->piph[nxt];
+    S.dphi_n = h->dphi[nxt];
+    S.rot = h->rot[cur];
+    S.rot_n = h->rot[nxt];
     S.wiph = h->wiph;
     S.bcjobs = h->bcjobs;
     S.lbo = h->lbo;
@@ -184,26 +177,57 @@ static DevState make_state(disco_gpu* h, int substep) {
     return S;
 }
 
-// One step for the current parity: [dt from the running CFL minimum] +
-// onestep(0, dt, first) + onestep(1/2, dt/2, last) (timestep.c:45-48).
+// One step for the current parity (timestep.c:7-54): [dt from the running
+// CFL minimum] + the substeps of the run's Timestep, then add_diagnostics.
+//   Timestep 0, 2 (and any other value but 1, 3): onestep(0, dt, first) +
+//     onestep(1/2, dt/2, last), RK2 (timestep.c:45-48);
+//   Timestep 3: onestep(0, dt, first) + onestep(3/4, dt/4) +
+//     onestep(1/3, 2dt/3, last), the three-stage SSP Runge-Kutta (:36-43);
+//   Timestep 1: onestep(0, dt, first, last) and then RK2 as above: timestep.c
+//     tests 3 with an if of its own, so its else-branch runs after the Euler
+//     substep (:29-48).  The mesh moves twice and t advances by 2 dt.
+// The RK copy of the conserved state and face fluxes (timestep.c:18-24) is
+// taken by the step's first substep (first == 1); the later substeps that
+// move the mesh pass first == 2.  The mesh moves from buffer `parity` to the
+// other one in a moving substep; step_parity_flip says whether a step ends
+// with it in the other buffer.
 // Invariant between steps: dtbits holds the CFL minimum over the current
 // state (the last substep's update computes it; create/upload seed it).
 // fk: run the independent branches of each substep on two streams (graph
 // capture); null: every kernel in order on the handle's stream.
 static void enqueue_step(disco_gpu* h, bool with_dt, const LaunchHook* hook, const Fork* fk = nullptr) {
-    DevState S0 = make_state(h, 0);
-    DevState S1 = make_state(h, 1);
+    const int p = h->parity, q = 1 - p;
+    const StepShape& sh = h->shape;
+    DevState S0 = make_state(h, p, q);
     if (with_dt) launch_dt_final(S0, 1, h->stream, hook);
-    launch_substep(h->shape, S0, 0.0, 1.0, 1, 0, h->stream, hook, fk);
-    launch_substep(h->shape, S1, 0.5, 0.5, 0, 1, h->stream, hook, fk);
-    launch_diag(S1, h->nann, h->nq, h->stream, hook);
+    launch_substep(sh, S0, 0.0, 1.0, 1, 0, h->stream, hook, fk);
+    if (sh.stepper == 3) {
+        DevState S1 = make_state(h, q, q);
+        launch_substep(sh, S1, 0.75, 0.25, 0, 0, h->stream, hook, fk);
+        launch_substep(sh, S1, 1.0 / 3.0, 2.0 / 3.0, 0, 1, h->stream, hook, fk);
+        launch_diag(S1, h->nann, h->nq, h->stream, hook);
+    } else if (sh.stepper == 1) {
+        DevState S1 = make_state(h, q, p);
+        launch_substep(sh, S1, 0.0, 1.0, 2, 0, h->stream, hook, fk);
+        DevState S2 = make_state(h, p, p);
+        launch_substep(sh, S2, 0.5, 0.5, 0, 1, h->stream, hook, fk);
+        launch_diag(S2, h->nann, h->nq, h->stream, hook);
+    } else {
+        DevState S1 = make_state(h, q, q);
+        launch_substep(sh, S1, 0.5, 0.5, 0, 1, h->stream, hook, fk);
+        launch_diag(S1, h->nann, h->nq, h->stream, hook);
+    }
 }
+
+// 1 when a step leaves the mesh in the other buffer (one moving substep), 0
+// for Timestep 1 (two)
+static int step_parity_flip(const disco_gpu* h) { return h->shape.stepper == 1 ? 0 : 1; }
 
 // Recomputes the overlap-face partners and the CFL minimum over the current
 // state (after create/upload: the steps search partners only after the mesh
 // moved).
 static int seed_dt(disco_gpu* h) {
-    DevState S = make_state(h, 0);
+    DevState S = make_state(h, h->parity, 1 - h->parity);
     launch_partners(S, h->stream);
     launch_mindt(h->shape, S, h->stream, nullptr);
     HIPCHK(hipGetLastError());
@@ -554,6 +578,10 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
             delete h;
             return fail("all runs of a batch must use the same Hydro module");
         }
+        if (stepper_of(h->runs[r].par.Timestep) != stepper_of(h->runs[0].par.Timestep)) {
+            delete h;
+            return fail("all runs of a batch must use the same Timestep scheme");
+        }
     }
     h->nq = h->runs[0].nq;
     // constrained-transport buffers: ideal MHD and GRMHD
@@ -705,7 +733,7 @@ int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kp
 
 int disco_gpu_mindt(disco_gpu_t* h, double* dt_out) {
     if (!h) return fail("disco_gpu_mindt: null handle");
-    DevState S = make_state(h, 0);
+    DevState S = make_state(h, h->parity, 1 - h->parity);
     launch_mindt(h->shape, S, h->stream, nullptr);
     launch_dt_final(S, 0, h->stream, nullptr);
     HIPCHK(hipGetLastError());
@@ -720,7 +748,7 @@ int disco_gpu_step(disco_gpu_t* h, int nsteps) {
     if (!h || nsteps < 0) return fail("disco_gpu_step: bad arguments");
     for (int n = 0; n < nsteps; ++n) {
         HIPCHK(hipGraphLaunch(h->graph[h->parity], h->stream));
-        h->parity ^= 1;
+        h->parity ^= step_parity_flip(h);
     }
     return 0;
 }
@@ -782,7 +810,7 @@ int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_
                 }
             }
             HIPCHK(hipGraphLaunch(h->graph[h->parity], h->stream));
-            h->parity ^= 1;
+            h->parity ^= step_parity_flip(h);
             ++done;
             if (stop) break;
         }
@@ -812,7 +840,7 @@ int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) 
     double* dpart = nullptr;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMalloc((void**)&dpart, part.size() * sizeof(double)));
-    DevState S = make_state(h, 0);
+    DevState S = make_state(h, h->parity, 1 - h->parity);
     launch_report(S, a0, na, dpart, h->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(part.data(), dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -859,7 +887,7 @@ int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int
     // every slot of the run's Nr x Nz annuli is written by k_report (zeros
     // for ghost annuli) and every total by k_report_sum: nothing left from an
     // earlier call is read
-    DevState S = make_state(h, 0);
+    DevState S = make_state(h, h->parity, 1 - h->parity);
     launch_report(S, a0, na, part, h->stream);
     launch_report_sum(part, R.Nr, R.Nz, sums, h->stream);
     hipError_t e = hipGetLastError();
@@ -1075,7 +1103,7 @@ int disco_gpu_timestep(disco_gpu_t* h, const double* dt) {
     enqueue_step(h, false, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->parity ^= 1;
+    h->parity ^= step_parity_flip(h);
     return 0;
 }
 
@@ -1182,7 +1210,7 @@ int disco_gpu_kernel_times(disco_gpu_t* h, int nsteps, int maxk, char* names, do
     LaunchHook hook{timer_before, timer_after, &T};
     for (int n = 0; n < nsteps; ++n) {
         enqueue_step(h, true, &hook);
-        h->parity ^= 1;
+        h->parity ^= step_parity_flip(h);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -477,11 +477,12 @@ DHD void ct_update_cell(const DevState& S, int c, double RK, double dtfac, int f
     const size_t N = S.ncell;
     const double dt = S.dt[A.run] * dtfac;
     // the face fluxes at the substep's start, every cell: timestep.c:22's copy
-    // in the first substep, adjust_RK_cons's average (misc.c:239-241) in the
-    // second; no kernel reads them between the stage and here
+    // in the step's first substep (first == 1), adjust_RK_cons's average
+    // (misc.c:239-241) in the others; no kernel reads them between the stage
+    // and here
     _Pragma("unroll") for (int f = 0; f < 5; ++f) {
         const double p = S.Phi[f * N + c];
-        if (first)
+        if (first == 1)
             st_stream(&S.rkPhi[f * N + c], p);
         else
             S.Phi[f * N + c] = (1. - RK) * p + RK * S.rkPhi[f * N + c];

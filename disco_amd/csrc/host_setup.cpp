@@ -247,6 +247,10 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     disco_ic_from_params(&par, ic);
 }
 
+// timestep.c:29-49: Timestep 3 runs the three-stage Runge-Kutta, 1 a forward
+// Euler substep followed by RK2, every other value RK2
+int stepper_of(int Timestep) { return Timestep == 1 ? 1 : Timestep == 3 ? 3 : 2; }
+
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
@@ -275,7 +279,6 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED)
         m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
-    if (p.Timestep != 0) m += " Timestep!=0 (RK2 only)";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
     if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
         m += " Riemann_Solver";
@@ -1155,6 +1158,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         shape->bc_nlev = l + 1;
     }
     shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
+    shape->stepper = runs.empty() ? 2 : stepper_of(runs[0].par.Timestep);
     shape->has_z = 0;
     shape->zcopy = 0;
     shape->sink = 0;

@@ -60,6 +60,9 @@ void host_boundary(HostRun* run);
 // from annulus jk1's, overlap-weighted, in the reference's loop order.
 void copy_distant(HostRun& R, int jk, int jk1);
 
+// StepShape::stepper of a Timestep value (1, 3, else 2).
+int stepper_of(int Timestep);
+
 // Derived state of a run restarted from a checkpoint (see host_setup.cpp).
 void host_restart_fill(HostRun* R, bool have_cons, bool have_wiph);
 

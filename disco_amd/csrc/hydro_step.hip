@@ -504,9 +504,10 @@ struct Launch {
         // GRMHD's sources read only the substep's prims and mesh
         if (Ph::MHD && sh.ct && Ph::SOURCE_PASS)
             DISCO_LAUNCH(hook, "k_source", k_source<Ph>, G(KC_CT), dim3(B(KC_CT)), su, S, dtfac);
-        // partners: the first substep's are the last second substep's (the
-        // mesh has not moved since)
-        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, !first);
+        // partners: the step's first substep (first == 1) keeps the last
+        // substep's (the mesh has not moved since); every later substep
+        // searches (a moving substep before it moved the mesh)
+        DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, first != 1);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
         DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         if (Ph::MHD && sh.ct) {

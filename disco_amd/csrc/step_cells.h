@@ -806,7 +806,8 @@ DHD void nozzle_cell(const PhysParams& P, double* cons, const double* xp, const 
 }
 
 // First half of the update: the substep's start state (timestep.c:18-24's
-// copy in the first substep, adjust_RK_cons's average in the second,
+// copy in the step's first substep (first == 1), adjust_RK_cons's average in
+// the others,
 // misc.c:226-244), the conservative update from the gathered face increments
 // (riemann.c:314-317), add_source (misc.c:550-593) and, in the first substep,
 // move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns the updated
@@ -832,7 +833,7 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
         double prim[NQ];
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             const double u = S.cons[q * N + c];
-            if (first) {
+            if (first == 1) {
                 st_stream(&S.rkcons[q * N + c], u);
                 cons[q] = u;
             } else {

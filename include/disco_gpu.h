@@ -58,8 +58,10 @@ int disco_gpu_grid(disco_gpu_t* h, int run, int* Np, double* r_jph, double* z_kp
 int disco_gpu_mindt(disco_gpu_t* h, double* dt_out);
 
 /* One iteration of the reference main loop without output: getmindt,
- * check_dt's T_End clip (domain.c:283-292) and timestep (timestep.c:7-54,
- * RK2 = onestep(0, dt, first) + onestep(1/2, dt/2, last)), `nsteps` times,
+ * check_dt's T_End clip (domain.c:283-292) and timestep (timestep.c:7-54:
+ * RK2 = onestep(0, dt, first) + onestep(1/2, dt/2, last); Timestep 3 the
+ * three-stage Runge-Kutta; Timestep 1 a forward Euler substep followed by
+ * RK2, as timestep.c runs it), `nsteps` times,
  * for every run, dt kept on the device.  A run whose step to T_End has been
  * taken holds (dt = 0: state, mesh and time unchanged) while the others of
  * the batch go on.  Launched as a captured hipGraph per step; asynchronous

@@ -264,6 +264,21 @@ def fixtures_mhd_mm1() -> None:
     save_states("c2_mhd_small_mm1.npz", states)
 
 
+STEPPER_CASES = [  # timestep.c:29-49: Timestep 3 (three-stage SSP Runge-Kutta), Timestep 1 (forward Euler, then RK2)
+    ("c1_rk3.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=3), "euler_sun", 10, (1,)),
+    ("c1_fe.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=1), "euler_sun", 10, (1,)),
+    ("c3_fe.npz", "c3_binary_disk.par", dict(Num_R=24, Timestep=1), "euler_binary", 10, (1,)),
+    ("c2_mhd_small_rk3.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Timestep=3), "mhd_sun", 5, (1,)),
+    ("c2_mhd_small_fe.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Timestep=1), "mhd_sun", 5, (1,)),
+]
+
+
+def fixtures_steppers() -> None:
+    for name, base, over, variant, n, dumps in STEPPER_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_mhd_50() -> None:
     # full-size 3D MHD disk for the configuration's 50 steps: conserved totals
     # and dt after every step (the driver's DISCO_REF_TOTALS lines; no dumps)
@@ -397,6 +412,10 @@ def main() -> None:
         return
     if "--sink-z" in sys.argv:
         fixtures_sink_z()
+        print("fixtures written")
+        return
+    if "--steppers" in sys.argv:
+        fixtures_steppers()
         print("fixtures written")
         return
     if "--mhd-mm1" in sys.argv:

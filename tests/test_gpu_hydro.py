@@ -65,6 +65,11 @@ CASES = [
     ("c2e_nozzle_outflow.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Boundary="outflow3d", Nozzle_Type=1,
                                                         Nozzle_Par1=1.5, Nozzle_Par2=0.4, Nozzle_Par3=0.8,
                                                         Nozzle_Par4=10.0)),
+    # the other steppers of timestep.c:29-49: the three-stage Runge-Kutta, and a forward Euler substep
+    # followed by RK2 (the mesh and the binary move twice, t advances by 2 dt)
+    ("c1_rk3.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=3)),
+    ("c1_fe.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=1)),
+    ("c3_fe.npz", "c3_binary_disk.par", dict(Num_R=24, Timestep=1)),
 ]
 
 

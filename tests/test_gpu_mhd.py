@@ -53,10 +53,15 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            # noise off: the damping target is the initial condition at the moving centre)
            ("c2_mhd_sinkterms.npz", dict(Init_Par5=0.0, Cool_Type=2, Cool_Par1=50.0, DampOuter_Type=1, DampOuter_Time=1.0,
                                          DampOuter_Len=0.8, Nozzle_Type=1, Nozzle_Par1=2.0, Nozzle_Par2=0.4,
-                                         Nozzle_Par3=0.7, Nozzle_Par4=20.0))]
+                                         Nozzle_Par3=0.7, Nozzle_Par4=20.0)),
+           # the other steppers of timestep.c:29-49: three-stage Runge-Kutta, and forward Euler then RK2
+           # (two mesh moves, two rounds of flux flips and t + 2 dt per step)
+           ("c2_mhd_small_rk3.npz", {"Timestep": 3}),
+           ("c2_mhd_small_fe.npz", {"Timestep": 1})]
 
 
-@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms"])
+@pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms", "rk3",
+                                                       "fe"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver
@@ -79,6 +84,7 @@ def test_mhd_matches_reference(fixture, over):
         r = ref[s]
         tm = dom.time()
         assert abs(tm["dt"][0] - r["dt"]) <= 1e-12 * r["dt"], (tm["dt"][0], r["dt"])
+        assert abs(tm["t"][0] - r["t"]) <= 1e-11 * max(1.0, r["t"]), (tm["t"][0], r["t"])
         rc = cell_radius(r["Nr"], r["Nz"], r["Np"], r["r_jph"])
         e = state_errors(st, r, rc, real)
         ep = _phi_errors(st, r, real)

@@ -227,9 +227,12 @@ def test_parser_semantics():
 
 
 def test_unsupported_configuration_is_rejected():
-    p = params("c1_kepler2d.par", Timestep=1)
-    with pytest.raises(RuntimeError, match="Timestep"):
+    p = params("c1_kepler2d.par", R_Periodic=1)
+    with pytest.raises(RuntimeError, match="R_Periodic"):
         D.setup_run(p)
+    # every Timestep value sets up (timestep.c: 1 and 3 have their own substeps, the rest run RK2)
+    for ts in (0, 1, 2, 3):
+        D.setup_run(params("c1_kepler2d.par", Timestep=ts))
     # sink.c terms that are not built are refused, not ignored
     for key, msg in (("Sink_Type", "Sink_Type"), ("Nozzle_Type", "Nozzle_Type"), ("DampInner_Type", "Damp"),
                      ("Cool_Type", "Cool_Type")):
