@@ -142,7 +142,7 @@ struct StepShape {
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
-    int sink;         // some run cools or has damping zones: the update kernels with sink.c's terms run
+    int sink;         // some run cools, damps, has a nozzle or a rotating frame: the update kernels with those terms run
     int stepper;      // substeps of a step (timestep.c:29-49): 2 RK2, 3 three-stage RK, 1 Euler then RK2
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
     // zero-gradient ghost fills (zerograd.c, outflow3d.c): DevState::bcjobs

@@ -209,6 +209,8 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->mesh_om_choice = par.Exact_Mesh_Omega;
     ph->en_om_choice = par.Energy_Omega;
     ph->omega0 = par.RotOmega;
+    ph->rot_frame = par.RotFrame;
+    ph->rot_d = par.RotD;
     ph->phi_max = par.phimax;
     ph->plm = par.PLM;
     ph->cfl = par.CFL;
@@ -300,7 +302,7 @@ static int unsupported(const disco_params& p, std::string* err) {
         if (t < 0 || t > 2) m += " Damp*_Type";
     if ((dt4[2] || dt4[3]) && !dt4[0] && !dt4[1] && p.Num_Z == 1) m += " Damp{Upper,Lower}_Type alone in 2D";
     if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && p.hydro == DISCO_HYDRO_GRMHD) m += " Damp*_Type with grmhd";
-    if (p.RotFrame) m += " RotFrame";
+    if (p.RotFrame && p.hydro == DISCO_HYDRO_GRMHD) m += " RotFrame with grmhd";
     if (p.noiseType) m += " Noise_Type";
     if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";
     if (p.Num_R < 1 || p.Num_Z < 1) m += " Num_R/Num_Z";
@@ -1169,7 +1171,9 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         if (R.Nz > 1) shape->has_z = 1;
         if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;
         const auto& q = R.par;
-        if (q.coolType || q.nozzleType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType) shape->sink = 1;
+        if (q.coolType || q.nozzleType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType ||
+            q.RotFrame)
+            shape->sink = 1;
     }
 }
 

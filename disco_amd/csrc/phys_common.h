@@ -25,6 +25,8 @@ struct PhysParams {
     int mesh_om_choice; // Exact_Mesh_Omega
     int en_om_choice;   // Energy_Omega
     double omega0;      // RotOmega
+    int rot_frame;      // RotFrame: the rotating frame's forces (rotframe.c) are on
+    double rot_d;       // RotD: distance of the rotation axis from the grid's axis
     double phi_max;
     double plm;
     double cfl;
@@ -153,6 +155,33 @@ DHD double get_om(const PhysParams& P, const double* x) {
     if (P.en_om_choice == 4) return 10. * exp(-.5 * r * r) - P.omega0;
     return 0.0 - P.omega0;
 }
+// rotframe.c:38-81 (omega_src, cylindrical geometry): centrifugal and
+// Coriolis force per unit mass of a frame rotating at Omega0 about an axis a
+// distance RotD from the grid's, at the cell centroid, added to the momenta
+// and, with the velocity relative to the energy frame (get_om), to the
+// energy.  The basis changes of rotframe.c are spelled out as the
+// cylindrical scale factors (1, r, 1), in its order of operations.
+DHD void omega_src(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                   double dVdt) {
+    if (!P.rot_frame) return;
+    const double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
+    const double r = x[0], phi = x[1];
+    const double rho = prim[RHO];
+    double v[3] = {prim[URR], prim[UPP] * r, prim[UZZ]};  // orthonormal
+    const double vr = v[0], om = v[1] / r;                // rpz, contravariant
+    const double W = P.omega0;
+    const double Rr = r - P.rot_d * cos(phi);
+    const double Rp = P.rot_d * sin(phi);
+    double f[3] = {W * W * Rr + 2. * W * r * om, W * W * Rp - 2. * W * vr, 0.0};  // orthonormal
+    v[1] -= r * get_om(P, x);
+    const double vf = v[0] * f[0] + v[1] * f[1] + v[2] * f[2];
+    f[1] *= r;  // covariant
+    cons[SRR] += rho * f[0] * dVdt;
+    cons[LLL] += rho * f[1] * dVdt;
+    cons[SZZ] += rho * f[2] * dVdt;
+    cons[TAU] += rho * vf * dVdt;
+}
+
 // omega.c:103-126
 DHD double get_om1(const PhysParams& P, const double* x) {
     double r = x[0];

@@ -812,8 +812,8 @@ DHD void nozzle_cell(const PhysParams& P, double* cons, const double* xp, const 
 // (riemann.c:314-317), add_source (misc.c:550-593) and, in the first substep,
 // move_cells/clean_pi/calc_dp (misc.c:266-298, 8-36).  Returns the updated
 // conserved state of a real cell in cons[] (and stores it).  SINK compiles in
-// sink.c's cooling, damping and nozzle (a separate instantiation, so that the
-// update without them keeps its registers).
+// sink.c's cooling, damping and nozzle and rotframe.c's omega_src (a separate
+// instantiation, so that the update without them keeps its registers).
 template <class Ph, bool SINK = false>
 DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int first, double* cons) {
     constexpr int NQ = Ph::NQ;
@@ -881,8 +881,10 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
             euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);  // planet.c:138-158 (module independent)
         }
-        if constexpr (Ph::NEWTONIAN && SINK)  // misc.c:589, after the planets
-            if (P.nozzle_type) nozzle_cell(P, cons, xp, xm, dV, dt);
+        if constexpr (Ph::NEWTONIAN && SINK) {
+            omega_src(P, prim, cons, xp, xm, dV * dt);                  // misc.c:588
+            if (P.nozzle_type) nozzle_cell(P, cons, xp, xm, dV, dt);  // misc.c:589, after the planets
+        }
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.cons[q * N + c], cons[q]);
     }
 
@@ -999,6 +1001,7 @@ DHD void update_b_ghost(const DevState& S, int c) {
         x[1] = 0.5 * (phip + (phip - dph));
     double p8[8];
     disco_initial(&Rn.ic, p8, x);
+    if (Rn.ph.rot_frame) p8[UPP] -= Rn.ph.omega0;  // subtract_omega (fixed.c:37,54; boundary_functions.c:30)
     _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) st_stream(&S.prim[q * N + c], p8[q]);
 }
 

@@ -273,6 +273,22 @@ STEPPER_CASES = [  # timestep.c:29-49: Timestep 3 (three-stage SSP Runge-Kutta),
 ]
 
 
+ROT_CASES = [  # rotframe.c: a frame rotating at RotOmega about an axis RotD from the grid's
+    ("c3_rot.npz", "c3_binary_disk.par", dict(Num_R=24, RotFrame=1, RotOmega=1.0, RotD=0.1), "euler_binary", 10,
+     (1,)),
+    ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05), "euler_sun", 10,
+     (1,)),
+    ("c2_mhd_small_rot.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, RotFrame=1, RotOmega=0.3, RotD=0.05),
+     "mhd_sun", 5, (1,)),
+]
+
+
+def fixtures_rot() -> None:
+    for name, base, over, variant, n, dumps in ROT_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -412,6 +428,10 @@ def main() -> None:
         return
     if "--sink-z" in sys.argv:
         fixtures_sink_z()
+        print("fixtures written")
+        return
+    if "--rot" in sys.argv:
+        fixtures_rot()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

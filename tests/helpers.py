@@ -144,8 +144,10 @@ def diag_errors(dg: dict, ref: dict) -> tuple:
     quantities such as rho v_r cancel to far below their terms, so the
     average itself is no scale); quantities the reference holds at exactly
     zero must be exactly zero.  Also the relative error of t_avg.  The
-    per-cell magnitudes follow Diagnostics/diag_euler.c:17-44 (torque: |Qrz|
-    itself)."""
+    per-cell magnitudes follow Diagnostics/diag_euler.c:17-44; the torque
+    density's is 2 pi r rho r |f| with |f| planet 1's point-mass force at the
+    cell's centre (planet.c:97-102: M d / (d^2 + eps^2)^1.5), since its
+    annulus average cancels around the ring like rho v_r's."""
     a, b = dg["Qrz"], ref["Qrz"]
     rc = cell_radius(ref["Nr"], ref["Nz"], ref["Np"], ref["r_jph"])
     p = ref["prim"]
@@ -153,7 +155,14 @@ def diag_errors(dg: dict, ref: dict) -> tuple:
     real = ref["real"] == 1
     vs = np.abs(p[3][real] * rc[real]).max()  # flow speed scale
     tw = 2 * np.pi * rc * rho
-    mag = [rho, tw * vs, tw * vs, None, tw * rc * rc * om * vs, om, tw * rc * rc * om, P]
+    torque = None
+    pl = ref.get("planets")
+    if pl is not None and len(pl) > 1:
+        rp, pp, M, _, eps = pl[1]
+        phi = ref["piph"] - 0.5 * ref["dphi"]
+        d = np.sqrt((rc * np.cos(phi) - rp * np.cos(pp)) ** 2 + (rc * np.sin(phi) - rp * np.sin(pp)) ** 2)
+        torque = tw * rc * M * d / (d * d + eps * eps) ** 1.5
+    mag = [rho, tw * vs, tw * vs, torque, tw * rc * rc * om * vs, om, tw * rc * rc * om, P]
     e = np.zeros(a.shape[-1])
     for q in range(a.shape[-1]):
         d = np.abs(a[..., q] - b[..., q]).max()

@@ -70,6 +70,10 @@ CASES = [
     ("c1_rk3.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=3)),
     ("c1_fe.npz", "c1_kepler2d.par", dict(Num_R=16, Timestep=1)),
     ("c3_fe.npz", "c3_binary_disk.par", dict(Num_R=24, Timestep=1)),
+    # a rotating frame (rotframe.c): Coriolis and centrifugal forces about an axis RotD off the grid's, the
+    # initial and the fixed ghosts' azimuthal velocity measured in the frame
+    ("c3_rot.npz", "c3_binary_disk.par", dict(Num_R=24, RotFrame=1, RotOmega=1.0, RotD=0.1)),
+    ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05)),
 ]
 
 

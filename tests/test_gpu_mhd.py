@@ -57,11 +57,13 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            # the other steppers of timestep.c:29-49: three-stage Runge-Kutta, and forward Euler then RK2
            # (two mesh moves, two rounds of flux flips and t + 2 dt per step)
            ("c2_mhd_small_rk3.npz", {"Timestep": 3}),
-           ("c2_mhd_small_fe.npz", {"Timestep": 1})]
+           ("c2_mhd_small_fe.npz", {"Timestep": 1}),
+           # a rotating frame (rotframe.c)
+           ("c2_mhd_small_rot.npz", dict(RotFrame=1, RotOmega=0.3, RotD=0.05))]
 
 
 @pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms", "rk3",
-                                                       "fe"])
+                                                       "fe", "rot"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver
