@@ -80,6 +80,7 @@ class Params(C.Structure):
         ("dampTimeLower", C.c_double), ("dampLenLower", C.c_double),
         ("nozzlePar1", C.c_double), ("nozzlePar2", C.c_double), ("nozzlePar3", C.c_double),
         ("nozzlePar4", C.c_double),
+        ("viscosity", C.c_double), ("alpha_flag", C.c_int), ("include_atmos", C.c_int),
     ]
 
     def copy(self) -> "Params":

@@ -94,6 +94,9 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     D("RotD", &p->RotD);
     I("Riemann_Solver", &p->Riemann_Solver);
     I("Use_Viscosity", &p->visc_flag);
+    D("Viscosity", &p->viscosity);
+    I("Use_As_Alpha", &p->alpha_flag);
+    I("Include_Atmos", &p->include_atmos);
     I("P_Times_2pi", &pTimes_2pi);
     I("T_Times_2pi", &tTimes_2pi);
     I("Z_Times_pi", &zTimes_pi);
@@ -249,6 +252,29 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     disco_ic_from_params(&par, ic);
 }
 
+// noise.c:15-51 (addNoise): glibc rand() deviates in [-1, 1], drawn per real
+// cell in setupCells' loop order (domain.c:172-205), after the annulus
+// phases of setup_phases; type 1 perturbs v_r and Omega, 2 the pressure, 3
+// replaces v_r and v_z by fractions of the orthonormal azimuthal speed.
+static double noise_dev() { return 2 * ((double)std::rand() / (double)(RAND_MAX)) - 1; }
+
+static void add_noise(const disco_params& par, double* prim, const double* x) {
+    const double rel = par.noiseRel, ab = par.noiseAbs;
+    if (par.noiseType == 1) {
+        const double x1 = noise_dev(), x2 = noise_dev(), x3 = noise_dev(), x4 = noise_dev();
+        prim[URR] = (1.0 + x1 * rel) * prim[URR] + x2 * ab;
+        prim[UPP] = (1.0 + x3 * rel) * prim[UPP] + x4 * ab;
+    } else if (par.noiseType == 2) {
+        const double x1 = noise_dev(), x2 = noise_dev();
+        prim[PPP] = (1.0 + x1 * rel) * prim[PPP] + x2 * ab;
+    } else if (par.noiseType == 3) {
+        const double x1 = noise_dev(), x2 = noise_dev(), x3 = noise_dev(), x4 = noise_dev();
+        const double vp = prim[UPP] * x[0];  // get_vec_covariant: the orthonormal azimuthal speed
+        prim[URR] = x1 * rel * vp + x2 * ab;
+        prim[UZZ] = x3 * rel * vp + x4 * ab;
+    }
+}
+
 // timestep.c:29-49: Timestep 3 runs the three-stage Runge-Kutta, 1 a forward
 // Euler substep followed by RK2, every other value RK2
 int stepper_of(int Timestep) { return Timestep == 1 ? 1 : Timestep == 3 ? 3 : 2; }
@@ -303,7 +329,8 @@ static int unsupported(const disco_params& p, std::string* err) {
     if ((dt4[2] || dt4[3]) && !dt4[0] && !dt4[1] && p.Num_Z == 1) m += " Damp{Upper,Lower}_Type alone in 2D";
     if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && p.hydro == DISCO_HYDRO_GRMHD) m += " Damp*_Type with grmhd";
     if (p.RotFrame && p.hydro == DISCO_HYDRO_GRMHD) m += " RotFrame with grmhd";
-    if (p.noiseType) m += " Noise_Type";
+    if (p.noiseType < 0 || p.noiseType > 3) m += " Noise_Type";
+    if (p.include_atmos) m += " Include_Atmos";
     if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";
     if (p.Num_R < 1 || p.Num_Z < 1) m += " Num_R/Num_Z";
     if (!m.empty()) {
@@ -897,6 +924,7 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 double dV = get_dV(xp, xm, R.par.phimax);
                 disco_initial(&R.ic, prim, x);
                 if (par.RotFrame) prim[UPP] -= par.RotOmega;
+                if (par.noiseType) add_noise(par, prim, x);
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
                 host_prim2cons(R, prim, cons, x, dV);
                 for (int q = 0; q < R.nq; ++q) R.cons[(size_t)q * R.ncell + c] = cons[q];

@@ -90,6 +90,11 @@ typedef struct disco_params {
     double dampTimeInner, dampLenInner, dampTimeOuter, dampLenOuter;
     double dampTimeUpper, dampLenUpper, dampTimeLower, dampLenLower;
     double nozzlePar1, nozzlePar2, nozzlePar3, nozzlePar4;
+    /* readpar.c:115-118: viscosity (Use_Viscosity above), its alpha form, and
+       the planets' atmospheres at set-up (adjust_gas) */
+    double viscosity;
+    int alpha_flag;
+    int include_atmos;
 } disco_params;
 
 #ifdef __cplusplus

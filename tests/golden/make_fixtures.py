@@ -289,6 +289,22 @@ def fixtures_rot() -> None:
         save_states(name, states)
 
 
+NOISE_CASES = [  # noise.c: glibc rand() deviates per real cell after the annulus phases
+    ("c1_noise1.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=1, Noise_Abs=1e-3, Noise_Rel=1e-2), "euler_sun", 5,
+     (1,)),
+    ("c1_noise2.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=2, Noise_Abs=1e-4, Noise_Rel=1e-2), "euler_sun", 5,
+     (1,)),
+    ("c2e_noise3.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Noise_Type=3, Noise_Abs=1e-3,
+                                               Noise_Rel=1e-2), "euler_sun", 5, (1,)),
+]
+
+
+def fixtures_noise() -> None:
+    for name, base, over, variant, n, dumps in NOISE_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -432,6 +448,10 @@ def main() -> None:
         return
     if "--rot" in sys.argv:
         fixtures_rot()
+        print("fixtures written")
+        return
+    if "--noise" in sys.argv:
+        fixtures_noise()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

@@ -74,6 +74,11 @@ CASES = [
     # initial and the fixed ghosts' azimuthal velocity measured in the frame
     ("c3_rot.npz", "c3_binary_disk.par", dict(Num_R=24, RotFrame=1, RotOmega=1.0, RotD=0.1)),
     ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05)),
+    # initial noise (noise.c, types 1-3)
+    ("c1_noise1.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=1, Noise_Abs=1e-3, Noise_Rel=1e-2)),
+    ("c1_noise2.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=2, Noise_Abs=1e-4, Noise_Rel=1e-2)),
+    ("c2e_noise3.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Noise_Type=3, Noise_Abs=1e-3,
+                                                Noise_Rel=1e-2)),
 ]
 
 

@@ -188,7 +188,16 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                        ("c2_mhd_outflow.npz", "c2_mhd3d.par",
                         dict(Num_R=12, Num_Z=4, CFL=0.3, Z_Periodic=0, Boundary="outflow3d")),
                        ("c5_grmhd_outflow.npz", "c5_grmhd_torus.par",
-                        dict(Num_R=16, Num_Z=12, Metric_Par3=0.0, Boundary="outflow3d"))]
+                        dict(Num_R=16, Num_Z=12, Metric_Par3=0.0, Boundary="outflow3d")),
+                       # Noise_Type 1-3 (noise.c): the glibc rand() stream continued after the phases
+                       ("c1_noise1.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=1, Noise_Abs=1e-3,
+                                                                 Noise_Rel=1e-2)),
+                       ("c1_noise2.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=2, Noise_Abs=1e-4,
+                                                                 Noise_Rel=1e-2)),
+                       ("c2e_noise3.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Noise_Type=3,
+                                                                  Noise_Abs=1e-3, Noise_Rel=1e-2)),
+                       # the rotating frame's initial and fixed-ghost data
+                       ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05))]
 
 
 @pytest.mark.parametrize("fixture,base,over", SETUP_CASES, ids=[c[0] for c in SETUP_CASES])
@@ -233,6 +242,8 @@ def test_unsupported_configuration_is_rejected():
     # every Timestep value sets up (timestep.c: 1 and 3 have their own substeps, the rest run RK2)
     for ts in (0, 1, 2, 3):
         D.setup_run(params("c1_kepler2d.par", Timestep=ts))
+    with pytest.raises(RuntimeError, match="Include_Atmos"):
+        D.setup_run(params("c1_kepler2d.par", Include_Atmos=1))
     # sink.c terms that are not built are refused, not ignored
     for key, msg in (("Sink_Type", "Sink_Type"), ("Nozzle_Type", "Nozzle_Type"), ("DampInner_Type", "Damp"),
                      ("Cool_Type", "Cool_Type")):
