@@ -143,6 +143,7 @@ struct StepShape {
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
     int sink;         // some run cools, damps, has a nozzle or a rotating frame: the update kernels with those terms run
+    int visc;         // some run is viscous: the flux kernel with the viscous fluxes runs
     int stepper;      // substeps of a step (timestep.c:29-49): 2 RK2, 3 three-stage RK, 1 Euler then RK2
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
     // zero-gradient ghost fills (zerograd.c, outflow3d.c): DevState::bcjobs

@@ -213,6 +213,9 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->en_om_choice = par.Energy_Omega;
     ph->omega0 = par.RotOmega;
     ph->rot_frame = par.RotFrame;
+    ph->visc = par.visc_flag;
+    ph->visc_alpha = par.alpha_flag;
+    ph->visc_nu = par.viscosity;
     ph->rot_d = par.RotD;
     ph->phi_max = par.phimax;
     ph->plm = par.PLM;
@@ -310,7 +313,9 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
     if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
         m += " Riemann_Solver";
-    if (p.visc_flag) m += " Use_Viscosity";
+    // euler.c holds the viscous terms; mhd.c and grmhd.c define no visc_source
+    if (p.visc_flag && p.hydro != DISCO_HYDRO_EULER) m += " Use_Viscosity (euler only)";
+    if (p.visc_flag && !p.alpha_flag && p.viscosity <= 0.0) m += " Use_Viscosity needs Viscosity > 0";
     // add_source's sink.c terms (misc.c:579-591) are not built
     if (p.sinkType) m += " Sink_Type";
     // sink.c:274-322: beta cooling (COOL_BETA, COOL_BETA_RELAX) of a Newtonian gas
@@ -1192,6 +1197,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->has_z = 0;
     shape->zcopy = 0;
     shape->sink = 0;
+    shape->visc = 0;
     shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
                     ? 1
                     : 0;
@@ -1200,8 +1206,9 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         if (R.Nz > 1 && R.par.Z_Periodic) shape->zcopy = 1;
         const auto& q = R.par;
         if (q.coolType || q.nozzleType || q.dampInnerType || q.dampOuterType || q.dampUpperType || q.dampLowerType ||
-            q.RotFrame)
+            q.RotFrame || q.visc_flag)
             shape->sink = 1;
+        if (q.visc_flag) shape->visc = 1;
     }
 }
 

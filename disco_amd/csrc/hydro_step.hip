@@ -166,10 +166,12 @@ __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::PLM) void k_plm_trans(DevSt
     if (c < S.ncell) plm_trans_cell<Ph>(S, c, has_z);
 }
 
-template <class Ph>
+// VISC: with euler.c's viscous fluxes (an instantiation of its own, launched
+// only for viscous runs)
+template <class Ph, bool VISC>
 __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::FLUX) void k_flux(DevState S, double dtfac) {
     int c = cell_index();
-    if (c < S.ncell) flux_cell<Ph>(S, c, dtfac);
+    if (c < S.ncell) flux_cell<Ph, VISC>(S, c, dtfac);
 }
 
 // Fused update for hydro: conserved state stays in registers between the
@@ -509,7 +511,14 @@ struct Launch {
         // searches (a moving substep before it moved the mesh)
         DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, first != 1);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
-        DISCO_LAUNCH(hook, "k_flux", k_flux<Ph>, G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+        if constexpr (!Ph::MHD) {
+            if (sh.visc)
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, true>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+            else
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+        } else {
+            DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+        }
         if (Ph::MHD && sh.ct) {
             if (fork) {
                 (void)hipEventRecord(fk->ev[1], st);

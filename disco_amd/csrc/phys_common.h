@@ -46,6 +46,9 @@ struct PhysParams {
     int nozzle_type;          // Nozzle_Type: 0 none, 1 a mass source in a ball
     double nozzle_par[4];     // Nozzle_Par1-4: ball centre radius, ball radius, injection speed, its Mach number
     int two_d;                // Num_Z == 1: the z zones are off
+    int visc;                 // Use_Viscosity: euler.c's viscous fluxes, source and time step
+    int visc_alpha;           // Use_As_Alpha: nu = alpha c_s^2 / Omega_height
+    double visc_nu;           // Viscosity: nu, or alpha with Use_As_Alpha
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face
@@ -193,6 +196,12 @@ DHD double get_om1(const PhysParams& P, const double* x) {
         return -1.5 * pow(r, -1 + 1.5 * n) / pow(pow(r, 1.5 * n) + 1., 1.0 + 1. / n);
     }
     if (P.en_om_choice == 4) return 10. * r * exp(-.5 * r * r);
+    return 0.0;
+}
+// omega.c:128-130
+DHD double get_om2(const PhysParams& P, const double* x) {
+    (void)P;
+    (void)x;
     return 0.0;
 }
 // omega.c:157-209 (profiles 1-4; profile 5 needs the planet list and is not

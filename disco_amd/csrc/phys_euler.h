@@ -146,9 +146,11 @@ DHD void vel(const PhysParams& P, const double* prim1, const double* prim2, doub
 // HLL / HLLC branches of riemann.c:216-292 followed by the per-face
 // increment of riemann.c:314-317: returns dU[q] = (F[q] - w U*[q]) * dAdt,
 // which the reference subtracts from the left cell and adds to the right.
+// vF (optional): a viscous flux added to the face flux before the increment
+// (riemann.c:297-312).
 template <int NQ>
 DHD void riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
-                 const double* n, double w, double dAdt, double* dU) {
+                 const double* n, double w, double dAdt, double* dU, const double* vF = nullptr) {
     double Flux[NQ];
     double Ustr[NQ];
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
@@ -187,7 +189,70 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
         flux<NQ>(P, pk, Fk, x, n);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sk * (Ustr[q] - Uk[q]);
     }
+    if (vF) {
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] += vF[q];
+    }
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+}
+
+// euler.c:215-256 (visc_flux): the viscous stress through a face of
+// orthonormal normal n at x, from the face's mean primitives and gradients
+// (coordinate components: d/dr, d/dphi, d/dz), with kinematic viscosity nu;
+// only the momenta and the energy carry it (flux[] of the others is 0).
+template <int NQ>
+DHD void visc_flux(const PhysParams& P, double nu, const double* prim, const double* gradr, const double* gradp,
+                   const double* gradz, double* flux, const double* x, const double* n) {
+    const double r = x[0];
+    const double rho = prim[RHO];
+    const double vr = prim[URR];
+    const double om = prim[UPP];
+    const double om_off = om - get_om(P, x);
+    const double vz = prim[UZZ];
+    const double divV_o_d = (gradr[URR] + gradp[UPP] + gradz[UZZ] + vr / r) / 3.0;
+    const double srr = gradr[URR] - divV_o_d;
+    const double spp = r * (r * gradp[UPP] + vr - r * divV_o_d);
+    const double szz = gradz[UZZ] - divV_o_d;
+    const double srp = 0.5 * (r * r * gradr[UPP] + gradp[URR]);
+    const double srz = 0.5 * (gradr[UZZ] + gradz[URR]);
+    const double spz = 0.5 * (gradp[UZZ] + r * r * gradz[UPP]);
+    const double nc[3] = {n[0], n[1] / r, n[2]};
+    const double srn = srr * nc[0] + srp * nc[1] + srz * nc[2];
+    const double spn = srp * nc[0] + spp * nc[1] + spz * nc[2];
+    const double szn = srz * nc[0] + spz * nc[1] + szz * nc[2];
+    _Pragma("unroll") for (int q = 0; q < NQ; ++q) flux[q] = 0.0;
+    flux[SRR] = -2 * nu * rho * srn;
+    flux[LLL] = -2 * nu * rho * spn;
+    flux[SZZ] = -2 * nu * rho * szn;
+    flux[TAU] = -2 * nu * rho * (vr * srn + om_off * spn + vz * szn);
+}
+
+// euler.c:258-290 (visc_source): the geometric viscous terms of the radial
+// momentum and of the energy in the frame of get_om, at the centroid x.
+DHD void visc_source(const PhysParams& P, double nu, const double* prim, const double* gradr, const double* gradp,
+                     const double* gradz, double* cons, const double* x, double dVdt) {
+    const double r = x[0];
+    const double rho = prim[RHO];
+    const double vr = prim[URR];
+    const double divV_o_d = (gradr[URR] + gradp[UPP] + gradz[UZZ] + vr / r) / 3.0;
+    const double spp = (r * gradp[UPP] + vr - r * divV_o_d) / (r * r * r);
+    cons[SRR] += (-2 * r * rho * nu * spp) * dVdt;
+    const double srp = 0.5 * (r * r * gradr[UPP] + gradp[URR]);
+    const double spz = 0.5 * (gradp[UZZ] + r * r * gradz[UPP]);
+    const double om_r = get_om1(P, x);
+    const double om_z = get_om2(P, x);
+    cons[TAU] += (2 * rho * nu * (srp * om_r + spz * om_z)) * dVdt;
+}
+
+// euler.c:351-373: the viscous limit 0.5 dx^2 / nu of the time step, dx the
+// cell's smallest extent
+DHD double visc_dt(const PhysParams& P, double nu, const double* xp, const double* xm) {
+    const double dL0 = get_dL(xp, xm, 0, P.phi_max);
+    const double dL1 = get_dL(xp, xm, 1, P.phi_max);
+    const double dL2 = get_dL(xp, xm, 2, P.phi_max);
+    double dx = dL0;
+    if (dx > dL1) dx = dL1;
+    if (dx > dL2) dx = dL2;
+    return 0.5 * dx * dx / nu;
 }
 
 // euler.c:172-213 (polar sources are off: NoBC_Rmin = 0)

@@ -298,7 +298,50 @@ DHD TFace t_face(double piphL, double dphiL, double piphR, double dphiR, int LR,
     return F;
 }
 
+// ------------------------------------------------------------------ viscosity
+// omega.c:132-155 (get_height_om): the frequency sqrt(sum_p f_p / (d_p +
+// eps_p / 16)) of the run's point-mass planets at x (r, phi) in the plane
+DHD double height_om(const DevState& S, const RunDev& Rn, int run, const double* x) {
+    double omtot2 = 0.0;
+    const double r = x[0], phi = x[1];
+    const double gx = r * cos(phi), gy = r * sin(phi);
+    for (int p = 0; p < Rn.ph.npl; ++p) {
+        const Planet& pl = Rn.planets[p];
+        const double pphi = S.planet_phi[run * DISCO_MAX_PLANETS + p];
+        const double px = pl.r * cos(pphi), py = pl.r * sin(pphi);
+        const double sr = sqrt((px - gx) * (px - gx) + (py - gy) * (py - gy));
+        const double f1 = fgrav_pointmass(pl.M, sr, pl.eps);
+        omtot2 += f1 / (sr + 0.0625 * pl.eps);
+    }
+    return sqrt(omtot2);
+}
+
+// euler.c:220-225: the kinematic viscosity nu, or alpha c_s^2 / Omega_height
+// with Use_As_Alpha, for the primitives prim at x
+DHD double visc_nu(const DevState& S, const RunDev& Rn, int run, const double* prim, const double* x) {
+    const PhysParams& P = Rn.ph;
+    if (!P.visc_alpha) return P.visc_nu;
+    const double c2 = P.gamma_law * prim[PPP] / prim[RHO];
+    return P.visc_nu * c2 / height_om(S, Rn, run, x);
+}
+
 // ------------------------------------------------------------------ dt
+// euler.c:351-373: a viscous run's time step is at most 0.5 dx^2 / nu
+template <class Ph>
+DHD double with_visc_dt(const DevState& S, const AnnDev& A, double dt, const double* prim, const double* xp,
+                        const double* xm) {
+    if constexpr (!Ph::MHD) {
+        const RunDev& Rn = S.runs[A.run];
+        if (Rn.ph.visc) {
+            const double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0),
+                                 get_centroid(xp[2], xm[2], 2)};
+            const double dtv = euler::visc_dt(Rn.ph, visc_nu(S, Rn, A.run, prim, x), xp, xm);
+            if (dt > dtv) dt = dtv;
+        }
+    }
+    return dt;
+}
+
 // misc.c:39-84 for one real cell: mindt(prim, w = mean of the two phi-face
 // mesh speeds, xp, xm)
 template <class Ph>
@@ -310,7 +353,7 @@ DHD double cell_mindt(const DevState& S, const AnnDev& A, const PhysParams& P, i
     double xp[3] = {A.rp, phip, A.zp};
     double xm[3] = {A.rm, phim, A.zm};
     double w = .5 * (S.wiph[cm] + S.wiph[c]);
-    return Ph::mindt(P, prim, w, xp, xm);
+    return with_visc_dt<Ph>(S, A, Ph::mindt(P, prim, w, xp, xm), prim, xp, xm);
 }
 
 // CFL candidate of cell c over the current state (1e300 for ghosts)
@@ -558,7 +601,7 @@ DHD void plm_trans_cell(const DevState& S, int c, int has_z) {
 // (riemann.c:114-213) on its owned faces; solve_riemann's increment
 // (F - w U*) dA dt is stored per owner cell, and for MHD the face's EMFs
 // (flux_to_E) beside it.
-template <class Ph, int D>
+template <class Ph, int D, bool VISC = false>
 DHD FaceEMF t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, const TFace& F, const AnnDev& AL,
                         double dt, double* dU) {
     constexpr int NQ = Ph::NQ;
@@ -578,10 +621,32 @@ DHD FaceEMF t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, 
         primR[BT] = Bavg;
     }
     const double n[3] = {D == DIR_R ? 1.0 : 0.0, 0.0, D == DIR_Z ? 1.0 : 0.0};
-    return Ph::riemann(P, primL, primR, F.x, n, 0.0, F.dA * dt, D, dU);
+    if constexpr (VISC && !Ph::MHD) {
+        // riemann.c:114-140, 297-312: the face's mean state; across the face
+        // the difference of the two phi-reconstructed values, along it the
+        // mean of the cells' gradients
+        const AnnDev& Ann = S.ann[S.cell_ann[cL]];
+        const RunDev& Rn = S.runs[Ann.run];
+        double pm[NQ], g[3][NQ], vF[NQ];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+            pm[q] = .5 * (primL[q] + primR[q]);
+            const double gf = ((S.prim[q * N + cR] - S.gradp[q * N + cR] * F.dpR) -
+                               (S.prim[q * N + cL] + S.gradp[q * N + cL] * F.dpL)) /
+                              (dxL + dxR);
+            g[0][q] = D == DIR_R ? gf : .5 * (S.gradr[q * N + cL] + S.gradr[q * N + cR]);
+            g[1][q] = .5 * (S.gradp[q * N + cL] + S.gradp[q * N + cR]);
+            g[2][q] = D == DIR_Z ? gf : .5 * (S.gradz[q * N + cL] + S.gradz[q * N + cR]);
+        }
+        const double nu = visc_nu(S, Rn, Ann.run, pm, F.x);
+        euler::visc_flux<NQ>(P, nu, pm, g[0], g[1], g[2], vF, F.x, n);
+        euler::riemann<NQ>(P, primL, primR, F.x, n, 0.0, F.dA * dt, dU, vF);
+        return FaceEMF{0.0, 0.0, 0.0, 0.0};
+    } else {
+        return Ph::riemann(P, primL, primR, F.x, n, 0.0, F.dA * dt, D, dU);
+    }
 }
 
-template <class Ph, int D>
+template <class Ph, int D, bool VISC = false>
 DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& P, int c, double dt) {
     constexpr int NQ = Ph::NQ;
     using Dr = Dir<D>;
@@ -594,7 +659,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         const AnnDev& B = S.ann[Dr::hi(A)];
         int cb = B.off + Dr::part_hi(S)[c];
         TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
-        FaceEMF e = t_face_flux<Ph, D>(S, P, c, cb, F, A, dt, dU);
+        FaceEMF e = t_face_flux<Ph, D, VISC>(S, P, c, cb, F, A, dt, dU);
         double* f = Dr::f_hi(S);
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
@@ -607,7 +672,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         const AnnDev& W = S.ann[Dr::lo(A)];
         int ca = W.off + Dr::part_lo(S)[c];
         TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
-        FaceEMF e = t_face_flux<Ph, D>(S, P, ca, c, F, W, dt, dU);
+        FaceEMF e = t_face_flux<Ph, D, VISC>(S, P, ca, c, F, W, dt, dU);
         double* f = Dr::f_lo(S);
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
@@ -618,7 +683,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
     }
 }
 
-template <class Ph>
+template <class Ph, bool VISC = false>
 DHD void flux_cell(const DevState& S, int c, double dtfac) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
@@ -648,7 +713,24 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
         const double nv[3] = {0.0, 1.0, 0.0};
         double dA = (A.rp - A.rm) * (A.zp - A.zm);  // get_dA(xp, xm, 0)
         double w = A.rc * S.wiph[c];                // get_scale_factor(x, 0) * wiph
-        FaceEMF e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
+        FaceEMF e;
+        if constexpr (VISC && !Ph::MHD) {
+            // riemann.c:39-70, 297-312: across the face the difference of the
+            // two cells' prims, along it the mean of their gradients
+            double pm[NQ], g[3][NQ], vF[NQ];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+                pm[q] = .5 * (primL[q] + primR[q]);
+                g[0][q] = .5 * (S.gradr[q * N + c] + S.gradr[q * N + cp]);
+                g[1][q] = (S.prim[q * N + cp] - S.prim[q * N + c]) / (0.5 * (dphR + dph));
+                g[2][q] = .5 * (S.gradz[q * N + c] + S.gradz[q * N + cp]);
+            }
+            const double nu = visc_nu(S, S.runs[A.run], A.run, pm, x);
+            euler::visc_flux<NQ>(P, nu, pm, g[0], g[1], g[2], vF, x, nv);
+            euler::riemann<NQ>(P, primL, primR, x, nv, w, dA * dt, dU, vF);
+            e = FaceEMF{0.0, 0.0, 0.0, 0.0};
+        } else {
+            e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
+        }
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&S.fphi[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_phi) {
             st_stream(&S.emf_phi[0 * N + c], e.E1);  // E_z
@@ -657,8 +739,8 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
             st_stream(&S.emf_phi[3 * N + c], e.B2);  // B_z
         }
     }
-    owned_faces_flux<Ph, DIR_R>(S, A, P, c, dt);
-    owned_faces_flux<Ph, DIR_Z>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_R, VISC>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_Z, VISC>(S, A, P, c, dt);
 }
 
 // ------------------------------------------------------------------ update
@@ -881,6 +963,20 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
             pl.phi = S.planet_phi[A.run * DISCO_MAX_PLANETS + p];
             euler::planet_src(P, pl, prim, cons, xp, xm, dV * dt);  // planet.c:138-158 (module independent)
         }
+        if constexpr (Ph::NEWTONIAN && SINK && !Ph::MHD) {
+            if (P.visc) {  // misc.c:585-587
+                const double xc[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0),
+                                      get_centroid(xp[2], xm[2], 2)};
+                double g[3][NQ];
+                _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+                    g[0][q] = S.gradr[q * N + c];
+                    g[1][q] = S.gradp[q * N + c];
+                    g[2][q] = S.gradz[q * N + c];
+                }
+                const double nu = visc_nu(S, Rn, A.run, prim, xc);
+                euler::visc_source(P, nu, prim, g[0], g[1], g[2], cons, xc, dV * dt);
+            }
+        }
         if constexpr (Ph::NEWTONIAN && SINK) {
             omega_src(P, prim, cons, xp, xm, dV * dt);                  // misc.c:588
             if (P.nozzle_type) nozzle_cell(P, cons, xp, xm, dV, dt);  // misc.c:589, after the planets
@@ -981,7 +1077,7 @@ DHD double update_b_post(const DevState& S, int c, const double* prim, const Cel
     int i = c - A.off;
     int cm = A.off + (i == 0 ? A.np - 1 : i - 1);
     double w = .5 * (S.wiph[cm] + S.wiph[c]);
-    return Ph::mindt(S.runs[A.run].ph, prim, w, G.xp, G.xm);
+    return with_visc_dt<Ph>(S, A, Ph::mindt(S.runs[A.run].ph, prim, w, G.xp, G.xm), prim, G.xp, G.xm);
 }
 
 template <class Ph>

@@ -305,6 +305,22 @@ def fixtures_noise() -> None:
         save_states(name, states)
 
 
+VISC_CASES = [  # euler.c's viscosity (riemann.c:297-312, misc.c:585-587, the dt limit of euler.c:351-373), 3D
+    ("c2e_visc.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02),
+     "euler_sun", 10, (1, 5)),
+    ("c2e_visc_alpha.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Use_As_Alpha=1,
+                                                   Viscosity=0.1), "euler_sun", 5, (1,)),
+    ("c3z_visc_alpha.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Use_Viscosity=1,
+                                                      Use_As_Alpha=1, Viscosity=0.05), "euler_binary", 5, (1,)),
+]
+
+
+def fixtures_visc() -> None:
+    for name, base, over, variant, n, dumps in VISC_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -452,6 +468,10 @@ def main() -> None:
         return
     if "--noise" in sys.argv:
         fixtures_noise()
+        print("fixtures written")
+        return
+    if "--visc" in sys.argv:
+        fixtures_visc()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

@@ -79,6 +79,13 @@ CASES = [
     ("c1_noise2.npz", "c1_kepler2d.par", dict(Num_R=16, Noise_Type=2, Noise_Abs=1e-4, Noise_Rel=1e-2)),
     ("c2e_noise3.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Noise_Type=3, Noise_Abs=1e-3,
                                                 Noise_Rel=1e-2)),
+    # viscosity (euler.c): constant nu with its time-step limit in force, and the alpha form with the sun's and
+    # the binary's height frequency (3D: the reference's 2D runs read z-gradients it never sets)
+    ("c2e_visc.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02)),
+    ("c2e_visc_alpha.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Use_As_Alpha=1,
+                                                    Viscosity=0.1)),
+    ("c3z_visc_alpha.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Use_Viscosity=1,
+                                                       Use_As_Alpha=1, Viscosity=0.05)),
 ]
 
 

@@ -242,6 +242,11 @@ def test_unsupported_configuration_is_rejected():
     # every Timestep value sets up (timestep.c: 1 and 3 have their own substeps, the rest run RK2)
     for ts in (0, 1, 2, 3):
         D.setup_run(params("c1_kepler2d.par", Timestep=ts))
+    # viscosity is euler.c's (mhd.c defines no visc_source) and needs nu > 0
+    with pytest.raises(RuntimeError, match="euler only"):
+        D.setup_run(params("c2_mhd3d.par", Num_R=12, Num_Z=4, Use_Viscosity=1, Viscosity=0.01))
+    with pytest.raises(RuntimeError, match="Viscosity > 0"):
+        D.setup_run(params("c1_kepler2d.par", Use_Viscosity=1))
     with pytest.raises(RuntimeError, match="Include_Atmos"):
         D.setup_run(params("c1_kepler2d.par", Include_Atmos=1))
     # sink.c terms that are not built are refused, not ignored
