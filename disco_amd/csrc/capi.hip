@@ -24,6 +24,7 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
 void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
 void launch_partners(const DevState& S, hipStream_t st);
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st);
+void launch_sink_reset(const DevState& S, int run, hipStream_t st);
 void launch_report_sum(const double* part, int Nr, int Nz, double* sums, hipStream_t st);
 void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const LaunchHook* hook);
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
@@ -105,6 +106,8 @@ struct disco_gpu {
     size_t bcjobs_cap = 0;
     double *diag = nullptr, *t_avg = nullptr;
     double* src = nullptr;  // GRMHD source increments
+    // accretion sinks (DevState::sinkc ...; null without them)
+    double *sinkc = nullptr, *sink_ann = nullptr, *sink_acc = nullptr, *sink_rk = nullptr;
     int parity = 0;
     hipStream_t stream = nullptr;
     Fork fork{nullptr, {nullptr, nullptr, nullptr}};  // second stream of the captured steps
@@ -128,6 +131,7 @@ static DevState make_state(disco_gpu* h, int cur, int nxt) {
     S.runs = h->druns;
     S.ncell = h->ncell;
     S.nruns = h->nruns;
+    S.nann = h->nann;
     S.prim = h->prim;
     S.cons = h->cons;
     S.rkcons = h->rkcons;
@@ -160,6 +164,10 @@ static DevState make_state(disco_gpu* h, int cur, int nxt) {
     S.src = h->src;
     S.diag = h->diag;
     S.t_avg = h->t_avg;
+    S.sinkc = h->sinkc;
+    S.sink_ann = h->sink_ann;
+    S.sink_acc = h->sink_acc;
+    S.sink_rk = h->sink_rk;
     S.Phi = h->Phi;
     S.rkPhi = h->rkPhi;
     S.emf_phi = h->emf_phi;
@@ -501,6 +509,10 @@ static int snapshot(disco_gpu* h, Snapshot* S, bool restore) {
         list.push_back({h->t_avg, n * sizeof(double)});
         list.push_back({h->planet_phi, n * DISCO_MAX_PLANETS * sizeof(double)});
         if (h->c2p_stats) list.push_back({h->c2p_stats, 4 * sizeof(unsigned long long)});
+        if (h->sink_acc) {
+            list.push_back({h->sink_acc, n * DISCO_NSINKQ * DISCO_MAX_PLANETS * sizeof(double)});
+            list.push_back({h->sink_rk, n * DISCO_NSINKQ * DISCO_MAX_PLANETS * sizeof(double)});
+        }
         for (auto& l : list) {
             void* c = nullptr;
             HIPCHK(hipMalloc(&c, l.second));
@@ -549,7 +561,8 @@ void disco_gpu_destroy(disco_gpu_t* h) {
                     h->fout,  h->fin,      h->fup,   h->fdn,    h->dtbits,  h->dt,         h->planet_phi,
                     h->t,     h->Phi,      h->rkPhi, h->emf_phi, h->emf_t,  h->esum,
                     h->efin,  h->fflags,   h->err,   h->pdg,   h->mlp_wt,  h->mlp_w,      h->c2p_stats, h->diag,
-                    h->t_avg, h->src,      h->final_step, h->bcjobs};
+                    h->t_avg, h->src,      h->final_step, h->bcjobs, h->sinkc, h->sink_ann, h->sink_acc,
+                    h->sink_rk};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -635,6 +648,13 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->diag, (size_t)h->nann * DISCO_NDIAG);
     rc |= dalloc(&h->t_avg, nruns);
     if (h->runs[0].par.hydro == DISCO_HYDRO_GRMHD) rc |= dalloc(&h->src, 4 * N);
+    constexpr size_t SQ = (size_t)DISCO_NSINKQ * DISCO_MAX_PLANETS;
+    if (h->shape.accrete) {
+        rc |= dalloc(&h->sinkc, SQ * N);
+        rc |= dalloc(&h->sink_ann, SQ * h->nann);
+        rc |= dalloc(&h->sink_acc, SQ * nruns);
+        rc |= dalloc(&h->sink_rk, SQ * nruns);
+    }
     if (rc) {
         std::string m = g_err;
         disco_gpu_destroy(h);
@@ -665,6 +685,9 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         hipMemset(h->t_avg, 0, nruns * sizeof(double)) != hipSuccess)
         return cleanup("hipMemset");
     if (hipMemset(h->err, 0, sizeof(int)) != hipSuccess) return cleanup("hipMemset");
+    if (h->shape.accrete && (hipMemset(h->sink_acc, 0, SQ * nruns * sizeof(double)) != hipSuccess ||
+                             hipMemset(h->sink_rk, 0, SQ * nruns * sizeof(double)) != hipSuccess))
+        return cleanup("hipMemset");
     if (mhd) {
         struct {
             double* p;
@@ -827,6 +850,31 @@ int disco_gpu_advance(disco_gpu_t* h, int max_steps, int poll_files, int* steps_
     return 0;
 }
 
+// The report line's accretion fields of run `run` (report.c:278-303: gravL,
+// dM, accL, Ls, kin, therm, linXmom, linYmom, each for every planet) into
+// out[8 * npl], then the accumulators restart from zero (report.c:91-106;
+// a kernel, not a memset, so that the call may be captured).  No host
+// allocation.  who: the entry point for error messages (null: fail_c).
+static int sink_line(disco_gpu* h, int run, int npl, double* out, const char* who) {
+    for (int p = 0; p < 8 * npl; ++p) out[p] = 0.0;
+    if (!h->sink_acc) return 0;
+    double acc[DISCO_NSINKQ * DISCO_MAX_PLANETS];
+    hipError_t e = hipMemcpyAsync(acc, h->sink_acc + (size_t)run * DISCO_NSINKQ * DISCO_MAX_PLANETS, sizeof acc,
+                                  hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) {
+        launch_sink_reset(make_state(h, h->parity, 1 - h->parity), run, h->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        if (who) return fail(std::string(who) + ": " + hipGetErrorString(e));
+        return fail_c("disco_gpu_report_noalloc: a kernel or copy failed");
+    }
+    for (int k = 0; k < DISCO_NSINKQ; ++k)
+        for (int p = 0; p < npl; ++p) out[k * npl + p] = acc[p * DISCO_NSINKQ + k];
+    return 0;
+}
+
 int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) {
     if (!h || run < 0 || run >= h->nruns || !out) return fail("disco_gpu_report: bad arguments");
     const HostRun& R = h->runs[run];
@@ -858,8 +906,10 @@ int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout) 
     int o = 0;
     out[o++] = t;
     out[o++] = npl > 1 ? R.planets[1].M / R.planets[0].M : 0.0;
-    // the planets' accreted and exchanged quantities (sink.c): no sinks run
-    for (int p = 0; p < 8 * npl; ++p) out[o++] = 0.0;
+    // the planets' accretion accumulators (sink.c; zero without sinks), then
+    // zeroed as report.c:91-106 does
+    if (sink_line(h, run, npl, out + o, "disco_gpu_report")) return -1;
+    o += 8 * npl;
     for (int q = 0; q < 3; ++q) out[o++] = sum[q];
     for (int q = 3; q < DISCO_NREPORT; ++q) out[o++] = sum[q];
     return 0;
@@ -897,8 +947,7 @@ int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_c("disco_gpu_report_noalloc: a kernel or copy failed");
     out[1] = npl > 1 ? R.planets[1].M / R.planets[0].M : 0.0;
-    for (int p = 0; p < 8 * npl; ++p) out[2 + p] = 0.0;
-    return 0;
+    return sink_line(h, run, npl, out + 2, nullptr);
 }
 
 // ---------------------------------------------------------------- checkpoints

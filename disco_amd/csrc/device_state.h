@@ -21,6 +21,9 @@ constexpr int DISCO_NDIAG = 8;
 // report.c's gas sums per annulus: mass, the torques of planets 1 and 0, and
 // both torques outside (n + 1) x Sink_Par3 of planet 1 for n = 0..4
 constexpr int DISCO_NREPORT = 13;
+// accumulators of one planet's accretion sink (struct planet's gravL, dM,
+// accL, Ls, kin, therm, linXmom, linYmom; report.c prints them in this order)
+constexpr int DISCO_NSINKQ = 8;
 // levels of the zero-gradient ghost fill: NUM_G r-layers then NUM_G z-layers
 constexpr int DISCO_MAX_BC_LEVELS = 8;
 // one ghost-fill job: {ghost annulus, source annulus, velocity field cut by
@@ -79,6 +82,7 @@ struct DevState {
     const RunDev* runs;
     int ncell;
     int nruns;
+    int nann;
     double* prim;
     double* cons;
     double* rkcons;
@@ -111,6 +115,13 @@ struct DevState {
     double* src;         // [4][ncell] GRMHD: source increments of S_r, L, S_z, tau (null: in the update)
     double* diag;        // [annulus][DISCO_NDIAG] time-weighted annulus averages (analysis.c theTools.Qrz)
     double* t_avg;       // per run: their accumulated time (theTools.t_avg)
+    // ---- accretion sinks (sink.c:143-270; null without them) ----
+    // per-planet quantities, in report.c's order: gravL, dM, accL, Ls, kin,
+    // therm, linXmom, linYmom (DISCO_NSINKQ per planet)
+    double* sinkc;       // [DISCO_NSINKQ * DISCO_MAX_PLANETS][ncell] this substep's contribution of each real cell
+    double* sink_ann;    // [annulus][DISCO_NSINKQ * DISCO_MAX_PLANETS] their sums per annulus
+    double* sink_acc;    // [run][DISCO_NSINKQ * DISCO_MAX_PLANETS] the planets' accumulators
+    double* sink_rk;     // the same at the start of the step (planet_RK_copy)
     // ---- constrained transport (MHD with CT; null otherwise) ----
     // Face fluxes (bfields.c, paul.h:196-216 Phi[NUM_FACES]): slot 0 = the
     // cell's phi face (at piph), 1 / 2 = its owned r face in the inner / outer
@@ -143,6 +154,7 @@ struct StepShape {
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
     int sink;         // some run cools, damps, has a nozzle or a rotating frame: the update kernels with those terms run
+    int accrete;      // some run has accretion sinks: their per-planet sums run after the update
     int visc;         // some run is viscous: the flux kernel with the viscous fluxes runs
     int stepper;      // substeps of a step (timestep.c:29-49): 2 RK2, 3 three-stage RK, 1 Euler then RK2
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)

@@ -216,6 +216,13 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->visc = par.visc_flag;
     ph->visc_alpha = par.alpha_flag;
     ph->visc_nu = par.viscosity;
+    ph->sink_type = par.sinkType;
+    ph->sink_par[0] = par.sinkPar1;
+    ph->sink_par[1] = par.sinkPar2;
+    ph->sink_par[2] = par.sinkPar3;
+    ph->sink_par[3] = par.sinkPar4;
+    ph->sink_par[4] = par.sinkPar5;
+    ph->sink_rho0 = 1.0 / (3.0 * M_PI * par.viscosity);
     ph->rot_d = par.RotD;
     ph->phi_max = par.phimax;
     ph->plm = par.PLM;
@@ -316,8 +323,9 @@ static int unsupported(const disco_params& p, std::string* err) {
     // euler.c holds the viscous terms; mhd.c and grmhd.c define no visc_source
     if (p.visc_flag && p.hydro != DISCO_HYDRO_EULER) m += " Use_Viscosity (euler only)";
     if (p.visc_flag && !p.alpha_flag && p.viscosity <= 0.0) m += " Use_Viscosity needs Viscosity > 0";
-    // add_source's sink.c terms (misc.c:579-591) are not built
-    if (p.sinkType) m += " Sink_Type";
+    // sink.c:143-270: the accretion sinks, types 1-3
+    if (p.sinkType < 0 || p.sinkType > 3) m += " Sink_Type";
+    if (p.sinkType && p.hydro == DISCO_HYDRO_GRMHD) m += " Sink_Type with grmhd";
     // sink.c:274-322: beta cooling (COOL_BETA, COOL_BETA_RELAX) of a Newtonian gas
     if (p.coolType < 0 || p.coolType > 2) m += " Cool_Type";
     if (p.coolType && p.hydro == DISCO_HYDRO_GRMHD) m += " Cool_Type with grmhd";
@@ -444,6 +452,9 @@ static void setup_planets(HostRun& R) {
         R.planets.push_back(b1);
     }
     R.ph.npl = (int)R.planets.size();
+    // sink.c:162-163
+    R.ph.sink_n = R.ph.npl;
+    if ((p.sinkNumber > 0) & (p.sinkNumber < R.ph.sink_n)) R.ph.sink_n = p.sinkNumber;
 }
 
 // domain.c:26-89: one glibc rand() phase per annulus, layout-independent
@@ -1198,6 +1209,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->zcopy = 0;
     shape->sink = 0;
     shape->visc = 0;
+    shape->accrete = 0;
     shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
                     ? 1
                     : 0;
@@ -1209,6 +1221,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
             q.RotFrame || q.visc_flag)
             shape->sink = 1;
         if (q.visc_flag) shape->visc = 1;
+        if (q.sinkType) shape->sink = shape->accrete = 1;
     }
 }
 

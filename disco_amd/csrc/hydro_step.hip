@@ -401,6 +401,64 @@ __global__ __launch_bounds__(256) void k_diag(DevState S) {
     }
 }
 
+// Accretion sinks: the substep's contributions of the cells of each real
+// annulus summed per planet quantity (one block per annulus; fixed butterfly
+// and wavefront order), zero for the other annuli.
+constexpr int DISCO_SINK_NQ = DISCO_NSINKQ * DISCO_MAX_PLANETS;
+
+__global__ __launch_bounds__(256) void k_sink_ann(DevState S) {
+    __shared__ double red[DISCO_SINK_NQ][4];
+    const int a = blockIdx.x;
+    const AnnDev& A = S.ann[a];
+    const size_t N = S.ncell;
+    double acc[DISCO_SINK_NQ];
+    for (int q = 0; q < DISCO_SINK_NQ; ++q) acc[q] = 0.0;
+    if (A.flags & ANN_REAL)
+        for (int i = threadIdx.x; i < A.np; i += blockDim.x)
+            for (int q = 0; q < DISCO_SINK_NQ; ++q) acc[q] += S.sinkc[q * N + A.off + i];
+    for (int q = 0; q < DISCO_SINK_NQ; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < DISCO_SINK_NQ; ++q) red[q][wave] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < DISCO_SINK_NQ) {
+        const int q = threadIdx.x;
+        const int nw = (blockDim.x + 63) >> 6;
+        double sq = red[q][0];
+        for (int w = 1; w < nw; ++w) sq += red[q][w];
+        S.sink_ann[(size_t)a * DISCO_SINK_NQ + q] = sq;
+    }
+}
+
+// The planets' accumulators of each run (one block per run, one thread per
+// quantity): planet_RK_copy at the start of the step (first == 1),
+// planet_RK_adjust_aux's average (planet.c:236-270, onestep.c:94), then the
+// substep's sum over the run's annuli in table order.
+__global__ void k_sink_run(DevState S, double RK, int first) {
+    const int r = blockIdx.x, q = threadIdx.x;
+    if (r >= S.nruns || q >= DISCO_SINK_NQ) return;
+    const RunDev& Rn = S.runs[r];
+    double sum = 0.0;
+    for (int a = Rn.ann0; a < Rn.ann0 + Rn.nann; ++a) sum += S.sink_ann[(size_t)a * DISCO_SINK_NQ + q];
+    double* acc = S.sink_acc + (size_t)r * DISCO_SINK_NQ;
+    double* rk = S.sink_rk + (size_t)r * DISCO_SINK_NQ;
+    double v = acc[q];
+    if (first == 1) rk[q] = v;
+    v = (1. - RK) * v + RK * rk[q];
+    acc[q] = v + sum;
+}
+
+// report.c:91-106: the accumulators (and their RK copies) of run r restart
+// from zero after a report
+__global__ void k_sink_reset(DevState S, int r) {
+    const int q = threadIdx.x;
+    if (q < DISCO_SINK_NQ) {
+        S.sink_acc[(size_t)r * DISCO_SINK_NQ + q] = 0.0;
+        S.sink_rk[(size_t)r * DISCO_SINK_NQ + q] = 0.0;
+    }
+}
+
 // report.c's gas sums (report_quantities) of the annuli a0, a0+1, ... of one
 // run: one block per annulus, each writing its DISCO_NREPORT sums (zero for
 // ghost annuli) to the annulus' slot j + Nr k of `out`; the annuli are then
@@ -478,6 +536,12 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
         if (hook) hook->after(hook->ctx, name);                    \
     } while (0)
 
+// the accretion sinks' per-planet sums of a substep (after the cell update)
+static void sink_sums(const DevState& S, double RK, int first, hipStream_t st, const LaunchHook* hook) {
+    DISCO_LAUNCH(hook, "k_sink_ann", k_sink_ann, dim3(S.nann), dim3(256), st, S);
+    DISCO_LAUNCH(hook, "k_sink_run", k_sink_run, dim3(S.nruns), dim3(64), st, S, RK, first);
+}
+
 template <class Ph>
 struct Launch {
     // Full CFL reduction over the current state (first step, getmindt calls).
@@ -530,6 +594,7 @@ struct Launch {
             else
                 DISCO_LAUNCH(hook, "k_update_a", (k_update_a<Ph, false>), G(KC_CT), dim3(B(KC_CT)), su, S, RK, dtfac,
                              first);
+            if (sh.accrete) sink_sums(S, RK, first, su, hook);
             DISCO_LAUNCH(hook, "k_emf_sum", k_emf_sum, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             DISCO_LAUNCH(hook, "k_emf_face", k_emf_face, G(KC_EMF), dim3(B(KC_EMF)), st, S, sh.has_z);
             DISCO_LAUNCH(hook, "k_ct_update", k_ct_update, G(KC_CT), dim3(B(KC_CT)), st, S, RK, dtfac, first,
@@ -555,6 +620,7 @@ struct Launch {
             else
                 DISCO_LAUNCH(hook, "k_update", (k_update<Ph, false>), G(KC_UPDATE), dim3(B(KC_UPDATE)), st, S, RK,
                              dtfac, first, want_dt);
+            if (sh.accrete) sink_sums(S, RK, first, st, hook);
         }
         // boundary_trans (onestep.c:166-173): zero-gradient ghosts level by level, then the periodic z copy
         for (int l = 0; l < sh.bc_nlev; ++l)
@@ -570,6 +636,10 @@ void launch_diag(const DevState& S, int nann, int nq, hipStream_t st, const Laun
     (void)nq;
     DISCO_LAUNCH(hook, "k_diag", k_diag, dim3(nann), dim3(256), st, S);
     DISCO_LAUNCH(hook, "k_diag_time", k_diag_time, dim3(nblk(S.nruns, 64)), dim3(64), st, S);
+}
+
+void launch_sink_reset(const DevState& S, int run, hipStream_t st) {
+    hipLaunchKernelGGL(k_sink_reset, dim3(1), dim3(64), 0, st, S, run);
 }
 
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st) {

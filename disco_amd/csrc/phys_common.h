@@ -49,6 +49,10 @@ struct PhysParams {
     int visc;                 // Use_Viscosity: euler.c's viscous fluxes, source and time step
     int visc_alpha;           // Use_As_Alpha: nu = alpha c_s^2 / Omega_height
     double visc_nu;           // Viscosity: nu, or alpha with Use_As_Alpha
+    int sink_type;            // Sink_Type: 0 none, 1 polynomial, 2 exponential, 3 constant (sink.c:143-270)
+    int sink_n;               // planets that accrete: Sink_Number when 0 < it < Npl, else every planet
+    double sink_par[5];       // Sink_Par1-5: rate / planet omega, delta, size, powers
+    double sink_rho0;         // 1 / (3 pi Viscosity): reference density of the sinks' torque accounting
 };
 
 // EMFs of one face as flux_to_E returns them (mhd.c:293-316): phi face

@@ -868,13 +868,15 @@ DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double
 // relative to the frame speed r get_om.  The clip compares the injected mass
 // with the cell's mass and then sets the rate to cons[DDD] / dV * dt, as the
 // reference writes it.
-DHD void nozzle_cell(const PhysParams& P, double* cons, const double* xp, const double* xm, double dV, double dt) {
+// Returns false for a cell outside the ball: sink_src returns there before
+// its accretion sinks.
+DHD bool nozzle_cell(const PhysParams& P, double* cons, const double* xp, const double* xm, double dV, double dt) {
     const double R0 = P.nozzle_par[0], dR = P.nozzle_par[1], v = P.nozzle_par[2], mach = P.nozzle_par[3];
     const double r = 0.5 * (xp[0] + xm[0]);
     double phi = 0.5 * (xp[1] + xm[1]);
     const double z = 0.5 * (xp[2] + xm[2]);
     if (phi > M_PI) phi -= 2 * M_PI;
-    if ((r - R0) * (r - R0) + r * r * phi * phi + z * z > dR * dR) return;
+    if ((r - R0) * (r - R0) + r * r * phi * phi + z * z > dR * dR) return false;
     double rhodot = 1.0 * 3.0 / (4.0 * M_PI * dR * dR * dR);
     if (P.two_d) rhodot *= 2 * sqrt(dR * dR - (r - R0) * (r - R0) - r * r * phi * phi);
     if (rhodot * dV * dt > cons[DDD]) rhodot = cons[DDD] / dV * dt;
@@ -885,6 +887,96 @@ DHD void nozzle_cell(const PhysParams& P, double* cons, const double* xp, const 
     cons[DDD] += rhodot * dV * dt;
     cons[LLL] += rhodot * r * v * dV * dt;
     cons[TAU] += (0.5 * rhodot * (v - r * om) * (v - r * om) + rhodot * eps) * dV * dt;
+    return true;
+}
+
+// sink.c:140-270 (the accretion sinks) for one real cell: each accreting
+// planet removes mass at rate Sink_Par1 x its omega x the kernel of its
+// distance (1: (1 - (d/eps)^M)^N inside eps, 2: exp(-d^P / (eps^P f)), 3: 1
+// inside Sink_Par3), with the momentum of the gas velocity relative to the
+// planet's (Sink_Par2 = delta of it removed in the planet's direction, the
+// rest in the tangential one), its kinetic and thermal energy.  The planet's
+// accumulators gain the removed quantities and the torque accounting
+// -(rho - rho0) r_p f_phi dV dt; this cell's contributions go to
+// S.sinkc[(p * DISCO_NSINKQ + k) * ncell + c] (zero for planets that do not
+// accrete, and for every planet when `active` is false: the cell lies outside
+// the nozzle's ball).  The planets' radial speed is zero for the built
+// planet modules.
+DHD void sink_cell(const DevState& S, const RunDev& Rn, int run, int c, bool active, const double* prim, double* cons,
+                   const double* xp, const double* xm, double dV, double dt) {
+    const PhysParams& P = Rn.ph;
+    const size_t N = S.ncell;
+    const double x[3] = {get_centroid(xp[0], xm[0], 1), get_centroid(xp[1], xm[1], 0), get_centroid(xp[2], xm[2], 2)};
+    const double r = x[0], phi = x[1];
+    const double rho = prim[RHO], vr = prim[URR], vp = prim[UPP] * r, vz = prim[UZZ];
+    const double cosg = cos(phi), sing = sin(phi);
+    const double gx = r * cosg, gy = r * sing;
+    for (int p = 0; p < DISCO_MAX_PLANETS; ++p) {
+        double a[DISCO_NSINKQ] = {};
+        if (active && p < P.sink_n) {
+            Planet pl = Rn.planets[p];
+            pl.phi = S.planet_phi[run * DISCO_MAX_PLANETS + p];
+            const double cosp = cos(pl.phi), sinp = sin(pl.phi);
+            const double px = pl.r * cosp, py = pl.r * sinp;
+            const double dx = gx - px, dy = gy - py;
+            double mag = dx * dx + dy * dy;
+            mag = sqrt(mag);
+            double arg = 0.0;
+            if (P.sink_type == 3) {
+                if (mag <= P.sink_par[2]) arg = 1.0;
+            } else if (P.sink_type == 2) {
+                const double eps = pow(P.sink_par[2], P.sink_par[3]);
+                double epsfactor = P.sink_par[4];
+                if (epsfactor <= 0.0) epsfactor = 1.0;
+                const double magPow = pow(mag, P.sink_par[3]);
+                arg = exp(-magPow / (eps * epsfactor));
+            } else if (P.sink_type == 1) {
+                const double eps = P.sink_par[2];
+                arg = 1.0 - pow((mag / eps), P.sink_par[3]);
+                arg = pow(arg, P.sink_par[4]);
+                if (mag >= eps) arg = 0.0;
+            }
+            const double rate = P.sink_par[0] * pl.omega;
+            const double surfdiff = rho * rate * arg;
+            const double delta = fmax(0.0, P.sink_par[1]);  // sink.c:206-207 keeps only the lower clip
+            const double vp_p = pl.r * pl.omega, vp_r = 0.0;
+            const double vxp = vp_r * cosp - vp_p * sinp;
+            const double vyp = vp_r * sinp + vp_p * cosp;
+            double vxg = vr * cosg - vp * sing;
+            double vyg = vr * sing + vp * cosg;
+            const double vxr = vxg - vxp, vyr = vyg - vyp;
+            const double cphi = dx / mag, sphi = dy / mag;
+            const double acc_factor = dV * dt * surfdiff;
+            const double vpr = cphi * vyr - sphi * vxr;
+            a[3] = (1.0 - delta) * mag * vpr * acc_factor;  // Ls
+            const double vxn = (cphi * cphi + (1.0 - delta) * sphi * sphi) * vxr + delta * sphi * cphi * vyr;
+            const double vyn = delta * cphi * sphi * vxr + (sphi * sphi + (1.0 - delta) * cphi * cphi) * vyr;
+            vxg = vxn + vxp;
+            vyg = vyn + vyp;
+            double vg_r = vxg * cosg + vyg * sing;
+            double vg_p = -vxg * sing + vyg * cosg;
+            a[2] = vg_p * r * acc_factor;  // accL
+            cons[DDD] -= acc_factor;
+            if (delta == 0.0) {
+                vg_r = vr;
+                vg_p = vp;
+            }
+            cons[SRR] -= vg_r * acc_factor;
+            cons[LLL] -= r * vg_p * acc_factor;
+            cons[SZZ] -= vz * acc_factor;
+            const double v2 = vg_p * vg_p + vg_r * vg_r + vz * vz;
+            cons[TAU] -= acc_factor * (0.5 * v2 + prim[PPP] / (rho * (P.gamma_law - 1.0)));
+            a[1] = acc_factor;                                           // dM
+            a[4] = 0.5 * v2 * acc_factor;                                // kin
+            a[5] = prim[PPP] * acc_factor / (rho * (P.gamma_law - 1.0));  // therm
+            a[6] = acc_factor * vxg;                                     // linXmom
+            a[7] = acc_factor * vyg;                                     // linYmom
+            double fr, fp, fz;
+            planetary_force(pl, P.grav2D, r, phi, 0.0, &fr, &fp, &fz, 1);
+            a[0] = -((rho - P.sink_rho0) * pl.r * fp * dV * dt);  // gravL
+        }
+        _Pragma("unroll") for (int k = 0; k < DISCO_NSINKQ; ++k) S.sinkc[(p * DISCO_NSINKQ + k) * N + c] = a[k];
+    }
 }
 
 // First half of the update: the substep's start state (timestep.c:18-24's
@@ -978,8 +1070,11 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
             }
         }
         if constexpr (Ph::NEWTONIAN && SINK) {
-            omega_src(P, prim, cons, xp, xm, dV * dt);                  // misc.c:588
-            if (P.nozzle_type) nozzle_cell(P, cons, xp, xm, dV, dt);  // misc.c:589, after the planets
+            omega_src(P, prim, cons, xp, xm, dV * dt);  // misc.c:588
+            // misc.c:589 (sink_src): the nozzle, then the accretion sinks
+            bool in_ball = true;
+            if (P.nozzle_type) in_ball = nozzle_cell(P, cons, xp, xm, dV, dt);
+            if (S.sinkc) sink_cell(S, Rn, A.run, c, P.sink_type != 0 && in_ball, prim, cons, xp, xm, dV, dt);
         }
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.cons[q * N + c], cons[q]);
     }

@@ -108,11 +108,13 @@ int disco_gpu_restart(disco_gpu_t* h, int run, const char* path);
 /* report (report.c:6-326) of run `run` at the current state: the values of
  * its report.dat line, in order: t, q = M_1/M_0 (0 with fewer than two
  * planets), the planets' Lg, M_acc, La, Ls, kin, therm, xMom, yMom (Npl each;
- * sink.c's accumulators, zero: sinks are not run), Mass, Torque, Torque2,
- * then Torque1_cut[n], Torque2_cut[n] for n = 0..4 (cut at (n+1) x
- * Sink_Par3).  *nout = 15 + 8 Npl; fails if nmax is smaller.  The gas sums
- * run on the device (one block per annulus) and are added on the host in the
- * reference's loop order.  Synchronous; changes no state. */
+ * the accretion sinks' accumulators since the last report, sink.c:143-270,
+ * zero without sinks), Mass, Torque, Torque2, then Torque1_cut[n],
+ * Torque2_cut[n] for n = 0..4 (cut at (n+1) x Sink_Par3).  *nout = 15 +
+ * 8 Npl; fails if nmax is smaller.  The gas sums run on the device (one
+ * block per annulus) and are added on the host in the reference's loop
+ * order.  Synchronous; like report.c:91-106 it restarts the sink
+ * accumulators from zero, and changes nothing else. */
 int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
 
 /* No counterpart in the reference.  For callers that cannot allocate (HIP
@@ -123,7 +125,8 @@ int disco_gpu_report(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
  * at most 65536 annuli (Nr x Nz, ghosts included) per run, larger runs fail
  * with a message.  The scratch is set up by disco_gpu_create on the current
  * device and shared by every call on that GPU: calls that overlap on one GPU
- * (from several threads or handles) race. */
+ * (from several threads or handles) race.  Restarts the sink accumulators
+ * like disco_gpu_report (with a kernel, no memset). */
 int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int* nout);
 
 /* timestep(dom, dt) with caller-given dt per run (no CFL reduction). */

@@ -79,10 +79,10 @@ typedef struct disco_params {
     /* module selection (compile-time in the reference) */
     int hydro, boundary, planets, initial;
     int num_c, num_n;
-    /* sink.c's source terms: beta cooling (Cool_Type 1, 2), the damping
-       zones (Damp*_Type 1, 2) and the mass nozzle (Nozzle_Type 1) are built,
-       the sinks are not (a non-zero type is rejected);
-       Sink_Par3 is also report.c's torque-cut length */
+    /* sink.c's source terms: the accretion sinks (Sink_Type 1-3), beta
+       cooling (Cool_Type 1, 2), the damping zones (Damp*_Type 1, 2) and the
+       mass nozzle (Nozzle_Type 1); Sink_Par3 is also report.c's torque-cut
+       length */
     int sinkType, sinkNumber;
     double sinkPar1, sinkPar2, sinkPar3, sinkPar4, sinkPar5;
     int coolType, nozzleType, dampInnerType, dampOuterType, dampUpperType, dampLowerType;

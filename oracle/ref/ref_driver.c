@@ -22,6 +22,9 @@
  *   With the environment variable DISCO_REF_TOTALS set, every step (and
  *   step 0) also prints "tot STEP DT Q0 .. Q(NUM_Q-1)": the conserved totals
  *   over the real cells (jk order, then array order) with 17 digits.
+ *   With DISCO_REF_SINKS set, every step also prints, per planet p,
+ *   "sink STEP p gravL dM accL Ls kin therm linXmom linYmom": the accretion
+ *   sinks' accumulators (sink.c:143-270) as report.c would print them.
  */
 #include "paul.h"
 #include <stdint.h>
@@ -101,6 +104,16 @@ static void print_totals(struct domain* d, int step, double dt) {
     fflush(stdout);
 }
 
+static void print_sinks(struct domain* d, int step) {
+    int p;
+    for (p = 0; p < d->Npl; ++p) {
+        const struct planet* pl = d->thePlanets + p;
+        printf("sink %d %d %.17e %.17e %.17e %.17e %.17e %.17e %.17e %.17e\n", step, p, pl->gravL, pl->dM, pl->accL,
+               pl->Ls, pl->kin, pl->therm, pl->linXmom, pl->linYmom);
+    }
+    fflush(stdout);
+}
+
 static double now(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -132,6 +145,7 @@ int main(int argc, char* argv[]) {
     }
     if (strcmp(prefix, "-") != 0) dump(&theDomain, prefix, 0, 0.0);
     const int want_totals = getenv("DISCO_REF_TOTALS") != NULL;
+    const int want_sinks = getenv("DISCO_REF_SINKS") != NULL;
     if (want_totals) print_totals(&theDomain, 0, 0.0);
 
     long ncells = 0;
@@ -154,6 +168,7 @@ int main(int argc, char* argv[]) {
             print_totals(&theDomain, n, dt);
             t0 = now();
         }
+        if (want_sinks) print_sinks(&theDomain, n);
         int want = to_end ? theDomain.final_step : (n == nsteps);
         int a;
         for (a = 0; a < ndump; ++a)

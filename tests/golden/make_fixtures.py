@@ -48,6 +48,24 @@ def run_ref(variant: str, par_text: str, nsteps: int, dumps=()):
     return states, float(m.group(1))
 
 
+def run_ref_sinks(variant: str, par_text: str, nsteps: int, dumps=()):
+    """run_ref with the driver's sink lines (DISCO_REF_SINKS): states, and the
+    planets' accretion accumulators after every step, [step - 1][planet][8]
+    in report.c's order gravL, dM, accL, Ls, kin, therm, linXmom, linYmom."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_{variant}")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "in.par"), "w").write(par_text)
+        out = subprocess.run([exe, str(nsteps), "out", *map(str, dumps)], cwd=d, check=True, timeout=600,
+                             capture_output=True, text=True, env={**os.environ, "DISCO_REF_SINKS": "1"}).stdout
+        states = {s: read_dump(os.path.join(d, f"out_{s}.bin")) for s in [0, *dumps, nsteps]}
+    rows = [ln.split()[1:] for ln in out.splitlines() if ln.startswith("sink ")]
+    npl = max(int(r[1]) for r in rows) + 1
+    acc = np.zeros((nsteps, npl, 8))
+    for r in rows:
+        acc[int(r[0]) - 1, int(r[1])] = [float(v) for v in r[2:]]
+    return states, acc
+
+
 def run_ref_to_end(variant: str, par_text: str):
     """The reference's own loop to T_End (driver mode NSTEPS = 0): states at
     step 0 and at the final step."""
@@ -321,6 +339,31 @@ def fixtures_visc() -> None:
         save_states(name, states)
 
 
+SINK_CASES = [  # sink.c:143-270, the accretion sinks (Viscosity only sets the torque accounting's rho0)
+    ("c3_sink1.npz", "c3_binary_disk.par", dict(Num_R=24, Viscosity=1e-3, Sink_Type=1, Sink_Par1=1.0, Sink_Par2=0.5,
+                                                 Sink_Par3=0.9, Sink_Par4=2.0, Sink_Par5=2.0), "euler_binary", 10, (1,)),
+    ("c3_sink2.npz", "c3_binary_disk.par", dict(Num_R=24, Viscosity=1e-3, Sink_Type=2, Sink_Number=1, Sink_Par1=0.5,
+                                                 Sink_Par2=0.0, Sink_Par3=0.5, Sink_Par4=2.0, Sink_Par5=1.0),
+     "euler_binary", 10, (1,)),
+    ("c3_sink3.npz", "c3_binary_disk.par", dict(Num_R=24, Viscosity=1e-3, Sink_Type=3, Sink_Par1=0.3, Sink_Par2=1.0,
+                                                 Sink_Par3=0.7), "euler_binary", 5, (1,)),
+    # inside sink_src the nozzle returns early outside its ball, so with a nozzle the sinks act only inside it
+    ("c1_sink_nozzle.npz", "c1_kepler2d.par", dict(Num_R=16, Viscosity=1e-3, Sink_Type=2, Sink_Par1=0.5, Sink_Par2=0.3,
+                                                    Sink_Par3=1.0, Sink_Par4=2.0, Sink_Par5=1.0, Nozzle_Type=1,
+                                                    Nozzle_Par1=1.5, Nozzle_Par2=0.4, Nozzle_Par3=0.8,
+                                                    Nozzle_Par4=10.0), "euler_sun", 5, (1,)),
+    ("c2_mhd_small_sink.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Viscosity=1e-3, Sink_Type=2,
+                                                    Sink_Par1=0.5, Sink_Par2=0.3, Sink_Par3=1.0, Sink_Par4=2.0,
+                                                    Sink_Par5=1.0), "mhd_sun", 5, (1,)),
+]
+
+
+def fixtures_sinks() -> None:
+    for name, base, over, variant, n, dumps in SINK_CASES:
+        states, acc = run_ref_sinks(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states, sink_acc=acc)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -472,6 +515,10 @@ def main() -> None:
         return
     if "--visc" in sys.argv:
         fixtures_visc()
+        print("fixtures written")
+        return
+    if "--sinks" in sys.argv:
+        fixtures_sinks()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

@@ -249,11 +249,11 @@ def test_unsupported_configuration_is_rejected():
         D.setup_run(params("c1_kepler2d.par", Use_Viscosity=1))
     with pytest.raises(RuntimeError, match="Include_Atmos"):
         D.setup_run(params("c1_kepler2d.par", Include_Atmos=1))
-    # sink.c terms that are not built are refused, not ignored
+    # sink.c's types beyond the built ones are refused, not ignored
     for key, msg in (("Sink_Type", "Sink_Type"), ("Nozzle_Type", "Nozzle_Type"), ("DampInner_Type", "Damp"),
                      ("Cool_Type", "Cool_Type")):
         with pytest.raises(RuntimeError, match=msg):
-            D.setup_run(params("c1_kepler2d.par", **{key: 3}))
+            D.setup_run(params("c1_kepler2d.par", **{key: 4}))
     with pytest.raises(RuntimeError, match="Cool_Par1"):
         D.setup_run(params("c1_kepler2d.par", Cool_Type=1))
     with pytest.raises(RuntimeError, match="alone in 2D"):
@@ -267,6 +267,9 @@ def test_unsupported_configuration_is_rejected():
     (dict(Metric_Par3=1.2), "Kerr needs"),
     (dict(Mesh_Motion=1), "grmhd: 0 or 4"),
     (dict(Constrained_Transport=0), "Constrained_Transport"),
+    (dict(Sink_Type=1), "Sink_Type with grmhd"),
+    (dict(RotFrame=1), "RotFrame with grmhd"),
+    (dict(Use_Viscosity=1, Viscosity=0.1), "euler only"),
 ])
 def test_unsupported_grmhd_configuration_is_rejected(over, msg):
     """GRMHD set-ups the step does not run are refused with the key named."""
