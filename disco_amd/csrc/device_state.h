@@ -39,6 +39,8 @@ enum AnnFlags {
     ANN_UPFLUX = 32,     // the z-track (this | up) carries fluxes
     ANN_DNFLUX = 64,     // the z-track (down | this) carries fluxes
     ANN_ZCOPY = 128,     // periodic-z ghost: copy of annulus `zsrc` after each substep
+    ANN_GEOM_R = 256,    // innermost annulus of a run without inner ghosts (NoBC_Rmin) whose inner edge is
+                         // r = 0: plm_geom_boundary's geom_grad fixes its radial gradients
 };
 
 // Transverse directions of the overlap faces (the reference's dim 1 and 2).

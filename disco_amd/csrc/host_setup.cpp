@@ -213,6 +213,7 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->en_om_choice = par.Energy_Omega;
     ph->omega0 = par.RotOmega;
     ph->rot_frame = par.RotFrame;
+    ph->polar_sources = par.NoBC_Rmin == 1;
     ph->visc = par.visc_flag;
     ph->visc_alpha = par.alpha_flag;
     ph->visc_nu = par.viscosity;
@@ -310,11 +311,13 @@ static int unsupported(const disco_params& p, std::string* err) {
     } else if (p.hydro == DISCO_HYDRO_EULER && p.Riemann_Solver == DISCO_HLLD) {
         m += " Riemann_Solver=HLLD needs mhd (riemann.c:19-22)";
     }
-    if (p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) m += " NoBC_*";
+    // geometric boundaries without ghost annuli (gridsetup.c:34-51), euler
+    if ((p.NoBC_Rmin || p.NoBC_Rmax || p.NoBC_Zmin || p.NoBC_Zmax) && p.hydro != DISCO_HYDRO_EULER)
+        m += " NoBC_* (euler only)";
     if (p.R_Periodic) m += " R_Periodic";
     // a serial run fills ghost z-layers only through periodic copies or the
     // fixed3d boundary (Boundary/fixed.c leaves them unset)
-    if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED)
+    if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED && !(p.NoBC_Zmin && p.NoBC_Zmax))
         m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
@@ -1134,6 +1137,10 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
                     A.zsrc = ann0 + pos[j + R.Nr * ks];
                     f |= ANN_ZCOPY;
                 }
+                // plm.c:184-189 with cylindrical.c:141-156: geom_grad acts on the
+                // inner edge only, and only where it lies at r = 0
+                if (R.NgRa == 0 && j == 0 && !(R.rj(j) < 0.0 || std::fabs(R.rj(j - 1)) > 1.0e-10 * R.rj(j)))
+                    f |= ANN_GEOM_R;
                 A.flags = f;
                 A.rm = R.rj(j - 1);
                 A.rp = R.rj(j);

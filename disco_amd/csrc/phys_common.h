@@ -26,6 +26,7 @@ struct PhysParams {
     int en_om_choice;   // Energy_Omega
     double omega0;      // RotOmega
     int rot_frame;      // RotFrame: the rotating frame's forces (rotframe.c) are on
+    int polar_sources;  // NoBC_Rmin == 1: euler.c's and planet.c's sources integrated as at r = 0
     double rot_d;       // RotD: distance of the rotation axis from the grid's axis
     double phi_max;
     double plm;
@@ -223,6 +224,13 @@ DHD double get_cs2(const PhysParams& P, const double* x) {
 }
 
 // ---------------------------------------------------------------- gravity
+// cylindrical.c:175-183 (geom_polar_vec_adjust): sin(dphi/2) / (dphi/2) for
+// the r and phi components (1 for z)
+DHD double polar_adjust(const double* xp, const double* xm, double phi_max) {
+    const double dphi = get_dp(xp[1], xm[1], phi_max);
+    return sin(0.5 * dphi) / (0.5 * dphi);
+}
+
 // planet.c:97-102 (PLPOINTMASS, PHI_ORDER = 2)
 DHD double fgrav_pointmass(double M, double r, double eps) {
     double n = 2.0;

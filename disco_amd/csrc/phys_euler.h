@@ -255,7 +255,8 @@ DHD double visc_dt(const PhysParams& P, double nu, const double* xp, const doubl
     return 0.5 * dx * dx / nu;
 }
 
-// euler.c:172-213 (polar sources are off: NoBC_Rmin = 0)
+// euler.c:172-213 (with NoBC_Rmin the centrifugal term is integrated over
+// the cell as at r = 0: polar_sources)
 DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                 double dVdt) {
     double rp = xp[0];
@@ -270,6 +271,7 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double vz = prim[UZZ];
     double x[3] = {r, 0.5 * (xm[1] + xp[1]), z};
     double centrifugal = rho * omega * omega * r;
+    if (P.polar_sources) centrifugal *= polar_adjust(xp, xm, P.phi_max);  // euler.c:195-200
     double press_bal = Pp / r_1;
     cons[SRR] += dVdt * (centrifugal + press_bal);
     double om = get_om(P, x);
@@ -290,6 +292,11 @@ DHD void planet_src(const PhysParams& P, const Planet& pl, const double* prim, d
     double x[3] = {get_centroid(xp[0], xm[0], 1), phi, get_centroid(xp[2], xm[2], 2)};
     double F[3];
     planetary_force(pl, P.grav2D, x[0], x[1], x[2], &F[0], &F[1], &F[2]);
+    if (P.polar_sources) {  // planet.c:217-227, cylindrical: the r and phi components
+        const double adjust = polar_adjust(xp, xm, P.phi_max);
+        F[0] *= adjust;
+        F[1] *= adjust;
+    }
     double hr = 1.0;
     double hp = x[0];
     double hz = 1.0;

@@ -568,6 +568,7 @@ DHD void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P,
     double dAtot = dAp + dAm;
     if (alo < 0) dAtot = dAp;
     if (ahi < 0) dAtot = dAm;
+    const bool geom = D == DIR_R && (A.flags & ANN_GEOM_R);
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double g = acc.sum[q] / dAtot;
         if (!acc.any) {
@@ -579,6 +580,19 @@ DHD void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P,
             } else {
                 double lim = P.plm * acc.smin[q];
                 if (fabs(lim) < fabs(g)) g = lim;
+            }
+        }
+        if (geom) {
+            // plm.c:184-189 + geom_grad (cylindrical.c:141-173) at r = 0: the
+            // radial components limited by prim / r, every other field flat
+            if (q == URR || (NQ > BZZ && q == BRR)) {
+                const double SL = S.prim[q * N + c] / A.rc;
+                if (g * SL < 0.0)
+                    g = 0.0;
+                else if (fabs(P.plm * SL) < fabs(g))
+                    g = P.plm * SL;
+            } else {
+                g = 0.0;
             }
         }
         st_stream(&grad[q * N + c], g);

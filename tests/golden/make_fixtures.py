@@ -364,6 +364,24 @@ def fixtures_sinks() -> None:
         save_states(name, states, sink_acc=acc)
 
 
+NOBC_CASES = [  # geometric boundaries without ghost annuli (gridsetup.c:34-51; euler)
+    # r = 0 on the binary disk: geom_grad of the innermost annulus and the polar sources
+    ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3), "euler_binary",
+     10, (1, 5)),
+    # NoBC_Rmin away from r = 0 (polar sources, geom_grad inactive) and NoBC_Rmax
+    ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1), "euler_sun", 10, (1,)),
+    ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1), "euler_sun", 10, (1,)),
+    # no z ghosts at either end (the fixed module then needs none)
+    ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1), "euler_sun", 5, (1,)),
+]
+
+
+def fixtures_nobc() -> None:
+    for name, base, over, variant, n, dumps in NOBC_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -519,6 +537,10 @@ def main() -> None:
         return
     if "--sinks" in sys.argv:
         fixtures_sinks()
+        print("fixtures written")
+        return
+    if "--nobc" in sys.argv:
+        fixtures_nobc()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

@@ -86,6 +86,12 @@ CASES = [
                                                     Viscosity=0.1)),
     ("c3z_visc_alpha.npz", "c3_binary_disk.par", dict(Num_R=16, Num_Z=6, Z_Periodic=1, Use_Viscosity=1,
                                                        Use_As_Alpha=1, Viscosity=0.05)),
+    # geometric boundaries without ghost annuli: r = 0 (geom_grad of the innermost annulus, polar sources),
+    # NoBC_Rmin away from r = 0, NoBC_Rmax, and no z ghosts
+    ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3)),
+    ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1)),
+    ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
+    ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
 ]
 
 

@@ -196,6 +196,10 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                                                                  Noise_Rel=1e-2)),
                        ("c2e_noise3.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Noise_Type=3,
                                                                   Noise_Abs=1e-3, Noise_Rel=1e-2)),
+                       # geometric boundaries without ghost annuli
+                       ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3)),
+                       ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
+                       ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
                        # the rotating frame's initial and fixed-ghost data
                        ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05))]
 
@@ -268,6 +272,7 @@ def test_unsupported_configuration_is_rejected():
     (dict(Mesh_Motion=1), "grmhd: 0 or 4"),
     (dict(Constrained_Transport=0), "Constrained_Transport"),
     (dict(Sink_Type=1), "Sink_Type with grmhd"),
+    (dict(NoBC_Rmax=1), "euler only"),
     (dict(RotFrame=1), "RotFrame with grmhd"),
     (dict(Use_Viscosity=1, Viscosity=0.1), "euler only"),
 ])
