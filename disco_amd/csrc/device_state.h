@@ -156,6 +156,7 @@ struct StepShape {
     int ct;     // MHD with constrained transport
     int c2p_learned;  // GRMHD: the cons2prim Newton starts from the guess network
     int sink;         // some run cools, damps, has a nozzle or a rotating frame: the update kernels with those terms run
+    int wmean;        // some run moves its annuli at their mean speed (Mesh_Motion 3)
     int accrete;      // some run has accretion sinks: their per-planet sums run after the update
     int visc;         // some run is viscous: the flux kernel with the viscous fluxes runs
     int stepper;      // substeps of a step (timestep.c:29-49): 2 RK2, 3 three-stage RK, 1 Euler then RK2

@@ -300,8 +300,8 @@ static int unsupported(const disco_params& p, std::string* err) {
         // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
         if (!p.CT) m += " mhd/grmhd need Constrained_Transport 1";
         if (p.Num_Z < 2) m += " mhd/grmhd with CT_MODE 2 need Num_Z > 1";
-        if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4 && !(p.hydro == DISCO_HYDRO_MHD && p.Mesh_Motion == 1))
-            m += " Mesh_Motion (mhd: 0, 1 or 4; grmhd: 0 or 4)";
+        if (p.hydro == DISCO_HYDRO_GRMHD && p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
+            m += " Mesh_Motion (grmhd: 0 or 4)";
     }
     if (p.hydro == DISCO_HYDRO_GRMHD) {
         // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one
@@ -320,7 +320,7 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.Num_Z > 1 && !p.Z_Periodic && p.boundary == DISCO_BC_FIXED && !(p.NoBC_Zmin && p.NoBC_Zmax))
         m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
-    if (p.Mesh_Motion != 0 && p.Mesh_Motion != 1 && p.Mesh_Motion != 4) m += " Mesh_Motion";
+    if (p.Mesh_Motion < 0 || p.Mesh_Motion > 4) m += " Mesh_Motion";
     if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
         m += " Riemann_Solver";
     // euler.c holds the viscous terms; mhd.c and grmhd.c define no visc_source
@@ -675,7 +675,7 @@ static void set_wcell(HostRun& R) {
                 double phip = R.piph[o + i];
                 double phim = phip - R.dphi[o + i];
                 double x[3] = {r, 0.5 * (phim + phip), z};
-                if (R.par.Mesh_Motion == 1) {
+                if (R.par.Mesh_Motion == 1 || R.par.Mesh_Motion == 2 || R.par.Mesh_Motion == 3) {
                     int ip = (i + 1) % n;
                     double wL = R.prim[(size_t)UPP * R.ncell + o + i];
                     double wR = R.prim[(size_t)UPP * R.ncell + o + ip];
@@ -684,6 +684,12 @@ static void set_wcell(HostRun& R) {
                     w = mesh_om(R.ph, x);
                 }
                 R.wiph[o + i] = w;
+            }
+            if (R.par.Mesh_Motion == 3) {  // misc.c:137-150: the annulus mean
+                double w = 0.0;
+                for (int i = 0; i < n; ++i) w += R.wiph[o + i];
+                w /= (double)n;
+                for (int i = 0; i < n; ++i) R.wiph[o + i] = w;
             }
         }
     }
@@ -1217,6 +1223,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->sink = 0;
     shape->visc = 0;
     shape->accrete = 0;
+    shape->wmean = 0;
     shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
                     ? 1
                     : 0;
@@ -1228,6 +1235,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
             q.RotFrame || q.visc_flag)
             shape->sink = 1;
         if (q.visc_flag) shape->visc = 1;
+        if (q.Mesh_Motion == 3) shape->wmean = 1;
         if (q.sinkType) shape->sink = shape->accrete = 1;
     }
 }

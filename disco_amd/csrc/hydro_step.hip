@@ -160,6 +160,23 @@ __global__ __launch_bounds__(256) void k_partners(DevState S) {
     if (c < S.ncell) partners_cell(S, c);
 }
 
+// set_wcell's Mesh_Motion 3 (misc.c:137-150): every cell of an annulus
+// moves at the mean of its cells' speeds, summed in array order (one block
+// per annulus; the annuli of other runs keep theirs)
+__global__ void k_wmean(DevState S) {
+    const int a = blockIdx.x;
+    const AnnDev& A = S.ann[a];
+    if (S.runs[A.run].ph.mesh_motion != 3) return;
+    __shared__ double wm;
+    if (threadIdx.x == 0) {
+        double w = 0.0;
+        for (int i = 0; i < A.np; ++i) w += S.wiph[A.off + i];
+        wm = w / (double)A.np;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < A.np; i += blockDim.x) S.wiph[A.off + i] = wm;
+}
+
 template <class Ph>
 __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::PLM) void k_plm_trans(DevState S, int has_z) {
     int c = cell_index();
@@ -574,6 +591,7 @@ struct Launch {
         // substep's (the mesh has not moved since); every later substep
         // searches (a moving substep before it moved the mesh)
         DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, first != 1);
+        if (first && sh.wmean) DISCO_LAUNCH(hook, "k_wmean", k_wmean, dim3(S.nann), dim3(64), st, S);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
         if constexpr (!Ph::MHD) {
             if (sh.visc)

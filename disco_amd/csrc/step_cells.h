@@ -427,7 +427,9 @@ DHD void stage_a_cell(const DevState& S, int c, int first, int search) {
 
     if (first) {
         double w = 0.0;
-        if (P.mesh_motion == 1) {
+        if (P.mesh_motion == 1 || P.mesh_motion == 2 || P.mesh_motion == 3) {
+            // misc.c:117-132 (every non-zero mode; 3 then takes the annulus
+            // mean in k_wmean)
             w = .5 * (S.prim[UPP * N + c] + S.prim[UPP * N + cp]);
         } else if (P.mesh_motion == 4) {
             double phip = S.piph[c];

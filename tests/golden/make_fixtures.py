@@ -382,6 +382,19 @@ def fixtures_nobc() -> None:
         save_states(name, states)
 
 
+MM_CASES = [  # set_wcell (misc.c:100-168): Mesh_Motion 2 (as 1) and 3 (the annulus mean of the face speeds)
+    ("c1_mm2.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=2), "euler_sun", 5, (1,)),
+    ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3), "euler_sun", 10, (1,)),
+    ("c2_mhd_small_mm3.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Mesh_Motion=3), "mhd_sun", 5, (1,)),
+]
+
+
+def fixtures_mm() -> None:
+    for name, base, over, variant, n, dumps in MM_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -541,6 +554,10 @@ def main() -> None:
         return
     if "--nobc" in sys.argv:
         fixtures_nobc()
+        print("fixtures written")
+        return
+    if "--mm" in sys.argv:
+        fixtures_mm()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

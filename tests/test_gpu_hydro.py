@@ -92,6 +92,9 @@ CASES = [
     ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1)),
     ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
     ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
+    # set_wcell's Mesh_Motion 2 (the face speeds, as 1) and 3 (their annulus mean)
+    ("c1_mm2.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=2)),
+    ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
 ]
 
 

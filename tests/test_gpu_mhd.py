@@ -59,11 +59,13 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            ("c2_mhd_small_rk3.npz", {"Timestep": 3}),
            ("c2_mhd_small_fe.npz", {"Timestep": 1}),
            # a rotating frame (rotframe.c)
-           ("c2_mhd_small_rot.npz", dict(RotFrame=1, RotOmega=0.3, RotD=0.05))]
+           ("c2_mhd_small_rot.npz", dict(RotFrame=1, RotOmega=0.3, RotD=0.05)),
+           # annuli at the mean of their face speeds (Mesh_Motion 3)
+           ("c2_mhd_small_mm3.npz", {"Mesh_Motion": 3})]
 
 
 @pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms", "rk3",
-                                                       "fe", "rot"])
+                                                       "fe", "rot", "mm3"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver

@@ -200,6 +200,8 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                        ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3)),
                        ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
                        ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
+                       # Mesh_Motion 3: the annulus mean of the initial face speeds
+                       ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
                        # the rotating frame's initial and fixed-ghost data
                        ("c1_rot_fixed.npz", "c1_kepler2d.par", dict(Num_R=16, RotFrame=1, RotOmega=0.3, RotD=0.05))]
 
