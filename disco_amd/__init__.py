@@ -23,7 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libdisco_gpu.so")
 
-HYDRO = {"euler": 0, "mhd": 1, "grmhd": 2}
+HYDRO = {"euler": 0, "mhd": 1, "grmhd": 2, "greuler": 3}
 NQ_OF = {0: 5, 1: 8, 2: 8}
 
 

@@ -951,7 +951,9 @@ int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int
 }
 
 // ---------------------------------------------------------------- checkpoints
-static const char* hydro_name(int v) { return v == DISCO_HYDRO_MHD ? "mhd" : v == DISCO_HYDRO_GRMHD ? "grmhd" : "euler"; }
+static const char* hydro_name(int v) {
+    return v == DISCO_HYDRO_MHD ? "mhd" : v == DISCO_HYDRO_GRMHD ? "grmhd" : v == DISCO_HYDRO_GREULER ? "greuler" : "euler";
+}
 static const char* boundary_name(int v) {
     static const char* n[] = {"fixed", "fixed3d", "zerograd", "outflow3d"};
     return v >= 0 && v < 4 ? n[v] : "?";

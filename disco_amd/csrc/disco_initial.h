@@ -202,7 +202,9 @@ DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
             prim[RHO] = rho;
             prim[PPP] = P;
             prim[URR] = us * r / s;
-            prim[UPP] = -sqrt(ell2) * ut / (r * r);
+            /* grmhd.c keeps u_phi / R^2 in prim[UPP], greuler.c (Num_C 5)
+             * u_phi itself */
+            prim[UPP] = ic->num_c > BZZ ? -sqrt(ell2) * ut / (r * r) : -sqrt(ell2) * ut;
             prim[UZZ] = us * z / s;
         } else {
             prim[RHO] = ic->rho_atm;

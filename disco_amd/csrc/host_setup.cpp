@@ -10,6 +10,7 @@
 
 #include "phys_euler.h"
 #include "phys_mhd.h"
+#include "phys_greuler.h"
 #include "phys_grmhd_fv.h"
 
 namespace disco {
@@ -158,6 +159,7 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
     if (!read_var(text, "Hydro", 2, s)) {
         if (!std::strcmp(s, "mhd")) p->hydro = DISCO_HYDRO_MHD;
         else if (!std::strcmp(s, "grmhd")) p->hydro = DISCO_HYDRO_GRMHD;
+        else if (!std::strcmp(s, "greuler")) p->hydro = DISCO_HYDRO_GREULER;
         else if (std::strcmp(s, "euler")) { ++e; missing += " Hydro=" + std::string(s); }
     }
     p->boundary = DISCO_BC_FIXED;
@@ -180,7 +182,7 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
         else if (!std::strcmp(s, "gr_torus")) p->initial = DISCO_IC_GR_TORUS;
         else if (std::strcmp(s, "kepler_noise")) { ++e; missing += " Initial=" + std::string(s); }
     }
-    p->num_c = p->hydro == DISCO_HYDRO_EULER ? 5 : 8;
+    p->num_c = (p->hydro == DISCO_HYDRO_EULER || p->hydro == DISCO_HYDRO_GREULER) ? 5 : 8;
     read_var(text, "Num_C", 1, &p->num_c);
     p->num_n = 0;
     read_var(text, "Num_N", 1, &p->num_n);
@@ -295,6 +297,14 @@ static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
     if (p.num_n != 0) m += " Num_N>0";
     if (p.hydro == DISCO_HYDRO_EULER && p.num_c != 5) m += " Num_C!=5 for euler";
+    if (p.hydro == DISCO_HYDRO_GREULER) {
+        // Hydro/greuler.c: GR hydrodynamics in the same metric as grmhd
+        if (p.num_c != 5) m += " Num_C!=5 for greuler";
+        if (p.Riemann_Solver == DISCO_HLLD) m += " Riemann_Solver=HLLD with greuler";
+        if (p.isothermal_flag) m += " isothermal greuler";
+        if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4) m += " Mesh_Motion (greuler: 0 or 4)";
+        if (p.metricPar2 <= 0.0 || std::fabs(p.metricPar3) >= p.metricPar2) m += " Kerr needs M > 0 and |a| < M";
+    }
     if (p.hydro == DISCO_HYDRO_MHD || p.hydro == DISCO_HYDRO_GRMHD) {
         if (p.num_c != 8) m += " Num_C!=8 for mhd/grmhd";
         // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
@@ -328,14 +338,15 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.visc_flag && !p.alpha_flag && p.viscosity <= 0.0) m += " Use_Viscosity needs Viscosity > 0";
     // sink.c:143-270: the accretion sinks, types 1-3
     if (p.sinkType < 0 || p.sinkType > 3) m += " Sink_Type";
-    if (p.sinkType && p.hydro == DISCO_HYDRO_GRMHD) m += " Sink_Type with grmhd";
+    const bool gr = p.hydro == DISCO_HYDRO_GRMHD || p.hydro == DISCO_HYDRO_GREULER;
+    if (p.sinkType && gr) m += " Sink_Type with grmhd/greuler";
     // sink.c:274-322: beta cooling (COOL_BETA, COOL_BETA_RELAX) of a Newtonian gas
     if (p.coolType < 0 || p.coolType > 2) m += " Cool_Type";
-    if (p.coolType && p.hydro == DISCO_HYDRO_GRMHD) m += " Cool_Type with grmhd";
+    if (p.coolType && gr) m += " Cool_Type with grmhd/greuler";
     if (p.coolType && p.coolPar1 == 0.0) m += " Cool_Type needs Cool_Par1 (beta) != 0";
     // sink.c:107-139: the nozzle (type 1), a Newtonian source
     if (p.nozzleType < 0 || p.nozzleType > 1) m += " Nozzle_Type";
-    if (p.nozzleType && p.hydro == DISCO_HYDRO_GRMHD) m += " Nozzle_Type with grmhd";
+    if (p.nozzleType && gr) m += " Nozzle_Type with grmhd/greuler";
     if (p.nozzleType && (p.nozzlePar2 <= 0.0 || p.nozzlePar4 == 0.0)) m += " Nozzle_Par2 > 0 and Nozzle_Par4 != 0";
     // sink.c:329-426 damping zones: types 0-2; with only the z zones in a 2D
     // run the reference's damping factor is never set
@@ -343,8 +354,8 @@ static int unsupported(const disco_params& p, std::string* err) {
     for (int t : dt4)
         if (t < 0 || t > 2) m += " Damp*_Type";
     if ((dt4[2] || dt4[3]) && !dt4[0] && !dt4[1] && p.Num_Z == 1) m += " Damp{Upper,Lower}_Type alone in 2D";
-    if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && p.hydro == DISCO_HYDRO_GRMHD) m += " Damp*_Type with grmhd";
-    if (p.RotFrame && p.hydro == DISCO_HYDRO_GRMHD) m += " RotFrame with grmhd";
+    if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && gr) m += " Damp*_Type with grmhd/greuler";
+    if (p.RotFrame && gr) m += " RotFrame with grmhd/greuler";
     if (p.noiseType < 0 || p.noiseType > 3) m += " Noise_Type";
     if (p.include_atmos) m += " Include_Atmos";
     if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";
@@ -907,6 +918,10 @@ static void host_prim2cons(const HostRun& R, const double* prim, double* cons, c
         grmhd::Metric g;
         grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
         grmhd::prim2cons(R.ph.gamma_law, g, prim, cons, x, dV);
+    } else if (R.par.hydro == DISCO_HYDRO_GREULER) {
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
+        greuler::prim2cons(R.ph.gamma_law, g, prim, cons, dV);
     } else {
         euler::prim2cons<5>(R.ph, prim, cons, x, dV);
     }
@@ -918,6 +933,8 @@ static void host_cons2prim(const HostRun& R, const double* cons, double* prim, c
         mhd::cons2prim(R.ph, cons, prim, x, dV);
     else if (R.par.hydro == DISCO_HYDRO_GRMHD)
         grmhd::cons2prim(R.ph, cons, prim, x, dV);
+    else if (R.par.hydro == DISCO_HYDRO_GREULER)
+        greuler::cons2prim(R.ph, cons, prim, x, dV);
     else
         euler::cons2prim<5>(R.ph, cons, prim, x, dV);
 }

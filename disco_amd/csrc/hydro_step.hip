@@ -50,6 +50,10 @@ template <>
 struct Occ<PhysGRMHD> {
     static constexpr int FLUX = 0, PLM = 0, CT = 0;
 };
+template <>
+struct Occ<PhysGREuler> {
+    static constexpr int FLUX = 0, PLM = 0, CT = 0;
+};
 
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Remap so that XCD x works on one contiguous band of cells: partner cells in
@@ -683,6 +687,8 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
         Launch<PhysMHD>::mindt(S, st, hook);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::mindt(S, st, hook);
+    else if (sh.hydro == DISCO_HYDRO_GREULER)
+        Launch<PhysGREuler>::mindt(S, st, hook);
     else
         Launch<PhysEuler>::mindt(S, st, hook);
 }
@@ -693,6 +699,8 @@ void launch_substep(const StepShape& sh, const DevState& S, double RK, double dt
         Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
+    else if (sh.hydro == DISCO_HYDRO_GREULER)
+        Launch<PhysGREuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else
         Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
 }

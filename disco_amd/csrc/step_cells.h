@@ -36,6 +36,7 @@
 #include "device_state.h"
 #include "phys_euler.h"
 #include "phys_mhd.h"
+#include "phys_greuler.h"
 #include "phys_grmhd_fv.h"
 
 namespace disco {
@@ -129,6 +130,34 @@ struct PhysGRMHD {
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return grmhd::mindt(P, prim, w, xp, xm);
+    }
+};
+
+// GR hydrodynamics (greuler.c) in the same metric: no field, no constrained
+// transport; its cons2prim also starts from the cell's previous primitives.
+struct PhysGREuler {
+    static constexpr int NQ = 5;
+    static constexpr int NF = 5;
+    static constexpr bool MHD = false;
+    static constexpr bool C2P_GUESS = true;
+    static constexpr bool SOURCE_PASS = false;
+    static constexpr bool NEWTONIAN = false;  // sink.c's terms are rejected at set-up
+    DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
+                        double w, double dAdt, int dim, double* dU) {
+        (void)dim;
+        greuler::riemann(P, pL, pR, x, n, w, dAdt, dU);
+        return FaceEMF{0.0, 0.0, 0.0, 0.0};
+    }
+    DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
+                    double dVdt) {
+        greuler::source(P, prim, cons, xp, xm, dVdt);
+    }
+    DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
+                      int* okp = nullptr) {
+        return greuler::cons2prim(P, cons, prim, x, dV, okp);
+    }
+    DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
+        return greuler::mindt(P, prim, w, xp, xm);
     }
 };
 

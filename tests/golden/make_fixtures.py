@@ -395,6 +395,19 @@ def fixtures_mm() -> None:
         save_states(name, states)
 
 
+def fixtures_greuler() -> None:
+    # GR hydrodynamics (greuler.c) on the small torus at spin 0 (the reference's metric), no field: HLL 0/1/5/20
+    # steps, HLLC 1/5 steps
+    txt = par_with("c5_grmhd_torus.par", Hydro="greuler", Num_C=5, Constrained_Transport=0, Num_R=16, Num_Z=12,
+                   Metric_Par3=0.0)
+    states, _ = run_ref("greuler_torus", txt, 20, dumps=(1, 5))
+    save_states("c6_greuler_small.npz", states)
+    txt = par_with("c5_grmhd_torus.par", Hydro="greuler", Num_C=5, Constrained_Transport=0, Num_R=16, Num_Z=12,
+                   Metric_Par3=0.0, Riemann_Solver=1)
+    states, _ = run_ref("greuler_torus", txt, 5, dumps=(1,))
+    save_states("c6_greuler_small_hllc.npz", states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -453,13 +466,17 @@ LEAF_CASES = {
     "grmhd_hll": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0}, "grmhd_torus"),
     "grmhd_hllc": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1},
                    "grmhd_torus"),
+    "greuler_hll": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
+                                           "Num_Z": 12, "Metric_Par3": 0.0}, "greuler_torus"),
+    "greuler_hllc": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
+                                            "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1}, "greuler_torus"),
 }
 
 
 def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
     """Random left/right face states, positions, normals, face speeds."""
     rng = np.random.default_rng(seed)
-    nq = 5 if kind.startswith("euler") else 8
+    nq = 5 if kind.startswith(("euler", "greuler")) else 8
     U = lambda a, b, size=None: rng.uniform(a, b, size)  # noqa: E731
     out = []
     for k in range(n):
@@ -468,6 +485,9 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
             r, z = U(4.0, 20.0), U(-5.0, 5.0)
             pL = np.array([U(0.1, 1.0), U(1e-3, 5e-2), U(-0.1, 0.1), U(0.1, 0.4) / r, U(-0.1, 0.1),
                            U(-0.02, 0.02), U(-0.02, 0.02), U(-0.02, 0.02)])
+        elif kind.startswith("greuler"):  # prim[UPP] is the covariant u_phi
+            r, z = U(4.0, 20.0), U(-5.0, 5.0)
+            pL = np.array([U(0.1, 1.0), U(1e-3, 5e-2), U(-0.1, 0.1), U(0.1, 0.4) * r, U(-0.1, 0.1)])
         else:
             r, z = U(0.6, 2.9), U(-0.4, 0.4)
             pL = np.array([U(0.5, 2.0), U(5e-3, 5e-2), U(-0.05, 0.05), U(0.2, 1.0), U(-0.05, 0.05),
@@ -480,8 +500,11 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
         x = np.array([r, U(0.0, 2 * np.pi), z])
         nrm = np.zeros(3)
         nrm[[1, 0, 2][dim]] = 1.0
-        w = r * pL[3] * U(0.8, 1.2) if dim == 0 and not kind.startswith("grmhd") else (
-            pL[3] * U(0.8, 1.2) if dim == 0 else 0.0)
+        if kind.startswith("greuler"):
+            w = 0.1 * U(0.8, 1.2) if dim == 0 else 0.0
+        else:
+            w = r * pL[3] * U(0.8, 1.2) if dim == 0 and not kind.startswith("grmhd") else (
+                pL[3] * U(0.8, 1.2) if dim == 0 else 0.0)
         out.append(np.concatenate([pL, pR, x, nrm, [w, U(1e-3, 1e-2), float(dim)]]))
     return np.array(out)
 
@@ -558,6 +581,10 @@ def main() -> None:
         return
     if "--mm" in sys.argv:
         fixtures_mm()
+        print("fixtures written")
+        return
+    if "--greuler" in sys.argv:
+        fixtures_greuler()
         print("fixtures written")
         return
     if "--steppers" in sys.argv:

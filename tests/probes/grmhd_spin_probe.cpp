@@ -11,6 +11,11 @@
 //   riemann(prim1, prim2, w = 0, dAdt = 1): dU[8], E1, B1, E2, B2   (12)
 //   source(prim1) over the cell x +- (0.05, 0.02, 0.03), dVdt = 0.7 (8)
 //   mindt(prim1, wc = 0.1) over the same cell                 (1)
+// With riemann >= 10 the record is greuler.c's (GR hydrodynamics; riemann
+// - 10 the solver): prim[0..4] with prim[UPP] = u_phi, and the same 62
+// columns with zeros where greuler has no value (cons[5..7], F[5..7],
+// dU[5..7], the EMFs); the source's columns hold cons2prim(prim2cons(prim1))
+// instead (the round trip, started from prim2).
 #include <cstdio>
 #include <iostream>
 #include <sstream>
@@ -36,10 +41,36 @@ int main() {
         P.gamma_law = gam_law;
         P.gr_M = M;
         P.gr_a = a;
-        P.riemann = rs;
+        P.riemann = rs % 10;
         P.phi_max = 2.0 * M_PI;
         grmhd::Metric g;
         grmhd::kerr_metric(x, M, a, &g);
+        if (rs >= 10) {
+            std::printf("%.17g", g.al);
+            for (double v : g.be) std::printf(" %.17g", v);
+            for (double v : g.gam) std::printf(" %.17g", v);
+            for (double v : g.igam) std::printf(" %.17g", v);
+            double cons[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            greuler::prim2cons(gam_law, g, p1, cons, 1.0);
+            for (double v : cons) std::printf(" %.17g", v);
+            const double n[3] = {dir == 0 ? 1.0 : 0.0, dir == 1 ? 1.0 : 0.0, dir == 2 ? 1.0 : 0.0};
+            double F[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            greuler::flux(gam_law, g, p1, F, x, n);
+            for (double v : F) std::printf(" %.17g", v);
+            double Sl, Sr, Ss;
+            greuler::vel(gam_law, g, p1, p2, &Sl, &Sr, &Ss, n, x);
+            std::printf(" %.17g %.17g %.17g", Sl, Sr, Ss);
+            double dU[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            greuler::riemann(P, p1, p2, x, n, 0.0, 1.0, dU);
+            for (double v : dU) std::printf(" %.17g", v);
+            std::printf(" 0 0 0 0");
+            double back[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < 5; ++q) back[q] = p2[q];
+            greuler::cons2prim(P, cons, back, x, 1.0);
+            for (double v : back) std::printf(" %.17g", v);
+            std::printf(" 0\n");
+            continue;
+        }
         std::printf("%.17g", g.al);
         for (double v : g.be) std::printf(" %.17g", v);
         for (double v : g.gam) std::printf(" %.17g", v);

@@ -1,7 +1,7 @@
 // Test probe (built by tests/test_leaves_cpu.py with the host compiler): the
 // library's per-face and per-cell physics leaves -- the same source the GPU
 // kernels compile (step_cells.h policies over phys_euler.h, phys_mhd.h,
-// phys_grmhd_fv.h) -- evaluated on records read from a file, in the layout
+// phys_grmhd_fv.h, phys_greuler.h) -- evaluated on records read from a file, in the layout
 // of oracle/ref/ref_leaf.c, so the two can be compared record by record.
 //
 // usage: leaf_probe PARFILE IN.bin OUT.bin
@@ -38,6 +38,10 @@ static void record(const PhysParams& P, int hydro, const double* in, double* out
         grmhd::Metric g;
         grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
         grmhd::prim2cons(P.gamma_law, g, pL, cons, x, 1.37);
+    } else if (hydro == DISCO_HYDRO_GREULER) {
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
+        greuler::prim2cons(P.gamma_law, g, pL, cons, 1.37);
     } else if (hydro == DISCO_HYDRO_MHD) {
         mhd::prim2cons(P, pL, cons, x, 1.37);
     } else {
@@ -81,6 +85,8 @@ int main(int argc, char** argv) {
         if (std::fread(in.data(), sizeof(double), in.size(), fi) != in.size()) return 1;
         if (par.hydro == DISCO_HYDRO_GRMHD)
             record<PhysGRMHD>(P, par.hydro, in.data(), out.data());
+        else if (par.hydro == DISCO_HYDRO_GREULER)
+            record<PhysGREuler>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_MHD)
             record<PhysMHD>(P, par.hydro, in.data(), out.data());
         else

@@ -53,11 +53,11 @@ def probe(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-w", "-I", CSRC, "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "probes", "grmhd_spin_probe.cpp"), "-o", exe], check=True, timeout=300)
 
-    def run(a, recs):
+    def run(a, recs, mode=0):
         lines = []
         for p1, p2, x, d in recs:
             vals = [*p1, *p2, *x]
-            lines.append(f"{M!r} {a!r} {GAMMA!r} 0 " + " ".join(repr(float(v)) for v in vals) + f" {int(d)}")
+            lines.append(f"{M!r} {a!r} {GAMMA!r} {mode} " + " ".join(repr(float(v)) for v in vals) + f" {int(d)}")
         out = subprocess.run([exe], input="\n".join(lines) + "\n", capture_output=True, text=True, check=True,
                              timeout=120).stdout
         return np.array([[float(v) for v in ln.split()] for ln in out.strip().splitlines()])
@@ -217,3 +217,43 @@ def test_planted_spin_sign_error_is_caught(probe):
     finally:
         GC.metric_cart = orig
     assert max(errs) > 1e-6, max(errs)
+
+
+# ------------------------------------------------------------ greuler.c
+@pytest.mark.parametrize("a", [0.5, 0.9])
+def test_greuler_leaves_equal_covariant_oracle(probe, a):
+    """greuler.c's GR hydrodynamics (phys_greuler.h) is the GRMHD stress
+    tensor without field: its prim2cons, fluxes, HLL wave speeds and HLL
+    increment must equal the covariant oracle's at B = 0 (greuler keeps
+    u_phi in prim[UPP], the oracle u_phi / R^2), to 1e-12 of each quantity's
+    scale; and its cons2prim (the quartic for W - 1) must invert prim2cons
+    to 1e-10 from another start.  Its geometric source and CFL step differ
+    from grmhd.c's by construction (greuler.c drops the z row of the frame
+    terms and uses centroids) and are pinned at a = 0 by the reference's leaf
+    vectors only."""
+    recs = []
+    for p1, p2, x, d in _states(240, 21 if a == 0.9 else 22):
+        q1, q2 = p1[:5].copy(), p2[:5].copy()
+        q1[3] *= x[0] ** 2
+        q2[3] *= x[0] ** 2
+        recs.append((np.concatenate([q1, np.zeros(3)]), np.concatenate([q2, np.zeros(3)]), x, d))
+    lib = probe(a, recs, mode=10)
+    worst = {"cons": 0.0, "flux": 0.0, "speeds": 0.0, "riemann": 0.0, "roundtrip": 0.0}
+    for row, (p1, p2, x, d) in zip(lib, recs):
+        g1, g2 = p1.copy(), p2.copy()
+        g1[3] /= x[0] ** 2
+        g2[3] /= x[0] ** 2
+        pt = GC.Point(x, M, a)
+        s1 = GC.State(pt, g1, GAMMA)
+        for k, mine, ora in (("cons", row[22:27], GC.prim2cons(pt, s1)[:5]),
+                             ("flux", row[30:35], GC.flux(pt, s1, d)[:5])):
+            worst[k] = max(worst[k], np.abs(mine - ora).max() / max(np.abs(ora).max(), 1e-300))
+        dU, _, (Sl, Sr) = GC.riemann_hll(x, g1, g2, d, 0.0, 1.0, M, a, GAMMA)
+        worst["speeds"] = max(worst["speeds"], np.abs(row[38:40] - [Sl, Sr]).max() / max(abs(Sl), abs(Sr)))
+        worst["riemann"] = max(worst["riemann"], np.abs(row[41:46] - dU[:5]).max() / max(np.abs(dU[:5]).max(), 1e-300))
+        back = row[53:58]
+        sc = np.array([p1[0], p1[1], *([max(np.abs(p1[2:5]).max(), 1e-300)] * 3)])
+        worst["roundtrip"] = max(worst["roundtrip"], (np.abs(back - p1[:5]) / sc).max())
+    print(f"greuler a = {a}: {worst}")
+    assert max(worst["cons"], worst["flux"], worst["speeds"], worst["riemann"]) < 1e-12, worst
+    assert worst["roundtrip"] < 1e-10, worst

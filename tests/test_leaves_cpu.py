@@ -3,11 +3,11 @@ reference's own functions.
 
 tests/golden/leaf_vectors.npz holds random face states and what the
 reference build returns for them (oracle/ref/ref_leaf.c, linked with the
-reference's riemann.c, hlld.c and Hydro/euler.c, mhd.c or grmhd.c +
-schw_ks_rpz.c): solve_riemann's conserved increments (F - w U*) dA dt and
+reference's riemann.c, hlld.c and Hydro/euler.c, mhd.c, grmhd.c or
+greuler.c + schw_ks_rpz.c): solve_riemann's conserved increments (F - w U*) dA dt and
 flux_to_E's EMFs for HLL, HLLC and HLLD on phi, r and z faces, prim2cons,
 cons2prim back, mindt and source.  The library's leaves (the step_cells.h
-policies over phys_euler.h, phys_mhd.h and phys_grmhd_fv.h -- the source the
+policies over phys_euler.h, phys_mhd.h, phys_grmhd_fv.h and phys_greuler.h -- the source the
 GPU kernels compile) are built here with the host compiler into
 tests/probes/leaf_probe.cpp and fed the same records.  Both sides are
 compiled without fused multiply-adds and call the same libm, so every value
@@ -34,6 +34,10 @@ CASES = {
     "mhd_hll": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 0}),
     "grmhd_hll": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0}),
     "grmhd_hllc": ("c5_grmhd_torus.par", {"Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1}),
+    "greuler_hll": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
+                                           "Num_Z": 12, "Metric_Par3": 0.0}),
+    "greuler_hllc": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
+                                            "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1}),
 }
 
 

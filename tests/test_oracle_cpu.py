@@ -200,6 +200,10 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                        ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3)),
                        ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
                        ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
+                       # GR hydrodynamics (greuler.c) on the torus at spin 0
+                       ("c6_greuler_small.npz", "c5_grmhd_torus.par",
+                        dict(Hydro="greuler", Num_C=5, Constrained_Transport=0, Num_R=16, Num_Z=12,
+                             Metric_Par3=0.0)),
                        # Mesh_Motion 3: the annulus mean of the initial face speeds
                        ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
                        # the rotating frame's initial and fixed-ghost data
