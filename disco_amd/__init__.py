@@ -23,7 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libdisco_gpu.so")
 
-HYDRO = {"euler": 0, "mhd": 1, "grmhd": 2, "greuler": 3}
+HYDRO = {"euler": 0, "mhd": 1, "grmhd": 2, "greuler": 3, "grmhd2": 4}
 NQ_OF = {0: 5, 1: 8, 2: 8}
 
 
@@ -333,7 +333,7 @@ class Domain:
                                         _dptr(wiph)), "disco_gpu_download")
         out = dict(prim=prim.reshape(I.nq, I.ncell), cons=cons.reshape(I.nq, I.ncell), piph=piph, dphi=dphi,
                    wiph=wiph)
-        if self.params[run].hydro in (HYDRO["mhd"], HYDRO["grmhd"]):
+        if self.params[run].hydro in (HYDRO["mhd"], HYDRO["grmhd"], HYDRO["grmhd2"]):
             out["Phi"] = self.faces(run)
         return out
 

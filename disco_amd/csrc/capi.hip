@@ -598,7 +598,9 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     }
     h->nq = h->runs[0].nq;
     // constrained-transport buffers: ideal MHD and GRMHD
-    const bool mhd = h->runs[0].par.hydro == DISCO_HYDRO_MHD || h->runs[0].par.hydro == DISCO_HYDRO_GRMHD;
+    const int hyd = h->runs[0].par.hydro;
+    const bool grmhd = hyd == DISCO_HYDRO_GRMHD || hyd == DISCO_HYDRO_GRMHD2;
+    const bool mhd = hyd == DISCO_HYDRO_MHD || grmhd;
     build_tables(h);
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;
@@ -647,7 +649,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     rc |= dalloc(&h->final_step, nruns);
     rc |= dalloc(&h->diag, (size_t)h->nann * DISCO_NDIAG);
     rc |= dalloc(&h->t_avg, nruns);
-    if (h->runs[0].par.hydro == DISCO_HYDRO_GRMHD) rc |= dalloc(&h->src, 4 * N);
+    if (grmhd) rc |= dalloc(&h->src, 4 * N);
     constexpr size_t SQ = (size_t)DISCO_NSINKQ * DISCO_MAX_PLANETS;
     if (h->shape.accrete) {
         rc |= dalloc(&h->sinkc, SQ * N);
@@ -952,7 +954,11 @@ int disco_gpu_report_noalloc(disco_gpu_t* h, int run, double* out, int nmax, int
 
 // ---------------------------------------------------------------- checkpoints
 static const char* hydro_name(int v) {
-    return v == DISCO_HYDRO_MHD ? "mhd" : v == DISCO_HYDRO_GRMHD ? "grmhd" : v == DISCO_HYDRO_GREULER ? "greuler" : "euler";
+    return v == DISCO_HYDRO_MHD       ? "mhd"
+           : v == DISCO_HYDRO_GRMHD   ? "grmhd"
+           : v == DISCO_HYDRO_GRMHD2  ? "grmhd2"
+           : v == DISCO_HYDRO_GREULER ? "greuler"
+                                      : "euler";
 }
 static const char* boundary_name(int v) {
     static const char* n[] = {"fixed", "fixed3d", "zerograd", "outflow3d"};

@@ -41,6 +41,7 @@ typedef struct disco_ic {
     double tor_rin;     /* Init_Par6: inner edge (spherical radius, equator) */
     double tor_rmax;    /* Init_Par7: pressure maximum */
     double rho_atm;     /* Density_Floor: atmosphere density */
+    int upp_covariant;  /* prim[UPP] holds u_phi (greuler.c, grmhd2.c), else u_phi/R^2 (grmhd.c) */
 } disco_ic;
 
 static inline void disco_ic_from_params(const disco_params* p, disco_ic* ic) {
@@ -64,6 +65,7 @@ static inline void disco_ic_from_params(const disco_params* p, disco_ic* ic) {
     ic->tor_rin = p->initPar6;
     ic->tor_rmax = p->initPar7;
     ic->rho_atm = p->Density_Floor;
+    ic->upp_covariant = p->hydro == DISCO_HYDRO_GREULER || p->hydro == DISCO_HYDRO_GRMHD2;
 }
 
 /* U(-1,1) deviate keyed on the exact cell centre and a stream index. */
@@ -202,9 +204,9 @@ DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
             prim[RHO] = rho;
             prim[PPP] = P;
             prim[URR] = us * r / s;
-            /* grmhd.c keeps u_phi / R^2 in prim[UPP], greuler.c (Num_C 5)
-             * u_phi itself */
-            prim[UPP] = ic->num_c > BZZ ? -sqrt(ell2) * ut / (r * r) : -sqrt(ell2) * ut;
+            /* grmhd.c keeps u_phi / R^2 in prim[UPP], greuler.c and
+             * grmhd2.c u_phi itself */
+            prim[UPP] = ic->upp_covariant ? -sqrt(ell2) * ut : -sqrt(ell2) * ut / (r * r);
             prim[UZZ] = us * z / s;
         } else {
             prim[RHO] = ic->rho_atm;

@@ -160,6 +160,7 @@ int read_par_text(const std::string& text, disco_params* p, std::string* err) {
         if (!std::strcmp(s, "mhd")) p->hydro = DISCO_HYDRO_MHD;
         else if (!std::strcmp(s, "grmhd")) p->hydro = DISCO_HYDRO_GRMHD;
         else if (!std::strcmp(s, "greuler")) p->hydro = DISCO_HYDRO_GREULER;
+        else if (!std::strcmp(s, "grmhd2")) p->hydro = DISCO_HYDRO_GRMHD2;
         else if (std::strcmp(s, "euler")) { ++e; missing += " Hydro=" + std::string(s); }
     }
     p->boundary = DISCO_BC_FIXED;
@@ -292,6 +293,9 @@ static void add_noise(const disco_params& par, double* prim, const double* x) {
 // Euler substep followed by RK2, every other value RK2
 int stepper_of(int Timestep) { return Timestep == 1 ? 1 : Timestep == 3 ? 3 : 2; }
 
+// grmhd.c or grmhd2.c (the same module in two conventions for prim[UPP])
+static bool is_grmhd(int hydro) { return hydro == DISCO_HYDRO_GRMHD || hydro == DISCO_HYDRO_GRMHD2; }
+
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
@@ -305,15 +309,15 @@ static int unsupported(const disco_params& p, std::string* err) {
         if (p.Mesh_Motion != 0 && p.Mesh_Motion != 4) m += " Mesh_Motion (greuler: 0 or 4)";
         if (p.metricPar2 <= 0.0 || std::fabs(p.metricPar3) >= p.metricPar2) m += " Kerr needs M > 0 and |a| < M";
     }
-    if (p.hydro == DISCO_HYDRO_MHD || p.hydro == DISCO_HYDRO_GRMHD) {
+    if (p.hydro == DISCO_HYDRO_MHD || is_grmhd(p.hydro)) {
         if (p.num_c != 8) m += " Num_C!=8 for mhd/grmhd";
         // CT_MODE 2 (3D constrained transport) is the MHD mode that is built
         if (!p.CT) m += " mhd/grmhd need Constrained_Transport 1";
         if (p.Num_Z < 2) m += " mhd/grmhd with CT_MODE 2 need Num_Z > 1";
-        if (p.hydro == DISCO_HYDRO_GRMHD && p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
+        if (is_grmhd(p.hydro) && p.Mesh_Motion != 0 && p.Mesh_Motion != 4)
             m += " Mesh_Motion (grmhd: 0 or 4)";
     }
-    if (p.hydro == DISCO_HYDRO_GRMHD) {
+    if (is_grmhd(p.hydro)) {
         // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one
         if (p.Riemann_Solver == DISCO_HLLD) m += " Riemann_Solver=HLLD with grmhd";
         if (p.isothermal_flag) m += " isothermal grmhd (cons2prim_solve_isothermal is a stub)";
@@ -338,7 +342,7 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.visc_flag && !p.alpha_flag && p.viscosity <= 0.0) m += " Use_Viscosity needs Viscosity > 0";
     // sink.c:143-270: the accretion sinks, types 1-3
     if (p.sinkType < 0 || p.sinkType > 3) m += " Sink_Type";
-    const bool gr = p.hydro == DISCO_HYDRO_GRMHD || p.hydro == DISCO_HYDRO_GREULER;
+    const bool gr = is_grmhd(p.hydro) || p.hydro == DISCO_HYDRO_GREULER;
     if (p.sinkType && gr) m += " Sink_Type with grmhd/greuler";
     // sink.c:274-322: beta cooling (COOL_BETA, COOL_BETA_RELAX) of a Newtonian gas
     if (p.coolType < 0 || p.coolType > 2) m += " Cool_Type";
@@ -918,6 +922,10 @@ static void host_prim2cons(const HostRun& R, const double* prim, double* cons, c
         grmhd::Metric g;
         grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
         grmhd::prim2cons(R.ph.gamma_law, g, prim, cons, x, dV);
+    } else if (R.par.hydro == DISCO_HYDRO_GRMHD2) {
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
+        grmhd::prim2cons<true>(R.ph.gamma_law, g, prim, cons, x, dV);
     } else if (R.par.hydro == DISCO_HYDRO_GREULER) {
         grmhd::Metric g;
         grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
@@ -933,6 +941,8 @@ static void host_cons2prim(const HostRun& R, const double* cons, double* prim, c
         mhd::cons2prim(R.ph, cons, prim, x, dV);
     else if (R.par.hydro == DISCO_HYDRO_GRMHD)
         grmhd::cons2prim(R.ph, cons, prim, x, dV);
+    else if (R.par.hydro == DISCO_HYDRO_GRMHD2)
+        grmhd::cons2prim<true>(R.ph, cons, prim, x, dV);
     else if (R.par.hydro == DISCO_HYDRO_GREULER)
         greuler::cons2prim(R.ph, cons, prim, x, dV);
     else
@@ -953,7 +963,7 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     R.prim.assign((size_t)R.nq * R.ncell, 0.0);
     R.cons.assign((size_t)R.nq * R.ncell, 0.0);
     R.wiph.assign(R.ncell, 0.0);
-    const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD || par.hydro == DISCO_HYDRO_GRMHD;
+    const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD || is_grmhd(par.hydro);
     // domain.c:163-209: initial data and prim2cons on real cells
     double prim[8], cons[8];
     for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
@@ -1071,7 +1081,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         // phi_flux and trans_flux ranges (misc.c:393-407, 459-500): r-tracks
         // of real z-layers, z-tracks of real r-columns; with constrained
         // transport every range grows by one ghost layer across the track
-        bool ct = (R.par.hydro == DISCO_HYDRO_MHD || R.par.hydro == DISCO_HYDRO_GRMHD) && R.par.CT;
+        bool ct = (R.par.hydro == DISCO_HYDRO_MHD || is_grmhd(R.par.hydro)) && R.par.CT;
         int jmin = R.NgRa == 0 ? 0 : R.NgRa - 1;
         int jmax = R.NgRb == 0 ? R.Nr - 1 : R.Nr - R.NgRb;
         int kmin = R.NgZa, kmax = R.Nz - R.NgZb;
@@ -1241,7 +1251,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->visc = 0;
     shape->accrete = 0;
     shape->wmean = 0;
-    shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || runs[0].par.hydro == DISCO_HYDRO_GRMHD))
+    shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || is_grmhd(runs[0].par.hydro)))
                     ? 1
                     : 0;
     for (const HostRun& R : runs) {

@@ -51,6 +51,10 @@ struct Occ<PhysGRMHD> {
     static constexpr int FLUX = 0, PLM = 0, CT = 0;
 };
 template <>
+struct Occ<PhysGRMHD2> {
+    static constexpr int FLUX = 0, PLM = 0, CT = 0;
+};
+template <>
 struct Occ<PhysGREuler> {
     static constexpr int FLUX = 0, PLM = 0, CT = 0;
 };
@@ -687,6 +691,8 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
         Launch<PhysMHD>::mindt(S, st, hook);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::mindt(S, st, hook);
+    else if (sh.hydro == DISCO_HYDRO_GRMHD2)
+        Launch<PhysGRMHD2>::mindt(S, st, hook);
     else if (sh.hydro == DISCO_HYDRO_GREULER)
         Launch<PhysGREuler>::mindt(S, st, hook);
     else
@@ -699,6 +705,8 @@ void launch_substep(const StepShape& sh, const DevState& S, double RK, double dt
         Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
+    else if (sh.hydro == DISCO_HYDRO_GRMHD2)
+        Launch<PhysGRMHD2>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GREULER)
         Launch<PhysGREuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else

@@ -103,6 +103,9 @@ DHD void kerr_metric(const double* x, double M, double a, Metric* g) {
 }
 
 // grmhd.c:65-155 (passive scalars absent: NUM_Q = 8)
+// L2: grmhd2.c's convention, prim[UPP] = u_phi (grmhd.c: u_phi / R^2); the
+// two modules are otherwise the same here.
+template <bool L2 = false>
 DHD void prim2cons(double gamma_law, const Metric& g, const double* prim, double* cons, const double* x, double dV) {
     double r = x[0];
     double al = g.al;
@@ -113,7 +116,7 @@ DHD void prim2cons(double gamma_law, const Metric& g, const double* prim, double
     double sqrtgam = jac / al;
     double rho = prim[RHO];
     double Pp = prim[PPP];
-    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double l[3] = {prim[URR], L2 ? prim[UPP] : r * r * prim[UPP], prim[UZZ]};
     double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
     const double* U = g.U;
     double igaml[3];
@@ -191,10 +194,11 @@ DHD void c2p_invariants(const Metric& g, const double* cons, const double* x, C2
 
 // The reference's initial guess from a primitive state (grmhd.c:885-893):
 // v2 = u^2/(1+u^2), eta = W (1 + Gamma/(Gamma-1) P/rho).
+template <bool L2 = false>
 DHD void c2p_guess_from_prim(double gamma_law, const Metric& g, const double* prim, const double* x, double* v2,
                              double* eta) {
     double r = x[0];
-    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double l[3] = {prim[URR], L2 ? prim[UPP] : r * r * prim[UPP], prim[UZZ]};
     double u2 = 0.0;
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) u2 += g.igam[3 * i + j] * l[i] * l[j];
@@ -246,6 +250,7 @@ DHD int c2p_newton(double gamma_law, const C2PInvariants& v, double v20, double 
 }
 
 // Primitive recovery after the solve, grmhd.c:970-1003.
+template <bool L2 = false>
 DHD void c2p_recover(double gamma_law, double rho_floor, double pre_floor, const Metric& g, const C2PInvariants& v,
                      double v2, double eta, const double* x, double* prim) {
     double r = x[0];
@@ -269,7 +274,7 @@ DHD void c2p_recover(double gamma_law, double rho_floor, double pre_floor, const
     for (int i = 0; i < 3; ++i) l[i] = (v.S[i] + sg * BS * Bd[i] / (D * h * w)) / (D * h + sg * B2 / w);
     prim[RHO] = rho;
     prim[URR] = l[0];
-    prim[UPP] = l[1] / (r * r);
+    prim[UPP] = L2 ? l[1] : l[1] / (r * r);
     prim[UZZ] = l[2];
     prim[PPP] = Pp;
     prim[BRR] = sg * B[0];

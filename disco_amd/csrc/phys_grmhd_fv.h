@@ -137,6 +137,7 @@ DHD void metric_der(const double* x, double M, double a, MetricDer* D) {
 
 // ------------------------------------------------------------------ leaves
 // grmhd.c:247-337
+template <bool L2 = false>
 DHD void flux(double gamma_law, const Metric& g, const double* prim, double* F, const double* x, const double* n) {
     double r = x[0];
     const double al = g.al;
@@ -148,7 +149,7 @@ DHD void flux(double gamma_law, const Metric& g, const double* prim, double* F, 
     double sqrtgam = jac / al;
     double rho = prim[RHO];
     double Pp = prim[PPP];
-    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double l[3] = {prim[URR], L2 ? prim[UPP] : r * r * prim[UPP], prim[UZZ]};
     double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
     double igaml[3];
     for (int i = 0; i < 3; i++) {
@@ -246,6 +247,7 @@ DHD void getUstar(double gamma_law, const Metric& g, const double* prim, double*
 }
 
 // grmhd.c:489-646: fast-magnetosonic bounds and the HLLC contact speed
+template <bool L2 = false>
 DHD void vel(double gamma_law, const Metric& g, const double* prim1, const double* prim2, double* Sl, double* Sr,
              double* Ss, const double* n, const double* x) {
     double r = x[0];
@@ -256,12 +258,12 @@ DHD void vel(double gamma_law, const Metric& g, const double* prim1, const doubl
     double sqrtgam = x[0] / (al * r);  // metric_jacobian / (al r), as grmhd.c:499 rounds it
     double rho1 = prim1[RHO];
     double P1 = prim1[PPP];
-    double l1[3] = {prim1[URR], r * r * prim1[UPP], prim1[UZZ]};
+    double l1[3] = {prim1[URR], L2 ? prim1[UPP] : r * r * prim1[UPP], prim1[UZZ]};
     double B1[3] = {prim1[BRR] / sqrtgam, prim1[BPP] / (r * sqrtgam), prim1[BZZ] / sqrtgam};
     double cs21 = gamma_law * P1 / (rho1 + gamma_law / (gamma_law - 1.0) * P1);
     double rho2 = prim2[RHO];
     double P2 = prim2[PPP];
-    double l2[3] = {prim2[URR], r * r * prim2[UPP], prim2[UZZ]};
+    double l2[3] = {prim2[URR], L2 ? prim2[UPP] : r * r * prim2[UPP], prim2[UZZ]};
     double B2[3] = {prim2[BRR] / sqrtgam, prim2[BPP] / (r * sqrtgam), prim2[BZZ] / sqrtgam};
     double cs22 = gamma_law * P2 / (rho2 + gamma_law / (gamma_law - 1.0) * P2);
     double u21 = 0.0, u22 = 0.0;
@@ -368,6 +370,7 @@ DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, i
 // solve_riemann (riemann.c:216-320) for GRMHD with HLL or HLLC (HLLD needs
 // grmhd2.c's calc_msfast and is not part of this module); one metric
 // evaluation at the face centroid serves every leaf.
+template <bool L2 = false>
 DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
                     const double* n, double w, double dAdt, int dim, double* dU) {
     const double gl = P.gamma_law;
@@ -379,21 +382,21 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
         Ustr[q] = 0.0;
     }
     double Sl, Sr, Ss;
-    vel(gl, g, primL, primR, &Sl, &Sr, &Ss, n, x);
+    vel<L2>(gl, g, primL, primR, &Sl, &Sr, &Ss, n, x);
     if (w < Sl) {
-        flux(gl, g, primL, Flux, x, n);
-        prim2cons(gl, g, primL, Ustr, x, 1.0);
+        flux<L2>(gl, g, primL, Flux, x, n);
+        prim2cons<L2>(gl, g, primL, Ustr, x, 1.0);
     } else if (w > Sr) {
-        flux(gl, g, primR, Flux, x, n);
-        prim2cons(gl, g, primR, Ustr, x, 1.0);
+        flux<L2>(gl, g, primR, Flux, x, n);
+        prim2cons<L2>(gl, g, primR, Ustr, x, 1.0);
     } else if (P.riemann == DISCO_HLL) {
         double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
         double aL = Sr;
         double aR = -Sl;
-        prim2cons(gl, g, primL, Ul, x, 1.0);
-        prim2cons(gl, g, primR, Ur, x, 1.0);
-        flux(gl, g, primL, Fl, x, n);
-        flux(gl, g, primR, Fr, x, n);
+        prim2cons<L2>(gl, g, primL, Ul, x, 1.0);
+        prim2cons<L2>(gl, g, primR, Ur, x, 1.0);
+        flux<L2>(gl, g, primL, Fl, x, n);
+        flux<L2>(gl, g, primR, Fr, x, n);
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
             Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
             Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
@@ -401,14 +404,14 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
     } else {
         double Uk[NQ], Fk[NQ];
         if (w < Ss) {
-            prim2cons(gl, g, primL, Uk, x, 1.0);
+            prim2cons<L2>(gl, g, primL, Uk, x, 1.0);
             getUstar(gl, g, primL, Ustr, x, Sl, Ss, n);
-            flux(gl, g, primL, Fk, x, n);
+            flux<L2>(gl, g, primL, Fk, x, n);
             _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sl * (Ustr[q] - Uk[q]);
         } else {
-            prim2cons(gl, g, primR, Uk, x, 1.0);
+            prim2cons<L2>(gl, g, primR, Uk, x, 1.0);
             getUstar(gl, g, primR, Ustr, x, Sr, Ss, n);
-            flux(gl, g, primR, Fk, x, n);
+            flux<L2>(gl, g, primR, Fk, x, n);
             _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sr * (Ustr[q] - Uk[q]);
         }
     }
@@ -420,6 +423,7 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
 // The metric derivatives are taken one direction at a time (frame_der_U,
 // Frame/normal.c:21-42, restated per direction) to keep the live state
 // small; the sums run in the reference's order.
+template <bool L2 = false>
 DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                 double dVdt) {
     const double gamma_law = P.gamma_law;
@@ -436,7 +440,7 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double sqrtgam = jac / al;
     double rho = prim[RHO];
     double Pp = prim[PPP];
-    double l[4] = {0.0, prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double l[4] = {0.0, prim[URR], L2 ? prim[UPP] : r * r * prim[UPP], prim[UZZ]};
     double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
     double ia2 = 1.0 / (al * al);
     double igaml[3];
@@ -472,8 +476,9 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
     double rm = xm[0];
     double r2_3 = (rp * rp + rp * rm + rm * rm) / 3.0;
     double dphi = get_dp(xp[1], xm[1], P.phi_max);
-    double rcorr = r2_3 / (r * r);
-    double pcorr = sin(0.5 * dphi) / (0.5 * dphi);
+    // grmhd2.c:456-457 sets both cell-average corrections to 1
+    double rcorr = L2 ? 1.0 : r2_3 / (r * r);
+    double pcorr = L2 ? 1.0 : sin(0.5 * dphi) / (0.5 * dphi);
     double ia = 1.0 / al;
     double Sk[3] = {0.0, 0.0, 0.0};  // phi (Sk[1]) is a Killing direction
     double t[2][4];                  // the dU terms of mu = r, z
@@ -519,6 +524,7 @@ DHD void source(const PhysParams& P, const double* prim, double* cons, const dou
 }
 
 // grmhd.c:648-721
+template <bool L2 = false>
 DHD double mindt(const PhysParams& P, const double* prim, double wc, const double* xp, const double* xm) {
     const double gamma_law = P.gamma_law;
     double x[3] = {0.5 * (xm[0] + xp[0]), 0.5 * (xm[1] + xp[1]), 0.5 * (xm[2] + xp[2])};
@@ -532,7 +538,7 @@ DHD double mindt(const PhysParams& P, const double* prim, double wc, const doubl
     double sqrtgam = x[0] / (r * al);  // grmhd.c:655
     double rho = prim[RHO];
     double Pp = prim[PPP];
-    double l[3] = {prim[URR], r * r * prim[UPP], prim[UZZ]};
+    double l[3] = {prim[URR], L2 ? prim[UPP] : r * r * prim[UPP], prim[UZZ]};
     double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
     double cs = sqrt(gamma_law * Pp / (rho + gamma_law / (gamma_law - 1) * Pp));
     double uS[3];
@@ -578,6 +584,7 @@ DHD double mindt(const PhysParams& P, const double* prim, double wc, const doubl
 // grmhd.c:228-245 with cons2prim_solve_adiabatic_noble2d (:813-1029): the
 // Newton solve starts from the cell's previous primitives (prim on entry).
 // Returns the Newton iteration count; *okp (if given) = converged.
+template <bool L2 = false>
 DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
                   int* okp = nullptr) {
     double cons1[NQ];
@@ -587,11 +594,11 @@ DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const d
     C2PInvariants v;
     c2p_invariants(g, cons1, x, &v);
     double v20, eta0;
-    c2p_guess_from_prim(P.gamma_law, g, prim, x, &v20, &eta0);
+    c2p_guess_from_prim<L2>(P.gamma_law, g, prim, x, &v20, &eta0);
     double v2, eta;
     int ok;
     int it = c2p_newton(P.gamma_law, v, v20, eta0, C2P_PREC, C2P_MAX_ITER, C2P_NEXTRA, &v2, &eta, &ok);
-    c2p_recover(P.gamma_law, P.rho_floor, P.pre_floor, g, v, v2, eta, x, prim);
+    c2p_recover<L2>(P.gamma_law, P.rho_floor, P.pre_floor, g, v, v2, eta, x, prim);
     if (okp) *okp = ok;
     return it;
 }

@@ -108,8 +108,11 @@ struct PhysMHD {
 
 // GRMHD (grmhd.c) in the Kerr-Schild metric: the same CT layout as ideal MHD;
 // its Noble 2D Newton cons2prim starts from the cell's previous primitives,
-// so update_b_cell hands them in through prim[].
-struct PhysGRMHD {
+// so update_b_cell hands them in through prim[].  L2: grmhd2.c, the same
+// module with prim[UPP] = u_phi (grmhd.c: u_phi / R^2) and without the
+// source's cell-average corrections.
+template <bool L2>
+struct PhysGRMHDT {
     static constexpr int NQ = 8;
     static constexpr int NF = 5;
     static constexpr bool MHD = true;
@@ -118,20 +121,22 @@ struct PhysGRMHD {
     static constexpr bool NEWTONIAN = false;  // sink.c's terms are rejected at set-up
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
-        return grmhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
+        return grmhd::riemann<L2>(P, pL, pR, x, n, w, dAdt, dim, dU);
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                     double dVdt) {
-        grmhd::source(P, prim, cons, xp, xm, dVdt);
+        grmhd::source<L2>(P, prim, cons, xp, xm, dVdt);
     }
     DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
                       int* okp = nullptr) {
-        return grmhd::cons2prim(P, cons, prim, x, dV, okp);
+        return grmhd::cons2prim<L2>(P, cons, prim, x, dV, okp);
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
-        return grmhd::mindt(P, prim, w, xp, xm);
+        return grmhd::mindt<L2>(P, prim, w, xp, xm);
     }
 };
+using PhysGRMHD = PhysGRMHDT<false>;
+using PhysGRMHD2 = PhysGRMHDT<true>;
 
 // GR hydrodynamics (greuler.c) in the same metric: no field, no constrained
 // transport; its cons2prim also starts from the cell's previous primitives.

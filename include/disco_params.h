@@ -23,7 +23,7 @@ enum { DDD = 0, TAU = 1, SRR = 2, LLL = 3, SZZ = 4 };
 #endif
 
 /* Hydro module (Makefile_opt.in HYDRO slot) */
-enum { DISCO_HYDRO_EULER = 0, DISCO_HYDRO_MHD = 1, DISCO_HYDRO_GRMHD = 2, DISCO_HYDRO_GREULER = 3 };
+enum { DISCO_HYDRO_EULER = 0, DISCO_HYDRO_MHD = 1, DISCO_HYDRO_GRMHD = 2, DISCO_HYDRO_GREULER = 3, DISCO_HYDRO_GRMHD2 = 4 };
 /* Boundary module (BOUNDARY slot): fixed.c, fixed3d.c, zerograd.c,
  * outflow3d.c (zero gradient with the inflow velocity cut) */
 enum { DISCO_BC_FIXED = 0, DISCO_BC_FIXED3D = 1, DISCO_BC_ZEROGRAD = 2, DISCO_BC_OUTFLOW3D = 3 };

@@ -6,6 +6,7 @@
  * CPU build and the GPU library start from identical data.
  */
 #include "paul.h"
+#include <string.h>
 #include "disco_initial.h"
 
 static disco_ic the_ic;
@@ -32,6 +33,8 @@ void setICparams(struct domain* theDomain) {
     the_ic.tor_rin = p->initPar6;
     the_ic.tor_rmax = p->initPar7;
     the_ic.rho_atm = p->Density_Floor;
+    /* the Hydro module's convention for prim[UPP] */
+    the_ic.upp_covariant = strcmp(HYDRO, "greuler") == 0 || strcmp(HYDRO, "grmhd2") == 0;
 }
 
 void initial(double* prim, double* x) {

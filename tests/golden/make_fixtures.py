@@ -408,6 +408,17 @@ def fixtures_greuler() -> None:
     save_states("c6_greuler_small_hllc.npz", states)
 
 
+def fixtures_grmhd2() -> None:
+    # grmhd2.c (prim[UPP] = u_phi, no cell-average source corrections) on the small torus at spin 0: HLL
+    # 0/1/5/20 steps, HLLC 1/5 steps
+    txt = par_with("c5_grmhd_torus.par", Hydro="grmhd2", Num_R=16, Num_Z=12, Metric_Par3=0.0)
+    states, _ = run_ref("grmhd2_torus", txt, 20, dumps=(1, 5))
+    save_states("c7_grmhd2_small.npz", states)
+    txt = par_with("c5_grmhd_torus.par", Hydro="grmhd2", Num_R=16, Num_Z=12, Metric_Par3=0.0, Riemann_Solver=1)
+    states, _ = run_ref("grmhd2_torus", txt, 5, dumps=(1,))
+    save_states("c7_grmhd2_small_hllc.npz", states)
+
+
 def fixtures_steppers() -> None:
     for name, base, over, variant, n, dumps in STEPPER_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -470,6 +481,10 @@ LEAF_CASES = {
                                            "Num_Z": 12, "Metric_Par3": 0.0}, "greuler_torus"),
     "greuler_hllc": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
                                             "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1}, "greuler_torus"),
+    "grmhd2_hll": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0},
+                   "grmhd2_torus"),
+    "grmhd2_hllc": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                           "Riemann_Solver": 1}, "grmhd2_torus"),
 }
 
 
@@ -481,7 +496,11 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
     out = []
     for k in range(n):
         dim = k % 3
-        if kind.startswith("grmhd"):
+        if kind.startswith("grmhd2"):  # prim[UPP] is the covariant u_phi
+            r, z = U(4.0, 20.0), U(-5.0, 5.0)
+            pL = np.array([U(0.1, 1.0), U(1e-3, 5e-2), U(-0.1, 0.1), U(0.1, 0.4) * r, U(-0.1, 0.1),
+                           U(-0.02, 0.02), U(-0.02, 0.02), U(-0.02, 0.02)])
+        elif kind.startswith("grmhd"):
             r, z = U(4.0, 20.0), U(-5.0, 5.0)
             pL = np.array([U(0.1, 1.0), U(1e-3, 5e-2), U(-0.1, 0.1), U(0.1, 0.4) / r, U(-0.1, 0.1),
                            U(-0.02, 0.02), U(-0.02, 0.02), U(-0.02, 0.02)])
@@ -500,7 +519,7 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
         x = np.array([r, U(0.0, 2 * np.pi), z])
         nrm = np.zeros(3)
         nrm[[1, 0, 2][dim]] = 1.0
-        if kind.startswith("greuler"):
+        if kind.startswith(("greuler", "grmhd2")):
             w = 0.1 * U(0.8, 1.2) if dim == 0 else 0.0
         else:
             w = r * pL[3] * U(0.8, 1.2) if dim == 0 and not kind.startswith("grmhd") else (
@@ -581,6 +600,10 @@ def main() -> None:
         return
     if "--mm" in sys.argv:
         fixtures_mm()
+        print("fixtures written")
+        return
+    if "--grmhd2" in sys.argv:
+        fixtures_grmhd2()
         print("fixtures written")
         return
     if "--greuler" in sys.argv:

@@ -11,6 +11,8 @@
 //   riemann(prim1, prim2, w = 0, dAdt = 1): dU[8], E1, B1, E2, B2   (12)
 //   source(prim1) over the cell x +- (0.05, 0.02, 0.03), dVdt = 0.7 (8)
 //   mindt(prim1, wc = 0.1) over the same cell                 (1)
+// With riemann >= 20 the record is grmhd2.c's (prim[UPP] = u_phi; riemann -
+// 20 the solver), in the same 62 columns.
 // With riemann >= 10 the record is greuler.c's (GR hydrodynamics; riemann
 // - 10 the solver): prim[0..4] with prim[UPP] = u_phi, and the same 62
 // columns with zeros where greuler has no value (cons[5..7], F[5..7],
@@ -42,10 +44,11 @@ int main() {
         P.gr_M = M;
         P.gr_a = a;
         P.riemann = rs % 10;
+        const bool l2 = rs >= 20;
         P.phi_max = 2.0 * M_PI;
         grmhd::Metric g;
         grmhd::kerr_metric(x, M, a, &g);
-        if (rs >= 10) {
+        if (rs >= 10 && !l2) {
             std::printf("%.17g", g.al);
             for (double v : g.be) std::printf(" %.17g", v);
             for (double v : g.gam) std::printf(" %.17g", v);
@@ -76,26 +79,39 @@ int main() {
         for (double v : g.gam) std::printf(" %.17g", v);
         for (double v : g.igam) std::printf(" %.17g", v);
         double cons[8];
-        grmhd::prim2cons(gam_law, g, p1, cons, x, 1.0);
+        if (l2)
+            grmhd::prim2cons<true>(gam_law, g, p1, cons, x, 1.0);
+        else
+            grmhd::prim2cons(gam_law, g, p1, cons, x, 1.0);
         for (double v : cons) std::printf(" %.17g", v);
         const double n[3] = {dir == 0 ? 1.0 : 0.0, dir == 1 ? 1.0 : 0.0, dir == 2 ? 1.0 : 0.0};
         double F[8];
-        grmhd::flux(gam_law, g, p1, F, x, n);
+        if (l2)
+            grmhd::flux<true>(gam_law, g, p1, F, x, n);
+        else
+            grmhd::flux(gam_law, g, p1, F, x, n);
         for (double v : F) std::printf(" %.17g", v);
         double Sl, Sr, Ss;
-        grmhd::vel(gam_law, g, p1, p2, &Sl, &Sr, &Ss, n, x);
+        if (l2)
+            grmhd::vel<true>(gam_law, g, p1, p2, &Sl, &Sr, &Ss, n, x);
+        else
+            grmhd::vel(gam_law, g, p1, p2, &Sl, &Sr, &Ss, n, x);
         std::printf(" %.17g %.17g %.17g", Sl, Sr, Ss);
         double dU[8];
         const int dim = dir == 0 ? 1 : (dir == 1 ? 0 : 2);  // the reference's dim: 0 phi, 1 r, 2 z
-        FaceEMF e = PhysGRMHD::riemann(P, p1, p2, x, n, 0.0, 1.0, dim, dU);
+        FaceEMF e = l2 ? PhysGRMHD2::riemann(P, p1, p2, x, n, 0.0, 1.0, dim, dU)
+                       : PhysGRMHD::riemann(P, p1, p2, x, n, 0.0, 1.0, dim, dU);
         for (double v : dU) std::printf(" %.17g", v);
         std::printf(" %.17g %.17g %.17g %.17g", e.E1, e.B1, e.E2, e.B2);
         double xp[3] = {x[0] + 0.05, x[1] + 0.02, x[2] + 0.03};
         double xm[3] = {x[0] - 0.05, x[1] - 0.02, x[2] - 0.03};
         double src[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        PhysGRMHD::source(P, p1, src, xp, xm, 0.7);
+        if (l2)
+            PhysGRMHD2::source(P, p1, src, xp, xm, 0.7);
+        else
+            PhysGRMHD::source(P, p1, src, xp, xm, 0.7);
         for (double v : src) std::printf(" %.17g", v);
-        std::printf(" %.17g\n", PhysGRMHD::mindt(P, p1, 0.1, xp, xm));
+        std::printf(" %.17g\n", l2 ? PhysGRMHD2::mindt(P, p1, 0.1, xp, xm) : PhysGRMHD::mindt(P, p1, 0.1, xp, xm));
     }
     return 0;
 }

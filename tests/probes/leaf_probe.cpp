@@ -38,6 +38,10 @@ static void record(const PhysParams& P, int hydro, const double* in, double* out
         grmhd::Metric g;
         grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
         grmhd::prim2cons(P.gamma_law, g, pL, cons, x, 1.37);
+    } else if (hydro == DISCO_HYDRO_GRMHD2) {
+        grmhd::Metric g;
+        grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
+        grmhd::prim2cons<true>(P.gamma_law, g, pL, cons, x, 1.37);
     } else if (hydro == DISCO_HYDRO_GREULER) {
         grmhd::Metric g;
         grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
@@ -87,6 +91,8 @@ int main(int argc, char** argv) {
             record<PhysGRMHD>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_GREULER)
             record<PhysGREuler>(P, par.hydro, in.data(), out.data());
+        else if (par.hydro == DISCO_HYDRO_GRMHD2)
+            record<PhysGRMHD2>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_MHD)
             record<PhysMHD>(P, par.hydro, in.data(), out.data());
         else

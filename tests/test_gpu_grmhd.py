@@ -41,8 +41,11 @@ def _phi_errors(st, ref, real):
 
 
 @pytest.mark.parametrize("fixture,over", [("c5_grmhd_small.npz", {}),
-                                           ("c5_grmhd_outflow.npz", {"Boundary": "outflow3d"})],
-                         ids=["fixed3d", "outflow3d"])
+                                           ("c5_grmhd_outflow.npz", {"Boundary": "outflow3d"}),
+                                           # grmhd2.c: prim[UPP] = u_phi, no cell-average source corrections
+                                           ("c7_grmhd2_small.npz", {"Hydro": "grmhd2"}),
+                                           ("c7_grmhd2_small_hllc.npz", {"Hydro": "grmhd2", "Riemann_Solver": 1})],
+                         ids=["fixed3d", "outflow3d", "grmhd2", "grmhd2_hllc"])
 def test_grmhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     dom = D.Domain(params("c5_grmhd_torus.par", **SMALL, **over))

@@ -257,3 +257,42 @@ def test_greuler_leaves_equal_covariant_oracle(probe, a):
     print(f"greuler a = {a}: {worst}")
     assert max(worst["cons"], worst["flux"], worst["speeds"], worst["riemann"]) < 1e-12, worst
     assert worst["roundtrip"] < 1e-10, worst
+
+
+# ------------------------------------------------------------ grmhd2.c
+@pytest.mark.parametrize("a", [0.5, 0.9])
+def test_grmhd2_leaves_equal_covariant_oracle(probe, a):
+    """grmhd2.c is grmhd.c with prim[UPP] = u_phi: the 3+1 metric, prim2cons,
+    the fluxes, the HLL wave speeds and the HLL increment with its EMFs must
+    equal the covariant oracle's (given u_phi / R^2) to 1e-12 of each
+    quantity's scale; the CFL step too (the same formula).  Its source drops
+    grmhd.c's cell-average corrections and is pinned at a = 0 by the
+    reference's leaf vectors only."""
+    recs = []
+    for p1, p2, x, d in _states(240, 31 if a == 0.9 else 32):
+        q1, q2 = p1.copy(), p2.copy()
+        q1[3] *= x[0] ** 2
+        q2[3] *= x[0] ** 2
+        recs.append((q1, q2, x, d))
+    lib = probe(a, recs, mode=20)
+    worst = {}
+    for k in ("metric", "cons", "flux", "speeds", "riemann", "dt"):
+        worst[k] = 0.0
+    for row, (p1, p2, x, d) in zip(lib, recs):
+        g1, g2 = p1.copy(), p2.copy()
+        g1[3] /= x[0] ** 2
+        g2[3] /= x[0] ** 2
+        pt = GC.Point(x, M, a)
+        s1 = GC.State(pt, g1, GAMMA)
+        dU, E, (Sl, Sr) = GC.riemann_hll(x, g1, g2, d, 0.0, 1.0, M, a, GAMMA)
+        xp = x + np.array([0.05, 0.02, 0.03])
+        xm = x - np.array([0.05, 0.02, 0.03])
+        for k, sl, ora in (("metric", slice(0, 22), np.concatenate([[pt.al], pt.be, pt.gam.ravel(), pt.igam.ravel()])),
+                           ("cons", slice(22, 30), GC.prim2cons(pt, s1)),
+                           ("flux", slice(30, 38), GC.flux(pt, s1, d)),
+                           ("speeds", slice(38, 40), np.array([Sl, Sr])),
+                           ("riemann", slice(41, 53), np.concatenate([dU, E])),
+                           ("dt", slice(61, 62), np.array([GC.mindt(g1, 0.1, xp, xm, M, a, GAMMA)]))):
+            worst[k] = max(worst[k], np.abs(row[sl] - ora).max() / max(np.abs(ora).max(), 1e-300))
+    print(f"grmhd2 a = {a}: {worst}")
+    assert max(worst.values()) < 1e-12, worst

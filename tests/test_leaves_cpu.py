@@ -3,8 +3,8 @@ reference's own functions.
 
 tests/golden/leaf_vectors.npz holds random face states and what the
 reference build returns for them (oracle/ref/ref_leaf.c, linked with the
-reference's riemann.c, hlld.c and Hydro/euler.c, mhd.c, grmhd.c or
-greuler.c + schw_ks_rpz.c): solve_riemann's conserved increments (F - w U*) dA dt and
+reference's riemann.c, hlld.c and Hydro/euler.c, mhd.c, grmhd.c, grmhd2.c
+or greuler.c + schw_ks_rpz.c): solve_riemann's conserved increments (F - w U*) dA dt and
 flux_to_E's EMFs for HLL, HLLC and HLLD on phi, r and z faces, prim2cons,
 cons2prim back, mindt and source.  The library's leaves (the step_cells.h
 policies over phys_euler.h, phys_mhd.h, phys_grmhd_fv.h and phys_greuler.h -- the source the
@@ -38,6 +38,9 @@ CASES = {
                                            "Num_Z": 12, "Metric_Par3": 0.0}),
     "greuler_hllc": ("c5_grmhd_torus.par", {"Hydro": "greuler", "Num_C": 5, "Constrained_Transport": 0, "Num_R": 16,
                                             "Num_Z": 12, "Metric_Par3": 0.0, "Riemann_Solver": 1}),
+    "grmhd2_hll": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0}),
+    "grmhd2_hllc": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                           "Riemann_Solver": 1}),
 }
 
 

@@ -204,6 +204,8 @@ SETUP_CASES = CASES + [("c2e_small.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z
                        ("c6_greuler_small.npz", "c5_grmhd_torus.par",
                         dict(Hydro="greuler", Num_C=5, Constrained_Transport=0, Num_R=16, Num_Z=12,
                              Metric_Par3=0.0)),
+                       ("c7_grmhd2_small.npz", "c5_grmhd_torus.par",
+                        dict(Hydro="grmhd2", Num_R=16, Num_Z=12, Metric_Par3=0.0)),
                        # Mesh_Motion 3: the annulus mean of the initial face speeds
                        ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
                        # the rotating frame's initial and fixed-ghost data
