@@ -59,9 +59,11 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
     double vn_off = vn - om * r * n[1];
     double Ss_off = Ss + vn_off - vn;
     double rhoe = Pp / (P.gamma_law - 1.);
-    double rhostar = rho * (Sk - vn) / (Sk - Ss);
-    double Pstar = Pp * (Ss - vn) / (Sk - Ss);
-    double Us = rhoe * (Sk - vn) / (Sk - Ss);
+    const double dS = Sk - Ss;
+    const double ydS = 1.0 / dS;  // three quotients, one division (div_rn: exact)
+    double rhostar = div_rn(rho * (Sk - vn), dS, ydS);
+    double Pstar = div_rn(Pp * (Ss - vn), dS, ydS);
+    double Us = div_rn(rhoe * (Sk - vn), dS, ydS);
     Ustar[DDD] = rhostar;
     Ustar[SRR] = rhostar * (vr + (Ss - vn) * n[0]);
     Ustar[LLL] = r * rhostar * (vp + (Ss - vn) * n[1]);
@@ -74,17 +76,20 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
 template <int NQ>
 DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
     double r = x[0];
-    double rho = cons[DDD] / dV;
+    // quotients by dV, rho and r share one division each (div_rn: exact)
+    const double yV = 1.0 / dV, yr = 1.0 / r;
+    double rho = div_rn(cons[DDD], dV, yV);
     if (rho < P.rho_floor) rho = P.rho_floor;
-    double Sr = cons[SRR] / dV;
-    double Sp = cons[LLL] / dV / r;
-    double Sz = cons[SZZ] / dV;
-    double E = cons[TAU] / dV;
+    const double yrho = 1.0 / rho;
+    double Sr = div_rn(cons[SRR], dV, yV);
+    double Sp = div_rn(div_rn(cons[LLL], dV, yV), r, yr);
+    double Sz = div_rn(cons[SZZ], dV, yV);
+    double E = div_rn(cons[TAU], dV, yV);
     double om = get_om(P, x);
-    double vr = Sr / rho;
-    double vp = Sp / rho;
+    double vr = div_rn(Sr, rho, yrho);
+    double vp = div_rn(Sp, rho, yrho);
     double vp_off = vp - om * r;
-    double vz = Sz / rho;
+    double vz = div_rn(Sz, rho, yrho);
     double KE = .5 * (Sr * vr + rho * vp_off * vp_off + Sz * vz);
     double rhoe = E - KE;
     double Pp = (P.gamma_law - 1.) * rhoe;
@@ -96,7 +101,7 @@ DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const 
     prim[RHO] = rho;
     prim[PPP] = Pp;
     prim[URR] = vr;
-    prim[UPP] = vp / r;
+    prim[UPP] = div_rn(vp, r, yr);
     prim[UZZ] = vz;
     _Pragma("unroll") for (int q = NUM_C; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
 }
@@ -173,9 +178,11 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
         prim2cons<NQ>(P, primR, Ur, x, 1.0);
         flux<NQ>(P, primL, Fl, x, n);
         flux<NQ>(P, primR, Fr, x, n);
+        const double aS = aL + aR;
+        const double yS = 1.0 / aS;
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-            Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
-            Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+            Flux[q] = div_rn(aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q]), aS, yS);
+            Ustr[q] = div_rn(aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q], aS, yS);
         }
     } else {
         // riemann.c:272-290 on the side of the contact; the lanes of a

@@ -77,11 +77,13 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
     double E = .5 * rho * v2 + rhoe + .5 * B2;
     double Bs2 = Bsr * Bsr + Bsp * Bsp + Bsz * Bsz;
     double Ps = rho * (Sk - vn) * (Ss - vn) + (Pp + .5 * B2 - Bn * Bn) - .5 * Bs2 + Bsn * Bsn;
-    double Dstar = (Sk - vn) * D / (Sk - Ss);
-    double Msr = ((Sk - vn) * mr + (Br * Bn - Bsr * Bsn)) / (Sk - Ss);
-    double Msp = ((Sk - vn) * mp + (Bp * Bn - Bsp * Bsn)) / (Sk - Ss);
-    double Msz = ((Sk - vn) * mz + (Bz * Bn - Bsz * Bsn)) / (Sk - Ss);
-    double Estar = ((Sk - vn) * E + (Ps + .5 * Bs2) * Ss - (Pp + .5 * B2) * vn - vBs * Bsn + vB * Bn) / (Sk - Ss);
+    const double dS = Sk - Ss;
+    const double ydS = 1.0 / dS;  // five quotients, one division (div_rn: exact)
+    double Dstar = div_rn((Sk - vn) * D, dS, ydS);
+    double Msr = div_rn((Sk - vn) * mr + (Br * Bn - Bsr * Bsn), dS, ydS);
+    double Msp = div_rn((Sk - vn) * mp + (Bp * Bn - Bsp * Bsn), dS, ydS);
+    double Msz = div_rn((Sk - vn) * mz + (Bz * Bn - Bsz * Bsn), dS, ydS);
+    double Estar = div_rn((Sk - vn) * E + (Ps + .5 * Bs2) * Ss - (Pp + .5 * B2) * vn - vBs * Bsn + vB * Bn, dS, ydS);
     double Msn = Dstar * Ss;
     double mn = Msr * n[0] + Msp * n[1] + Msz * n[2];
     Msr += n[0] * (Msn - mn);
@@ -100,20 +102,23 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
 // mhd.c:138-185
 DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
     double r = x[0];
-    double rho = cons[DDD] / dV;
+    // quotients by dV, rho and r share one division each (div_rn: exact)
+    const double yV = 1.0 / dV, yr = 1.0 / r;
+    double rho = div_rn(cons[DDD], dV, yV);
     if (rho < P.rho_floor) rho = P.rho_floor;
-    double Sr = cons[SRR] / dV;
-    double Sp = cons[LLL] / dV / r;
-    double Sz = cons[SZZ] / dV;
-    double E = cons[TAU] / dV;
+    const double yrho = 1.0 / rho;
+    double Sr = div_rn(cons[SRR], dV, yV);
+    double Sp = div_rn(div_rn(cons[LLL], dV, yV), r, yr);
+    double Sz = div_rn(cons[SZZ], dV, yV);
+    double E = div_rn(cons[TAU], dV, yV);
     double om = get_om(P, x);
-    double vr = Sr / rho;
-    double vp = Sp / rho;
+    double vr = div_rn(Sr, rho, yrho);
+    double vp = div_rn(Sp, rho, yrho);
     double vp_off = vp - om * r;
-    double vz = Sz / rho;
+    double vz = div_rn(Sz, rho, yrho);
     double Br = cons[BRR] / (dV * bscale_r(r));
-    double Bp = cons[BPP] / dV * r;
-    double Bz = cons[BZZ] / dV;
+    double Bp = div_rn(cons[BPP], dV, yV) * r;
+    double Bz = div_rn(cons[BZZ], dV, yV);
     double B2 = Br * Br + Bp * Bp + Bz * Bz;
     double KE = .5 * (Sr * vr + rho * vp_off * vp_off + Sz * vz);
     double rhoe = E - KE - .5 * B2;
@@ -126,7 +131,7 @@ DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const 
     prim[RHO] = rho;
     prim[PPP] = Pp;
     prim[URR] = vr;
-    prim[UPP] = vp / r;
+    prim[UPP] = div_rn(vp, r, yr);
     prim[UZZ] = vz;
     prim[BRR] = Br;
     prim[BPP] = Bp;
@@ -222,22 +227,25 @@ DHD void vel(const PhysParams& P, const double* prim1, const double* prim2, doub
 
     double aL = L_Plus;
     double aR = -L_Mins;
-    double Br = (aR * Br1 + aL * Br2 + FrL - FrR) / (aL + aR);
-    double Bp = (aR * Bp1 + aL * Bp2 + FpL - FpR) / (aL + aR);
-    double Bz = (aR * Bz1 + aL * Bz2 + FzL - FzR) / (aL + aR);
+    const double aS = aL + aR;
+    const double yS = 1.0 / aS;  // seven quotients by aL + aR, one division (div_rn: exact)
+    double Br = div_rn(aR * Br1 + aL * Br2 + FrL - FrR, aS, yS);
+    double Bp = div_rn(aR * Bp1 + aL * Bp2 + FpL - FpR, aS, yS);
+    double Bz = div_rn(aR * Bz1 + aL * Bz2 + FzL - FzR, aS, yS);
     double Bn = Br * n[0] + Bp * n[1] + Bz * n[2];
-    double mr = (aR * mrL + aL * mrR + FmrL - FmrR) / (aL + aR);
-    double mp = (aR * mpL + aL * mpR + FmpL - FmpR) / (aL + aR);
-    double mz = (aR * mzL + aL * mzR + FmzL - FmzR) / (aL + aR);
+    double mr = div_rn(aR * mrL + aL * mrR + FmrL - FmrR, aS, yS);
+    double mp = div_rn(aR * mpL + aL * mpR + FmpL - FmpR, aS, yS);
+    double mz = div_rn(aR * mzL + aL * mzR + FmzL - FmzR, aS, yS);
     double mnL = mrL * n[0] + mpL * n[1] + mzL * n[2];
     double mnR = mrR * n[0] + mpR * n[1] + mzR * n[2];
-    double rho = (aR * rho1 + aL * rho2 + mnL - mnR) / (aL + aR);
+    double rho = div_rn(aR * rho1 + aL * rho2 + mnL - mnR, aS, yS);
     double L_Star = (rho2 * vn2 * (L_Plus - vn2) - rho1 * vn1 * (L_Mins - vn1) + (P1 + .5 * B21 - Bn1 * Bn1) -
                      (P2 + .5 * B22 - Bn2 * Bn2)) /
                     (rho2 * (L_Plus - vn2) - rho1 * (L_Mins - vn1));
-    double vr = mr / rho;
-    double vp = mp / rho;
-    double vz = mz / rho;
+    const double yrho = 1.0 / rho;
+    double vr = div_rn(mr, rho, yrho);
+    double vp = div_rn(mp, rho, yrho);
+    double vz = div_rn(mz, rho, yrho);
     double vdotB = vr * Br + vp * Bp + vz * Bz;
     Bpack[0] = Bn;
     Bpack[1] = Br;
@@ -368,16 +376,20 @@ DHD void single_star(double g, const double* prim, double* Ustar, double S_K, do
     double B2 = Bx * Bx + By * By + Bz * Bz;
     double Pt = Pp + .5 * B2;
     double e = .5 * rho * v2 + Pp / (g - 1.) + .5 * B2;
-    double rhostar = rho * (S_K - u_K) / (S_K - S_M);
+    const double dSM = S_K - S_M;
+    const double ydSM = 1.0 / dSM;
+    double rhostar = div_rn(rho * (S_K - u_K), dSM, ydSM);
     double denom = rho * (S_K - u_K) * (S_K - S_M) - Bn * Bn;
     if (fabs(denom) < 1e-10) denom = 1e-10;
-    double vx_star = vx - Bn * Bx * (S_M - u_K) / denom;
-    double vy_star = vy - Bn * By * (S_M - u_K) / denom;
-    double vz_star = vz - Bn * Bz * (S_M - u_K) / denom;
+    // six quotients by denom: one division, then div_rn's exact quotients
+    const double yden = 1.0 / denom;
+    double vx_star = vx - div_rn(Bn * Bx * (S_M - u_K), denom, yden);
+    double vy_star = vy - div_rn(Bn * By * (S_M - u_K), denom, yden);
+    double vz_star = vz - div_rn(Bn * Bz * (S_M - u_K), denom, yden);
     double sk2 = (S_K - u_K) * (S_K - u_K);
-    double Bx_star = Bx * (rho * sk2 - Bn * Bn) / denom;
-    double By_star = By * (rho * sk2 - Bn * Bn) / denom;
-    double Bz_star = Bz * (rho * sk2 - Bn * Bn) / denom;
+    double Bx_star = div_rn(Bx * (rho * sk2 - Bn * Bn), denom, yden);
+    double By_star = div_rn(By * (rho * sk2 - Bn * Bn), denom, yden);
+    double Bz_star = div_rn(Bz * (rho * sk2 - Bn * Bn), denom, yden);
     double vn_star = vx_star * n[0] + vy_star * n[1] + vz_star * n[2];
     double Bn_star = Bx_star * n[0] + By_star * n[1] + Bz_star * n[2];
     vx_star += (S_M - vn_star) * n[0];
@@ -387,7 +399,7 @@ DHD void single_star(double g, const double* prim, double* Ustar, double S_K, do
     By_star += (Bn - Bn_star) * n[1];
     Bz_star += (Bn - Bn_star) * n[2];
     double vBstar = vx_star * Bx_star + vy_star * By_star + vz_star * Bz_star;
-    double e_star = ((S_K - u_K) * e - Pt * u_K + Pt_star * S_M + Bn * (vdotB - vBstar)) / (S_K - S_M);
+    double e_star = div_rn((S_K - u_K) * e - Pt * u_K + Pt_star * S_M + Bn * (vdotB - vBstar), dSM, ydSM);
     Ustar[DDD] = rhostar;
     Ustar[SRR] = rhostar * vx_star;
     Ustar[LLL] = rhostar * vy_star;
@@ -407,15 +419,16 @@ DHD void double_star(const double* UsL, const double* UsR, double* Uss, const do
     double rrh_L = sqrt(rho_L);
     double rrh_R = sqrt(rho_R);
     double rrh_K = sqrt(rhostar_K);
-    double vx_L = UsL[SRR] / rho_L;
-    double vy_L = UsL[LLL] / rho_L;
-    double vz_L = UsL[SZZ] / rho_L;
+    const double yL = 1.0 / rho_L, yR = 1.0 / rho_R;
+    double vx_L = div_rn(UsL[SRR], rho_L, yL);
+    double vy_L = div_rn(UsL[LLL], rho_L, yL);
+    double vz_L = div_rn(UsL[SZZ], rho_L, yL);
     double Bx_L = UsL[BRR];
     double By_L = UsL[BPP];
     double Bz_L = UsL[BZZ];
-    double vx_R = UsR[SRR] / rho_R;
-    double vy_R = UsR[LLL] / rho_R;
-    double vz_R = UsR[SZZ] / rho_R;
+    double vx_R = div_rn(UsR[SRR], rho_R, yR);
+    double vy_R = div_rn(UsR[LLL], rho_R, yR);
+    double vz_R = div_rn(UsR[SZZ], rho_R, yR);
     double Bx_R = UsR[BRR];
     double By_R = UsR[BPP];
     double Bz_R = UsR[BZZ];
@@ -424,16 +437,17 @@ DHD void double_star(const double* UsL, const double* UsR, double* Uss, const do
     double Bn = .5 * (Bn_L + Bn_R);
     double signBn = Bn > 0.0 ? 1.0 : -1.0;
     double denom = rrh_L + rrh_R;
-    double vx_ss = (rrh_L * vx_L + rrh_R * vx_R + (Bx_R - Bx_L) * signBn) / denom;
-    double vy_ss = (rrh_L * vy_L + rrh_R * vy_R + (By_R - By_L) * signBn) / denom;
-    double vz_ss = (rrh_L * vz_L + rrh_R * vz_R + (Bz_R - Bz_L) * signBn) / denom;
+    const double yden = 1.0 / denom;
+    double vx_ss = div_rn(rrh_L * vx_L + rrh_R * vx_R + (Bx_R - Bx_L) * signBn, denom, yden);
+    double vy_ss = div_rn(rrh_L * vy_L + rrh_R * vy_R + (By_R - By_L) * signBn, denom, yden);
+    double vz_ss = div_rn(rrh_L * vz_L + rrh_R * vz_R + (Bz_R - Bz_L) * signBn, denom, yden);
     double vn_ss = vx_ss * n[0] + vy_ss * n[1] + vz_ss * n[2];
     vx_ss += (v[2] - vn_ss) * n[0];
     vy_ss += (v[2] - vn_ss) * n[1];
     vz_ss += (v[2] - vn_ss) * n[2];
-    double Bx_ss = (rrh_L * Bx_R + rrh_R * Bx_L + rrh_L * rrh_R * (vx_R - vx_L) * signBn) / denom;
-    double By_ss = (rrh_L * By_R + rrh_R * By_L + rrh_L * rrh_R * (vy_R - vy_L) * signBn) / denom;
-    double Bz_ss = (rrh_L * Bz_R + rrh_R * Bz_L + rrh_L * rrh_R * (vz_R - vz_L) * signBn) / denom;
+    double Bx_ss = div_rn(rrh_L * Bx_R + rrh_R * Bx_L + rrh_L * rrh_R * (vx_R - vx_L) * signBn, denom, yden);
+    double By_ss = div_rn(rrh_L * By_R + rrh_R * By_L + rrh_L * rrh_R * (vy_R - vy_L) * signBn, denom, yden);
+    double Bz_ss = div_rn(rrh_L * Bz_R + rrh_R * Bz_L + rrh_L * rrh_R * (vz_R - vz_L) * signBn, denom, yden);
     double Bn_ss = Bx_ss * n[0] + By_ss * n[1] + Bz_ss * n[2];
     Bx_ss += (Bn - Bn_ss) * n[0];
     By_ss += (Bn - Bn_ss) * n[1];
@@ -584,9 +598,11 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
             prim2cons(P, primR, Ur, x, 1.0);
             flux(P, primL, Fl, x, n);
             flux(P, primR, Fr, x, n);
+            const double aS = aL + aR;
+            const double yS = 1.0 / aS;
             _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-                Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
-                Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+                Flux[q] = div_rn(aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q]), aS, yS);
+                Ustr[q] = div_rn(aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q], aS, yS);
             }
         } else {
             // riemann.c:272-290 on the side of the contact, one path for the

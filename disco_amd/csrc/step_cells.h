@@ -476,16 +476,19 @@ DHD void stage_a_cell(const DevState& S, int c, int first, int search) {
     double dpL = S.dphi[cm];
     double dpC = S.dphi[c];
     double dpR = S.dphi[cp];
+    // the three centre distances are shared by every field: one division
+    // each, then the exactly rounded quotients of div_rn
+    const double dL = .5 * (dpC + dpL);
+    const double dR = .5 * (dpR + dpC);
+    const double dC = .5 * (dpL + dpR) + dpC;
+    const double yL = 1.0 / dL, yR = 1.0 / dR, yC = 1.0 / dC;
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double pL = S.prim[q * N + cm];
         double pC = S.prim[q * N + c];
         double pR = S.prim[q * N + cp];
-        double sL = pC - pL;
-        sL /= .5 * (dpC + dpL);
-        double sR = pR - pC;
-        sR /= .5 * (dpR + dpC);
-        double sC = pR - pL;
-        sC /= .5 * (dpL + dpR) + dpC;
+        double sL = div_rn(pC - pL, dL, yL);
+        double sR = div_rn(pR - pC, dR, yR);
+        double sC = div_rn(pR - pL, dC, yC);
         st_stream(&S.gradp[q * N + c], minmod(P.plm * sL, sC, P.plm * sR));
     }
     if (search) partners_cell(S, c);
@@ -518,10 +521,12 @@ DHD void acc_add(SlopeAcc<NQ>& a, const double* S, double dA) {
 template <int NQ>
 DHD void face_slope(const DevState& S, int cL, int cR, const TFace& F, double dxL, double dxR, double* Sq) {
     const size_t N = S.ncell;
+    const double dx = dxR + dxL;
+    const double y = 1.0 / dx;
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         double WL = S.prim[q * N + cL] + F.dpL * S.gradp[q * N + cL];
         double WR = S.prim[q * N + cR] - F.dpR * S.gradp[q * N + cR];
-        Sq[q] = (WR - WL) / (dxR + dxL);
+        Sq[q] = div_rn(WR - WL, dx, y);
     }
 }
 
@@ -605,8 +610,9 @@ DHD void trans_gradient(const DevState& S, const AnnDev& A, const PhysParams& P,
     if (alo < 0) dAtot = dAp;
     if (ahi < 0) dAtot = dAm;
     const bool geom = D == DIR_R && (A.flags & ANN_GEOM_R);
+    const double yA = 1.0 / dAtot;
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-        double g = acc.sum[q] / dAtot;
+        double g = div_rn(acc.sum[q], dAtot, yA);
         if (!acc.any) {
             g = 0.0;
         } else if (g != 0.0) {
