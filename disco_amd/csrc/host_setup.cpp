@@ -318,8 +318,9 @@ static int unsupported(const disco_params& p, std::string* err) {
             m += " Mesh_Motion (grmhd: 0 or 4)";
     }
     if (is_grmhd(p.hydro)) {
-        // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one
-        if (p.Riemann_Solver == DISCO_HLLD) m += " Riemann_Solver=HLLD with grmhd";
+        // grmhd.c links only with hlld.c, whose HLLD is the Newtonian one;
+        // grmhd2.c takes the relativistic one of hlld_grmhd.c
+        if (p.Riemann_Solver == DISCO_HLLD && p.hydro == DISCO_HYDRO_GRMHD) m += " Riemann_Solver=HLLD with grmhd";
         if (p.isothermal_flag) m += " isothermal grmhd (cons2prim_solve_isothermal is a stub)";
         if (p.metricPar2 <= 0.0 || std::fabs(p.metricPar3) >= p.metricPar2) m += " Kerr needs M > 0 and |a| < M";
     } else if (p.hydro == DISCO_HYDRO_EULER && p.Riemann_Solver == DISCO_HLLD) {

@@ -16,6 +16,7 @@
 
 #include "phys_common.h"
 #include "phys_grmhd.h"
+#include "phys_grhlld.h"
 
 namespace disco {
 namespace grmhd {
@@ -367,8 +368,9 @@ DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, i
     return e;
 }
 
-// solve_riemann (riemann.c:216-320) for GRMHD with HLL or HLLC (HLLD needs
-// grmhd2.c's calc_msfast and is not part of this module); one metric
+// solve_riemann (riemann.c:216-320) for GRMHD with HLL or HLLC, and for
+// grmhd2.c with the relativistic HLLD of hlld_grmhd.c (riemann.c:293-295:
+// get_Ustar_HLLD; grmhd.c cannot link it, it lacks calc_msfast); one metric
 // evaluation at the face centroid serves every leaf.
 template <bool L2 = false>
 DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
@@ -380,6 +382,13 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
     _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
         Flux[q] = 0.0;
         Ustr[q] = 0.0;
+    }
+    if constexpr (L2) {
+        if (P.riemann == DISCO_HLLD) {
+            grhlld::face_state(gl, g, w, primL, primR, Flux, Ustr, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+            return flux_to_E(Flux, Ustr, x, dim);
+        }
     }
     double Sl, Sr, Ss;
     vel<L2>(gl, g, primL, primR, &Sl, &Sr, &Ss, n, x);

@@ -408,6 +408,13 @@ def fixtures_greuler() -> None:
     save_states("c6_greuler_small_hllc.npz", states)
 
 
+def fixtures_grmhd2_hlld() -> None:
+    # grmhd2.c with the relativistic HLLD of hlld_grmhd.c (Riemann_Solver 2) on the small torus at spin 0
+    txt = par_with("c5_grmhd_torus.par", Hydro="grmhd2", Num_R=16, Num_Z=12, Metric_Par3=0.0, Riemann_Solver=2)
+    states, _ = run_ref("grmhd2_torus_hlld", txt, 5, dumps=(1,))
+    save_states("c7_grmhd2_small_hlld.npz", states)
+
+
 def fixtures_grmhd2() -> None:
     # grmhd2.c (prim[UPP] = u_phi, no cell-average source corrections) on the small torus at spin 0: HLL
     # 0/1/5/20 steps, HLLC 1/5 steps
@@ -485,6 +492,10 @@ LEAF_CASES = {
                    "grmhd2_torus"),
     "grmhd2_hllc": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
                                            "Riemann_Solver": 1}, "grmhd2_torus"),
+    "grmhd2_hlld": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                           "Riemann_Solver": 2}, "grmhd2_torus_hlld"),
+    "grmhd2_hlld_strong": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                                  "Riemann_Solver": 2}, "grmhd2_torus_hlld"),
 }
 
 
@@ -511,9 +522,16 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
             r, z = U(0.6, 2.9), U(-0.4, 0.4)
             pL = np.array([U(0.5, 2.0), U(5e-3, 5e-2), U(-0.05, 0.05), U(0.2, 1.0), U(-0.05, 0.05),
                            U(-0.05, 0.05), U(-0.05, 0.05), U(-0.05, 0.05)])[:nq]
-        pR = pL * (1.0 + 0.2 * U(-1, 1, nq))
-        pR[2] = pL[2] + 0.03 * U(-1, 1)
-        pR[4] = pL[4] + 0.03 * U(-1, 1)
+        if kind.endswith("_strong"):  # strong jumps and fields: HLLD's retries and its HLL fallback
+            pL[2], pL[4] = U(-0.3, 0.3), U(-0.3, 0.3)
+            pL[5:8] = U(-1.0, 1.0, 3)
+            pR = pL * (1.0 + 0.9 * U(-1, 1, nq))
+            pR[2] = pL[2] + 0.45 * U(-1, 1)
+            pR[4] = pL[4] + 0.45 * U(-1, 1)
+        else:
+            pR = pL * (1.0 + 0.2 * U(-1, 1, nq))
+            pR[2] = pL[2] + 0.03 * U(-1, 1)
+            pR[4] = pL[4] + 0.03 * U(-1, 1)
         if k % 17 == 0:
             pR = pL.copy()  # identical states
         x = np.array([r, U(0.0, 2 * np.pi), z])
@@ -531,16 +549,16 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
 def leaf_fixtures(n: int = 192) -> None:
     arrs = {}
     for i, (name, (base, over, variant)) in enumerate(LEAF_CASES.items()):
-        rec = leaf_records(name, n, 1000 + i)
+        rec = leaf_records(name, 4 * n if name.endswith("_strong") else n, 1000 + i)
         nq = (rec.shape[1] - 9) // 2
         exe = os.path.join(ROOT, "oracle", "_ref", f"disco_ref_leaf_{variant}")
         with tempfile.TemporaryDirectory() as d:
             open(os.path.join(d, "in.par"), "w").write(par_with(base, **over))
             with open(os.path.join(d, "in.bin"), "wb") as f:
-                f.write(np.int32(n).tobytes())
+                f.write(np.int32(rec.shape[0]).tobytes())
                 f.write(rec.astype(np.float64).tobytes())
             subprocess.run([exe, "in.bin", "out.bin"], cwd=d, check=True, timeout=120, capture_output=True)
-            out = np.fromfile(os.path.join(d, "out.bin"), np.float64).reshape(n, 4 * nq + 5)
+            out = np.fromfile(os.path.join(d, "out.bin"), np.float64).reshape(rec.shape[0], 4 * nq + 5)
         arrs[f"{name}_in"] = rec
         arrs[f"{name}_out"] = out
     np.savez_compressed(os.path.join(HERE, "leaf_vectors.npz"), cases=np.array(list(LEAF_CASES)), **arrs)
@@ -600,6 +618,10 @@ def main() -> None:
         return
     if "--mm" in sys.argv:
         fixtures_mm()
+        print("fixtures written")
+        return
+    if "--grmhd2-hlld" in sys.argv:
+        fixtures_grmhd2_hlld()
         print("fixtures written")
         return
     if "--grmhd2" in sys.argv:

@@ -41,6 +41,12 @@ CASES = {
     "grmhd2_hll": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0}),
     "grmhd2_hllc": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
                                            "Riemann_Solver": 1}),
+    "grmhd2_hlld": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                           "Riemann_Solver": 2}),
+    # strong jumps and fields: the relativistic HLLD's retried starting
+    # pressures and its HLL fallback
+    "grmhd2_hlld_strong": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
+                                                  "Riemann_Solver": 2}),
 }
 
 
