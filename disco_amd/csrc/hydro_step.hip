@@ -34,8 +34,14 @@ namespace disco {
 #ifndef DISCO_OCC_PLM
 #define DISCO_OCC_PLM 0
 #endif
+// The conserved update (k_update_a) runs uncapped: held to 4 waves it
+// spilled 100 B per lane to scratch and took 10% longer than at the 3 waves
+// its 162 VGPRs allow.  The Stokes update (k_ct_update) is held to 6 waves.
 #ifndef DISCO_OCC_CT
-#define DISCO_OCC_CT 4
+#define DISCO_OCC_CT 0
+#endif
+#ifndef DISCO_OCC_CTU
+#define DISCO_OCC_CTU 6
 #endif
 #define DISCO_OCC(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, 8)))
 
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(256) void k_source(DevState S, double dtfac) {
 // own: fused, the two need 127 VGPRs (4 waves per SIMD) and run 19% slower
 // than apart.  The Stokes update is held to 6 waves per SIMD (80 VGPRs; 5%
 // faster than its natural 83 at 5 waves).
-__global__ __launch_bounds__(256) DISCO_OCC(6) void k_ct_update(DevState S, double RK, double dtfac, int first,
+__global__ __launch_bounds__(256) DISCO_OCC(DISCO_OCC_CTU) void k_ct_update(DevState S, double RK, double dtfac, int first,
                                                                int has_z) {
     int c = cell_index();
     if (c < S.ncell) ct_update_cell(S, c, RK, dtfac, first, has_z);
