@@ -26,6 +26,19 @@ DHD uint64_t disco_bits(double x) {
     return v.u;
 }
 
+/* a / b rounded to nearest, given y = 1 / b rounded to nearest: q = a y lies
+ * within one ulp of a / b, the residual a - b q is exact in one fused
+ * multiply-add, and q + (a - b q) y then rounds to the correctly rounded
+ * quotient (Markstein's theorem, for quotients and residuals clear of the
+ * subnormal range).  Several quotients by one divisor share one division:
+ * the results are bit-identical to a / b, at three operations each instead
+ * of a full FP64 division sequence. */
+DHD double div_rn(double a, double b, double y) {
+    const double q = a * y;
+    const double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
 /* SplitMix64 finaliser (Steele, Lea & Flood 2014). */
 DHD uint64_t disco_splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;

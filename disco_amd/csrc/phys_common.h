@@ -79,19 +79,6 @@ DHD double add_mul_rn(double a, double b, double c) {
 #endif
 }
 
-// a / b rounded to nearest, given y = 1 / b rounded to nearest: q = a y lies
-// within one ulp of a / b, the residual a - b q is exact in one fused
-// multiply-add, and q + (a - b q) y then rounds to the correctly rounded
-// quotient (Markstein's theorem, for quotients and residuals clear of the
-// subnormal range).  Several quotients by one divisor share one division:
-// the results are bit-identical to a / b, at three operations each instead
-// of a full FP64 division sequence.
-DHD double div_rn(double a, double b, double y) {
-    const double q = a * y;
-    const double r = fma(-b, q, a);
-    return fma(r, y, q);
-}
-
 // Store of a per-cell result that only a later kernel reads: non-temporal on
 // the device, so the stream of results does not evict from L2 the neighbour
 // cells' state the kernel is still gathering.

@@ -222,15 +222,18 @@ DHD int c2p_newton(double gamma_law, const C2PInvariants& v, double v20, double 
         v2 = v21;
         eta = eta1;
         double w = 1.0 / sqrt(1.0 - v2);
-        double fa = (eta + Q) * (eta + Q) * v2 - psi * (2 * eta + Q) / (eta * eta) - s2;
-        double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - 0.5 * psi / (eta * eta) - e;
+        // two quotients by eta^2 and two by the Jacobian: one division each (div_rn: exact)
+        const double e2 = eta * eta, ye2 = 1.0 / e2;
+        double fa = (eta + Q) * (eta + Q) * v2 - div_rn(psi * (2 * eta + Q), e2, ye2) - s2;
+        double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - div_rn(0.5 * psi, e2, ye2) - e;
         double dfadv2 = (eta + Q) * (eta + Q);
         double dfadet = 2 * (eta + Q) * (v2 + psi / (eta * eta * eta));
         double dfbdv2 = n * (eta - 0.5 * w) + 0.5 * Q;
         double dfbdet = 1 - n * (1 - v2) + psi / (eta * eta * eta);
         double detj = dfadv2 * dfbdet - dfbdv2 * dfadet;
-        v21 = v2 - (dfbdet * fa - dfadet * fb) / detj;
-        eta1 = eta - (-dfbdv2 * fa + dfadv2 * fb) / detj;
+        const double yj = 1.0 / detj;
+        v21 = v2 - div_rn(dfbdet * fa - dfadet * fb, detj, yj);
+        eta1 = eta - div_rn(-dfbdv2 * fa + dfadv2 * fb, detj, yj);
         i++;
         if (v21 > 1.0) v21 = 0.5 * (v2 + 1.0);
         if (v21 < 0.0) v21 = 0.5 * v2;
@@ -271,7 +274,9 @@ DHD void c2p_recover(double gamma_law, double rho_floor, double pre_floor, const
     Bd[2] = gam[6] * B[0] + gam[7] * B[1] + gam[8] * B[2];
     double D = v.D, sg = v.sqrtgam, BS = v.BS, B2 = v.B2;
     double l[3];
-    for (int i = 0; i < 3; ++i) l[i] = (v.S[i] + sg * BS * Bd[i] / (D * h * w)) / (D * h + sg * B2 / w);
+    const double d1 = D * h * w, d2 = D * h + sg * B2 / w;  // shared divisors (div_rn: exact quotients)
+    const double y1 = 1.0 / d1, y2 = 1.0 / d2;
+    for (int i = 0; i < 3; ++i) l[i] = div_rn(v.S[i] + div_rn(sg * BS * Bd[i], d1, y1), d2, y2);
     prim[RHO] = rho;
     prim[URR] = l[0];
     prim[UPP] = L2 ? l[1] : l[1] / (r * r);

@@ -162,14 +162,15 @@ DHD void flux(double gamma_law, const Metric& g, const double* prim, double* F, 
     double u0 = w / al;
     double u[3], v[3];
     for (int i = 0; i < 3; i++) u[i] = igaml[i] - be[i] * u0;
-    for (int i = 0; i < 3; i++) v[i] = u[i] / u0;
+    const double yu0 = 1.0 / u0, yw = 1.0 / w;  // shared divisors (div_rn: exact quotients)
+    for (int i = 0; i < 3; i++) v[i] = div_rn(u[i], u0, yu0);
     double l0 = -al * w + be[0] * l[0] + be[1] * l[1] + be[2] * l[2];
     double uU = U[0] * l0 + U[1] * l[0] + U[2] * l[1] + U[3] * l[2];
     double uB = l[0] * B[0] + l[1] * B[1] + l[2] * B[2];
     double bd[3];
-    bd[0] = (gam[0] * B[0] + gam[1] * B[1] + gam[2] * B[2] + l[0] * uB) / w;
-    bd[1] = (gam[3] * B[0] + gam[4] * B[1] + gam[5] * B[2] + l[1] * uB) / w;
-    bd[2] = (gam[6] * B[0] + gam[7] * B[1] + gam[8] * B[2] + l[2] * uB) / w;
+    bd[0] = div_rn(gam[0] * B[0] + gam[1] * B[1] + gam[2] * B[2] + l[0] * uB, w, yw);
+    bd[1] = div_rn(gam[3] * B[0] + gam[4] * B[1] + gam[5] * B[2] + l[1] * uB, w, yw);
+    bd[2] = div_rn(gam[6] * B[0] + gam[7] * B[1] + gam[8] * B[2] + l[2] * uB, w, yw);
     double B2 = 0.0, UB = 0.0;
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
@@ -177,13 +178,13 @@ DHD void flux(double gamma_law, const Metric& g, const double* prim, double* F, 
             UB += gam[3 * i + j] * B[i] * (U[0] * be[j] + U[j + 1]);
         }
     double b2 = (B2 + uB * uB) / (w * w);
-    double Ub = (UB + uU * uB) / w;
+    double Ub = div_rn(UB + uU * uB, w, yw);
     double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
     double Bn = B[0] * n[0] + B[1] * n[1] + B[2] * n[2];
     double Un = U[1] * n[0] + U[2] * n[1] + U[3] * n[2];
     double vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
     double hn = n[0] + r * n[1] + n[2];
-    double bn = (Bn + uB * un) / w;
+    double bn = div_rn(Bn + uB * un, w, yw);
     double rhoh = rho + gamma_law / (gamma_law - 1.0) * Pp + b2;
     double rhoe = Pp / (gamma_law - 1.0);
     double Pt = Pp + 0.5 * b2;
@@ -234,9 +235,10 @@ DHD void getUstar(double gamma_law, const Metric& g, const double* prim, double*
     double ssS = (ss + bn) / al;
     double skS = (sk + bn) / al;
     double Pstar = (qMn - ssS * qE) / (al * (ign - ssS * skS));
-    double kappa = (vn - sk) / (ss - sk);
-    double alpha1 = (Pp - Pstar) / (ss - sk);
-    double alpha2 = (vn * Pp - ss * Pstar) / (ss - sk);
+    const double dsk = ss - sk, ydsk = 1.0 / dsk;  // three quotients, one division (div_rn)
+    double kappa = div_rn(vn - sk, dsk, ydsk);
+    double alpha1 = div_rn(Pp - Pstar, dsk, ydsk);
+    double alpha2 = div_rn(vn * Pp - ss * Pstar, dsk, ydsk);
     double rhoe = Pp / (gamma_law - 1.0);
     double tau = -rhoe * uU * u0 - Pp * (u0 * uU + U[0]) - rho * (uU + 1.0) * u0;
     Ustar[DDD] = jac * rho * u0 * kappa;
@@ -324,12 +326,14 @@ DHD void vel(double gamma_law, const Metric& g, const double* prim1, const doubl
     double FMR[3] = {MR[0] * vnR + n[0] * al * P2, MR[1] * vnR + n[1] * al * P2, MR[2] * vnR + n[2] * al * P2};
     double FEL = EL * vnL + P1 * (vnL + bn);
     double FER = ER * vnR + P2 * (vnR + bn);
-    double UE = (sR * ER - sL * EL - FER + FEL) / (sR - sL);
-    double FE = ((sR * FEL - sL * FER + sL * sR * (ER - EL)) / (sR - sL) - bn * UE) / al;
+    // eight quotients by sR - sL and four by al: one division each (div_rn)
+    const double dS = sR - sL, ydS = 1.0 / dS, yal = 1.0 / al;
+    double UE = div_rn(sR * ER - sL * EL - FER + FEL, dS, ydS);
+    double FE = div_rn(div_rn(sR * FEL - sL * FER + sL * sR * (ER - EL), dS, ydS) - bn * UE, al, yal);
     double UM = 0.0, FM = 0.0;
     for (int i = 0; i < 3; i++) {
-        double UMh = (sR * MR[i] - sL * ML[i] - FMR[i] + FML[i]) / (sR - sL);
-        double FMh = ((sR * FML[i] - sL * FMR[i] + sL * sR * (MR[i] - ML[i])) / (sR - sL) - bn * UMh) / al;
+        double UMh = div_rn(sR * MR[i] - sL * ML[i] - FMR[i] + FML[i], dS, ydS);
+        double FMh = div_rn(div_rn(sR * FML[i] - sL * FMR[i] + sL * sR * (MR[i] - ML[i]), dS, ydS) - bn * UMh, al, yal);
         double igi = n[0] * igam[3 * i] + n[1] * igam[3 * i + 1] + n[2] * igam[3 * i + 2];
         UM += igi * UMh;
         FM += igi * FMh;
@@ -406,9 +410,11 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
         prim2cons<L2>(gl, g, primR, Ur, x, 1.0);
         flux<L2>(gl, g, primL, Fl, x, n);
         flux<L2>(gl, g, primR, Fr, x, n);
+        const double aS = aL + aR;
+        const double yS = 1.0 / aS;  // sixteen quotients, one division (div_rn: exact)
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
-            Flux[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
-            Ustr[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+            Flux[q] = div_rn(aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q]), aS, yS);
+            Ustr[q] = div_rn(aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q], aS, yS);
         }
     } else {
         double Uk[NQ], Fk[NQ];
