@@ -231,10 +231,18 @@ DHD double polar_adjust(const double* xp, const double* xm, double phi_max) {
     return sin(0.5 * dphi) / (0.5 * dphi);
 }
 
-// planet.c:97-102 (PLPOINTMASS, PHI_ORDER = 2)
+// planet.c:97-102 (PLPOINTMASS, PHI_ORDER = 2): M r / (r^2 + eps^2)^(3/2).
+// The device evaluates the 3/2 power as s sqrt(s) (within 2 ulp of the
+// libm pow the host and the reference call; the device pow costs a
+// double-double log and exp).
 DHD double fgrav_pointmass(double M, double r, double eps) {
     double n = 2.0;
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(DISCO_DEVICE_POW)
+    const double s = pow(r, n) + pow(eps, n);
+    return M * pow(r, n - 1.) / (s * sqrt(s));
+#else
     return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
+#endif
 }
 
 // planet.c:138-187: force per unit mass from one planet, returned as
