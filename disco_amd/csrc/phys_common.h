@@ -237,7 +237,7 @@ DHD double polar_adjust(const double* xp, const double* xm, double phi_max) {
 // double-double log and exp).
 DHD double fgrav_pointmass(double M, double r, double eps) {
     double n = 2.0;
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(DISCO_DEVICE_POW)
+#if defined(__HIP_DEVICE_COMPILE__)
     const double s = pow(r, n) + pow(eps, n);
     return M * pow(r, n - 1.) / (s * sqrt(s));
 #else
