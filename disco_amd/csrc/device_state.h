@@ -159,6 +159,7 @@ struct StepShape {
     int wmean;        // some run moves its annuli at their mean speed (Mesh_Motion 3)
     int accrete;      // some run has accretion sinks: their per-planet sums run after the update
     int visc;         // some run is viscous: the flux kernel with the viscous fluxes runs
+    int dampc;        // some run uses HLLC_DAMPCENTER: the flux kernel with its per-face choice of HLL runs
     int stepper;      // substeps of a step (timestep.c:29-49): 2 RK2, 3 three-stage RK, 1 Euler then RK2
     int blk[KC_N];    // threads per block of each kernel class (64, 128 or 256; 0 = 256)
     // zero-gradient ghost fills (zerograd.c, outflow3d.c): DevState::bcjobs

@@ -336,8 +336,7 @@ static int unsupported(const disco_params& p, std::string* err) {
         m += " Num_Z>1 needs Z_Periodic or a boundary that fills z (fixed3d, zerograd, outflow3d)";
     if (p.Num_Z > 1 && p.Z_Periodic && p.Num_Z < 2 * 2) m += " Z_Periodic needs Num_Z >= 2*NUM_G";
     if (p.Mesh_Motion < 0 || p.Mesh_Motion > 4) m += " Mesh_Motion";
-    if (p.Riemann_Solver != DISCO_HLL && p.Riemann_Solver != DISCO_HLLC && p.Riemann_Solver != DISCO_HLLD)
-        m += " Riemann_Solver";
+    if (p.Riemann_Solver < DISCO_HLL || p.Riemann_Solver > DISCO_HLLC_DAMPCENTER) m += " Riemann_Solver";
     // euler.c holds the viscous terms; mhd.c and grmhd.c define no visc_source
     if (p.visc_flag && p.hydro != DISCO_HYDRO_EULER) m += " Use_Viscosity (euler only)";
     if (p.visc_flag && !p.alpha_flag && p.viscosity <= 0.0) m += " Use_Viscosity needs Viscosity > 0";
@@ -1250,6 +1249,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
     shape->zcopy = 0;
     shape->sink = 0;
     shape->visc = 0;
+    shape->dampc = 0;
     shape->accrete = 0;
     shape->wmean = 0;
     shape->ct = (!runs.empty() && (runs[0].par.hydro == DISCO_HYDRO_MHD || is_grmhd(runs[0].par.hydro)))
@@ -1263,6 +1263,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
             q.RotFrame || q.visc_flag)
             shape->sink = 1;
         if (q.visc_flag) shape->visc = 1;
+        if (q.Riemann_Solver == DISCO_HLLC_DAMPCENTER) shape->dampc = 1;
         if (q.Mesh_Motion == 3) shape->wmean = 1;
         if (q.sinkType) shape->sink = shape->accrete = 1;
     }

@@ -199,10 +199,12 @@ __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::PLM) void k_plm_trans(DevSt
 
 // VISC: with euler.c's viscous fluxes (an instantiation of its own, launched
 // only for viscous runs)
-template <class Ph, bool VISC>
+// DC: with HLLC_DAMPCENTER's per-face choice of HLL (launched only when a run
+// uses that solver)
+template <class Ph, bool VISC, bool DC = false>
 __global__ __launch_bounds__(256) DISCO_OCC(Occ<Ph>::FLUX) void k_flux(DevState S, double dtfac) {
     int c = cell_index();
-    if (c < S.ncell) flux_cell<Ph, VISC>(S, c, dtfac);
+    if (c < S.ncell) flux_cell<Ph, VISC, DC>(S, c, dtfac);
 }
 
 // Fused update for hydro: conserved state stays in registers between the
@@ -607,13 +609,20 @@ struct Launch {
         DISCO_LAUNCH(hook, "k_stage_a", k_stage_a<Ph>, G(KC_STAGE), dim3(B(KC_STAGE)), st, S, first, first != 1);
         if (first && sh.wmean) DISCO_LAUNCH(hook, "k_wmean", k_wmean, dim3(S.nann), dim3(64), st, S);
         DISCO_LAUNCH(hook, "k_plm_trans", k_plm_trans<Ph>, G(KC_PLM), dim3(B(KC_PLM)), st, S, sh.has_z);
-        if constexpr (!Ph::MHD) {
-            if (sh.visc)
+        if constexpr (!Ph::MHD && Ph::NEWTONIAN) {  // euler.c's viscous fluxes
+            if (sh.visc && sh.dampc)
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, true, true>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+            else if (sh.visc)
                 DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, true>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+            else if (sh.dampc)
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false, true>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
             else
                 DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         } else {
-            DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+            if (sh.dampc)
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false, true>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
+            else
+                DISCO_LAUNCH(hook, "k_flux", (k_flux<Ph, false>), G(KC_FLUX), dim3(B(KC_FLUX)), st, S, dtfac);
         }
         if (Ph::MHD && sh.ct) {
             if (fork) {

@@ -79,6 +79,13 @@ DHD double add_mul_rn(double a, double b, double c) {
 #endif
 }
 
+// Whether a face inside the fan takes the HLL state rather than HLLC's
+// (riemann.c:253): always with HLL, and with HLLC_DAMPCENTER for faces whose
+// x[0] is below 0.1, where the contact is damped near the axis.
+DHD bool hll_face(const PhysParams& P, const double* x) {
+    return P.riemann == DISCO_HLL || (x[0] < 0.1 && P.riemann == DISCO_HLLC_DAMPCENTER);
+}
+
 // Store of a per-cell result that only a later kernel reads: non-temporal on
 // the device, so the stream of results does not evict from L2 the neighbour
 // cells' state the kernel is still gathering.

@@ -152,8 +152,10 @@ DHD void vel(const PhysParams& P, const double* prim1, const double* prim2, doub
 // increment of riemann.c:314-317: returns dU[q] = (F[q] - w U*[q]) * dAdt,
 // which the reference subtracts from the left cell and adds to the right.
 // vF (optional): a viscous flux added to the face flux before the increment
-// (riemann.c:297-312).
-template <int NQ>
+// (riemann.c:297-312).  DC: the run may use HLLC_DAMPCENTER, whose choice of
+// HLL varies from face to face (an instantiation of its own, so that the
+// others keep a branch that is uniform over the wavefront).
+template <int NQ, bool DC = false>
 DHD void riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
                  const double* n, double w, double dAdt, double* dU, const double* vF = nullptr) {
     double Flux[NQ];
@@ -170,7 +172,7 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
     } else if (w > Sr) {
         flux<NQ>(P, primR, Flux, x, n);
         prim2cons<NQ>(P, primR, Ustr, x, 1.0);
-    } else if (P.riemann == DISCO_HLL) {
+    } else if (DC ? hll_face(P, x) : P.riemann == DISCO_HLL) {
         double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
         double aL = Sr;
         double aR = -Sl;

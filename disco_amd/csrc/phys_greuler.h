@@ -202,6 +202,8 @@ DHD void vel(double gamma_law, const Metric& g, const double* prim1, const doubl
 
 // solve_riemann (riemann.c:216-320) with HLL or HLLC over greuler.c's
 // leaves; one metric evaluation at the face point serves every leaf.
+// DC: HLLC_DAMPCENTER's per-face choice of HLL is compiled in.
+template <bool DC = false>
 DHD void riemann(const PhysParams& P, const double* primL, const double* primR, const double* x, const double* n,
                  double w, double dAdt, double* dU) {
     const double gl = P.gamma_law;
@@ -220,7 +222,7 @@ DHD void riemann(const PhysParams& P, const double* primL, const double* primR, 
     } else if (w > Sr) {
         greuler::flux(gl, g, primR, Flux, x, n);
         greuler::prim2cons(gl, g, primR, Ustr, 1.0);
-    } else if (P.riemann == DISCO_HLL) {
+    } else if (DC ? hll_face(P, x) : P.riemann == DISCO_HLL) {
         double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
         const double aL = Sr;
         const double aR = -Sl;

@@ -375,8 +375,9 @@ DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, i
 // solve_riemann (riemann.c:216-320) for GRMHD with HLL or HLLC, and for
 // grmhd2.c with the relativistic HLLD of hlld_grmhd.c (riemann.c:293-295:
 // get_Ustar_HLLD; grmhd.c cannot link it, it lacks calc_msfast); one metric
-// evaluation at the face centroid serves every leaf.
-template <bool L2 = false>
+// evaluation at the face centroid serves every leaf.  DC: HLLC_DAMPCENTER's
+// per-face choice of HLL is compiled in.
+template <bool L2 = false, bool DC = false>
 DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
                     const double* n, double w, double dAdt, int dim, double* dU) {
     const double gl = P.gamma_law;
@@ -402,7 +403,7 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
     } else if (w > Sr) {
         flux<L2>(gl, g, primR, Flux, x, n);
         prim2cons<L2>(gl, g, primR, Ustr, x, 1.0);
-    } else if (P.riemann == DISCO_HLL) {
+    } else if (DC ? hll_face(P, x) : P.riemann == DISCO_HLL) {
         double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
         double aL = Sr;
         double aR = -Sl;

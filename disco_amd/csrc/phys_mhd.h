@@ -572,6 +572,10 @@ DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, i
 // solve_riemann (riemann.c:216-320) for MHD: HLL / HLLC (mhd.c vel, getUstar)
 // or HLLD (hlld.c).  Returns the per-face increment dU = (F - w U*) dAdt
 // that the reference subtracts from L and adds to R, and the face's EMFs.
+// DC: the run may use HLLC_DAMPCENTER, whose choice of HLL varies from face
+// to face (an instantiation of its own, so that the others keep a branch that
+// is uniform over the wavefront).
+template <bool DC = false>
 DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
                     const double* n, double w, double dAdt, int dim, double* dU) {
     double Flux[NQ], Ustr[NQ];
@@ -590,7 +594,7 @@ DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* prim
         } else if (w > Sr) {
             flux(P, primR, Flux, x, n);
             prim2cons(P, primR, Ustr, x, 1.0);
-        } else if (P.riemann == DISCO_HLL) {
+        } else if (DC ? hll_face(P, x) : P.riemann == DISCO_HLL) {
             double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
             double aL = Sr;
             double aR = -Sl;

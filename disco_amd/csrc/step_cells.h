@@ -49,10 +49,11 @@ struct PhysEuler {
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
     static constexpr bool SOURCE_PASS = false;  // sources are added inline in update_a_cell
+    template <bool DC = false>
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         (void)dim;
-        euler::riemann<NQ>(P, pL, pR, x, n, w, dAdt, dU);
+        euler::riemann<NQ, DC>(P, pL, pR, x, n, w, dAdt, dU);
         return FaceEMF{0.0, 0.0, 0.0, 0.0};
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
@@ -83,9 +84,10 @@ struct PhysMHD {
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = false;
     static constexpr bool SOURCE_PASS = false;
+    template <bool DC = false>
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
-        return mhd::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
+        return mhd::riemann<DC>(P, pL, pR, x, n, w, dAdt, dim, dU);
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                     double dVdt) {
@@ -119,9 +121,10 @@ struct PhysGRMHDT {
     static constexpr bool C2P_GUESS = true;
     static constexpr bool SOURCE_PASS = true;
     static constexpr bool NEWTONIAN = false;  // sink.c's terms are rejected at set-up
+    template <bool DC = false>
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
-        return grmhd::riemann<L2>(P, pL, pR, x, n, w, dAdt, dim, dU);
+        return grmhd::riemann<L2, DC>(P, pL, pR, x, n, w, dAdt, dim, dU);
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                     double dVdt) {
@@ -147,10 +150,11 @@ struct PhysGREuler {
     static constexpr bool C2P_GUESS = true;
     static constexpr bool SOURCE_PASS = false;
     static constexpr bool NEWTONIAN = false;  // sink.c's terms are rejected at set-up
+    template <bool DC = false>
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
         (void)dim;
-        greuler::riemann(P, pL, pR, x, n, w, dAdt, dU);
+        greuler::riemann<DC>(P, pL, pR, x, n, w, dAdt, dU);
         return FaceEMF{0.0, 0.0, 0.0, 0.0};
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
@@ -657,7 +661,7 @@ DHD void plm_trans_cell(const DevState& S, int c, int has_z) {
 // (riemann.c:114-213) on its owned faces; solve_riemann's increment
 // (F - w U*) dA dt is stored per owner cell, and for MHD the face's EMFs
 // (flux_to_E) beside it.
-template <class Ph, int D, bool VISC = false>
+template <class Ph, int D, bool VISC = false, bool DC = false>
 DHD FaceEMF t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, const TFace& F, const AnnDev& AL,
                         double dt, double* dU) {
     constexpr int NQ = Ph::NQ;
@@ -695,14 +699,14 @@ DHD FaceEMF t_face_flux(const DevState& S, const PhysParams& P, int cL, int cR, 
         }
         const double nu = visc_nu(S, Rn, Ann.run, pm, F.x);
         euler::visc_flux<NQ>(P, nu, pm, g[0], g[1], g[2], vF, F.x, n);
-        euler::riemann<NQ>(P, primL, primR, F.x, n, 0.0, F.dA * dt, dU, vF);
+        euler::riemann<NQ, DC>(P, primL, primR, F.x, n, 0.0, F.dA * dt, dU, vF);
         return FaceEMF{0.0, 0.0, 0.0, 0.0};
     } else {
-        return Ph::riemann(P, primL, primR, F.x, n, 0.0, F.dA * dt, D, dU);
+        return Ph::template riemann<DC>(P, primL, primR, F.x, n, 0.0, F.dA * dt, D, dU);
     }
 }
 
-template <class Ph, int D, bool VISC = false>
+template <class Ph, int D, bool VISC = false, bool DC = false>
 DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& P, int c, double dt) {
     constexpr int NQ = Ph::NQ;
     using Dr = Dir<D>;
@@ -715,7 +719,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         const AnnDev& B = S.ann[Dr::hi(A)];
         int cb = B.off + Dr::part_hi(S)[c];
         TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
-        FaceEMF e = t_face_flux<Ph, D, VISC>(S, P, c, cb, F, A, dt, dU);
+        FaceEMF e = t_face_flux<Ph, D, VISC, DC>(S, P, c, cb, F, A, dt, dU);
         double* f = Dr::f_hi(S);
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
@@ -728,7 +732,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         const AnnDev& W = S.ann[Dr::lo(A)];
         int ca = W.off + Dr::part_lo(S)[c];
         TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
-        FaceEMF e = t_face_flux<Ph, D, VISC>(S, P, ca, c, F, W, dt, dU);
+        FaceEMF e = t_face_flux<Ph, D, VISC, DC>(S, P, ca, c, F, W, dt, dU);
         double* f = Dr::f_lo(S);
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_t) {
@@ -739,7 +743,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
     }
 }
 
-template <class Ph, bool VISC = false>
+template <class Ph, bool VISC = false, bool DC = false>
 DHD void flux_cell(const DevState& S, int c, double dtfac) {
     constexpr int NQ = Ph::NQ;
     const AnnDev& A = S.ann[S.cell_ann[c]];
@@ -782,10 +786,10 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
             }
             const double nu = visc_nu(S, S.runs[A.run], A.run, pm, x);
             euler::visc_flux<NQ>(P, nu, pm, g[0], g[1], g[2], vF, x, nv);
-            euler::riemann<NQ>(P, primL, primR, x, nv, w, dA * dt, dU, vF);
+            euler::riemann<NQ, DC>(P, primL, primR, x, nv, w, dA * dt, dU, vF);
             e = FaceEMF{0.0, 0.0, 0.0, 0.0};
         } else {
-            e = Ph::riemann(P, primL, primR, x, nv, w, dA * dt, 0, dU);
+            e = Ph::template riemann<DC>(P, primL, primR, x, nv, w, dA * dt, 0, dU);
         }
         _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&S.fphi[q * N + c], dU[q]);
         if (Ph::MHD && S.emf_phi) {
@@ -795,8 +799,8 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
             st_stream(&S.emf_phi[3 * N + c], e.B2);  // B_z
         }
     }
-    owned_faces_flux<Ph, DIR_R, VISC>(S, A, P, c, dt);
-    owned_faces_flux<Ph, DIR_Z, VISC>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_R, VISC, DC>(S, A, P, c, dt);
+    owned_faces_flux<Ph, DIR_Z, VISC, DC>(S, A, P, c, dt);
 }
 
 // ------------------------------------------------------------------ update

@@ -33,7 +33,7 @@ enum { DISCO_PLANETS_NONE = 0, DISCO_PLANETS_SUN = 1, DISCO_PLANETS_BINARY = 2 }
 /* Initial module (INITIAL slot); the set is this project's own config ICs */
 enum { DISCO_IC_KEPLER_NOISE = 0, DISCO_IC_ISO_BINARY_DISK = 1, DISCO_IC_MHD_DISK = 2, DISCO_IC_GR_TORUS = 3 };
 /* Riemann solvers, riemann.c:1 */
-enum { DISCO_HLL = 0, DISCO_HLLC = 1, DISCO_HLLD = 2 };
+enum { DISCO_HLL = 0, DISCO_HLLC = 1, DISCO_HLLD = 2, DISCO_HLLC_DAMPCENTER = 3 };
 
 #define DISCO_MAX_PLANETS 2
 

@@ -368,6 +368,10 @@ NOBC_CASES = [  # geometric boundaries without ghost annuli (gridsetup.c:34-51; 
     # r = 0 on the binary disk: geom_grad of the innermost annulus and the polar sources
     ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3), "euler_binary",
      10, (1, 5)),
+    # the same disk with HLLC_DAMPCENTER (riemann.c:253): annulus 0's phi faces (centroid r = 0.056) and the
+    # r face at 0.083 take the HLL state, every face beyond r = 0.1 HLLC's
+    ("c3_nobc_r0_dampc.npz", "c3_binary_disk.par", dict(Num_R=48, R_Min=0.0, NoBC_Rmin=1, Softening=0.3,
+                                                         Riemann_Solver=3), "euler_binary", 10, (1, 5)),
     # NoBC_Rmin away from r = 0 (polar sources, geom_grad inactive) and NoBC_Rmax
     ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1), "euler_sun", 10, (1,)),
     ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1), "euler_sun", 10, (1,)),
@@ -496,6 +500,9 @@ LEAF_CASES = {
                                            "Riemann_Solver": 2}, "grmhd2_torus_hlld"),
     "grmhd2_hlld_strong": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
                                                   "Riemann_Solver": 2}, "grmhd2_torus_hlld"),
+    # HLLC_DAMPCENTER on faces at 0.02 < r < 0.2: HLL inside r = 0.1, HLLC outside
+    "euler_dampc": ("c1_kepler2d.par", {"Num_R": 16, "Riemann_Solver": 3}, "euler_sun"),
+    "mhd_dampc": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 3}, "mhd_sun"),
 }
 
 
@@ -534,6 +541,8 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
             pR[4] = pL[4] + 0.03 * U(-1, 1)
         if k % 17 == 0:
             pR = pL.copy()  # identical states
+        if "_dampc" in kind:  # faces either side of r = 0.1
+            r = U(0.02, 0.2)
         x = np.array([r, U(0.0, 2 * np.pi), z])
         nrm = np.zeros(3)
         nrm[[1, 0, 2][dim]] = 1.0

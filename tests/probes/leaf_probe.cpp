@@ -26,7 +26,9 @@ static void record(const PhysParams& P, int hydro, const double* in, double* out
     double w = n[3], dAdt = n[4];
     int dim = (int)n[5];
     double dU[NQ];
-    FaceEMF e = Ph::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
+    // the flux kernel's instantiation for runs with HLLC_DAMPCENTER
+    FaceEMF e = P.riemann == DISCO_HLLC_DAMPCENTER ? Ph::template riemann<true>(P, pL, pR, x, n, w, dAdt, dim, dU)
+                                                   : Ph::riemann(P, pL, pR, x, n, w, dAdt, dim, dU);
     double* o = out;
     for (int q = 0; q < NQ; ++q) *o++ = dU[q];
     *o++ = e.E1;

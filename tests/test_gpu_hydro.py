@@ -11,7 +11,10 @@ HIP path must match them per conserved/primitive field to
   helpers.state_errors) after 1 step,
 * 1e-10 after 10 steps and 1e-9 after 100 steps (round-off growth),
 
-and the mesh (piph) to the same absolute tolerances in radians; the
+and the mesh (piph) to the same absolute tolerances in radians, the CFL
+step dt to the same relative tolerances (it is the minimum over cells of a
+function of the state, so it inherits the state's round-off growth: near
+r = 0 the cells are small and their speeds set dt); the
 diagnostics every step accumulates (add_diagnostics: per-annulus averages
 times dt, ghosts included) to the same tolerances per quantity.  Summation
 order differs from the reference (gather instead of in-place scatter) and
@@ -89,6 +92,10 @@ CASES = [
     # geometric boundaries without ghost annuli: r = 0 (geom_grad of the innermost annulus, polar sources),
     # NoBC_Rmin away from r = 0, NoBC_Rmax, and no z ghosts
     ("c3_nobc_r0.npz", "c3_binary_disk.par", dict(Num_R=24, R_Min=0.0, NoBC_Rmin=1, Softening=0.3)),
+    # HLLC_DAMPCENTER (riemann.c:253) on that disk at Nr = 48: faces inside r = 0.1 take the HLL state (the
+    # reference's HLLC run of the same disk differs from this one by 5e-5 of the density after one step)
+    ("c3_nobc_r0_dampc.npz", "c3_binary_disk.par", dict(Num_R=48, R_Min=0.0, NoBC_Rmin=1, Softening=0.3,
+                                                         Riemann_Solver=3)),
     ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1)),
     ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
     ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
@@ -132,9 +139,11 @@ def test_hydro_matches_reference(fixture, base, over):
         done = s
         st = dom.state(0)
         tm = dom.time()
-        assert abs(tm["dt"][0] - ref[s]["dt"]) <= 1e-12 * ref[s]["dt"], (tm["dt"][0], ref[s]["dt"])
-        assert abs(tm["t"][0] - ref[s]["t"]) <= 1e-11 * max(1.0, ref[s]["t"])
         _compare(st, ref[s], TOL[s], f"{fixture} step {s}")
+        edt = abs(tm["dt"][0] - ref[s]["dt"]) / ref[s]["dt"]
+        print(f"{fixture} step {s}: dt {edt:.1e}", flush=True)
+        assert edt <= TOL[s], (tm["dt"][0], ref[s]["dt"])
+        assert abs(tm["t"][0] - ref[s]["t"]) <= 1e-11 * max(1.0, ref[s]["t"])
         ed, et = diag_errors(dom.diagnostics(0), ref[s])
         assert ed.max() < TOL[s] and et < 1e-12, (s, ed, et)
     dom.close()

@@ -5,7 +5,7 @@ tests/golden/leaf_vectors.npz holds random face states and what the
 reference build returns for them (oracle/ref/ref_leaf.c, linked with the
 reference's riemann.c, hlld.c and Hydro/euler.c, mhd.c, grmhd.c, grmhd2.c
 or greuler.c + schw_ks_rpz.c): solve_riemann's conserved increments (F - w U*) dA dt and
-flux_to_E's EMFs for HLL, HLLC and HLLD on phi, r and z faces, prim2cons,
+flux_to_E's EMFs for HLL, HLLC, HLLD and HLLC_DAMPCENTER on phi, r and z faces, prim2cons,
 cons2prim back, mindt and source.  The library's leaves (the step_cells.h
 policies over phys_euler.h, phys_mhd.h, phys_grmhd_fv.h and phys_greuler.h -- the source the
 GPU kernels compile) are built here with the host compiler into
@@ -47,6 +47,10 @@ CASES = {
     # pressures and its HLL fallback
     "grmhd2_hlld_strong": ("c5_grmhd_torus.par", {"Hydro": "grmhd2", "Num_R": 16, "Num_Z": 12, "Metric_Par3": 0.0,
                                                   "Riemann_Solver": 2}),
+    # HLLC_DAMPCENTER (riemann.c:253) on faces either side of r = 0.1: the
+    # records inside take HLL's state, those outside HLLC's
+    "euler_dampc": ("c1_kepler2d.par", {"Num_R": 16, "Riemann_Solver": 3}),
+    "mhd_dampc": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 3}),
 }
 
 
@@ -91,3 +95,34 @@ def test_leaves_match_reference(case, probe, vectors, tmp_path):
         assert e <= 1e-14, (case, g, e)
     exact = float((out == ref).mean())
     print(f"{case}: worst {worst}, bit-identical values {exact:.3f}")
+
+
+@pytest.mark.parametrize("case", ["euler_dampc", "mhd_dampc"])
+def test_dampcenter_switches_at_r_0p1(case, probe, vectors, tmp_path):
+    """The HLLC_DAMPCENTER vectors are not HLLC's in disguise: run as HLL the
+    library gives the reference's values bit for bit on the faces inside
+    r = 0.1, run as HLLC those on the faces outside, and each differs from the
+    other solver on its own side."""
+    base, over = CASES[case]
+    rec = vectors[f"{case}_in"]
+    ref = vectors[f"{case}_out"]
+    n = rec.shape[0]
+    nq = (rec.shape[1] - 9) // 2
+    inner = rec[:, 2 * nq] < 0.1
+    assert 0 < inner.sum() < n
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        f.write(np.int32(n).tobytes())
+        f.write(np.ascontiguousarray(rec, np.float64).tobytes())
+    outs = {}
+    for rs in (0, 1):
+        par, fout = tmp_path / f"in{rs}.par", tmp_path / f"out{rs}.bin"
+        par.write_text(par_text(base, **dict(over, Riemann_Solver=rs)))
+        subprocess.run([probe, str(par), str(fin), str(fout)], check=True, timeout=60)
+        outs[rs] = np.fromfile(fout, np.float64).reshape(n, 4 * nq + 5)
+    dU = slice(0, nq)
+    assert np.array_equal(outs[0][inner], ref[inner])
+    assert np.array_equal(outs[1][~inner], ref[~inner])
+    moving = ~np.all(rec[:, :nq] == rec[:, nq:2 * nq], axis=1)  # distinct left and right states
+    assert (outs[1][inner & moving, dU] != ref[inner & moving, dU]).any(axis=1).all()
+    assert (outs[0][~inner & moving, dU] != ref[~inner & moving, dU]).any(axis=1).all()
