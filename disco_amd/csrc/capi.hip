@@ -83,6 +83,7 @@ struct disco_gpu {
     int ktile = 8;                            // z-layers per annulus tile in HBM
     int blk[KC_N] = {256, 256, 256, 256, 256, 256, 256, 256};  // threads per block per kernel class
     int nruns = 0, ncell = 0, nann = 0, nq = 5;
+    int nf = 5;  // stored face-increment components per cell (the policies' NF: NUM_Q, or 5 with CT)
     AnnDev* ann = nullptr;
     int* cell_ann = nullptr;
     RunDev* druns = nullptr;
@@ -392,7 +393,7 @@ static std::vector<DevBuf> layout_buffers(disco_gpu* h) {
     };
     D(h->prim, nq), D(h->cons, nq), D(h->rkcons, nq), D(h->gradp, nq), D(h->gradr, nq), D(h->gradz, nq);
     D(h->piph[0], 1), D(h->piph[1], 1), D(h->dphi[0], 1), D(h->dphi[1], 1), D(h->wiph, 1);
-    D(h->fphi, 5), D(h->fout, 5), D(h->fin, 5), D(h->fup, 5), D(h->fdn, 5);
+    D(h->fphi, h->nf), D(h->fout, h->nf), D(h->fin, h->nf), D(h->fup, h->nf), D(h->fdn, h->nf);
     D(h->Phi, 5), D(h->rkPhi, 5), D(h->emf_phi, 4), D(h->emf_t, 12), D(h->esum, 8), D(h->efin, 4);
     D(h->src, 4);
     I(h->lbo, 1), I(h->ubi, 1), I(h->lbu, 1), I(h->ubd, 1), I(h->pdg, 4), I(h->fflags, 1);
@@ -591,6 +592,10 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
             delete h;
             return fail("all runs of a batch must use the same Hydro module");
         }
+        if (h->runs[r].par.num_n != h->runs[0].par.num_n) {
+            delete h;
+            return fail("all runs of a batch must have the same Num_N");
+        }
         if (stepper_of(h->runs[r].par.Timestep) != stepper_of(h->runs[0].par.Timestep)) {
             delete h;
             return fail("all runs of a batch must use the same Timestep scheme");
@@ -601,6 +606,10 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     const int hyd = h->runs[0].par.hydro;
     const bool grmhd = hyd == DISCO_HYDRO_GRMHD || hyd == DISCO_HYDRO_GRMHD2;
     const bool mhd = hyd == DISCO_HYDRO_MHD || grmhd;
+    // with constrained transport the field's face increments are never
+    // stored (the cell field is rebuilt from the face fluxes): 5 components;
+    // otherwise every conserved field's, passive scalars included
+    h->nf = mhd ? 5 : h->nq;
     build_tables(h);
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;
@@ -636,7 +645,7 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
         rc |= dalloc(&h->fflags, N);
         rc |= dalloc(&h->pdg, 4 * N);
     }
-    const size_t QF = 5 * N;  // stored face-increment components (NF)
+    const size_t QF = (size_t)h->nf * N;  // stored face-increment components (NF)
     rc |= dalloc(&h->fphi, QF);
     rc |= dalloc(&h->fout, QF);
     rc |= dalloc(&h->fin, QF);

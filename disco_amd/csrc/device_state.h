@@ -151,6 +151,7 @@ enum KernelClass { KC_STAGE = 0, KC_PLM, KC_FLUX, KC_EMF, KC_CT, KC_FLIP, KC_UPD
 // What a batch needs launched each substep (uniform over its runs).
 struct StepShape {
     int hydro;  // DISCO_HYDRO_* of every run
+    int nn;     // passive scalars (NUM_N) of every run
     int has_z;  // some run has Num_Z > 1: z-tracks exist
     int zcopy;  // some run is periodic in z: ghost z-layers are copies
     int ct;     // MHD with constrained transport

@@ -122,6 +122,20 @@ DHD double disco_torus_hm1(const disco_ic* ic, double R, double z, double* grad)
     return hm1;
 }
 
+/* Passive scalar n (prim[NUM_C + n]) of every disk: a smooth tracer that
+ * varies in r and, periodically, in phi,
+ *   X_n = 1/4 + s^(n+1)/2 + 2 u (1 - u),  s = (r - R_Min)/(R_Max - R_Min),
+ *   u = phi/(2 pi) mod 1,
+ * polynomials only, so host and device builds agree to the bit. */
+DHD double disco_ic_scalar(const disco_ic* ic, const double* x, int n) {
+    double s = (x[0] - ic->rmin) / (ic->rmax - ic->rmin);
+    double u = x[1] / (2.0 * M_PI);
+    u = u - floor(u);
+    double sp = s;
+    for (int i = 0; i < n; ++i) sp *= s;
+    return 0.25 + 0.5 * sp + 2.0 * u * (1.0 - u);
+}
+
 /* prim must have room for 8 values (NUM_Q <= 8); the first NUM_Q are set. */
 DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
     double r = x[0];
@@ -166,7 +180,7 @@ DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
             prim[URR] = ic->vnoise * cs * disco_ic_noise(ic, x, 1);
             prim[UZZ] = ic->vnoise * cs * disco_ic_noise(ic, x, 2);
         }
-        if (nq > BZZ) {
+        if (ic->num_c > BZZ) {
             double s = (r - ic->rmin) / (ic->rmax - ic->rmin);
             double env = (s > 0.0 && s < 1.0) ? sin(M_PI * s) : 0.0;
             prim[BRR] = 0.0;
@@ -212,13 +226,14 @@ DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
             prim[RHO] = ic->rho_atm;
             prim[PPP] = ic->p_over_rho * ic->rho_atm;
         }
-        if (nq > BZZ && rho > 0.2) {
+        if (ic->num_c > BZZ && rho > 0.2) {
             /* d rho = rho / ((Gamma-1) (h-1)) d(h-1) */
             double f = ic->bz * rho / ((gl - 1.0) * hm1);
             prim[BRR] = -f * g[1] / r;
             prim[BZZ] = f * g[0] / r;
         }
     }
+    for (int q = ic->num_c; q < nq; ++q) prim[q] = disco_ic_scalar(ic, x, q - ic->num_c);
 }
 
 #endif

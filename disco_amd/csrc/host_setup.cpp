@@ -299,7 +299,9 @@ static bool is_grmhd(int hydro) { return hydro == DISCO_HYDRO_GRMHD || hydro == 
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
-    if (p.num_n != 0) m += " Num_N>0";
+    // passive scalars (NUM_N): euler with one scalar, the reference's templates' NUM_N = 1
+    if (p.num_n < 0 || p.num_n > 1) m += " Num_N (0 or 1)";
+    if (p.num_n != 0 && p.hydro != DISCO_HYDRO_EULER) m += " Num_N=1 (euler only)";
     if (p.hydro == DISCO_HYDRO_EULER && p.num_c != 5) m += " Num_C!=5 for euler";
     if (p.hydro == DISCO_HYDRO_GREULER) {
         // Hydro/greuler.c: GR hydrodynamics in the same metric as grmhd
@@ -931,7 +933,10 @@ static void host_prim2cons(const HostRun& R, const double* prim, double* cons, c
         grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
         greuler::prim2cons(R.ph.gamma_law, g, prim, cons, dV);
     } else {
-        euler::prim2cons<5>(R.ph, prim, cons, x, dV);
+        if (R.nq == 6)
+            euler::prim2cons<6>(R.ph, prim, cons, x, dV);
+        else
+            euler::prim2cons<5>(R.ph, prim, cons, x, dV);
     }
 }
 
@@ -946,7 +951,10 @@ static void host_cons2prim(const HostRun& R, const double* cons, double* prim, c
     else if (R.par.hydro == DISCO_HYDRO_GREULER)
         greuler::cons2prim(R.ph, cons, prim, x, dV);
     else
-        euler::cons2prim<5>(R.ph, cons, prim, x, dV);
+        if (R.nq == 6)
+            euler::cons2prim<6>(R.ph, cons, prim, x, dV);
+        else
+            euler::cons2prim<5>(R.ph, cons, prim, x, dV);
 }
 
 int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
@@ -1244,6 +1252,7 @@ void build_annulus_tables(const std::vector<HostRun>& runs, std::vector<AnnDev>*
         shape->bc_nlev = l + 1;
     }
     shape->hydro = runs.empty() ? 0 : runs[0].par.hydro;
+    shape->nn = runs.empty() ? 0 : runs[0].par.num_n;
     shape->stepper = runs.empty() ? 2 : stepper_of(runs[0].par.Timestep);
     shape->has_z = 0;
     shape->zcopy = 0;

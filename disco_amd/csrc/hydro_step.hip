@@ -710,6 +710,8 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
         Launch<PhysGRMHD2>::mindt(S, st, hook);
     else if (sh.hydro == DISCO_HYDRO_GREULER)
         Launch<PhysGREuler>::mindt(S, st, hook);
+    else if (sh.nn == 1)
+        Launch<PhysEuler1>::mindt(S, st, hook);
     else
         Launch<PhysEuler>::mindt(S, st, hook);
 }
@@ -724,6 +726,8 @@ void launch_substep(const StepShape& sh, const DevState& S, double RK, double dt
         Launch<PhysGRMHD2>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GREULER)
         Launch<PhysGREuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
+    else if (sh.nn == 1)
+        Launch<PhysEuler1>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else
         Launch<PhysEuler>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
 }

@@ -43,9 +43,13 @@ namespace disco {
 
 // ------------------------------------------------------------------ physics
 // The Hydro module's per-face and per-cell leaves (hydro.h:6-35) as policies.
-struct PhysEuler {
-    static constexpr int NQ = 5;
-    static constexpr int NF = 5;  // stored face-increment components
+// NN: passive scalars (NUM_N, paul.h:21 NUM_Q = NUM_C + NUM_N), advected as
+// prim[q] * rho at q = NUM_C.. (euler.c:60-62, 97-99, 136-138, 166-168).
+template <int NN>
+struct PhysEulerT {
+    static constexpr int NC = 5;  // NUM_C: index of the first passive scalar
+    static constexpr int NQ = NC + NN;
+    static constexpr int NF = NQ;  // stored face-increment components
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
     static constexpr bool SOURCE_PASS = false;  // sources are added inline in update_a_cell
@@ -74,8 +78,11 @@ struct PhysEuler {
         return euler::mindt(P, prim, w, xp, xm);
     }
 };
+using PhysEuler = PhysEulerT<0>;
+using PhysEuler1 = PhysEulerT<1>;  // one passive scalar (the reference's templates mostly build NUM_N = 1)
 
 struct PhysMHD {
+    static constexpr int NC = 8;
     static constexpr int NQ = 8;
     // with constrained transport the cell field is rebuilt from the face
     // fluxes (B_faces_to_cells), so the face increments of cons[BRR..BZZ]
@@ -115,6 +122,7 @@ struct PhysMHD {
 // source's cell-average corrections.
 template <bool L2>
 struct PhysGRMHDT {
+    static constexpr int NC = 8;
     static constexpr int NQ = 8;
     static constexpr int NF = 5;
     static constexpr bool MHD = true;
@@ -144,6 +152,7 @@ using PhysGRMHD2 = PhysGRMHDT<true>;
 // GR hydrodynamics (greuler.c) in the same metric: no field, no constrained
 // transport; its cons2prim also starts from the cell's previous primitives.
 struct PhysGREuler {
+    static constexpr int NC = 5;
     static constexpr int NQ = 5;
     static constexpr int NF = 5;
     static constexpr bool MHD = false;
@@ -930,6 +939,7 @@ DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double
 // reference writes it.
 // Returns false for a cell outside the ball: sink_src returns there before
 // its accretion sinks.
+template <int NC, int NQ>
 DHD bool nozzle_cell(const PhysParams& P, double* cons, const double* xp, const double* xm, double dV, double dt) {
     const double R0 = P.nozzle_par[0], dR = P.nozzle_par[1], v = P.nozzle_par[2], mach = P.nozzle_par[3];
     const double r = 0.5 * (xp[0] + xm[0]);
@@ -947,6 +957,7 @@ DHD bool nozzle_cell(const PhysParams& P, double* cons, const double* xp, const 
     cons[DDD] += rhodot * dV * dt;
     cons[LLL] += rhodot * r * v * dV * dt;
     cons[TAU] += (0.5 * rhodot * (v - r * om) * (v - r * om) + rhodot * eps) * dV * dt;
+    if constexpr (NQ > NC) cons[NC] += rhodot * dV * dt;  // sink.c:137-138: the first passive scalar tags it
     return true;
 }
 
@@ -1133,7 +1144,7 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
             omega_src(P, prim, cons, xp, xm, dV * dt);  // misc.c:588
             // misc.c:589 (sink_src): the nozzle, then the accretion sinks
             bool in_ball = true;
-            if (P.nozzle_type) in_ball = nozzle_cell(P, cons, xp, xm, dV, dt);
+            if (P.nozzle_type) in_ball = nozzle_cell<Ph::NC, NQ>(P, cons, xp, xm, dV, dt);
             if (S.sinkc) sink_cell(S, Rn, A.run, c, P.sink_type != 0 && in_ball, prim, cons, xp, xm, dV, dt);
         }
         _Pragma("unroll") for (int q = 0; q < NQ; ++q) st_stream(&S.cons[q * N + c], cons[q]);

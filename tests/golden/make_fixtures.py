@@ -393,6 +393,23 @@ MM_CASES = [  # set_wcell (misc.c:100-168): Mesh_Motion 2 (as 1) and 3 (the annu
 ]
 
 
+SCALAR_CASES = [  # one passive scalar (NUM_N = 1; euler.c:60-62, 97-99, 136-138, 166-168, sink.c:137-138)
+    ("c1_small_n1.npz", "c1_kepler2d.par", dict(Num_R=16, Num_N=1), "euler_sun_n1", 10, (1,)),
+    ("c2e_small_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Num_N=1), "euler_sun_n1", 10,
+     (1, 5)),
+    ("c1_nozzle_n1.npz", "c1_kepler2d.par", dict(Num_R=16, Num_N=1, Nozzle_Type=1, Nozzle_Par1=1.5, Nozzle_Par2=0.4,
+                                                  Nozzle_Par3=0.8, Nozzle_Par4=10.0), "euler_sun_n1", 10, (1,)),
+    ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
+                                                 Num_N=1), "euler_sun_n1", 10, (1,)),
+]
+
+
+def fixtures_scalars() -> None:
+    for name, base, over, variant, n, dumps in SCALAR_CASES:
+        states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
+        save_states(name, states)
+
+
 def fixtures_mm() -> None:
     for name, base, over, variant, n, dumps in MM_CASES:
         states, _ = run_ref(variant, par_with(base, **over), n, dumps=dumps)
@@ -503,6 +520,9 @@ LEAF_CASES = {
     # HLLC_DAMPCENTER on faces at 0.02 < r < 0.2: HLL inside r = 0.1, HLLC outside
     "euler_dampc": ("c1_kepler2d.par", {"Num_R": 16, "Riemann_Solver": 3}, "euler_sun"),
     "mhd_dampc": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 3}, "mhd_sun"),
+    # one passive scalar
+    "euler_hllc_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1}, "euler_sun_n1"),
+    "euler_hll_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1, "Riemann_Solver": 0}, "euler_sun_n1"),
 }
 
 
@@ -510,6 +530,7 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
     """Random left/right face states, positions, normals, face speeds."""
     rng = np.random.default_rng(seed)
     nq = 5 if kind.startswith(("euler", "greuler")) else 8
+    nq += kind.endswith("_n1")  # one passive scalar after the NUM_C fields
     U = lambda a, b, size=None: rng.uniform(a, b, size)  # noqa: E731
     out = []
     for k in range(n):
@@ -529,6 +550,8 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
             r, z = U(0.6, 2.9), U(-0.4, 0.4)
             pL = np.array([U(0.5, 2.0), U(5e-3, 5e-2), U(-0.05, 0.05), U(0.2, 1.0), U(-0.05, 0.05),
                            U(-0.05, 0.05), U(-0.05, 0.05), U(-0.05, 0.05)])[:nq]
+            if kind.endswith("_n1"):
+                pL[nq - 1] = U(0.1, 1.0)
         if kind.endswith("_strong"):  # strong jumps and fields: HLLD's retries and its HLL fallback
             pL[2], pL[4] = U(-0.3, 0.3), U(-0.3, 0.3)
             pL[5:8] = U(-1.0, 1.0, 3)
@@ -624,6 +647,9 @@ def main() -> None:
     if "--nobc" in sys.argv:
         fixtures_nobc()
         print("fixtures written")
+        return
+    if "--scalars" in sys.argv:
+        fixtures_scalars()
         return
     if "--mm" in sys.argv:
         fixtures_mm()

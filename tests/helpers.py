@@ -69,7 +69,7 @@ def state_errors(st: dict, ref: dict, rcell: np.ndarray, mask: np.ndarray) -> di
     sc = np.maximum(np.abs(rc).max(axis=1), 1e-300)
     sp[2] = sp[4] = vscale
     sc[2] = sc[4] = sscale
-    if rp.shape[0] > 5:
+    if rp.shape[0] >= 8:
         # magnetic field components: normalise by the field magnitude scale
         bscale = np.sqrt((rp[5:8] ** 2).sum(axis=0)).max()
         sp[5:8] = bscale

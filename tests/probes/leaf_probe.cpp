@@ -85,7 +85,7 @@ int main(int argc, char** argv) {
     if (!fi || !fo) return 1;
     int n = 0;
     if (std::fread(&n, sizeof n, 1, fi) != 1) return 1;
-    const int NQ = par.num_c;
+    const int NQ = par.num_c + par.num_n;
     std::vector<double> in(2 * NQ + 9), out(4 * NQ + 5);
     for (int k = 0; k < n; ++k) {
         if (std::fread(in.data(), sizeof(double), in.size(), fi) != in.size()) return 1;
@@ -97,6 +97,8 @@ int main(int argc, char** argv) {
             record<PhysGRMHD2>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_MHD)
             record<PhysMHD>(P, par.hydro, in.data(), out.data());
+        else if (par.num_n == 1)
+            record<PhysEuler1>(P, par.hydro, in.data(), out.data());
         else
             record<PhysEuler>(P, par.hydro, in.data(), out.data());
         std::fwrite(out.data(), sizeof(double), out.size(), fo);

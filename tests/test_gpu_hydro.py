@@ -99,6 +99,14 @@ CASES = [
     ("c1_nobc_rmin.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmin=1)),
     ("c1_nobc_rmax.npz", "c1_kepler2d.par", dict(Num_R=16, NoBC_Rmax=1)),
     ("c2e_nobc_z.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, NoBC_Zmin=1, NoBC_Zmax=1)),
+    # one passive scalar (NUM_N = 1): 2D, 3D periodic, with the nozzle (which tags its mass in the scalar) and
+    # with viscosity (which leaves the scalar alone)
+    ("c1_small_n1.npz", "c1_kepler2d.par", dict(Num_R=16, Num_N=1)),
+    ("c2e_small_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Num_N=1)),
+    ("c1_nozzle_n1.npz", "c1_kepler2d.par", dict(Num_R=16, Num_N=1, Nozzle_Type=1, Nozzle_Par1=1.5,
+                                                  Nozzle_Par2=0.4, Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
+    ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
+                                                 Num_N=1)),
     # set_wcell's Mesh_Motion 2 (the face speeds, as 1) and 3 (their annulus mean)
     ("c1_mm2.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=2)),
     ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),

@@ -51,6 +51,10 @@ CASES = {
     # records inside take HLL's state, those outside HLLC's
     "euler_dampc": ("c1_kepler2d.par", {"Num_R": 16, "Riemann_Solver": 3}),
     "mhd_dampc": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Riemann_Solver": 3}),
+    # one passive scalar (NUM_N = 1): prim2cons, cons2prim, flux and getUstar
+    # carry it as prim[5] * rho
+    "euler_hllc_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1}),
+    "euler_hll_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1, "Riemann_Solver": 0}),
 }
 
 
