@@ -21,6 +21,7 @@
 
 namespace disco {
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook);
+int launch_nf(const StepShape& sh);
 void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchHook* hook);
 void launch_partners(const DevState& S, hipStream_t st);
 void launch_report(const DevState& S, int a0, int nann, double* out, hipStream_t st);
@@ -607,10 +608,15 @@ int disco_gpu_create(const disco_params* pars, int nruns, disco_gpu_t** out) {
     const bool grmhd = hyd == DISCO_HYDRO_GRMHD || hyd == DISCO_HYDRO_GRMHD2;
     const bool mhd = hyd == DISCO_HYDRO_MHD || grmhd;
     // with constrained transport the field's face increments are never
-    // stored (the cell field is rebuilt from the face fluxes): 5 components;
-    // otherwise every conserved field's, passive scalars included
-    h->nf = mhd ? 5 : h->nq;
+    // stored (the cell field is rebuilt from the face fluxes): 5 components
+    // and the passive scalars'; otherwise every conserved field's
+    h->nf = mhd ? 5 + h->runs[0].par.num_n : h->nq;
     build_tables(h);
+    // the kernels that will run store exactly this many components per cell
+    if (launch_nf(h->shape) != h->nf) {
+        delete h;
+        return fail("internal: the face-increment buffers do not match the flux kernel's components");
+    }
     size_t N = h->ncell;
     size_t Q = (size_t)h->nq * N;
     int rc = 0;

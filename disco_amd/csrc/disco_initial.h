@@ -136,11 +136,12 @@ DHD double disco_ic_scalar(const disco_ic* ic, const double* x, int n) {
     return 0.25 + 0.5 * sp + 2.0 * u * (1.0 - u);
 }
 
-/* prim must have room for 8 values (NUM_Q <= 8); the first NUM_Q are set. */
+/* prim must have room for max(8, NUM_Q) values; the first NUM_Q are set. */
 DHD void disco_initial(const disco_ic* ic, double* prim, const double* x) {
     double r = x[0];
     int nq = ic->num_c + ic->num_n;
     prim[0] = prim[1] = prim[2] = prim[3] = prim[4] = prim[5] = prim[6] = prim[7] = 0.0;
+    for (int q = 8; q < nq; ++q) prim[q] = 0.0;
 
     if (ic->id == DISCO_IC_KEPLER_NOISE) {
         /* 2D adiabatic Keplerian disk: rho = 1 + A U(-1,1), P = (P/rho) rho */

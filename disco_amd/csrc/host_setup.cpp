@@ -299,9 +299,9 @@ static bool is_grmhd(int hydro) { return hydro == DISCO_HYDRO_GRMHD || hydro == 
 // ---------------------------------------------------------------- checks
 static int unsupported(const disco_params& p, std::string* err) {
     std::string m;
-    // passive scalars (NUM_N): euler with one scalar, the reference's templates' NUM_N = 1
+    // passive scalars (NUM_N): euler and mhd with one scalar, the reference's templates' NUM_N = 1
     if (p.num_n < 0 || p.num_n > 1) m += " Num_N (0 or 1)";
-    if (p.num_n != 0 && p.hydro != DISCO_HYDRO_EULER) m += " Num_N=1 (euler only)";
+    if (p.num_n != 0 && p.hydro != DISCO_HYDRO_EULER && p.hydro != DISCO_HYDRO_MHD) m += " Num_N=1 (euler, mhd)";
     if (p.hydro == DISCO_HYDRO_EULER && p.num_c != 5) m += " Num_C!=5 for euler";
     if (p.hydro == DISCO_HYDRO_GREULER) {
         // Hydro/greuler.c: GR hydrodynamics in the same metric as grmhd
@@ -614,7 +614,7 @@ void host_boundary(HostRun* Rp) {
         host_zerograd(R, R.par.boundary == DISCO_BC_OUTFLOW3D);
         return;
     }
-    double prim[8];
+    double prim[DISCO_MAX_NQ];
     auto fill = [&](int j, int k) {
         int jk = j + R.Nr * k;
         for (int i = 0; i < R.Np[jk]; ++i) {
@@ -873,7 +873,7 @@ static void host_b_faces_to_cells(HostRun& R, const std::vector<HostFace>& f1, c
 static void host_set_B_fields(HostRun& R) {
     const size_t N = R.ncell;
     R.Phi.assign(5 * N, 0.0);
-    double prim[8];
+    double prim[DISCO_MAX_NQ];
     for (int k = 0; k < R.Nz; ++k) {
         double z = get_centroid(R.zk(k), R.zk(k - 1), 2);
         for (int j = 0; j < R.Nr; ++j) {
@@ -919,7 +919,10 @@ static void host_set_B_fields(HostRun& R) {
 // The Hydro module's prim2cons / cons2prim for one cell (hydro.h:6-12).
 static void host_prim2cons(const HostRun& R, const double* prim, double* cons, const double* x, double dV) {
     if (R.par.hydro == DISCO_HYDRO_MHD) {
-        mhd::prim2cons(R.ph, prim, cons, x, dV);
+        if (R.nq == 9)
+            mhd::prim2cons<9>(R.ph, prim, cons, x, dV);
+        else
+            mhd::prim2cons(R.ph, prim, cons, x, dV);
     } else if (R.par.hydro == DISCO_HYDRO_GRMHD) {
         grmhd::Metric g;
         grmhd::kerr_metric(x, R.ph.gr_M, R.ph.gr_a, &g);
@@ -943,7 +946,10 @@ static void host_prim2cons(const HostRun& R, const double* prim, double* cons, c
 // prim holds the cell's current prims on entry (the GRMHD Newton guess).
 static void host_cons2prim(const HostRun& R, const double* cons, double* prim, const double* x, double dV) {
     if (R.par.hydro == DISCO_HYDRO_MHD)
-        mhd::cons2prim(R.ph, cons, prim, x, dV);
+        if (R.nq == 9)
+            mhd::cons2prim<9>(R.ph, cons, prim, x, dV);
+        else
+            mhd::cons2prim(R.ph, cons, prim, x, dV);
     else if (R.par.hydro == DISCO_HYDRO_GRMHD)
         grmhd::cons2prim(R.ph, cons, prim, x, dV);
     else if (R.par.hydro == DISCO_HYDRO_GRMHD2)
@@ -973,7 +979,7 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
     R.wiph.assign(R.ncell, 0.0);
     const bool mhd_ct = par.hydro == DISCO_HYDRO_MHD || is_grmhd(par.hydro);
     // domain.c:163-209: initial data and prim2cons on real cells
-    double prim[8], cons[8];
+    double prim[DISCO_MAX_NQ], cons[DISCO_MAX_NQ];
     for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
         for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
             int jk = j + R.Nr * k;
@@ -1032,7 +1038,7 @@ void host_restart_fill(HostRun* Rp, bool have_cons, bool have_wiph) {
     HostRun& R = *Rp;
     calc_dp(R);
     if (!have_cons) {
-        double prim[8], cons[8];
+        double prim[DISCO_MAX_NQ], cons[DISCO_MAX_NQ];
         for (int k = R.NgZa; k < R.Nz - R.NgZb; ++k)
             for (int j = R.NgRa; j < R.Nr - R.NgRb; ++j) {
                 const int jk = j + R.Nr * k;

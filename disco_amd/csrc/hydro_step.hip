@@ -701,8 +701,20 @@ void launch_dt_final(const DevState& S, int check, hipStream_t st, const LaunchH
     DISCO_LAUNCH(hook, "k_dt_final", k_dt_final, dim3(nblk(S.nruns, 64)), dim3(64), st, S, check);
 }
 
+// Face-increment components per cell that the flux kernel of this shape
+// stores (its policy's NF), for the host's check of the buffers it sized.
+int launch_nf(const StepShape& sh) {
+    if (sh.hydro == DISCO_HYDRO_MHD) return sh.nn == 1 ? PhysMHD1::NF : PhysMHD::NF;
+    if (sh.hydro == DISCO_HYDRO_GRMHD) return PhysGRMHD::NF;
+    if (sh.hydro == DISCO_HYDRO_GRMHD2) return PhysGRMHD2::NF;
+    if (sh.hydro == DISCO_HYDRO_GREULER) return PhysGREuler::NF;
+    return sh.nn == 1 ? PhysEuler1::NF : PhysEuler::NF;
+}
+
 void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const LaunchHook* hook) {
-    if (sh.hydro == DISCO_HYDRO_MHD)
+    if (sh.hydro == DISCO_HYDRO_MHD && sh.nn == 1)
+        Launch<PhysMHD1>::mindt(S, st, hook);
+    else if (sh.hydro == DISCO_HYDRO_MHD)
         Launch<PhysMHD>::mindt(S, st, hook);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::mindt(S, st, hook);
@@ -718,7 +730,9 @@ void launch_mindt(const StepShape& sh, const DevState& S, hipStream_t st, const 
 
 void launch_substep(const StepShape& sh, const DevState& S, double RK, double dtfac, int first, int want_dt,
                     hipStream_t st, const LaunchHook* hook, const Fork* fk) {
-    if (sh.hydro == DISCO_HYDRO_MHD)
+    if (sh.hydro == DISCO_HYDRO_MHD && sh.nn == 1)
+        Launch<PhysMHD1>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
+    else if (sh.hydro == DISCO_HYDRO_MHD)
         Launch<PhysMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);
     else if (sh.hydro == DISCO_HYDRO_GRMHD)
         Launch<PhysGRMHD>::substep(S, RK, dtfac, first, want_dt, sh, st, hook, fk);

@@ -21,7 +21,8 @@ constexpr int NQ = 8;
 // mhd.c:518-528 for dim 0 (r); 1 for phi and z
 DHD double bscale_r(double r) { return 1.0 / r; }
 
-// mhd.c:35-69
+// mhd.c:35-69; NQS = NUM_Q: passive scalars after the 8 fields (mhd.c:66-68)
+template <int NQS = NQ>
 DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const double* x, double dV) {
     double r = x[0];
     double rho = prim[RHO];
@@ -45,9 +46,11 @@ DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const 
     cons[BRR] = Br * dV * bscale_r(r);
     cons[BPP] = Bp * dV / r;
     cons[BZZ] = Bz * dV;
+    _Pragma("unroll") for (int q = NQ; q < NQS; ++q) cons[q] = prim[q] * cons[DDD];
 }
 
 // mhd.c:71-136; Bpack = {Bn, Br, Bp, Bz, v.B} of the HLL state (vel)
+template <int NQS = NQ>
 DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const double* x, double Sk, double Ss,
                   const double* n, const double* Bpack) {
     double r = x[0];
@@ -97,9 +100,11 @@ DHD void getUstar(const PhysParams& P, const double* prim, double* Ustar, const 
     Ustar[BRR] = Bsr * bscale_r(r);
     Ustar[BPP] = Bsp / r;
     Ustar[BZZ] = Bsz;
+    _Pragma("unroll") for (int q = NQ; q < NQS; ++q) Ustar[q] = prim[q] * Ustar[DDD];  // mhd.c:133-135
 }
 
 // mhd.c:138-185
+template <int NQS = NQ>
 DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV) {
     double r = x[0];
     // quotients by dV, rho and r share one division each (div_rn: exact)
@@ -136,9 +141,11 @@ DHD void cons2prim(const PhysParams& P, const double* cons, double* prim, const 
     prim[BRR] = Br;
     prim[BPP] = Bp;
     prim[BZZ] = Bz;
+    _Pragma("unroll") for (int q = NQ; q < NQS; ++q) prim[q] = cons[q] / cons[DDD];  // mhd.c:181-183
 }
 
 // mhd.c:187-223
+template <int NQS = NQ>
 DHD void flux(const PhysParams& P, const double* prim, double* F, const double* x, const double* n) {
     double r = x[0];
     double rho = prim[RHO];
@@ -163,6 +170,7 @@ DHD void flux(const PhysParams& P, const double* prim, double* F, const double* 
     F[BRR] = (Br * vn - vr * Bn) * bscale_r(r);
     F[BPP] = (Bp * vn - vp * Bn) / r;
     F[BZZ] = Bz * vn - vz * Bn;
+    _Pragma("unroll") for (int q = NQ; q < NQS; ++q) F[q] = prim[q] * F[DDD];  // mhd.c:219-221
 }
 
 // mhd.c:318-449: fast-magnetosonic wave-speed bounds, contact speed, and the
@@ -473,13 +481,15 @@ DHD void double_star(const double* UsL, const double* UsR, double* Uss, const do
     Uss[BZZ] = Bz_ss;
 }
 
-// hlld.c:282-401 (cylindrical scale factors h = (1, r, 1))
+// hlld.c:282-401 (cylindrical scale factors h = (1, r, 1)); passive scalars
+// (q >= 8) ride on the density of the contact's side (hlld.c:345-348, 371-374)
+template <int NQS = NQ>
 DHD void solve(const PhysParams& P, double w, const double* primL, const double* primR, double* F, double* U,
                const double* x, const double* n) {
     const double g = P.gamma_law;
     const double hr = 1.0, hp = x[0], hz = 1.0;
-    double pL[NQ], pR[NQ];
-    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+    double pL[NQS], pR[NQS];
+    _Pragma("unroll") for (int q = 0; q < NQS; ++q) {
         pL[q] = primL[q];
         pR[q] = primR[q];
     }
@@ -498,8 +508,8 @@ DHD void solve(const PhysParams& P, double w, const double* primL, const double*
     // selected side's states keeps every lane on it, with the same arithmetic
     // per side as the reference's branches.
     const bool left = w < v[2];
-    double pK[NQ];
-    _Pragma("unroll") for (int q = 0; q < NQ; ++q) pK[q] = left ? pL[q] : pR[q];
+    double pK[NQS];
+    _Pragma("unroll") for (int q = 0; q < NQS; ++q) pK[q] = left ? pL[q] : pR[q];
     const double SK = left ? v[0] : v[4];   // outer wave of the side
     const double SsK = left ? v[1] : v[3];  // its Alfven wave
     double UK[NQ];
@@ -526,6 +536,10 @@ DHD void solve(const PhysParams& P, double w, const double* primL, const double*
             double_star(UsL, UsR, U, v, n, left ? 0 : 1);
             _Pragma("unroll") for (int q = 0; q < NQ; ++q) F[q] += SsK * (U[q] - UsK[q]) + SK * (UsK[q] - UK[q]);
         }
+    }
+    _Pragma("unroll") for (int q = NQ; q < NQS; ++q) {
+        U[q] = pK[q] * U[DDD];
+        F[q] = pK[q] * F[DDD];
     }
     const double brfac = bscale_r(x[0]), bpfac = 1.0, bzfac = 1.0;
     F[SRR] *= hr;
@@ -575,53 +589,53 @@ DHD FaceEMF flux_to_E(const double* Flux, const double* Ustr, const double* x, i
 // DC: the run may use HLLC_DAMPCENTER, whose choice of HLL varies from face
 // to face (an instantiation of its own, so that the others keep a branch that
 // is uniform over the wavefront).
-template <bool DC = false>
+template <bool DC = false, int NQS = NQ>
 DHD FaceEMF riemann(const PhysParams& P, const double* primL, const double* primR, const double* x,
                     const double* n, double w, double dAdt, int dim, double* dU) {
-    double Flux[NQ], Ustr[NQ];
-    _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+    double Flux[NQS], Ustr[NQS];
+    _Pragma("unroll") for (int q = 0; q < NQS; ++q) {
         Flux[q] = 0.0;
         Ustr[q] = 0.0;
     }
     if (P.riemann == DISCO_HLLD) {
-        hlld::solve(P, w, primL, primR, Flux, Ustr, x, n);
+        hlld::solve<NQS>(P, w, primL, primR, Flux, Ustr, x, n);
     } else {
         double Sl, Sr, Ss, Bpack[5];
         vel(P, primL, primR, &Sl, &Sr, &Ss, n, x, Bpack);
         if (w < Sl) {
-            flux(P, primL, Flux, x, n);
-            prim2cons(P, primL, Ustr, x, 1.0);
+            flux<NQS>(P, primL, Flux, x, n);
+            prim2cons<NQS>(P, primL, Ustr, x, 1.0);
         } else if (w > Sr) {
-            flux(P, primR, Flux, x, n);
-            prim2cons(P, primR, Ustr, x, 1.0);
+            flux<NQS>(P, primR, Flux, x, n);
+            prim2cons<NQS>(P, primR, Ustr, x, 1.0);
         } else if (DC ? hll_face(P, x) : P.riemann == DISCO_HLL) {
-            double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
+            double Fl[NQS], Fr[NQS], Ul[NQS], Ur[NQS];
             double aL = Sr;
             double aR = -Sl;
-            prim2cons(P, primL, Ul, x, 1.0);
-            prim2cons(P, primR, Ur, x, 1.0);
-            flux(P, primL, Fl, x, n);
-            flux(P, primR, Fr, x, n);
+            prim2cons<NQS>(P, primL, Ul, x, 1.0);
+            prim2cons<NQS>(P, primR, Ur, x, 1.0);
+            flux<NQS>(P, primL, Fl, x, n);
+            flux<NQS>(P, primR, Fr, x, n);
             const double aS = aL + aR;
             const double yS = 1.0 / aS;
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) {
+            _Pragma("unroll") for (int q = 0; q < NQS; ++q) {
                 Flux[q] = div_rn(aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q]), aS, yS);
                 Ustr[q] = div_rn(aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q], aS, yS);
             }
         } else {
             // riemann.c:272-290 on the side of the contact, one path for the
             // lanes of both sides
-            double Uk[NQ], Fk[NQ], pk[NQ];
+            double Uk[NQS], Fk[NQS], pk[NQS];
             const bool left = w < Ss;
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) pk[q] = left ? primL[q] : primR[q];
+            _Pragma("unroll") for (int q = 0; q < NQS; ++q) pk[q] = left ? primL[q] : primR[q];
             const double Sk = left ? Sl : Sr;
-            prim2cons(P, pk, Uk, x, 1.0);
-            getUstar(P, pk, Ustr, x, Sk, Ss, n, Bpack);
-            flux(P, pk, Fk, x, n);
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) Flux[q] = Fk[q] + Sk * (Ustr[q] - Uk[q]);
+            prim2cons<NQS>(P, pk, Uk, x, 1.0);
+            getUstar<NQS>(P, pk, Ustr, x, Sk, Ss, n, Bpack);
+            flux<NQS>(P, pk, Fk, x, n);
+            _Pragma("unroll") for (int q = 0; q < NQS; ++q) Flux[q] = Fk[q] + Sk * (Ustr[q] - Uk[q]);
         }
     }
-    _Pragma("unroll") for (int q = 0; q < NQ; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
+    _Pragma("unroll") for (int q = 0; q < NQS; ++q) dU[q] = (Flux[q] - w * Ustr[q]) * dAdt;
     return flux_to_E(Flux, Ustr, x, dim);
 }
 

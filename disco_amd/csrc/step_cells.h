@@ -42,6 +42,11 @@
 namespace disco {
 
 // ------------------------------------------------------------------ physics
+// Field of slot s of a cell's stored face increments: the first five
+// conserved fields, then (after a gap of the field components that
+// constrained transport does not store) the passive scalars.
+DHD constexpr int fslot(int s, int gap) { return s < 5 ? s : s + gap; }
+
 // The Hydro module's per-face and per-cell leaves (hydro.h:6-35) as policies.
 // NN: passive scalars (NUM_N, paul.h:21 NUM_Q = NUM_C + NUM_N), advected as
 // prim[q] * rho at q = NUM_C.. (euler.c:60-62, 97-99, 136-138, 166-168).
@@ -50,6 +55,7 @@ struct PhysEulerT {
     static constexpr int NC = 5;  // NUM_C: index of the first passive scalar
     static constexpr int NQ = NC + NN;
     static constexpr int NF = NQ;  // stored face-increment components
+    static constexpr int FGAP = 0;  // slot s of the stored increments is field fslot(s, FGAP)
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = false;  // cons2prim reads the previous prims
     static constexpr bool SOURCE_PASS = false;  // sources are added inline in update_a_cell
@@ -81,20 +87,23 @@ struct PhysEulerT {
 using PhysEuler = PhysEulerT<0>;
 using PhysEuler1 = PhysEulerT<1>;  // one passive scalar (the reference's templates mostly build NUM_N = 1)
 
-struct PhysMHD {
+template <int NN>
+struct PhysMHDT {
     static constexpr int NC = 8;
-    static constexpr int NQ = 8;
+    static constexpr int NQ = NC + NN;
     // with constrained transport the cell field is rebuilt from the face
     // fluxes (B_faces_to_cells), so the face increments of cons[BRR..BZZ]
-    // (riemann.c:314-317) are never used: only the first five are stored
-    static constexpr int NF = 5;
+    // (riemann.c:314-317) are never used: only the first five and the
+    // passive scalars' are stored (slot s holds field fslot(s, FGAP))
+    static constexpr int NF = 5 + NN;
+    static constexpr int FGAP = 3;
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = false;
     static constexpr bool SOURCE_PASS = false;
     template <bool DC = false>
     DHD FaceEMF riemann(const PhysParams& P, const double* pL, const double* pR, const double* x, const double* n,
                         double w, double dAdt, int dim, double* dU) {
-        return mhd::riemann<DC>(P, pL, pR, x, n, w, dAdt, dim, dU);
+        return mhd::riemann<DC, NQ>(P, pL, pR, x, n, w, dAdt, dim, dU);
     }
     DHD void source(const PhysParams& P, const double* prim, double* cons, const double* xp, const double* xm,
                     double dVdt) {
@@ -102,18 +111,20 @@ struct PhysMHD {
     }
     DHD int cons2prim(const PhysParams& P, const double* cons, double* prim, const double* x, double dV,
                       int* okp = nullptr) {
-        mhd::cons2prim(P, cons, prim, x, dV);
+        mhd::cons2prim<NQ>(P, cons, prim, x, dV);
         if (okp) *okp = 1;
         return 0;
     }
     static constexpr bool NEWTONIAN = true;
     DHD void prim2cons(const PhysParams& P, const double* prim, double* cons, const double* x, double dV) {
-        mhd::prim2cons(P, prim, cons, x, dV);
+        mhd::prim2cons<NQ>(P, prim, cons, x, dV);
     }
     DHD double mindt(const PhysParams& P, const double* prim, double w, const double* xp, const double* xm) {
         return mhd::mindt(P, prim, w, xp, xm);
     }
 };
+using PhysMHD = PhysMHDT<0>;
+using PhysMHD1 = PhysMHDT<1>;
 
 // GRMHD (grmhd.c) in the Kerr-Schild metric: the same CT layout as ideal MHD;
 // its Noble 2D Newton cons2prim starts from the cell's previous primitives,
@@ -125,6 +136,7 @@ struct PhysGRMHDT {
     static constexpr int NC = 8;
     static constexpr int NQ = 8;
     static constexpr int NF = 5;
+    static constexpr int FGAP = 0;
     static constexpr bool MHD = true;
     static constexpr bool C2P_GUESS = true;
     static constexpr bool SOURCE_PASS = true;
@@ -155,6 +167,7 @@ struct PhysGREuler {
     static constexpr int NC = 5;
     static constexpr int NQ = 5;
     static constexpr int NF = 5;
+    static constexpr int FGAP = 0;
     static constexpr bool MHD = false;
     static constexpr bool C2P_GUESS = true;
     static constexpr bool SOURCE_PASS = false;
@@ -730,7 +743,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(phip, dph, S.piph[cb], S.dphi[cb], 0, A, Pmax);
         FaceEMF e = t_face_flux<Ph, D, VISC, DC>(S, P, c, cb, F, A, dt, dU);
         double* f = Dr::f_hi(S);
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[fslot(q, Ph::FGAP)]);
         if (Ph::MHD && S.emf_t) {
             st_stream(&S.emf_t[(Dr::EMF_HI + 0) * N + c], e.E1);
             st_stream(&S.emf_t[(Dr::EMF_HI + 1) * N + c], e.B1);
@@ -743,7 +756,7 @@ DHD void owned_faces_flux(const DevState& S, const AnnDev& A, const PhysParams& 
         TFace F = t_face<D>(S.piph[ca], S.dphi[ca], phip, dph, 1, W, Pmax);
         FaceEMF e = t_face_flux<Ph, D, VISC, DC>(S, P, ca, c, F, W, dt, dU);
         double* f = Dr::f_lo(S);
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[q]);
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&f[q * N + c], dU[fslot(q, Ph::FGAP)]);
         if (Ph::MHD && S.emf_t) {
             st_stream(&S.emf_t[(Dr::EMF_LO + 0) * N + c], e.E1);
             st_stream(&S.emf_t[(Dr::EMF_LO + 1) * N + c], e.B1);
@@ -800,7 +813,7 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
         } else {
             e = Ph::template riemann<DC>(P, primL, primR, x, nv, w, dA * dt, 0, dU);
         }
-        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&S.fphi[q * N + c], dU[q]);
+        _Pragma("unroll") for (int q = 0; q < Ph::NF; ++q) st_stream(&S.fphi[q * N + c], dU[fslot(q, Ph::FGAP)]);
         if (Ph::MHD && S.emf_phi) {
             st_stream(&S.emf_phi[0 * N + c], e.E1);  // E_z
             st_stream(&S.emf_phi[1 * N + c], e.B1);  // B_r
@@ -816,7 +829,7 @@ DHD void flux_cell(const DevState& S, int c, double dtfac) {
 // Gather of the increments of every face of cell c in direction D: the
 // reference subtracts a face's increment from its L cell and adds it to its
 // R cell (riemann.c:314-317).
-template <int NQ, int D>
+template <int NQ, int D, int GAP = 0>
 DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double* cons) {
     using Dr = Dir<D>;
     const size_t N = S.ncell;
@@ -826,14 +839,14 @@ DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double*
         const double* fo = Dr::f_lo(S);
         const double* fw = Dr::f_hi(S);
         const int* part = Dr::part_lo(S);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fo[q * N + c];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[fslot(q, GAP)] += fo[q * N + c];
         int start = part[cm];
         int cnt = part[c] - start;
         if (cnt < 0) cnt += W.np;
         for (int t = 0; t < cnt; ++t) {
             int a = start + t;
             if (a >= W.np) a -= W.np;
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] += fw[q * N + W.off + a];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[fslot(q, GAP)] += fw[q * N + W.off + a];
         }
     }
     if (ahi >= 0 && (A.flags & Dr::FLAG_HI)) {  // this cell is L: loses
@@ -841,14 +854,14 @@ DHD void gather_trans(const DevState& S, const AnnDev& A, int c, int cm, double*
         const double* fo = Dr::f_hi(S);
         const double* fb = Dr::f_lo(S);
         const int* part = Dr::part_hi(S);
-        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fo[q * N + c];
+        _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[fslot(q, GAP)] -= fo[q * N + c];
         int start = part[cm];
         int cnt = part[c] - start;
         if (cnt < 0) cnt += B.np;
         for (int t = 0; t < cnt; ++t) {
             int b = start + t;
             if (b >= B.np) b -= B.np;
-            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[q] -= fb[q * N + B.off + b];
+            _Pragma("unroll") for (int q = 0; q < NQ; ++q) cons[fslot(q, GAP)] -= fb[q * N + B.off + b];
         }
     }
 }
@@ -918,7 +931,7 @@ DHD void damping_cell(const DevState& S, const RunDev& Rn, int run, const double
             dampFactor = pow(1.0 - pow(theta, 2.0), 2.0);
     }
     dampFactor = expm1(-dt * dampFactor);
-    double p0[8], c0[NQ], c1[NQ];
+    double p0[NQ > 8 ? NQ : 8], c0[NQ], c1[NQ];
     disco_initial(&Rn.ic, p0, x);
     Ph::prim2cons(P, p0, c0, x, dV);
     Ph::prim2cons(P, prim, c1, x, dV);
@@ -1090,15 +1103,15 @@ DHD void update_a_cell(const DevState& S, int c, double RK, double dtfac, int fi
         constexpr int NF = Ph::NF;
         if (A.flags & ANN_PHIFLUX) {
             if (i == 0) {
-                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] -= S.fphi[q * N + c];
-                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[fslot(q, Ph::FGAP)] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[fslot(q, Ph::FGAP)] += S.fphi[q * N + cm];
             } else {
-                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] += S.fphi[q * N + cm];
-                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[q] -= S.fphi[q * N + c];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[fslot(q, Ph::FGAP)] += S.fphi[q * N + cm];
+                _Pragma("unroll") for (int q = 0; q < NF; ++q) cons[fslot(q, Ph::FGAP)] -= S.fphi[q * N + c];
             }
         }
-        gather_trans<NF, DIR_R>(S, A, c, cm, cons);
-        gather_trans<NF, DIR_Z>(S, A, c, cm, cons);
+        gather_trans<NF, DIR_R, Ph::FGAP>(S, A, c, cm, cons);
+        gather_trans<NF, DIR_Z, Ph::FGAP>(S, A, c, cm, cons);
         // add_source with the pre-move geometry and the substep's prims
         double xp[3] = {A.rp, phip, A.zp};
         double xm[3] = {A.rm, phip - dph, A.zm};
@@ -1261,7 +1274,7 @@ DHD void update_b_ghost(const DevState& S, int c) {
         x[1] = phip - .5 * dph;
     else
         x[1] = 0.5 * (phip + (phip - dph));
-    double p8[8];
+    double p8[Ph::NQ > 8 ? Ph::NQ : 8];
     disco_initial(&Rn.ic, p8, x);
     if (Rn.ph.rot_frame) p8[UPP] -= Rn.ph.omega0;  // subtract_omega (fixed.c:37,54; boundary_functions.c:30)
     _Pragma("unroll") for (int q = 0; q < Ph::NQ; ++q) st_stream(&S.prim[q * N + c], p8[q]);

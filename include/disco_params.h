@@ -36,6 +36,8 @@ enum { DISCO_IC_KEPLER_NOISE = 0, DISCO_IC_ISO_BINARY_DISK = 1, DISCO_IC_MHD_DIS
 enum { DISCO_HLL = 0, DISCO_HLLC = 1, DISCO_HLLD = 2, DISCO_HLLC_DAMPCENTER = 3 };
 
 #define DISCO_MAX_PLANETS 2
+/* the largest NUM_Q built: 8 fields and one passive scalar */
+#define DISCO_MAX_NQ 9
 
 typedef struct disco_params {
     /* time range */

@@ -401,6 +401,15 @@ SCALAR_CASES = [  # one passive scalar (NUM_N = 1; euler.c:60-62, 97-99, 136-138
                                                   Nozzle_Par3=0.8, Nozzle_Par4=10.0), "euler_sun_n1", 10, (1,)),
     ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
                                                  Num_N=1), "euler_sun_n1", 10, (1,)),
+    # the small 3D MHD disk (CT) with HLLD and HLLC, and with its sink terms (the nozzle tags the scalar)
+    ("c2_mhd_small_n1.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Num_N=1), "mhd_sun_n1", 5, (1,)),
+    ("c2_mhd_small_hllc_n1.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Num_N=1, Riemann_Solver=1),
+     "mhd_sun_n1", 5, (1,)),
+    ("c2_mhd_sinkterms_n1.npz", "c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Num_N=1, Init_Par5=0.0, Cool_Type=2,
+                                                      Cool_Par1=50.0, DampOuter_Type=1, DampOuter_Time=1.0,
+                                                      DampOuter_Len=0.8, Nozzle_Type=1, Nozzle_Par1=2.0,
+                                                      Nozzle_Par2=0.4, Nozzle_Par3=0.7, Nozzle_Par4=20.0),
+     "mhd_sun_n1", 5, (1,)),
 ]
 
 
@@ -523,6 +532,9 @@ LEAF_CASES = {
     # one passive scalar
     "euler_hllc_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1}, "euler_sun_n1"),
     "euler_hll_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1, "Riemann_Solver": 0}, "euler_sun_n1"),
+    "mhd_hlld_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1}, "mhd_sun_n1"),
+    "mhd_hllc_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1, "Riemann_Solver": 1}, "mhd_sun_n1"),
+    "mhd_hll_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1, "Riemann_Solver": 0}, "mhd_sun_n1"),
 }
 
 
@@ -551,6 +563,8 @@ def leaf_records(kind: str, n: int, seed: int) -> np.ndarray:
             pL = np.array([U(0.5, 2.0), U(5e-3, 5e-2), U(-0.05, 0.05), U(0.2, 1.0), U(-0.05, 0.05),
                            U(-0.05, 0.05), U(-0.05, 0.05), U(-0.05, 0.05)])[:nq]
             if kind.endswith("_n1"):
+                if pL.size < nq:
+                    pL = np.append(pL, 0.0)
                 pL[nq - 1] = U(0.1, 1.0)
         if kind.endswith("_strong"):  # strong jumps and fields: HLLD's retries and its HLL fallback
             pL[2], pL[4] = U(-0.3, 0.3), U(-0.3, 0.3)

@@ -49,7 +49,7 @@ static void record(const PhysParams& P, int hydro, const double* in, double* out
         grmhd::kerr_metric(x, P.gr_M, P.gr_a, &g);
         greuler::prim2cons(P.gamma_law, g, pL, cons, 1.37);
     } else if (hydro == DISCO_HYDRO_MHD) {
-        mhd::prim2cons(P, pL, cons, x, 1.37);
+        mhd::prim2cons<NQ>(P, pL, cons, x, 1.37);
     } else {
         euler::prim2cons<NQ>(P, pL, cons, x, 1.37);
     }
@@ -95,6 +95,8 @@ int main(int argc, char** argv) {
             record<PhysGREuler>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_GRMHD2)
             record<PhysGRMHD2>(P, par.hydro, in.data(), out.data());
+        else if (par.hydro == DISCO_HYDRO_MHD && par.num_n == 1)
+            record<PhysMHD1>(P, par.hydro, in.data(), out.data());
         else if (par.hydro == DISCO_HYDRO_MHD)
             record<PhysMHD>(P, par.hydro, in.data(), out.data());
         else if (par.num_n == 1)

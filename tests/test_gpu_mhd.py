@@ -61,11 +61,18 @@ SOLVERS = [("c2_mhd_small.npz", {}),  # HLLD
            # a rotating frame (rotframe.c)
            ("c2_mhd_small_rot.npz", dict(RotFrame=1, RotOmega=0.3, RotD=0.05)),
            # annuli at the mean of their face speeds (Mesh_Motion 3)
-           ("c2_mhd_small_mm3.npz", {"Mesh_Motion": 3})]
+           ("c2_mhd_small_mm3.npz", {"Mesh_Motion": 3}),
+           # one passive scalar (NUM_N = 1): HLLD (hlld.c:345-348), HLLC, and the sink terms (the nozzle tags its
+           # mass in the scalar, sink.c:137-138)
+           ("c2_mhd_small_n1.npz", {"Num_N": 1}),
+           ("c2_mhd_small_hllc_n1.npz", {"Num_N": 1, "Riemann_Solver": 1}),
+           ("c2_mhd_sinkterms_n1.npz", dict(Num_N=1, Init_Par5=0.0, Cool_Type=2, Cool_Par1=50.0, DampOuter_Type=1,
+                                            DampOuter_Time=1.0, DampOuter_Len=0.8, Nozzle_Type=1, Nozzle_Par1=2.0,
+                                            Nozzle_Par2=0.4, Nozzle_Par3=0.7, Nozzle_Par4=20.0))]
 
 
 @pytest.mark.parametrize("fixture,over", SOLVERS, ids=["hlld", "hllc", "hll", "mm1", "outflow", "sinkterms", "rk3",
-                                                       "fe", "rot", "mm3"])
+                                                       "fe", "rot", "mm3", "hlld_n1", "hllc_n1", "sinkterms_n1"])
 def test_mhd_matches_reference(fixture, over):
     ref = load_fixture(fixture)
     # the same seeded start for every solver

@@ -55,6 +55,11 @@ CASES = {
     # carry it as prim[5] * rho
     "euler_hllc_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1}),
     "euler_hll_n1": ("c1_kepler2d.par", {"Num_R": 16, "Num_N": 1, "Riemann_Solver": 0}),
+    # ... and for MHD, HLLD's rule too (the scalar rides on the density of the
+    # contact's side, hlld.c:345-348)
+    "mhd_hlld_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1}),
+    "mhd_hllc_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1, "Riemann_Solver": 1}),
+    "mhd_hll_n1": ("c2_mhd3d.par", {"Num_R": 12, "Num_Z": 4, "Num_N": 1, "Riemann_Solver": 0}),
 }
 
 
