@@ -30,8 +30,10 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3)),
          ("c3_binary_disk.par", dict(Num_R=24)),
-         ("c5_grmhd_torus.par", dict(Num_R=16, Num_Z=12))]
-IDS = ["mhd", "binary2d", "grmhd_a09"]
+         ("c5_grmhd_torus.par", dict(Num_R=16, Num_Z=12)),
+         # a passive scalar rides in the cell records after the fields (h5out.c:78-82)
+         ("c2_mhd3d.par", dict(Num_R=12, Num_Z=4, CFL=0.3, Num_N=1))]
+IDS = ["mhd", "binary2d", "grmhd_a09", "mhd_scalar"]
 
 
 def _same_state(a, b, dt=True):
