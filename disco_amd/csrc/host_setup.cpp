@@ -211,6 +211,10 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
     ph->cs2_choice = par.Cs2_Profile;
     ph->mach = par.Disk_Mach;
     ph->gm = par.metricPar2;
+    {  // omega.c:39-43, 170: profile 3's constant, the same operations in the same order
+        const double M = par.metricPar2, H0 = par.initPar4, r1 = par.initPar2;
+        ph->cs2_3 = M * H0 * H0 / (2 * r1 * r1 * r1);
+    }
     ph->eps = par.grav_eps;
     ph->mesh_om_choice = par.Exact_Mesh_Omega;
     ph->en_om_choice = par.Energy_Omega;
@@ -364,7 +368,9 @@ static int unsupported(const disco_params& p, std::string* err) {
     if (p.RotFrame && gr) m += " RotFrame with grmhd/greuler";
     if (p.noiseType < 0 || p.noiseType > 3) m += " Noise_Type";
     if (p.include_atmos) m += " Include_Atmos";
-    if (p.Cs2_Profile == 3 || p.Cs2_Profile == 5) m += " Cs2_Profile";
+    if (p.Cs2_Profile == 5) m += " Cs2_Profile 5";
+    // profile 3 divides by the fiducial radius Init_Par2 (omega.c:170)
+    if (p.Cs2_Profile == 3 && p.initPar2 == 0.0) m += " Cs2_Profile 3 needs Init_Par2 (r1) != 0";
     if (p.Num_R < 1 || p.Num_Z < 1) m += " Num_R/Num_Z";
     if (!m.empty()) {
         if (err) *err = "unsupported configuration:" + m;

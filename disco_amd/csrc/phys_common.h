@@ -21,6 +21,7 @@ struct PhysParams {
     int cs2_choice;
     double mach;
     double gm;          // Metric_Par2, used by get_cs2 profile 4
+    double cs2_3;       // get_cs2 profile 3: M H0^2 / (2 r1^3), formed at set-up as omega.c:170 forms it
     double eps;         // Softening (omega.c, planets)
     int mesh_om_choice; // Exact_Mesh_Omega
     int en_om_choice;   // Energy_Omega
@@ -222,6 +223,7 @@ DHD double get_cs2(const PhysParams& P, const double* x) {
     double r = x[0];
     if (P.cs2_choice == 1) return 1. / (P.mach * P.mach);
     if (P.cs2_choice == 2) return .5 / P.mach / P.mach / pow(r, 2. * .5);
+    if (P.cs2_choice == 3) return P.cs2_3;  // constant: r1 = Init_Par2, H0 = Init_Par4, M = Metric_Par2
     if (P.cs2_choice == 4) {
         r = sqrt(r * r + P.eps * P.eps);
         double v2 = P.gm / r;

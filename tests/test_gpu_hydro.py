@@ -107,6 +107,8 @@ CASES = [
                                                   Nozzle_Par2=0.4, Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
     ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
                                                  Num_N=1)),
+    # get_cs2's constant profile 3 on the isothermal binary disk
+    ("c3_cs2p3.npz", "c3_binary_disk.par", dict(Num_R=24, Cs2_Profile=3, Init_Par2=1.2, Init_Par4=0.15)),
     # set_wcell's Mesh_Motion 2 (the face speeds, as 1) and 3 (their annulus mean)
     ("c1_mm2.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=2)),
     ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
