@@ -276,6 +276,23 @@ void derive_params(const disco_params& par, PhysParams* ph, disco_ic* ic) {
 // replaces v_r and v_z by fractions of the orthonormal azimuthal speed.
 static double noise_dev() { return 2 * ((double)std::rand() / (double)(RAND_MAX)) - 1; }
 
+// planet.c:114-136 (adjust_gas, called by domain.c:194-200 with Include_Atmos):
+// the gas brought to hydrostatic balance in a point mass's potential
+// (phigrav, planet.c:37-43, PLPOINTMASS with PHI_ORDER 2), at set-up only.
+static void adjust_gas(const Planet& pl, const double* x, double* prim, double gam) {
+    const double r = x[0], phi = x[1];
+    const double cosp = std::cos(phi), sinp = std::sin(phi);
+    const double dx = r * cosp - pl.r * std::cos(pl.phi);
+    const double dy = r * sinp - pl.r * std::sin(pl.phi);
+    const double d = std::sqrt(dx * dx + dy * dy);
+    const double n = 2.0;
+    const double pot = pl.M / std::pow(std::pow(d, n) + std::pow(pl.eps, n), 1. / n);
+    const double c2 = gam * prim[PPP] / prim[RHO];
+    const double factor = 1. + (gam - 1.) * pot / c2;
+    prim[RHO] *= factor;
+    prim[PPP] *= std::pow(factor, gam);
+}
+
 static void add_noise(const disco_params& par, double* prim, const double* x) {
     const double rel = par.noiseRel, ab = par.noiseAbs;
     if (par.noiseType == 1) {
@@ -367,7 +384,8 @@ static int unsupported(const disco_params& p, std::string* err) {
     if ((dt4[0] || dt4[1] || dt4[2] || dt4[3]) && gr) m += " Damp*_Type with grmhd/greuler";
     if (p.RotFrame && gr) m += " RotFrame with grmhd/greuler";
     if (p.noiseType < 0 || p.noiseType > 3) m += " Noise_Type";
-    if (p.include_atmos) m += " Include_Atmos";
+    if (p.include_atmos && p.hydro != DISCO_HYDRO_EULER && p.hydro != DISCO_HYDRO_MHD)
+        m += " Include_Atmos (euler, mhd)";
     if (p.Cs2_Profile == 5) m += " Cs2_Profile 5";
     // profile 3 divides by the fiducial radius Init_Par2 (omega.c:170)
     if (p.Cs2_Profile == 3 && p.initPar2 == 0.0) m += " Cs2_Profile 3 needs Init_Par2 (r1) != 0";
@@ -996,6 +1014,8 @@ int setup_run(const disco_params& par, HostRun* Rp, std::string* err) {
                 double dV = get_dV(xp, xm, R.par.phimax);
                 disco_initial(&R.ic, prim, x);
                 if (par.RotFrame) prim[UPP] -= par.RotOmega;
+                if (par.include_atmos)
+                    for (const Planet& pl : R.planets) adjust_gas(pl, x, prim, par.Adiabatic_Index);
                 if (par.noiseType) add_noise(par, prim, x);
                 for (int q = 0; q < R.nq; ++q) R.prim[(size_t)q * R.ncell + c] = prim[q];
                 host_prim2cons(R, prim, cons, x, dV);

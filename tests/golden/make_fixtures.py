@@ -401,6 +401,10 @@ SCALAR_CASES = [  # one passive scalar (NUM_N = 1; euler.c:60-62, 97-99, 136-138
                                                   Nozzle_Par3=0.8, Nozzle_Par4=10.0), "euler_sun_n1", 10, (1,)),
     ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
                                                  Num_N=1), "euler_sun_n1", 10, (1,)),
+    # Include_Atmos (domain.c:194-200, planet.c:114-136): the initial gas rescaled in each planet's potential,
+    # adiabatic around the sun and isothermal around the binary
+    ("c1_atmos.npz", "c1_kepler2d.par", dict(Num_R=16, Include_Atmos=1), "euler_sun", 10, (1,)),
+    ("c3_atmos.npz", "c3_binary_disk.par", dict(Num_R=24, Include_Atmos=1), "euler_binary", 10, (1,)),
     # the isothermal binary disk with get_cs2's constant profile 3 (omega.c:168-171; r1 = 1.2, H0 = 0.15)
     ("c3_cs2p3.npz", "c3_binary_disk.par", dict(Num_R=24, Cs2_Profile=3, Init_Par2=1.2, Init_Par4=0.15),
      "euler_binary", 10, (1,)),

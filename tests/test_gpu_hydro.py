@@ -107,12 +107,22 @@ CASES = [
                                                   Nozzle_Par2=0.4, Nozzle_Par3=0.8, Nozzle_Par4=10.0)),
     ("c2e_visc_n1.npz", "c1_kepler2d.par", dict(Num_R=12, Num_Z=6, Z_Periodic=1, Use_Viscosity=1, Viscosity=0.02,
                                                  Num_N=1)),
+    # Include_Atmos: the initial gas rescaled in the planets' potential (set-up only)
+    ("c1_atmos.npz", "c1_kepler2d.par", dict(Num_R=16, Include_Atmos=1)),
+    ("c3_atmos.npz", "c3_binary_disk.par", dict(Num_R=24, Include_Atmos=1)),
     # get_cs2's constant profile 3 on the isothermal binary disk
     ("c3_cs2p3.npz", "c3_binary_disk.par", dict(Num_R=24, Cs2_Profile=3, Init_Par2=1.2, Init_Par4=0.15)),
     # set_wcell's Mesh_Motion 2 (the face speeds, as 1) and 3 (their annulus mean)
     ("c1_mm2.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=2)),
     ("c1_mm3.npz", "c1_kepler2d.par", dict(Num_R=16, Mesh_Motion=3)),
 ]
+
+
+# set-ups whose initial prims agree with the reference's to an ulp rather
+# than to the bit: Include_Atmos around the binary evaluates cos, sin and pow
+# of the planets' offsets (planet.c:114-136), and one cell of the 3,079 comes
+# out one ulp apart between the two host compilers' calls
+ROUNDED_START = {"c3_atmos.npz"}
 
 
 def _compare(st, ref, tol, label):
@@ -139,7 +149,11 @@ def test_hydro_matches_reference(fixture, base, over):
     dom = D.Domain(params(base, **over))
     st0 = dom.state(0)
     real = ref[0]["real"] == 1
-    assert np.array_equal(st0["prim"][:, real], ref[0]["prim"][:, real])
+    if fixture in ROUNDED_START:
+        e0 = np.abs(st0["prim"][:, real] - ref[0]["prim"][:, real]).max(axis=1)
+        assert (e0 <= 1e-15 * np.abs(ref[0]["prim"][:, real]).max(axis=1)).all(), e0
+    else:
+        assert np.array_equal(st0["prim"][:, real], ref[0]["prim"][:, real])
     assert np.array_equal(st0["piph"], ref[0]["piph"])
     done = 0
     for s in sorted(ref):

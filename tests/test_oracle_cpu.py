@@ -259,8 +259,12 @@ def test_unsupported_configuration_is_rejected():
         D.setup_run(params("c2_mhd3d.par", Num_R=12, Num_Z=4, Use_Viscosity=1, Viscosity=0.01))
     with pytest.raises(RuntimeError, match="Viscosity > 0"):
         D.setup_run(params("c1_kepler2d.par", Use_Viscosity=1))
+    # the atmosphere is set-up work of the Newtonian modules (domain.c:194-200)
     with pytest.raises(RuntimeError, match="Include_Atmos"):
-        D.setup_run(params("c1_kepler2d.par", Include_Atmos=1))
+        D.setup_run(params("c5_grmhd_torus.par", Num_R=16, Num_Z=12, Include_Atmos=1))
+    # get_cs2's profile 5 needs the planets inside cons2prim: refused
+    with pytest.raises(RuntimeError, match="Cs2_Profile 5"):
+        D.setup_run(params("c3_binary_disk.par", Num_R=24, Cs2_Profile=5))
     # sink.c's types beyond the built ones are refused, not ignored
     for key, msg in (("Sink_Type", "Sink_Type"), ("Nozzle_Type", "Nozzle_Type"), ("DampInner_Type", "Damp"),
                      ("Cool_Type", "Cool_Type")):
